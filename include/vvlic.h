@@ -1,0 +1,223 @@
+/*
+ * vvlic.h — C ABI of the MI355X-native 3D LIC + ray-cast engine.
+ *
+ * This ABI replaces the reference's C++ entry points on the LIC-volume and
+ * ray-cast path (reference paths relative to VectorVisualization/, "VV/"):
+ *   VectorDataSet::loadData + createTextureIterp   VV/dataset.cpp:97-180, 290-366, 533-635
+ *   NoiseDataSet::loadData / createTexture          VV/dataset.cpp:1124-1344
+ *   computeGradients / filterGradients / quantize   VV/gradient.cpp:190-532
+ *   LICFilter::createBoxFilter / loadData           VV/dataset.cpp:1405-1512
+ *   TransferEdit::updateTextures                    VV/transferEdit.cpp:480-543
+ *   LICParams + Renderer::setRenderVolParams        VV/types.h:91-109, VV/renderer.cpp:925-996
+ *   Renderer::updateLICVolume / renderLICVolume     VV/renderer.cpp:1311-1379
+ *   Renderer::render / raycastLICVolume             VV/renderer.cpp:126-312, 1381-1405
+ *   Camera::setCamera, Renderer::updateLightPos     VV/camera.cpp:42-68, VV/renderer.cpp:431-466
+ *
+ * Conventions
+ *   - Every call returns vv_status; VV_OK is 0.  A failing call leaves the
+ *     context usable and stores a message readable with vv_last_error().
+ *     Nothing aborts or exits.
+ *   - Volumes are x-fastest arrays of dims[0]*dims[1]*dims[2] voxels; vector
+ *     fields are 3 interleaved float32 per voxel (VV/reader.cpp:290-295).
+ *   - Images are width*height RGBA float32, row 0 at the bottom (OpenGL
+ *     window convention).
+ *   - Pointer arguments marked "host or device" take a memory-kind flag
+ *     `on_device` (0 = host memory, 1 = device memory of the context's GPU).
+ *   - All device work is ordered on the context's stream (vv_stream());
+ *     calls that return data to host memory synchronise that stream.
+ *   - A context is not thread-safe; use one context per host thread.
+ */
+#ifndef VVLIC_H
+#define VVLIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VVLIC_ABI_VERSION 1
+
+typedef enum vv_status {
+    VV_OK = 0,
+    VV_EINVAL = 1,        /* bad argument (size, pointer, NaN, range) */
+    VV_ENOMEM = 2,        /* device or host allocation failed */
+    VV_EDEVICE = 3,       /* HIP runtime error */
+    VV_EUNSUPPORTED = 4,  /* valid request this build does not implement */
+    VV_ESTATE = 5         /* required input (field, noise, TF, ...) not loaded */
+} vv_status;
+
+typedef enum vv_storage {
+    VV_STORE_FP32 = 0,    /* 4 x float32 per voxel: encoded direction + |v|/max */
+    VV_STORE_FP16 = 1     /* 4 x float16 per voxel, FP32 arithmetic */
+} vv_storage;
+
+typedef enum vv_filter_kind {
+    VV_FILTER_BOX = 0,
+    VV_FILTER_TRIANGLE = 1,
+    VV_FILTER_COS2 = 2,
+    VV_FILTER_GAUSSIAN = 3
+} vv_filter_kind;
+
+typedef enum vv_noise_kind {
+    VV_NOISE_BINARY = 0,  /* {0,255}, 255 with probability `density` */
+    VV_NOISE_UNIFORM = 1  /* dense uniform bytes */
+} vv_noise_kind;
+
+typedef enum vv_illum {
+    VV_ILLUM_LIC = 0,      /* illumLIC, VV/shader/inc_illum.glsl:158-172 */
+    VV_ILLUM_GRADIENT = 1  /* illumGradient, VV/shader/inc_illum.glsl:1-41 */
+} vv_illum;
+
+typedef enum vv_tf_index {
+    VV_TF_INDEX_B = 0,    /* field .b, as both reference ray casters do */
+    VV_TF_INDEX_MAG = 1   /* field .a = |v|/max|v| ("vector-length TF") */
+} vv_tf_index;
+
+/* Mirrors LICParams (VV/types.h:91-109) plus the shader switches the
+ * reference selects with #defines.  vv_default_lic_params() fills the
+ * reference defaults (VV/types.h:93-98). */
+typedef struct vv_lic_params {
+    float step_size_vol;   /* ray step, default 1/128; alphaCorrection = 128*step */
+    float gradient_scale;  /* LIC intensity scale, default 30 */
+    float illum_scale;     /* default 1 */
+    float freq_scale;      /* noise frequency, default 1 */
+    int   num_iterations;  /* ray loop bound n (n*n samples max), default 255 */
+    int   steps_forward;   /* F, default 32 */
+    int   steps_backward;  /* B, default 32 */
+    float step_size_lic;   /* default 0.01 */
+    float eye_factor;      /* (0.5*logEyeDist + 0.3); logEyeDist is never set: 0.3 */
+    int   illum_mode;      /* vv_illum; VV_ILLUM_GRADIENT needs RGBA noise */
+    int   tf_index;        /* vv_tf_index */
+    int   gate_enable;     /* scalar gate of freqSampling (needs vv_upload_scalar) */
+    float gate_lo;         /* 0.1 */
+    float gate_hi;         /* 0.3 */
+} vv_lic_params;
+
+/* Mirrors Camera (VV/camera.cpp:42-68) and the light set by
+ * Renderer::updateLightPos (VV/renderer.cpp:431-466). */
+typedef struct vv_camera {
+    int   width, height;
+    float fovy_deg;        /* 35 */
+    float dist;            /* 4 */
+    float pos[3];          /* camera translation, (0,0,0) */
+    float quat[4];         /* orientation x,y,z,w, identity */
+    float light_pos[3];    /* eye-space light position (gl_LightSource[0].position) */
+    float light_diffuse;   /* 1 */
+    float light_specular;  /* 1 */
+    float light_ambient;   /* 0 */
+    int   spec_exponent;   /* GL_SPOT_EXPONENT, 40 (VV/3DLIC.cpp:736) */
+} vv_camera;
+
+typedef struct vv_ctx vv_ctx;
+
+/* ---- context ---------------------------------------------------- */
+vv_status vv_create(vv_ctx** out, int device);
+vv_status vv_destroy(vv_ctx* ctx);
+/* Message of the last failing call on ctx (or of vv_create when ctx is NULL). */
+const char* vv_last_error(const vv_ctx* ctx);
+int vv_abi_version(void);
+/* hipStream_t the context orders its work on. */
+void* vv_stream(vv_ctx* ctx);
+vv_status vv_synchronize(vv_ctx* ctx);
+vv_status vv_default_lic_params(vv_lic_params* p);
+/* camera defaults for a volume centred at center: fovy 35, dist 4, light at
+ * center + (0,0,1) (light distance 1, VV/3DLIC.cpp:681) */
+vv_status vv_default_camera(vv_camera* cam, int width, int height, const float center[3]);
+
+/* ---- a1: vector field (VectorDataSet::loadData + createTextureIterp) ----
+ * v0, v1: 3*N float32 (host or device); v1 may be NULL (no time lerp).
+ * v = v0 + lerp_t*(v1 - v0) is encoded on the GPU as
+ * rgb = 0.5 v/|v| + 0.5, a = |v|/max|v| and kept in `storage`.
+ * spacing = DAT slice distances; extent/scale/center follow
+ * VV/dataset.cpp:144-176. */
+vv_status vv_upload_field(vv_ctx* ctx, const float* v0, const float* v1, float lerp_t,
+                          const int dims[3], const float spacing[3], int storage,
+                          int on_device);
+/* extent[3], scale[3], center[3] of the loaded field (VolumeData). */
+vv_status vv_field_geometry(const vv_ctx* ctx, float extent[3], float scale[3],
+                            float center[3]);
+/* Encoded field, 4*N values (float32, also for FP16 storage). */
+vv_status vv_download_field(vv_ctx* ctx, float* out4, int on_device);
+
+/* ---- a2/a3: noise (NoiseDataSet) --------------------------------------
+ * noise: N bytes (host or device).  with_gradients != 0 builds the RGBA
+ * texture (grad.xyz, noise) of the reference's -g mode, computing the
+ * gradients on the GPU (computeGradients + filterGradients + quantize8).
+ * grad_rgb (3N bytes) may supply precomputed gradients instead (.grd cache). */
+vv_status vv_upload_noise(vv_ctx* ctx, const uint8_t* noise, const int dims[3],
+                          int with_gradients, const uint8_t* grad_rgb, int on_device);
+/* Seeded noise generated on the GPU (SplitMix64 of seed and voxel index). */
+vv_status vv_generate_noise(vv_ctx* ctx, uint64_t seed, int kind, double density,
+                            const int dims[3], int with_gradients);
+/* N bytes (1 channel) or 4N bytes (RGBA: grad.xyz, noise). */
+vv_status vv_download_noise(vv_ctx* ctx, uint8_t* out, int on_device);
+vv_status vv_noise_info(const vv_ctx* ctx, int dims[3], int* channels);
+
+/* ---- a4: LIC filter kernel (LICFilter) -------------------------------- */
+/* Synthesised 256-entry LUT: box (createBoxFilter), triangle, cos^2,
+ * Gaussian exp(-s^2/(2 sigma^2)), s = (2t-1)*length (sigma <= 0: length/4). */
+vv_status vv_make_filter(int kind, int length, float sigma, uint8_t lut[256],
+                         int* width, float* inv_area);
+/* Any LUT (e.g. a filter PNG row, VV/dataset.cpp:1416-1467): width bytes
+ * (width a power of two <= 4096).  inv_area <= 0 computes
+ * 0.5*W*255/sum (calcFilterKernelInvArea). */
+vv_status vv_set_filter(vv_ctx* ctx, const uint8_t* lut, int width, float inv_area);
+
+/* ---- a9: transfer function (TransferEdit::updateTextures) ------------- */
+/* rgba: 256x4 float32, opacity: 256 float32 (the LA texture's alpha). */
+vv_status vv_set_tf(vv_ctx* ctx, const float* rgba, const float* opacity);
+/* 256x5 bytes R,G,B,A,LIC-opacity as TransferEdit::_tfData. */
+vv_status vv_set_tf_u8(vv_ctx* ctx, const uint8_t* tf5);
+/* default table of TransferEdit's constructor (VV/transferEdit.cpp:76-82) */
+vv_status vv_default_tf_u8(uint8_t tf5[256 * 5]);
+
+/* ---- scalar gate volume (VolumeDataSet, freqSampling) ----------------- */
+vv_status vv_upload_scalar(vv_ctx* ctx, const uint8_t* scalar, const int dims[3],
+                           int on_device);
+
+/* ---- a11: parameters -------------------------------------------------- */
+vv_status vv_set_lic_params(vv_ctx* ctx, const vv_lic_params* p);
+vv_status vv_get_lic_params(const vv_ctx* ctx, vv_lic_params* p);
+
+/* ---- a5/a6: LIC volume (Renderer::updateLICVolume) -------------------- */
+/* Computes the LIC volume at the field's resolution into the context's
+ * volume buffer: 1 float per voxel (scalar noise) or 4 (RGBA noise). */
+vv_status vv_compute_lic(vv_ctx* ctx);
+vv_status vv_lic_info(const vv_ctx* ctx, int dims[3], int* channels);
+vv_status vv_download_lic(vv_ctx* ctx, float* out, int on_device);
+/* Replace the LIC volume with caller data (channels 1 or 4). */
+vv_status vv_upload_lic(vv_ctx* ctx, const float* lic, int channels, const int dims[3],
+                        int on_device);
+
+/* ---- a7/a8/a10: ray cast (Renderer::render, raycastLICVolume) --------- */
+/* image: width*height*4 float32 (host or device).  samples_out, if not
+ * NULL, receives the total number of compositing samples (host). */
+vv_status vv_render(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
+                    uint64_t* samples_out);
+
+/* ---- differentiation ---------------------------------------------------
+ * Per-voxel opacity scale sigma (N float32 at the LIC-volume resolution,
+ * host or device) multiplies each sample's alpha before opacity
+ * correction.  NULL removes it (sigma == 1). */
+vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
+/* Gradient of a loss L w.r.t. the 256x4 RGBA transfer function and the
+ * opacity scale, given dL/dimage (width*height*4).  dtf: 256*4 floats,
+ * dsigma: N floats or NULL.  Checkpoints every 64 samples; reductions are
+ * deterministic (fixed-point integer accumulation). */
+vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
+                            float* dtf, float* dsigma, int on_device);
+
+/* ---- helpers exposed for tests --------------------------------------- */
+/* The 24 floats + 2 ints of the per-frame ray setup (see vv_view_block in
+ * the implementation): cam[3] du[3] dv[3] dw[3] inv_w2 inv_h2 extent[3]
+ * scale[3] tex_max[3], then width, height. */
+vv_status vv_view_block(const vv_ctx* ctx, const vv_camera* cam, float out24[24]);
+/* Convolution weights k(u_i) for the current filter and steps: 1+B+F floats. */
+vv_status vv_lic_weights(const vv_ctx* ctx, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

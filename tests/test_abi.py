@@ -1,0 +1,91 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+include/vvlic.h declares, and its host-only entry points and error paths
+behave as documented."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import vectorvisualization_amd as vv
+from vectorvisualization_amd import _lib
+from oracle import oracle as orc
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "vvlic.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vv_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_what_the_binding_lists():
+    assert declared_functions() == sorted(_lib.EXPORTED)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(_lib.LIB_PATH))
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_abi_version():
+    assert vv.LIB.vv_abi_version() == 1
+
+
+def test_default_params_mirror_licparams():
+    p = vv.default_lic_params()
+    # VV/types.h:93-98
+    assert p.step_size_vol == np.float32(1.0 / 128.0)
+    assert (p.gradient_scale, p.illum_scale, p.freq_scale) == (30.0, 1.0, 1.0)
+    assert (p.num_iterations, p.steps_forward, p.steps_backward) == (255, 32, 32)
+    assert p.step_size_lic == np.float32(0.01)
+    assert p.eye_factor == np.float32(0.3)
+
+
+def test_default_camera_and_light():
+    c = vv.default_camera(640, 480, (0.5, 0.25, 0.5))
+    assert (c.fovy_deg, c.dist, list(c.quat)) == (35.0, 4.0, [0, 0, 0, 1])
+    np.testing.assert_allclose(list(c.light_pos), orc.light_position((0.5, 0.25, 0.5)), rtol=1e-7)
+    assert c.spec_exponent == 40
+
+
+def test_default_tf_matches_transferedit_constructor():
+    t = vv.default_tf_u8().reshape(256, 5)
+    i = np.arange(256)
+    assert np.all(t[:, :3] == i[:, None])
+    assert np.all(t[:, 3:] == np.where(i < 20, 0, i - 20)[:, None])
+
+
+@pytest.mark.parametrize("kind,L", [(vv.FILTER_BOX, 20), (vv.FILTER_TRIANGLE, 30),
+                                    (vv.FILTER_COS2, 60), (vv.FILTER_GAUSSIAN, 40)])
+def test_make_filter_matches_oracle(kind, L):
+    a, ia = vv.make_filter(kind, L)
+    b, ib = orc.make_filter(kind, L)
+    assert np.array_equal(a, b) and ia == ib
+
+
+def test_make_filter_rejects_bad_kind():
+    lut = np.zeros(256, np.uint8)
+    w, inv = C.c_int(), C.c_float()
+    assert vv.LIB.vv_make_filter(9, 10, 0.0, lut.ctypes.data, C.byref(w), C.byref(inv)) == 1
+
+
+def test_create_without_gpu_reports_error(monkeypatch):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(vv.VVError) as e:
+        vv.Context(0)
+    assert e.value.status == 3  # VV_EDEVICE
+    assert "device" in vv.LIB.vv_last_error(None).decode()
+
+
+def test_null_arguments_rejected():
+    assert vv.LIB.vv_default_lic_params(None) == 1
+    assert vv.LIB.vv_default_camera(None, 1, 1, None) == 1
+    assert vv.LIB.vv_synchronize(None) == 1
+    assert vv.LIB.vv_destroy(None) == 0

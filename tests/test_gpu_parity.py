@@ -1,0 +1,252 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the
+same seeded inputs.  Tolerance for floating point: 1e-5 relative (max abs
+difference over max abs value) in FP32; integer and byte outputs bit-exact.
+"""
+import numpy as np
+import pytest
+
+import vectorvisualization_amd as vv
+from oracle import oracle as orc
+
+from conftest import rel_err
+from workloads import lic_setup, oracle_lic, field
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _report(name, got, ref):
+    same = np.mean(np.asarray(got) == np.asarray(ref))
+    print(f"{name}: rel_err={rel_err(got, ref):.3e} bit-identical={same:.6f}", flush=True)
+
+
+# ---- a1: field encode -----------------------------------------------------
+@pytest.mark.parametrize("kind,n", [("abc", 24), ("rankine", 20), ("uniform", 8)])
+def test_field_encode_fp32_bitexact(ctx, kind, n):
+    v = field(kind, n)
+    v1 = field("abc", n) if kind != "abc" else field("uniform", n)
+    ctx.upload_field(v, (n, n, n), v1=v1, lerp_t=0.3)
+    got = ctx.download_field((n, n, n))
+    ref = orc.encode_field(v, (n, n, n), v1=v1, t=0.3)
+    _report("encode", got, ref)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_field_encode_fp16_bitexact(ctx):
+    n = 20
+    v = field("rankine", n)
+    ctx.upload_field(v, (n, n, n), storage=vv.STORE_FP16)
+    got = ctx.download_field((n, n, n))
+    ref = orc.f16_to_f32(orc.f32_to_f16(orc.encode_field(v, (n, n, n))))
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_field_zero_vector(ctx):
+    n = 4
+    v = np.zeros((n, n, n, 3), np.float32)
+    v[1, 2, 3] = (3.0, 0.0, 4.0)
+    ctx.upload_field(v, (n, n, n))
+    got = ctx.download_field((n, n, n)).reshape(n, n, n, 4)
+    assert np.all(got[0, 0, 0] == [0.5, 0.5, 0.5, 0.0])
+    np.testing.assert_allclose(got[1, 2, 3], [0.8, 0.5, 0.9, 1.0], rtol=0, atol=1e-7)
+
+
+# ---- a2/a3: noise ---------------------------------------------------------
+@pytest.mark.parametrize("kind,density", [(vv.NOISE_BINARY, 0.05), (vv.NOISE_UNIFORM, 0.0)])
+def test_noise_generation_bitexact(ctx, kind, density):
+    dims = (33, 17, 9)
+    ctx.generate_noise(7, kind, density, dims)
+    got = ctx.download_noise()
+    ref = orc.gen_noise(7, kind, density, dims)
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_noise_gradients_bitexact(ctx):
+    dims = (20, 18, 16)
+    ctx.generate_noise(3, vv.NOISE_BINARY, 0.2, dims, with_gradients=True)
+    got = ctx.download_noise().reshape(-1, 4)
+    n = orc.gen_noise(3, vv.NOISE_BINARY, 0.2, dims)
+    g = orc.noise_gradients(n, dims).reshape(-1, 3)
+    np.testing.assert_array_equal(got[:, 3], n)
+    np.testing.assert_array_equal(got[:, :3], g)
+
+
+def test_noise_upload_with_precomputed_gradients(ctx):
+    dims = (8, 8, 8)
+    n = orc.gen_noise(5, vv.NOISE_UNIFORM, 0.0, dims)
+    g = np.arange(3 * n.size, dtype=np.uint8)
+    ctx.upload_noise(n, dims, with_gradients=True, grad_rgb=g)
+    got = ctx.download_noise().reshape(-1, 4)
+    np.testing.assert_array_equal(got[:, :3].ravel(), g)
+    np.testing.assert_array_equal(got[:, 3], n)
+
+
+# ---- a4/a11: weights ------------------------------------------------------
+@pytest.mark.parametrize("filt,L", [(vv.FILTER_BOX, 20), (vv.FILTER_GAUSSIAN, 40),
+                                    (vv.FILTER_COS2, 60), (vv.FILTER_TRIANGLE, 30)])
+def test_lic_weights_equal_oracle(ctx, filt, L):
+    lut, inv = vv.make_filter(filt, L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    ctx.set_lic_params(p)
+    np.testing.assert_array_equal(ctx.lic_weights(), orc.lic_weights(lut, L, L))
+
+
+# ---- a5/a6: LIC volume ----------------------------------------------------
+LIC_CASES = [
+    dict(n=32, kind="abc", filt=vv.FILTER_BOX, L=20, h=0.01),
+    dict(n=24, kind="rankine", noise_kind=vv.NOISE_UNIFORM, filt=vv.FILTER_GAUSSIAN, L=40, h=0.005),
+    dict(n=20, kind="gyre", filt=vv.FILTER_TRIANGLE, L=30, h=0.02),
+    dict(n=16, kind="abc", filt=vv.FILTER_COS2, L=12, h=0.04, freq=2.0),
+    dict(n=16, kind="rankine", grad=True, L=10, density=0.3),
+    dict(n=20, kind="rankine", storage=vv.STORE_FP16, filt=vv.FILTER_COS2, L=15, h=0.005),
+]
+
+
+@pytest.mark.parametrize("case", LIC_CASES, ids=lambda c: "-".join(f"{k}={v}" for k, v in c.items()))
+def test_lic_parity(ctx, case):
+    c = lic_setup(ctx, **case)
+    ctx.compute_lic()
+    got = ctx.download_lic()
+    ref = oracle_lic(c)
+    _report("lic", got, ref)
+    assert got.shape == ref.shape
+    assert np.all(np.isfinite(got))
+    assert rel_err(got, ref) <= TOL
+
+
+def test_lic_scalar_gate(ctx):
+    n = 16
+    rng = np.random.default_rng(0)
+    gate = rng.integers(0, 120, n * n * n, dtype=np.uint8)
+    c = lic_setup(ctx, n=n, kind="abc", L=8, density=0.5, gate=gate)
+    ctx.compute_lic()
+    got = ctx.download_lic()
+    ref = oracle_lic(c)
+    assert np.any(ref != 0)
+    assert rel_err(got, ref) <= TOL
+
+
+def test_lic_non_cubic(ctx):
+    dims = (24, 16, 12)
+    rng = np.random.default_rng(1)
+    v = rng.normal(size=(dims[2], dims[1], dims[0], 3)).astype(np.float32)
+    ctx.upload_field(v, dims, spacing=(1.0, 1.0, 2.0))
+    ctx.generate_noise(2, vv.NOISE_UNIFORM, 0.0, (16, 16, 16))
+    p = vv.default_lic_params()
+    p.steps_forward, p.steps_backward = 6, 9
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    got = ctx.download_lic()
+    ext, sc, cen = ctx.field_geometry()
+    f4 = orc.encode_field(v, dims)
+    noise = orc.gen_noise(2, vv.NOISE_UNIFORM, 0.0, (16, 16, 16))
+    lut = np.full(256, 255, np.uint8)
+    op = orc.params(steps_forward=6, steps_backward=9)
+    ref = orc.lic(f4, dims, noise, 1, (16, 16, 16), lut, op, scale=sc)
+    assert rel_err(got, ref) <= TOL
+
+
+def test_lic_deterministic(ctx):
+    lic_setup(ctx, n=24, kind="rankine", L=16)
+    ctx.compute_lic()
+    a = ctx.download_lic()
+    ctx.compute_lic()
+    b = ctx.download_lic()
+    np.testing.assert_array_equal(a, b)
+
+
+# ---- a7-a10: ray cast ----------------------------------------------------
+def _render_case(ctx, c, w, h, tf_index=vv.TF_INDEX_B, step=1.0 / 128.0, quat=(0, 0, 0, 1),
+                 tf=None, sigma=None, illum_scale=1.0):
+    p = c["params"]
+    p.tf_index = tf_index
+    p.step_size_vol = step
+    p.illum_scale = illum_scale
+    ctx.set_lic_params(p)
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(w, h, cen)
+    for i in range(4):
+        cam.quat[i] = quat[i]
+    if tf is None:
+        rgba, op = vv.fields.tf_linear_ramp()
+    else:
+        rgba, op = tf
+    ctx.set_tf(rgba, op)
+    if sigma is not None:
+        ctx.set_opacity_scale(sigma)
+    img, nsamp = ctx.render(cam, count_samples=True)
+    view = orc.view_setup(w, h, quat=[float(np.float32(q)) for q in quat], center=cen, extent=ext,
+                          scale=sc)
+    np.testing.assert_array_equal(ctx.view_block(cam), view.as_array())
+    ldims, lch = ctx.lic_info()
+    licv = ctx.download_lic()
+    oparams = c["op"]
+    oparams.tf_index = tf_index
+    oparams.step_size_vol = step
+    oparams.alpha_correction = float(np.float32(step) * np.float32(128.0))
+    oparams.illum_scale = illum_scale
+    lp = list(cam.light_pos)
+    oparams.light[0], oparams.light[1], oparams.light[2] = lp
+    ref, ns, total = orc.raycast(c["f4"], c["dims"], licv, lch, ldims, rgba, op, oparams, view,
+                                 sigma=sigma)
+    return img, ref, nsamp, total
+
+
+@pytest.mark.parametrize("tf_index", [vv.TF_INDEX_B, vv.TF_INDEX_MAG])
+def test_render_illum_lic(ctx, tf_index):
+    c = lic_setup(ctx, n=32, kind="abc", L=10)
+    ctx.compute_lic()
+    img, ref, nsamp, total = _render_case(ctx, c, 96, 80, tf_index=tf_index)
+    _report("render", img, ref)
+    assert nsamp == total
+    assert np.max(ref) > 0
+    assert rel_err(img, ref) <= TOL
+
+
+def test_render_illum_gradient(ctx):
+    c = lic_setup(ctx, n=24, kind="rankine", grad=True, L=8, density=0.3)
+    ctx.compute_lic()
+    img, ref, nsamp, total = _render_case(ctx, c, 64, 64, quat=(0.2, 0.3, 0.1, 0.927))
+    _report("render-grad", img, ref)
+    assert nsamp == total
+    assert rel_err(img, ref) <= TOL
+
+
+def test_render_step_half_and_sigma(ctx):
+    c = lic_setup(ctx, n=20, kind="rankine", L=8)
+    ctx.compute_lic()
+    rng = np.random.default_rng(4)
+    sigma = rng.uniform(0.2, 1.5, 20 ** 3).astype(np.float32)
+    img, ref, nsamp, total = _render_case(ctx, c, 48, 40, step=1.0 / 256.0, sigma=sigma,
+                                          tf=vv.fields.tf_random(3), illum_scale=1.7)
+    assert nsamp == total
+    assert rel_err(img, ref) <= TOL
+
+
+def test_render_empty_tf_is_black(ctx):
+    c = lic_setup(ctx, n=16, kind="abc", L=4)
+    ctx.compute_lic()
+    rgba = np.zeros(1024, np.float32)
+    op = np.zeros(256, np.float32)
+    img, ref, _, _ = _render_case(ctx, c, 32, 32, tf=(rgba, op))
+    assert np.all(img == 0)
+
+
+# ---- error behaviour ------------------------------------------------------
+def test_errors_leave_context_usable(ctx):
+    with pytest.raises(vv.VVError) as e:
+        ctx.compute_lic()
+    assert e.value.status == 5  # VV_ESTATE
+    with pytest.raises(vv.VVError):
+        ctx.upload_field(np.zeros(3, np.float32), (1, 1, 1))
+    bad = np.ones((4, 4, 4, 3), np.float32)
+    bad[0, 0, 0, 0] = np.nan
+    with pytest.raises(vv.VVError) as e:
+        ctx.upload_field(bad, (4, 4, 4))
+    assert e.value.status == 1  # VV_EINVAL
+    lic_setup(ctx, n=8, L=2)
+    ctx.compute_lic()
+    assert ctx.download_lic().size == 8 ** 3

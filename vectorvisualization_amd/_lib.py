@@ -1,0 +1,362 @@
+"""ctypes binding of the C ABI in include/vvlic.h (libvvlic.so).
+
+The library is the only compute path: importing this module fails loudly when
+libvvlic.so is missing, and every volume operation runs in its HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = Path(os.environ.get("VVLIC_LIB", _PKG / "lib" / "libvvlic.so"))
+
+VV_OK, VV_EINVAL, VV_ENOMEM, VV_EDEVICE, VV_EUNSUPPORTED, VV_ESTATE = range(6)
+STATUS_NAMES = {0: "VV_OK", 1: "VV_EINVAL", 2: "VV_ENOMEM", 3: "VV_EDEVICE",
+                4: "VV_EUNSUPPORTED", 5: "VV_ESTATE"}
+STORE_FP32, STORE_FP16 = 0, 1
+FILTER_BOX, FILTER_TRIANGLE, FILTER_COS2, FILTER_GAUSSIAN = range(4)
+NOISE_BINARY, NOISE_UNIFORM = 0, 1
+ILLUM_LIC, ILLUM_GRADIENT = 0, 1
+TF_INDEX_B, TF_INDEX_MAG = 0, 1
+
+
+class LicParams(C.Structure):
+    """vv_lic_params, mirror of LICParams (VV/types.h:91-109)."""
+
+    _fields_ = [
+        ("step_size_vol", C.c_float),
+        ("gradient_scale", C.c_float),
+        ("illum_scale", C.c_float),
+        ("freq_scale", C.c_float),
+        ("num_iterations", C.c_int),
+        ("steps_forward", C.c_int),
+        ("steps_backward", C.c_int),
+        ("step_size_lic", C.c_float),
+        ("eye_factor", C.c_float),
+        ("illum_mode", C.c_int),
+        ("tf_index", C.c_int),
+        ("gate_enable", C.c_int),
+        ("gate_lo", C.c_float),
+        ("gate_hi", C.c_float),
+    ]
+
+
+class Camera(C.Structure):
+    """vv_camera, mirror of Camera (VV/camera.cpp:42-68) plus the light."""
+
+    _fields_ = [
+        ("width", C.c_int),
+        ("height", C.c_int),
+        ("fovy_deg", C.c_float),
+        ("dist", C.c_float),
+        ("pos", C.c_float * 3),
+        ("quat", C.c_float * 4),
+        ("light_pos", C.c_float * 3),
+        ("light_diffuse", C.c_float),
+        ("light_specular", C.c_float),
+        ("light_ambient", C.c_float),
+        ("spec_exponent", C.c_int),
+    ]
+
+
+class VVError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"libvvlic.so not found at {LIB_PATH}; build it with "
+            "`python -m vectorvisualization_amd.build` (hipcc, gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    P, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
+    IP = C.POINTER(C.c_int)
+    FP = C.POINTER(C.c_float)
+    u8p = C.c_void_p
+    sig = {
+        "vv_create": (I, [C.POINTER(P), I]),
+        "vv_destroy": (I, [P]),
+        "vv_last_error": (C.c_char_p, [P]),
+        "vv_abi_version": (I, []),
+        "vv_stream": (P, [P]),
+        "vv_synchronize": (I, [P]),
+        "vv_default_lic_params": (I, [C.POINTER(LicParams)]),
+        "vv_default_camera": (I, [C.POINTER(Camera), I, I, FP]),
+        "vv_upload_field": (I, [P, P, P, F, IP, FP, I, I]),
+        "vv_field_geometry": (I, [P, FP, FP, FP]),
+        "vv_download_field": (I, [P, P, I]),
+        "vv_upload_noise": (I, [P, u8p, IP, I, u8p, I]),
+        "vv_generate_noise": (I, [P, C.c_uint64, I, D, IP, I]),
+        "vv_download_noise": (I, [P, u8p, I]),
+        "vv_noise_info": (I, [P, IP, IP]),
+        "vv_make_filter": (I, [I, I, F, u8p, IP, FP]),
+        "vv_set_filter": (I, [P, u8p, I, F]),
+        "vv_set_tf": (I, [P, P, P]),
+        "vv_set_tf_u8": (I, [P, u8p]),
+        "vv_default_tf_u8": (I, [u8p]),
+        "vv_upload_scalar": (I, [P, u8p, IP, I]),
+        "vv_set_lic_params": (I, [P, C.POINTER(LicParams)]),
+        "vv_get_lic_params": (I, [P, C.POINTER(LicParams)]),
+        "vv_compute_lic": (I, [P]),
+        "vv_lic_info": (I, [P, IP, IP]),
+        "vv_download_lic": (I, [P, P, I]),
+        "vv_upload_lic": (I, [P, P, I, IP, I]),
+        "vv_render": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
+        "vv_set_opacity_scale": (I, [P, P, I]),
+        "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
+        "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
+        "vv_lic_weights": (I, [P, FP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+LIB = _load()
+
+# every function declared in include/vvlic.h
+EXPORTED = [
+    "vv_create", "vv_destroy", "vv_last_error", "vv_abi_version", "vv_stream", "vv_synchronize",
+    "vv_default_lic_params", "vv_default_camera", "vv_upload_field", "vv_field_geometry",
+    "vv_download_field", "vv_upload_noise", "vv_generate_noise", "vv_download_noise",
+    "vv_noise_info", "vv_make_filter", "vv_set_filter", "vv_set_tf", "vv_set_tf_u8",
+    "vv_default_tf_u8", "vv_upload_scalar", "vv_set_lic_params", "vv_get_lic_params",
+    "vv_compute_lic", "vv_lic_info", "vv_download_lic", "vv_upload_lic", "vv_render",
+    "vv_set_opacity_scale", "vv_render_adjoint", "vv_view_block", "vv_lic_weights",
+]
+
+
+def _i3(d) -> C.Array:
+    return (C.c_int * 3)(*[int(x) for x in d])
+
+
+def _f3(d) -> C.Array:
+    return (C.c_float * 3)(*[float(x) for x in d])
+
+
+def _ptr(a) -> tuple[int, int]:
+    """(address, on_device) of a numpy array or a torch tensor."""
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return a.ctypes.data, 0
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return a.data_ptr(), 1 if a.is_cuda else 0
+    raise TypeError(f"unsupported buffer type {type(a)}")
+
+
+def default_lic_params() -> LicParams:
+    p = LicParams()
+    LIB.vv_default_lic_params(C.byref(p))
+    return p
+
+
+def default_camera(width: int, height: int, center=(0.5, 0.5, 0.5)) -> Camera:
+    c = Camera()
+    LIB.vv_default_camera(C.byref(c), int(width), int(height), _f3(center))
+    return c
+
+
+def make_filter(kind: int, length: int = 0, sigma: float = 0.0):
+    """Synthesised 256-entry LIC filter LUT and its inverse area."""
+    lut = np.zeros(256, np.uint8)
+    w = C.c_int()
+    inv = C.c_float()
+    st = LIB.vv_make_filter(int(kind), int(length), float(sigma), lut.ctypes.data, C.byref(w),
+                            C.byref(inv))
+    if st:
+        raise VVError(st, "vv_make_filter")
+    return lut[: w.value].copy(), inv.value
+
+
+def default_tf_u8() -> np.ndarray:
+    t = np.zeros(256 * 5, np.uint8)
+    LIB.vv_default_tf_u8(t.ctypes.data)
+    return t
+
+
+class Context:
+    """One engine context on one GPU (vv_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        st = LIB.vv_create(C.byref(h), int(device))
+        if st:
+            raise VVError(st, LIB.vv_last_error(None).decode())
+        self._h = h
+
+    # -- plumbing --
+    def _chk(self, st: int) -> None:
+        if st:
+            raise VVError(st, LIB.vv_last_error(self._h).decode())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            LIB.vv_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self) -> int:
+        return LIB.vv_stream(self._h) or 0
+
+    def synchronize(self) -> None:
+        self._chk(LIB.vv_synchronize(self._h))
+
+    def last_error(self) -> str:
+        return LIB.vv_last_error(self._h).decode()
+
+    # -- inputs --
+    def upload_field(self, v0, dims, v1=None, lerp_t: float = 0.0, spacing=(1.0, 1.0, 1.0),
+                     storage: int = STORE_FP32) -> None:
+        p0, dev = _ptr(v0)
+        p1 = _ptr(v1)[0] if v1 is not None else None
+        self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
+                                      int(storage), dev))
+
+    def field_geometry(self):
+        e, s, c = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
+        self._chk(LIB.vv_field_geometry(self._h, e, s, c))
+        return list(e), list(s), list(c)
+
+    def download_field(self, dims) -> np.ndarray:
+        out = np.empty(int(np.prod(dims)) * 4, np.float32)
+        self._chk(LIB.vv_download_field(self._h, out.ctypes.data, 0))
+        return out
+
+    def upload_noise(self, noise, dims, with_gradients: bool = False, grad_rgb=None) -> None:
+        p, dev = _ptr(noise)
+        g = _ptr(grad_rgb)[0] if grad_rgb is not None else None
+        self._chk(LIB.vv_upload_noise(self._h, p, _i3(dims), int(bool(with_gradients)), g, dev))
+
+    def generate_noise(self, seed: int, kind: int, density: float, dims,
+                       with_gradients: bool = False) -> None:
+        self._chk(LIB.vv_generate_noise(self._h, C.c_uint64(seed), int(kind), float(density),
+                                        _i3(dims), int(bool(with_gradients))))
+
+    def noise_info(self):
+        d = (C.c_int * 3)()
+        ch = C.c_int()
+        self._chk(LIB.vv_noise_info(self._h, d, C.byref(ch)))
+        return list(d), ch.value
+
+    def download_noise(self) -> np.ndarray:
+        d, ch = self.noise_info()
+        out = np.empty(int(np.prod(d)) * ch, np.uint8)
+        self._chk(LIB.vv_download_noise(self._h, out.ctypes.data, 0))
+        return out
+
+    def set_filter(self, lut: np.ndarray, inv_area: float = 0.0) -> None:
+        lut = np.ascontiguousarray(lut, np.uint8)
+        self._chk(LIB.vv_set_filter(self._h, lut.ctypes.data, int(lut.size), float(inv_area)))
+
+    def set_tf(self, rgba: np.ndarray, opacity: np.ndarray) -> None:
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        opacity = np.ascontiguousarray(opacity, np.float32)
+        self._chk(LIB.vv_set_tf(self._h, rgba.ctypes.data, opacity.ctypes.data))
+
+    def set_tf_u8(self, tf5: np.ndarray) -> None:
+        tf5 = np.ascontiguousarray(tf5, np.uint8)
+        self._chk(LIB.vv_set_tf_u8(self._h, tf5.ctypes.data))
+
+    def upload_scalar(self, scalar, dims) -> None:
+        p, dev = _ptr(scalar)
+        self._chk(LIB.vv_upload_scalar(self._h, p, _i3(dims), dev))
+
+    def set_lic_params(self, p: LicParams) -> None:
+        self._chk(LIB.vv_set_lic_params(self._h, C.byref(p)))
+
+    def get_lic_params(self) -> LicParams:
+        p = LicParams()
+        self._chk(LIB.vv_get_lic_params(self._h, C.byref(p)))
+        return p
+
+    # -- LIC --
+    def compute_lic(self) -> None:
+        self._chk(LIB.vv_compute_lic(self._h))
+
+    def lic_info(self):
+        d = (C.c_int * 3)()
+        ch = C.c_int()
+        self._chk(LIB.vv_lic_info(self._h, d, C.byref(ch)))
+        return list(d), ch.value
+
+    def download_lic(self, out=None) -> np.ndarray:
+        d, ch = self.lic_info()
+        if out is None:
+            out = np.empty(int(np.prod(d)) * ch, np.float32)
+        p, dev = _ptr(out)
+        self._chk(LIB.vv_download_lic(self._h, p, dev))
+        return out
+
+    def upload_lic(self, lic, channels: int, dims) -> None:
+        p, dev = _ptr(lic)
+        self._chk(LIB.vv_upload_lic(self._h, p, int(channels), _i3(dims), dev))
+
+    def lic_weights(self) -> np.ndarray:
+        p = self.get_lic_params()
+        out = np.empty(1 + p.steps_forward + p.steps_backward, np.float32)
+        self._chk(LIB.vv_lic_weights(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    # -- render --
+    def render(self, cam: Camera, out=None, count_samples: bool = False):
+        if out is None:
+            out = np.empty(cam.width * cam.height * 4, np.float32)
+        p, dev = _ptr(out)
+        ns = C.c_uint64()
+        self._chk(LIB.vv_render(self._h, C.byref(cam), p, dev,
+                                C.byref(ns) if count_samples else None))
+        return (out, ns.value) if count_samples else out
+
+    def view_block(self, cam: Camera) -> np.ndarray:
+        out = (C.c_float * 24)()
+        self._chk(LIB.vv_view_block(self._h, C.byref(cam), out))
+        return np.array(out, np.float32)
+
+    def set_opacity_scale(self, sigma) -> None:
+        if sigma is None:
+            self._chk(LIB.vv_set_opacity_scale(self._h, None, 0))
+            return
+        p, dev = _ptr(sigma)
+        self._chk(LIB.vv_set_opacity_scale(self._h, p, dev))
+
+    def render_adjoint(self, cam: Camera, dimage, dtf=None, dsigma=None, want_sigma: bool = True):
+        host = isinstance(dimage, np.ndarray)
+        if host:
+            dimage = np.ascontiguousarray(dimage, np.float32)
+            if dtf is None:
+                dtf = np.zeros(1024, np.float32)
+            if dsigma is None and want_sigma:
+                d, _ = self.lic_info()
+                dsigma = np.zeros(int(np.prod(d)), np.float32)
+        pi, dev = _ptr(dimage)
+        pt = _ptr(dtf)[0]
+        ps = _ptr(dsigma)[0] if dsigma is not None else None
+        self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
+        return dtf, dsigma

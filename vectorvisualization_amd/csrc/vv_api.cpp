@@ -1,0 +1,1008 @@
+// vv_api.cpp — the C ABI declared in include/vvlic.h.
+//
+// Host-side responsibilities only: argument validation, device-buffer
+// ownership, the per-frame view block, the convolution-weight table and the
+// uniform values of Renderer::setRenderVolParams (VV/renderer.cpp:925-996).
+// Every volume operation runs in the HIP kernels of vv_lic.hip and
+// vv_render.hip; there is no host compute path.
+#include "../../include/vvlic.h"
+#include "vv_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace vv;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t ensure(size_t n)
+    {
+        if (n <= bytes && p) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&p, n ? n : 1);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        bytes = n;
+        return hipSuccess;
+    }
+};
+
+constexpr int kMaxSteps = 4096;
+constexpr long long kMaxVoxels = 1ll << 31;
+
+}  // namespace
+
+struct vv_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // field (VolumeData)
+    bool has_field = false;
+    int fdims[3] = {0, 0, 0};
+    float spacing[3] = {1, 1, 1};
+    float extent[3] = {1, 1, 1}, scale[3] = {1, 1, 1}, center[3] = {0.5f, 0.5f, 0.5f};
+    int storage = VV_STORE_FP32;
+    DevBuf field;
+
+    // noise
+    bool has_noise = false;
+    int ndims[3] = {0, 0, 0};
+    int nch = 1;
+    DevBuf noise;
+
+    // scalar gate volume
+    bool has_scalar = false;
+    int sdims[3] = {0, 0, 0};
+    DevBuf scalar;
+
+    // filter
+    std::vector<uint8_t> lut;
+    float inv_area = 0.5f;
+
+    // transfer function: 256*4 rgba + 256 opacity
+    float tf[1280];
+    DevBuf tf_dev;
+    bool tf_dirty = true;
+
+    vv_lic_params params;
+
+    // LIC volume (REPEAT layout)
+    bool has_lic = false;
+    int ldims[3] = {0, 0, 0};
+    int lch = 1;
+    DevBuf lic;
+    DevBuf weights;
+
+    // opacity scale (CLAMP layout, LIC grid)
+    bool has_sigma = false;
+    DevBuf sigma;
+
+    // staging / scratch
+    DevBuf stage, stage2, image, counter, adj_ckpt, adj_acc, adj_dimg, adj_out;
+};
+
+namespace {
+
+vv_status fail(vv_ctx* c, vv_status s, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c)
+        c->err = buf;
+    else
+        g_create_error = buf;
+    return s;
+}
+
+#define HIPCHK(c, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail((c), e_ == hipErrorOutOfMemory ? VV_ENOMEM : VV_EDEVICE, "%s: %s (%s:%d)", \
+                        #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
+    } while (0)
+
+bool dims_ok(const int d[3])
+{
+    if (!d) return false;
+    long long n = 1;
+    for (int a = 0; a < 3; ++a) {
+        if (d[a] < 2 || d[a] > 4096) return false;
+        n *= d[a];
+    }
+    return n < kMaxVoxels / 4;
+}
+
+long long nvox(const int d[3]) { return (long long)d[0] * d[1] * d[2]; }
+
+// copy n bytes from host or device memory into a device buffer
+vv_status upload(vv_ctx* c, DevBuf& b, const void* src, size_t n, int on_device)
+{
+    HIPCHK(c, b.ensure(n));
+    HIPCHK(c, hipMemcpyAsync(b.p, src, n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                             c->stream));
+    return VV_OK;
+}
+
+void default_params(vv_lic_params* p)
+{
+    p->step_size_vol = 1.0f / 128.0f;
+    p->gradient_scale = 30.0f;
+    p->illum_scale = 1.0f;
+    p->freq_scale = 1.0f;
+    p->num_iterations = 255;
+    p->steps_forward = 32;
+    p->steps_backward = 32;
+    p->step_size_lic = 0.01f;
+    p->eye_factor = 0.3f;
+    p->illum_mode = VV_ILLUM_LIC;
+    p->tf_index = VV_TF_INDEX_B;
+    p->gate_enable = 0;
+    p->gate_lo = 0.1f;
+    p->gate_hi = 0.3f;
+}
+
+void default_tf5(uint8_t* t)
+{
+    // TransferEdit::TransferEdit (VV/transferEdit.cpp:76-82)
+    for (int i = 0; i < 256; ++i) {
+        for (int ch = 0; ch < 3; ++ch) t[5 * i + ch] = (uint8_t)i;
+        for (int ch = 3; ch < 5; ++ch) t[5 * i + ch] = (uint8_t)(i < 20 ? 0 : i - 20);
+    }
+}
+
+void tf5_to_float(const uint8_t* t5, float* tf)
+{
+    // RGBA texture = channels 0..3, LA texture alpha = channel 4
+    // (VV/transferEdit.cpp:507-531); UNORM8 -> c/255
+    for (int i = 0; i < 256; ++i) {
+        for (int ch = 0; ch < 4; ++ch) tf[4 * i + ch] = (float)t5[5 * i + ch] / 255.0f;
+        tf[1024 + i] = (float)t5[5 * i + 4] / 255.0f;
+    }
+}
+
+// k(u): 1D linear CLAMP_TO_EDGE over the byte LUT, scaled by 1/255
+float lut_k(const std::vector<uint8_t>& lut, float u)
+{
+    const int n = (int)lut.size();
+    float x = std::fmaf(u, (float)n, -0.5f);
+    float fl = std::floor(x);
+    float f = x - fl;
+    if (!(fl > -1.0f)) fl = -1.0f;
+    if (fl > (float)(n - 1)) fl = (float)(n - 1);
+    int i = (int)fl;
+    int i0 = i < 0 ? 0 : i, i1 = i + 1 > n - 1 ? n - 1 : i + 1;
+    float a = (float)lut[i0], b = (float)lut[i1];
+    return std::fmaf(f, b - a, a) * (float)(1.0 / 255.0);
+}
+
+// convolution weights in sample order (inc_lic.glsl:168-199): k(0.5), then
+// B backward samples (u -= 0.5/B), then F forward samples (u += 0.5/F)
+void lic_weights(const vv_ctx* c, std::vector<float>& w)
+{
+    const int F = c->params.steps_forward, B = c->params.steps_backward;
+    const float kf = 0.5f / (float)F, kb = 0.5f / (float)B;
+    w.resize(1 + F + B);
+    float u = 0.5f;
+    w[0] = lut_k(c->lut, u);
+    for (int i = 0; i < B; ++i) {
+        u = u - kb;
+        w[1 + i] = lut_k(c->lut, u);
+    }
+    u = 0.5f;
+    for (int i = 0; i < F; ++i) {
+        u = u + kf;
+        w[1 + B + i] = lut_k(c->lut, u);
+    }
+}
+
+void quat_matrix(const float qf[4], double R[3][3])
+{
+    double x = qf[0], y = qf[1], z = qf[2], w = qf[3];
+    double n = std::sqrt(x * x + y * y + z * z + w * w);
+    if (n > 0) {
+        x /= n;
+        y /= n;
+        z /= n;
+        w /= n;
+    }
+    R[0][0] = 1 - 2 * (y * y + z * z);
+    R[0][1] = 2 * (x * y - z * w);
+    R[0][2] = 2 * (x * z + y * w);
+    R[1][0] = 2 * (x * y + z * w);
+    R[1][1] = 1 - 2 * (x * x + z * z);
+    R[1][2] = 2 * (y * z - x * w);
+    R[2][0] = 2 * (x * z - y * w);
+    R[2][1] = 2 * (y * z + x * w);
+    R[2][2] = 1 - 2 * (x * x + y * y);
+}
+
+// Camera::setCamera + glTranslatef(-center): MV = T(pos) T(0,0,-dist) R(q) T(-center).
+// Object-space camera = MV^-1 (0,0,0,1); pixel ray through NDC (x,y) is
+// R^T (x*aspect*tan(fovy/2), y*tan(fovy/2), -1).  Computed in double, rounded once.
+ViewBlock view_block(const vv_ctx* c, const vv_camera* cam)
+{
+    ViewBlock v;
+    double R[3][3];
+    quat_matrix(cam->quat, R);
+    const double pi = 3.14159265358979323846;
+    const double th = std::tan(0.5 * (double)cam->fovy_deg * pi / 180.0);
+    const double aspect = (double)cam->width / (double)cam->height;
+    const double e[3] = {-(double)cam->pos[0], -(double)cam->pos[1], (double)cam->dist - (double)cam->pos[2]};
+    for (int i = 0; i < 3; ++i) {
+        const double rt_e = R[0][i] * e[0] + R[1][i] * e[1] + R[2][i] * e[2];
+        v.cam[i] = (float)((double)c->center[i] + rt_e);
+        v.du[i] = (float)(R[0][i] * aspect * th);
+        v.dv[i] = (float)(R[1][i] * th);
+        v.dw[i] = (float)(-R[2][i]);
+        v.extent[i] = c->extent[i];
+        v.scale[i] = c->scale[i];
+        v.tex_max[i] = (float)((double)c->extent[i] * (double)c->scale[i]);
+    }
+    v.inv_w2 = (float)(2.0 / cam->width);
+    v.inv_h2 = (float)(2.0 / cam->height);
+    v.width = cam->width;
+    v.height = cam->height;
+    return v;
+}
+
+vv_status check_camera(vv_ctx* c, const vv_camera* cam)
+{
+    if (!cam) return fail(c, VV_EINVAL, "camera is NULL");
+    if (cam->width < 1 || cam->height < 1 || cam->width > 16384 || cam->height > 16384)
+        return fail(c, VV_EINVAL, "image size %dx%d out of range", cam->width, cam->height);
+    if (!(cam->fovy_deg > 0.0f && cam->fovy_deg < 180.0f))
+        return fail(c, VV_EINVAL, "fovy %g out of range", (double)cam->fovy_deg);
+    if (!std::isfinite(cam->dist)) return fail(c, VV_EINVAL, "camera distance not finite");
+    for (int a = 0; a < 3; ++a)
+        if (!std::isfinite(cam->pos[a]) || !std::isfinite(cam->light_pos[a]))
+            return fail(c, VV_EINVAL, "camera/light position not finite");
+    if (cam->spec_exponent < 0 || cam->spec_exponent > 1024)
+        return fail(c, VV_EINVAL, "spec exponent out of range");
+    return VV_OK;
+}
+
+vv_status check_params(vv_ctx* c, const vv_lic_params* p)
+{
+    if (!p) return fail(c, VV_EINVAL, "params NULL");
+    if (p->steps_forward < 1 || p->steps_forward > kMaxSteps || p->steps_backward < 1 ||
+        p->steps_backward > kMaxSteps)
+        return fail(c, VV_EINVAL, "steps forward/backward must be in [1,%d]", kMaxSteps);
+    if (!(p->step_size_vol > 0.0f) || !std::isfinite(p->step_size_vol))
+        return fail(c, VV_EINVAL, "step_size_vol must be > 0");
+    if (p->num_iterations < 1 || p->num_iterations > 4096)
+        return fail(c, VV_EINVAL, "num_iterations out of range");
+    const float fs[] = {p->gradient_scale, p->illum_scale, p->freq_scale, p->step_size_lic,
+                        p->eye_factor, p->gate_lo, p->gate_hi};
+    for (float f : fs)
+        if (!std::isfinite(f)) return fail(c, VV_EINVAL, "non-finite parameter");
+    if (p->illum_mode != VV_ILLUM_LIC && p->illum_mode != VV_ILLUM_GRADIENT)
+        return fail(c, VV_EINVAL, "illum_mode invalid");
+    if (p->tf_index != VV_TF_INDEX_B && p->tf_index != VV_TF_INDEX_MAG)
+        return fail(c, VV_EINVAL, "tf_index invalid");
+    return VV_OK;
+}
+
+vv_status ensure_tf(vv_ctx* c)
+{
+    if (!c->tf_dirty) return VV_OK;
+    HIPCHK(c, c->tf_dev.ensure(sizeof(c->tf)));
+    HIPCHK(c, hipMemcpyAsync(c->tf_dev.p, c->tf, sizeof(c->tf), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host table may change after return
+    c->tf_dirty = false;
+    return VV_OK;
+}
+
+vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunch& R)
+{
+    if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
+    if (!c->has_lic) return fail(c, VV_ESTATE, "no LIC volume (call vv_compute_lic or vv_upload_lic)");
+    vv_status st = check_camera(c, cam);
+    if (st) return st;
+    st = ensure_tf(c);
+    if (st) return st;
+    const vv_lic_params& p = c->params;
+    std::memset(&R, 0, sizeof(R));
+    R.field = make_clamp_desc(c->field.p, c->fdims);
+    R.field_fp16 = c->storage == VV_STORE_FP16;
+    R.tf_ch = p.tf_index == VV_TF_INDEX_MAG ? 3 : 2;
+    R.lic = make_repeat_desc(c->lic.p, c->ldims);
+    R.lc = c->lch;
+    if (c->has_sigma) R.sigma = make_clamp_desc(c->sigma.p, c->ldims);
+    R.tf = (const float*)c->tf_dev.p;
+    R.view = view_block(c, cam);
+    R.step = p.step_size_vol;
+    // alphaCorrection = stepSizeVol * 128 (VV/renderer.cpp:988-989)
+    R.alpha_corr = p.step_size_vol * 128.0f;
+    R.corr_mode = R.alpha_corr == 1.0f ? 0 : (R.alpha_corr == 0.5f ? 1 : (R.alpha_corr == 2.0f ? 2 : 3));
+    R.max_samples = (long long)p.num_iterations * (long long)p.num_iterations;
+    R.illum = p.illum_mode;
+    R.illum_scale = p.illum_scale;
+    // L = normalize(gl_LightSource[0].position.xyz) (scaleVolInv is never uploaded)
+    const float l0 = cam->light_pos[0], l1 = cam->light_pos[1], l2 = cam->light_pos[2];
+    const float ld = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+    if (ld > 0.0f) {
+        R.light[0] = l0 / ld;
+        R.light[1] = l1 / ld;
+        R.light[2] = l2 / ld;
+    }
+    R.light_diffuse = cam->light_diffuse;
+    R.light_specular = cam->light_specular;
+    R.light_ambient = cam->light_ambient;
+    R.spec_exp = cam->spec_exponent;
+    return VV_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vv_abi_version(void) { return VVLIC_ABI_VERSION; }
+
+const char* vv_last_error(const vv_ctx* ctx)
+{
+    return ctx ? ctx->err.c_str() : g_create_error.c_str();
+}
+
+vv_status vv_create(vv_ctx** out, int device)
+{
+    if (!out) return fail(nullptr, VV_EINVAL, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) return fail(nullptr, VV_EDEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(nullptr, VV_EINVAL, "device %d out of range", device);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return fail(nullptr, VV_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+    vv_ctx* c = new (std::nothrow) vv_ctx();
+    if (!c) return fail(nullptr, VV_ENOMEM, "out of host memory");
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(nullptr, VV_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    default_params(&c->params);
+    uint8_t t5[256 * 5];
+    default_tf5(t5);
+    tf5_to_float(t5, c->tf);
+    // default filter: box (VV/3DLIC.cpp:725-729 fallback)
+    c->lut.assign(256, 255);
+    c->inv_area = 0.5f;
+    *out = c;
+    return VV_OK;
+}
+
+vv_status vv_destroy(vv_ctx* c)
+{
+    if (!c) return VV_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    DevBuf* bufs[] = {&c->field, &c->noise, &c->scalar, &c->tf_dev, &c->lic, &c->weights,
+                      &c->sigma, &c->stage, &c->stage2, &c->image, &c->counter, &c->adj_ckpt,
+                      &c->adj_acc, &c->adj_dimg, &c->adj_out};
+    for (DevBuf* b : bufs) b->release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return VV_OK;
+}
+
+void* vv_stream(vv_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+vv_status vv_synchronize(vv_ctx* c)
+{
+    if (!c) return VV_EINVAL;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+vv_status vv_default_lic_params(vv_lic_params* p)
+{
+    if (!p) return VV_EINVAL;
+    default_params(p);
+    return VV_OK;
+}
+
+vv_status vv_default_camera(vv_camera* cam, int width, int height, const float center[3])
+{
+    if (!cam || width < 1 || height < 1) return VV_EINVAL;
+    std::memset(cam, 0, sizeof(*cam));
+    cam->width = width;
+    cam->height = height;
+    cam->fovy_deg = 35.0f;
+    cam->dist = 4.0f;
+    cam->quat[3] = 1.0f;
+    const float c0 = center ? center[0] : 0.5f, c1 = center ? center[1] : 0.5f,
+                c2 = center ? center[2] : 0.5f;
+    // updateLightPos with identity rotations: center + (0,0,light distance 1)
+    cam->light_pos[0] = c0;
+    cam->light_pos[1] = c1;
+    cam->light_pos[2] = c2 + 1.0f;
+    cam->light_diffuse = 1.0f;
+    cam->light_specular = 1.0f;
+    cam->light_ambient = 0.0f;
+    cam->spec_exponent = 40;
+    return VV_OK;
+}
+
+// ---- field -------------------------------------------------------------------
+vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float lerp_t, const int dims[3],
+                          const float spacing[3], int storage, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (storage != VV_STORE_FP32 && storage != VV_STORE_FP16)
+        return fail(c, VV_EINVAL, "unknown storage %d", storage);
+    if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
+    float sd[3] = {1, 1, 1};
+    if (spacing)
+        for (int a = 0; a < 3; ++a) {
+            if (!(spacing[a] > 0.0f) || !std::isfinite(spacing[a]))
+                return fail(c, VV_EINVAL, "spacing must be > 0");
+            sd[a] = spacing[a];
+        }
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = nvox(dims);
+    const size_t raw = sizeof(float) * 3 * (size_t)n;
+    const float* d0 = v0;
+    const float* d1 = v1 ? v1 : v0;
+    if (!on_device) {
+        for (long long i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(v0[i]) || (v1 && !std::isfinite(v1[i])))
+                return fail(c, VV_EINVAL, "field contains a non-finite value at %lld", i / 3);
+        vv_status st = upload(c, c->stage, v0, raw, 0);
+        if (st) return st;
+        d0 = (const float*)c->stage.p;
+        d1 = d0;
+        if (v1) {
+            st = upload(c, c->stage2, v1, raw, 0);
+            if (st) return st;
+            d1 = (const float*)c->stage2.p;
+        }
+    }
+    const int fp16 = storage == VV_STORE_FP16;
+    const size_t elem = fp16 ? 8 : 16;
+    HIPCHK(c, c->field.ensure(elem * (size_t)clamp_elems(dims)));
+    HIPCHK(c, c->counter.ensure(64));
+    uint32_t* maxbits = (uint32_t*)c->counter.p;
+    HIPCHK(c, launch_fill_u32(maxbits, 0u, 1, c->stream));
+    HIPCHK(c, launch_field_maxlen(d0, d1, lerp_t, n, maxbits, c->stream));
+    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, fp16, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
+    float vol[3], maxvol = 0.0f;
+    int maxtex = 0;
+    for (int a = 0; a < 3; ++a) {
+        c->fdims[a] = dims[a];
+        c->spacing[a] = sd[a];
+        vol[a] = (float)dims[a] * sd[a];
+        if (vol[a] > maxvol) maxvol = vol[a];
+        if (dims[a] > maxtex) maxtex = dims[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        c->scale[a] = (float)maxtex / ((float)dims[a] * sd[a]);
+        c->extent[a] = (float)dims[a] * sd[a] / maxvol;
+        c->center[a] = c->extent[a] / 2.0f;
+    }
+    c->storage = storage;
+    c->has_field = true;
+    return VV_OK;
+}
+
+vv_status vv_field_geometry(const vv_ctx* c, float extent[3], float scale[3], float center[3])
+{
+    if (!c || !c->has_field) return VV_ESTATE;
+    for (int a = 0; a < 3; ++a) {
+        if (extent) extent[a] = c->extent[a];
+        if (scale) scale[a] = c->scale[a];
+        if (center) center[a] = c->center[a];
+    }
+    return VV_OK;
+}
+
+vv_status vv_download_field(vv_ctx* c, float* out4, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!c->has_field) return fail(c, VV_ESTATE, "no field");
+    if (!out4) return fail(c, VV_EINVAL, "out is NULL");
+    const size_t bytes = sizeof(float) * 4 * (size_t)nvox(c->fdims);
+    float* dst = out4;
+    if (!on_device) {
+        HIPCHK(c, c->stage.ensure(bytes));
+        dst = (float*)c->stage.p;
+    }
+    HIPCHK(c, launch_field_decode(c->field.p, c->storage == VV_STORE_FP16, c->fdims, dst, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(out4, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+// ---- noise -------------------------------------------------------------------
+namespace {
+
+vv_status build_noise_tex(vv_ctx* c, const uint8_t* d_noise, const int dims[3], int with_gradients,
+                          const uint8_t* grad_rgb, int grad_on_device)
+{
+    const long long n = nvox(dims);
+    const int nc = with_gradients ? 4 : 1;
+    const uint8_t* d_rgb = nullptr;
+    if (with_gradients) {
+        if (grad_rgb) {
+            vv_status st = upload(c, c->stage2, grad_rgb, 3 * (size_t)n, grad_on_device);
+            if (st) return st;
+            d_rgb = (const uint8_t*)c->stage2.p;
+        } else {
+            // computeGradients -> filterGradients -> quantizeGradients
+            // (VV/dataset.cpp:1241-1252) on the GPU
+            HIPCHK(c, c->stage2.ensure(sizeof(float) * 3 * (size_t)n + 3 * (size_t)n));
+            float* g = (float*)c->stage2.p;
+            uint8_t* rgb = (uint8_t*)(g + 3 * n);
+            const float sd[3] = {1.0f, 1.0f, 1.0f};  // noise VolumeData sliceDist default
+            HIPCHK(c, launch_noise_grad_raw(d_noise, dims, sd, g, c->stream));
+            // filterGradients kernel weights: exp(-(i^2+j^2+k^2)/SIGMA2) over
+            // i,j,k in [-2,0], normalised (VV/gradient.cpp:394-409)
+            float w[27], sum = 0.0f;
+            for (int k = -2; k < 1; ++k)
+                for (int j = -2; j < 1; ++j)
+                    for (int i = -2; i < 1; ++i) {
+                        float v = std::exp(-(float)(i * i + j * j + k * k) / 5.0f);
+                        w[((k + 2) * 3 + (j + 2)) * 3 + (i + 2)] = v;
+                        sum += v;
+                    }
+            for (float& v : w) v /= sum;
+            HIPCHK(c, launch_noise_grad_filter(g, dims, w, rgb, c->stream));
+            d_rgb = rgb;
+        }
+    }
+    HIPCHK(c, c->noise.ensure((size_t)nc * (size_t)repeat_elems(dims)));
+    HIPCHK(c, launch_noise_tex(d_noise, d_rgb, dims, c->noise.p, nc, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) c->ndims[a] = dims[a];
+    c->nch = nc;
+    c->has_noise = true;
+    return VV_OK;
+}
+
+}  // namespace
+
+vv_status vv_upload_noise(vv_ctx* c, const uint8_t* noise, const int dims[3], int with_gradients,
+                          const uint8_t* grad_rgb, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!noise) return fail(c, VV_EINVAL, "noise is NULL");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims must be in [2,4096] per axis");
+    HIPCHK(c, hipSetDevice(c->device));
+    vv_status st = upload(c, c->stage, noise, (size_t)nvox(dims), on_device);
+    if (st) return st;
+    return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, grad_rgb, on_device);
+}
+
+vv_status vv_generate_noise(vv_ctx* c, uint64_t seed, int kind, double density, const int dims[3],
+                            int with_gradients)
+{
+    if (!c) return VV_EINVAL;
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims must be in [2,4096] per axis");
+    if (kind != VV_NOISE_BINARY && kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
+    if (kind == VV_NOISE_BINARY && !(density >= 0.0 && density <= 1.0))
+        return fail(c, VV_EINVAL, "density must be in [0,1]");
+    HIPCHK(c, hipSetDevice(c->device));
+    double t = std::floor((kind == VV_NOISE_BINARY ? density : 0.0) * 4294967296.0);
+    if (t < 0) t = 0;
+    if (t > 4294967296.0) t = 4294967296.0;
+    HIPCHK(c, c->stage.ensure((size_t)nvox(dims)));
+    HIPCHK(c, launch_noise_gen(seed, kind, (uint64_t)t, dims, (uint8_t*)c->stage.p, c->stream));
+    return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, nullptr, 0);
+}
+
+vv_status vv_download_noise(vv_ctx* c, uint8_t* out, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!c->has_noise) return fail(c, VV_ESTATE, "no noise");
+    if (!out) return fail(c, VV_EINVAL, "out is NULL");
+    const size_t bytes = (size_t)c->nch * (size_t)nvox(c->ndims);
+    uint8_t* dst = out;
+    if (!on_device) {
+        HIPCHK(c, c->stage.ensure(bytes));
+        dst = (uint8_t*)c->stage.p;
+    }
+    HIPCHK(c, launch_noise_untex(c->noise.p, c->nch, c->ndims, dst, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+vv_status vv_noise_info(const vv_ctx* c, int dims[3], int* channels)
+{
+    if (!c || !c->has_noise) return VV_ESTATE;
+    if (dims)
+        for (int a = 0; a < 3; ++a) dims[a] = c->ndims[a];
+    if (channels) *channels = c->nch;
+    return VV_OK;
+}
+
+// ---- filter ------------------------------------------------------------------
+vv_status vv_make_filter(int kind, int length, float sigma, uint8_t lut[256], int* width, float* inv_area)
+{
+    if (!lut || !width || !inv_area) return VV_EINVAL;
+    const int W = 256;
+    if (kind == VV_FILTER_BOX) {
+        // LICFilter::createBoxFilter (VV/dataset.cpp:1405-1413)
+        std::memset(lut, 255, W);
+        *width = W;
+        *inv_area = 0.5f;
+        return VV_OK;
+    }
+    if (kind < VV_FILTER_TRIANGLE || kind > VV_FILTER_GAUSSIAN || length <= 0) return VV_EINVAL;
+    double sg = sigma > 0.0f ? (double)sigma : length / 4.0;
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < W; ++i) {
+        const double t = (i + 0.5) / W;
+        const double cc = 2.0 * t - 1.0;
+        double w;
+        if (kind == VV_FILTER_TRIANGLE) {
+            w = 1.0 - std::fabs(cc);
+        } else if (kind == VV_FILTER_COS2) {
+            const double q = std::cos(pi * (t - 0.5));
+            w = q * q;
+        } else {
+            const double s = cc * length;
+            w = std::exp(-s * s / (2.0 * sg * sg));
+        }
+        double b = std::floor(255.0 * w + 0.5);
+        lut[i] = (uint8_t)(b < 0 ? 0 : (b > 255 ? 255 : b));
+    }
+    *width = W;
+    float area = 0.0f;
+    for (int i = 0; i < W; ++i) area += lut[i];
+    *inv_area = 0.5f * W * 255.0f / area;
+    return VV_OK;
+}
+
+vv_status vv_set_filter(vv_ctx* c, const uint8_t* lut, int width, float inv_area)
+{
+    if (!c) return VV_EINVAL;
+    if (!lut) return fail(c, VV_EINVAL, "lut is NULL");
+    if (width < 1 || width > 4096 || (width & (width - 1)))
+        return fail(c, VV_EINVAL, "filter width must be a power of two <= 4096");
+    float area = 0.0f;
+    for (int i = 0; i < width; ++i) area += lut[i];
+    if (!(area > 0.0f)) return fail(c, VV_EINVAL, "filter is all zero");
+    if (!std::isfinite(inv_area)) return fail(c, VV_EINVAL, "inv_area not finite");
+    c->lut.assign(lut, lut + width);
+    // calcFilterKernelInvArea (VV/dataset.cpp:1503-1512)
+    c->inv_area = inv_area > 0.0f ? inv_area : 0.5f * width * 255.0f / area;
+    return VV_OK;
+}
+
+// ---- transfer function -------------------------------------------------------
+vv_status vv_set_tf(vv_ctx* c, const float* rgba, const float* opacity)
+{
+    if (!c) return VV_EINVAL;
+    if (!rgba || !opacity) return fail(c, VV_EINVAL, "tf arrays NULL");
+    for (int i = 0; i < 1024; ++i)
+        if (!std::isfinite(rgba[i])) return fail(c, VV_EINVAL, "tf rgba not finite");
+    for (int i = 0; i < 256; ++i)
+        if (!std::isfinite(opacity[i])) return fail(c, VV_EINVAL, "tf opacity not finite");
+    std::memcpy(c->tf, rgba, sizeof(float) * 1024);
+    std::memcpy(c->tf + 1024, opacity, sizeof(float) * 256);
+    c->tf_dirty = true;
+    return VV_OK;
+}
+
+vv_status vv_set_tf_u8(vv_ctx* c, const uint8_t* tf5)
+{
+    if (!c) return VV_EINVAL;
+    if (!tf5) return fail(c, VV_EINVAL, "tf NULL");
+    tf5_to_float(tf5, c->tf);
+    c->tf_dirty = true;
+    return VV_OK;
+}
+
+vv_status vv_default_tf_u8(uint8_t tf5[256 * 5])
+{
+    if (!tf5) return VV_EINVAL;
+    default_tf5(tf5);
+    return VV_OK;
+}
+
+// ---- scalar gate volume --------------------------------------------------------
+vv_status vv_upload_scalar(vv_ctx* c, const uint8_t* scalar, const int dims[3], int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!scalar) return fail(c, VV_EINVAL, "scalar is NULL");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "scalar dims must be in [2,4096] per axis");
+    HIPCHK(c, hipSetDevice(c->device));
+    vv_status st = upload(c, c->stage, scalar, (size_t)nvox(dims), on_device);
+    if (st) return st;
+    HIPCHK(c, c->scalar.ensure((size_t)clamp_elems(dims)));
+    HIPCHK(c, launch_pad_clamp_u8((const uint8_t*)c->stage.p, dims, (uint8_t*)c->scalar.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) c->sdims[a] = dims[a];
+    c->has_scalar = true;
+    return VV_OK;
+}
+
+// ---- params ------------------------------------------------------------------
+vv_status vv_set_lic_params(vv_ctx* c, const vv_lic_params* p)
+{
+    if (!c) return VV_EINVAL;
+    vv_status st = check_params(c, p);
+    if (st) return st;
+    c->params = *p;
+    return VV_OK;
+}
+
+vv_status vv_get_lic_params(const vv_ctx* c, vv_lic_params* p)
+{
+    if (!c || !p) return VV_EINVAL;
+    *p = c->params;
+    return VV_OK;
+}
+
+vv_status vv_lic_weights(const vv_ctx* c, float* out)
+{
+    if (!c || !out) return VV_EINVAL;
+    std::vector<float> w;
+    lic_weights(c, w);
+    std::memcpy(out, w.data(), sizeof(float) * w.size());
+    return VV_OK;
+}
+
+// ---- LIC ---------------------------------------------------------------------
+vv_status vv_compute_lic(vv_ctx* c)
+{
+    if (!c) return VV_EINVAL;
+    if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
+    if (!c->has_noise) return fail(c, VV_ESTATE, "no noise uploaded");
+    const vv_lic_params& p = c->params;
+    if (p.gate_enable && c->nch == 1 && !c->has_scalar)
+        return fail(c, VV_ESTATE, "gate enabled but no scalar volume uploaded");
+    if (p.illum_mode == VV_ILLUM_GRADIENT && c->nch != 4)
+        return fail(c, VV_ESTATE, "gradient illumination needs RGBA noise (with_gradients)");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<float> w;
+    lic_weights(c, w);
+    HIPCHK(c, c->weights.ensure(sizeof(float) * w.size()));
+    HIPCHK(c, hipMemcpyAsync(c->weights.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    const int nc = c->nch;
+    HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
+    LicLaunch L;
+    std::memset(&L, 0, sizeof(L));
+    L.field = make_clamp_desc(c->field.p, c->fdims);
+    L.field_fp16 = c->storage == VV_STORE_FP16;
+    L.noise = make_repeat_desc(c->noise.p, c->ndims);
+    L.nc = nc;
+    if (c->has_scalar) L.scalar = make_clamp_desc(c->scalar.p, c->sdims);
+    L.gate = p.gate_enable && nc == 1;
+    L.gate_lo = p.gate_lo;
+    L.gate_hi = p.gate_hi;
+    L.w = (const float*)c->weights.p;
+    L.F = p.steps_forward;
+    L.B = p.steps_backward;
+    // licParams.z * (logEyeDist*0.5 + 0.3), logEyeDist == 0 (inc_lic.glsl:114)
+    L.he = p.step_size_lic * p.eye_factor;
+    // licKernel.b * gradient.r = invArea/(F+B) * gradientScale (VV/renderer.cpp:982-984)
+    L.scl = (c->inv_area / (float)(p.steps_forward + p.steps_backward)) * p.gradient_scale;
+    L.freq = p.freq_scale;
+    for (int a = 0; a < 3; ++a) L.scale[a] = c->scale[a];
+    L.out = make_repeat_desc(c->lic.p, c->fdims);
+    L.tile[0] = 8;
+    L.tile[1] = 8;
+    L.tile[2] = 4;
+    HIPCHK(c, launch_lic(L, c->stream));
+    HIPCHK(c, launch_repeat_halo_f32((float*)c->lic.p, nc, c->fdims, c->stream));
+    // the opacity scale lives on the LIC grid: drop it when that grid changes
+    if (c->ldims[0] != c->fdims[0] || c->ldims[1] != c->fdims[1] || c->ldims[2] != c->fdims[2])
+        c->has_sigma = false;
+    for (int a = 0; a < 3; ++a) c->ldims[a] = c->fdims[a];
+    c->lch = nc;
+    c->has_lic = true;
+    return VV_OK;
+}
+
+vv_status vv_lic_info(const vv_ctx* c, int dims[3], int* channels)
+{
+    if (!c || !c->has_lic) return VV_ESTATE;
+    if (dims)
+        for (int a = 0; a < 3; ++a) dims[a] = c->ldims[a];
+    if (channels) *channels = c->lch;
+    return VV_OK;
+}
+
+vv_status vv_download_lic(vv_ctx* c, float* out, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!c->has_lic) return fail(c, VV_ESTATE, "no LIC volume");
+    if (!out) return fail(c, VV_EINVAL, "out is NULL");
+    const size_t bytes = sizeof(float) * c->lch * (size_t)nvox(c->ldims);
+    float* dst = out;
+    if (!on_device) {
+        HIPCHK(c, c->stage.ensure(bytes));
+        dst = (float*)c->stage.p;
+    }
+    HIPCHK(c, launch_unpad_repeat_f32((const float*)c->lic.p, c->lch, c->ldims, dst, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+vv_status vv_upload_lic(vv_ctx* c, const float* lic, int channels, const int dims[3], int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!lic) return fail(c, VV_EINVAL, "lic is NULL");
+    if (channels != 1 && channels != 4) return fail(c, VV_EINVAL, "channels must be 1 or 4");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "lic dims must be in [2,4096] per axis");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t bytes = sizeof(float) * channels * (size_t)nvox(dims);
+    vv_status st = upload(c, c->stage, lic, bytes, on_device);
+    if (st) return st;
+    HIPCHK(c, c->lic.ensure(sizeof(float) * channels * (size_t)repeat_elems(dims)));
+    HIPCHK(c, launch_pad_repeat_f32((const float*)c->stage.p, channels, dims, (float*)c->lic.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) c->ldims[a] = dims[a];
+    c->lch = channels;
+    c->has_lic = true;
+    c->has_sigma = false;  // opacity scale lives on the LIC grid
+    return VV_OK;
+}
+
+// ---- render ------------------------------------------------------------------
+vv_status vv_view_block(const vv_ctx* c, const vv_camera* cam, float out24[24])
+{
+    if (!c || !cam || !out24) return VV_EINVAL;
+    ViewBlock v = view_block(c, cam);
+    float* o = out24;
+    for (int a = 0; a < 3; ++a) o[a] = v.cam[a];
+    for (int a = 0; a < 3; ++a) o[3 + a] = v.du[a];
+    for (int a = 0; a < 3; ++a) o[6 + a] = v.dv[a];
+    for (int a = 0; a < 3; ++a) o[9 + a] = v.dw[a];
+    o[12] = v.inv_w2;
+    o[13] = v.inv_h2;
+    for (int a = 0; a < 3; ++a) o[14 + a] = v.extent[a];
+    for (int a = 0; a < 3; ++a) o[17 + a] = v.scale[a];
+    for (int a = 0; a < 3; ++a) o[20 + a] = v.tex_max[a];
+    o[23] = 0.0f;
+    return VV_OK;
+}
+
+vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
+{
+    if (!c) return VV_EINVAL;
+    if (!image) return fail(c, VV_EINVAL, "image is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    RenderLaunch R;
+    vv_status st = fill_render_launch(c, cam, R);
+    if (st) return st;
+    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+    float* dst = image;
+    if (!on_device) {
+        HIPCHK(c, c->image.ensure(bytes));
+        dst = (float*)c->image.p;
+    }
+    R.image = dst;
+    if (samples_out) {
+        HIPCHK(c, c->counter.ensure(64));
+        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+    }
+    HIPCHK(c, launch_render(R, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (samples_out) {
+        unsigned long long t = 0;
+        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *samples_out = t;
+    } else if (!on_device) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return VV_OK;
+}
+
+// ---- differentiation -----------------------------------------------------------
+vv_status vv_set_opacity_scale(vv_ctx* c, const float* sigma, int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!sigma) {
+        c->has_sigma = false;
+        return VV_OK;
+    }
+    if (!c->has_lic) return fail(c, VV_ESTATE, "opacity scale lives on the LIC grid: no LIC volume");
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = nvox(c->ldims);
+    vv_status st = upload(c, c->stage, sigma, sizeof(float) * (size_t)n, on_device);
+    if (st) return st;
+    HIPCHK(c, c->sigma.ensure(sizeof(float) * (size_t)clamp_elems(c->ldims)));
+    HIPCHK(c, launch_pad_clamp_f32((const float*)c->stage.p, c->ldims, (float*)c->sigma.p, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->has_sigma = true;
+    return VV_OK;
+}
+
+vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf, float* dsigma,
+                            int on_device)
+{
+    if (!c) return VV_EINVAL;
+    if (!dimage || !dtf) return fail(c, VV_EINVAL, "dimage/dtf NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    AdjointLaunch A;
+    std::memset(&A, 0, sizeof(A));
+    vv_status st = fill_render_launch(c, cam, A.r);
+    if (st) return st;
+    const long long nrays = (long long)cam->width * cam->height;
+    // samples per ray are bounded by the box diagonal in texture space over
+    // the shortest per-step displacement (|dir| >= min scale)
+    double diag = 0.0, smin = 1e30;
+    for (int a = 0; a < 3; ++a) {
+        diag += (double)A.r.view.tex_max[a] * A.r.view.tex_max[a];
+        if (c->scale[a] < smin) smin = c->scale[a];
+    }
+    diag = std::sqrt(diag);
+    long long geo = (long long)(diag / (smin * (double)c->params.step_size_vol) * 1.01) + 4;
+    long long maxs = A.r.max_samples < geo ? A.r.max_samples : geo;
+    const int max_ckpt = (int)((maxs + 63) / 64);
+    const size_t ck_bytes = sizeof(float) * 4 * (size_t)(2 * max_ckpt + 2 * 64) * (size_t)nrays;
+    HIPCHK(c, c->adj_ckpt.ensure(ck_bytes));
+    const long long nv = nvox(c->ldims);
+    const size_t acc_bytes = sizeof(long long) * (size_t)(1024 + (dsigma ? nv : 0));
+    HIPCHK(c, c->adj_acc.ensure(acc_bytes));
+    const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
+    const float* d_dimg = dimage;
+    if (!on_device) {
+        st = upload(c, c->adj_dimg, dimage, img_bytes, 0);
+        if (st) return st;
+        d_dimg = (const float*)c->adj_dimg.p;
+    }
+    long long* acc = (long long*)c->adj_acc.p;
+    HIPCHK(c, launch_fill_i64(acc, 0, (long long)(acc_bytes / 8), c->stream));
+    A.dimage = d_dimg;
+    A.ckpt = (float*)c->adj_ckpt.p;
+    A.max_ckpt = max_ckpt;
+    A.r.max_samples = maxs;
+    A.dtf_fx = acc;
+    A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
+    A.fx_scale = 4294967296.0;  // 2^32
+    HIPCHK(c, launch_render_adjoint(A, c->stream));
+    const size_t out_bytes = sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0));
+    float* dout = nullptr;
+    if (on_device) {
+        HIPCHK(c, launch_fixed_to_float(acc, dtf, 1024, 1.0 / A.fx_scale, c->stream));
+        if (dsigma) HIPCHK(c, launch_fixed_to_float(acc + 1024, dsigma, nv, 1.0 / A.fx_scale, c->stream));
+    } else {
+        HIPCHK(c, c->adj_out.ensure(out_bytes));
+        dout = (float*)c->adj_out.p;
+        HIPCHK(c, launch_fixed_to_float(acc, dout, (long long)(out_bytes / 4), 1.0 / A.fx_scale, c->stream));
+        HIPCHK(c, hipMemcpyAsync(dtf, dout, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));
+        if (dsigma)
+            HIPCHK(c, hipMemcpyAsync(dsigma, dout + 1024, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost,
+                                     c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// vv_internal.hpp — launch descriptors shared by the C-ABI layer (vv_api.cpp)
+// and the HIP kernels (vv_lic.hip, vv_render.hip).  Plain structs only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vv {
+
+// Padded volume geometry (see vv_device.hpp for the two padded layouts).
+struct VolDesc {
+    void* p;
+    int n[3];      // logical voxels per axis
+    int sy, sz;    // element strides of the padded layout
+};
+
+inline VolDesc make_clamp_desc(void* p, const int n[3])
+{
+    VolDesc d;
+    d.p = p;
+    for (int a = 0; a < 3; ++a) d.n[a] = n[a];
+    d.sy = n[0] + 2;
+    d.sz = (n[0] + 2) * (n[1] + 2);
+    return d;
+}
+
+inline VolDesc make_repeat_desc(void* p, const int n[3])
+{
+    VolDesc d;
+    d.p = p;
+    for (int a = 0; a < 3; ++a) d.n[a] = n[a];
+    d.sy = n[0] + 1;
+    d.sz = (n[0] + 1) * (n[1] + 1);
+    return d;
+}
+
+inline long long clamp_elems(const int n[3]) { return (long long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2); }
+inline long long repeat_elems(const int n[3]) { return (long long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1); }
+
+struct LicLaunch {
+    VolDesc field;     // CLAMP, float4 or half4
+    int field_fp16;
+    VolDesc noise;     // REPEAT, uint8 (nc 1) or uchar4 (nc 4)
+    int nc;
+    VolDesc scalar;    // CLAMP uint8, used when gate
+    int gate;
+    float gate_lo, gate_hi;
+    const float* w;    // 1 + B + F convolution weights
+    int F, B;
+    float he;          // step_size_lic * eye_factor
+    float scl;         // (inv_area / (F+B)) * gradient_scale
+    float freq;
+    float scale[3];
+    VolDesc out;       // REPEAT layout, float (nc 1) or float4 (nc 4)
+    int tile[3];       // threads per block along x,y,z
+};
+
+struct ViewBlock {
+    float cam[3], du[3], dv[3], dw[3];
+    float inv_w2, inv_h2;
+    float extent[3], scale[3], tex_max[3];
+    int width, height;
+};
+
+struct RenderLaunch {
+    VolDesc field;     // CLAMP
+    int field_fp16;
+    int tf_ch;         // 2 (.b) or 3 (.a)
+    VolDesc lic;       // REPEAT, float or float4
+    int lc;            // 1 or 4
+    VolDesc sigma;     // CLAMP float, p == nullptr when absent
+    const float* tf;   // 256*4 rgba then 256 opacity (device)
+    ViewBlock view;
+    float step;
+    float alpha_corr;
+    int corr_mode;     // 0: c==1, 1: sqrt, 2: square, 3: powf
+    long long max_samples;
+    int illum;         // 0 illumLIC, 1 illumGradient
+    float illum_scale;
+    float light[3];    // normalised light direction
+    float light_diffuse, light_specular, light_ambient;
+    int spec_exp;
+    float* image;
+    unsigned long long* counter;  // total samples (may be null)
+};
+
+struct AdjointLaunch {
+    RenderLaunch r;
+    const float* dimage;          // width*height*4
+    float* ckpt;                  // scratch: per ray, per checkpoint 8 floats
+    int max_ckpt;                 // checkpoints per ray
+    long long* dtf_fx;            // 256*4 fixed-point accumulators
+    long long* dsigma_fx;         // sigma padded-clamp element count accumulators
+    double fx_scale;              // fixed-point scale (2^k)
+};
+
+// ---- launchers (vv_lic.hip / vv_render.hip) ----
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
+hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long long n,
+                               uint32_t* maxbits, hipStream_t s);
+hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int dims[3],
+                               const uint32_t* maxbits, void* out, int fp16, hipStream_t s);
+hipError_t launch_field_decode(const void* field, int fp16, const int dims[3], float* out4,
+                               hipStream_t s);
+hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int dims[3],
+                            uint8_t* out, hipStream_t s);
+hipError_t launch_noise_grad_raw(const uint8_t* noise, const int dims[3], const float sd[3],
+                                 float* g, hipStream_t s);
+hipError_t launch_noise_grad_filter(const float* g, const int dims[3], const float w27[27],
+                                    uint8_t* rgb, hipStream_t s);
+hipError_t launch_noise_tex(const uint8_t* noise, const uint8_t* rgb, const int dims[3],
+                            void* tex, int nc, hipStream_t s);
+hipError_t launch_noise_untex(const void* tex, int nc, const int dims[3], uint8_t* out,
+                              hipStream_t s);
+hipError_t launch_pad_clamp_u8(const uint8_t* src, const int dims[3], uint8_t* dst, hipStream_t s);
+hipError_t launch_pad_clamp_f32(const float* src, const int dims[3], float* dst, hipStream_t s);
+hipError_t launch_pad_repeat_f32(const float* src, int nc, const int dims[3], float* dst,
+                                 hipStream_t s);
+hipError_t launch_unpad_repeat_f32(const float* src, int nc, const int dims[3], float* dst,
+                                   hipStream_t s);
+hipError_t launch_repeat_halo_f32(float* vol, int nc, const int dims[3], hipStream_t s);
+hipError_t launch_lic(const LicLaunch& L, hipStream_t s);
+hipError_t launch_render(const RenderLaunch& R, hipStream_t s);
+hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s);
+hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
+                                 hipStream_t s);
+hipError_t launch_unpad_clamp_fixed(const long long* src, const int dims[3], float* dst,
+                                    double inv_scale, hipStream_t s);
+hipError_t launch_fill_i64(long long* p, long long v, long long n, hipStream_t s);
+
+}  // namespace vv

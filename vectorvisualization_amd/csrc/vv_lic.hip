@@ -1,0 +1,653 @@
+// vv_lic.hip — field encode, noise, noise-gradient and LIC-volume kernels.
+//
+//   field encode    VV/dataset.cpp:533-635 (fillTexDataFloatInterp)
+//   noise gradients VV/gradient.cpp:190-532 (computeGradients SOBEL,
+//                   filterGradients, quantize8)
+//   LIC volume      VV/shader/lic3d_volume_fragment.glsl:2-20 with
+//                   computeLIC / singleLICstep / freqSampling[Grad]
+//                   (VV/shader/inc_lic.glsl:61-202)
+//
+// All floating-point expressions keep the operation order of the
+// reference (left to right, no contraction: the library is built with
+// -ffp-contract=off) and use fmaf only where the sampling rules say so.
+#include "vv_device.hpp"
+#include "vv_internal.hpp"
+
+namespace vv {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i64(long long* p, long long v, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// |v| of the time-interpolated vector, 0 below EPS (VV/mmath.h:43)
+__device__ __forceinline__ float field_len(const float* v0, const float* v1, float t, long long i,
+                                           float tv[3])
+{
+    for (int c = 0; c < 3; ++c) {
+        float a = v0[3 * i + c], b = v1[3 * i + c];
+        tv[c] = a + t * (b - a);
+    }
+    float len = sqrtf(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+    return len < 1e-5f ? 0.0f : len;
+}
+
+__global__ void __launch_bounds__(kThreads) k_field_maxlen(const float* v0, const float* v1,
+                                                           float t, long long n, uint32_t* maxbits)
+{
+    __shared__ float red[kThreads / 64];
+    float m = 0.0f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+        float tv[3];
+        m = fmaxf(m, field_len(v0, v1, t, i, tv));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kThreads / 64; ++w) m = fmaxf(m, red[w]);
+        // non-negative floats order like their bit patterns
+        atomicMax(maxbits, __float_as_uint(m));
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_enc(T* out, long long i, float r, float g, float b, float a);
+
+template <>
+__device__ __forceinline__ void store_enc<float4>(float4* out, long long i, float r, float g, float b,
+                                                  float a)
+{
+    out[i] = make_float4(r, g, b, a);
+}
+
+template <>
+__device__ __forceinline__ void store_enc<half4_t>(half4_t* out, long long i, float r, float g,
+                                                   float b, float a)
+{
+    uint2 u;
+    u.x = (uint32_t)__half_as_ushort(__float2half(r)) | ((uint32_t)__half_as_ushort(__float2half(g)) << 16);
+    u.y = (uint32_t)__half_as_ushort(__float2half(b)) | ((uint32_t)__half_as_ushort(__float2half(a)) << 16);
+    reinterpret_cast<uint2*>(out)[i] = u;
+}
+
+// writes the CLAMP-padded encoded field; padded faces replicate the edge
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_field_encode(const float* v0, const float* v1, float t,
+                                                           int nx, int ny, int nz,
+                                                           const uint32_t* maxbits, T* out)
+{
+    const int px = nx + 2, py = ny + 2, pz = nz + 2;
+    const long long np = (long long)px * py * pz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    x = min(max(x - 1, 0), nx - 1);
+    y = min(max(y - 1, 0), ny - 1);
+    z = min(max(z - 1, 0), nz - 1);
+    long long i = ((long long)z * ny + y) * nx + x;
+    float tv[3];
+    float len = field_len(v0, v1, t, i, tv);
+    float r, g, b;
+    if (len == 0.0f) {
+        r = g = b = 0.5f;
+    } else {
+        r = 0.5f * tv[0] / len + 0.5f;
+        g = 0.5f * tv[1] / len + 0.5f;
+        b = 0.5f * tv[2] / len + 0.5f;
+    }
+    float maxlen = __uint_as_float(*maxbits);
+    float a = maxlen > 0.0f ? len / maxlen : 0.0f;
+    a = a > 1.0f ? 1.0f : (a < 0.0f ? 0.0f : a);
+    store_enc<T>(out, q, r, g, b, a);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_field_decode(const T* field, int nx, int ny, int nz,
+                                                           float4* out)
+{
+    long long n = (long long)nx * ny * nz;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+    long long q = ((long long)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + (x + 1);
+    out[i] = ld4(field, (int)q);
+}
+
+// ---- noise -----------------------------------------------------------------
+// noise[i] = (i+1)-th output of SplitMix64 seeded with `seed`
+__device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(kThreads) k_noise_gen(uint64_t seed, int kind, uint64_t thr,
+                                                        long long n, uint8_t* out)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t u = splitmix64_at(seed, (uint64_t)i);
+    out[i] = kind == 0 ? (((u >> 32) < thr) ? 255 : 0) : (uint8_t)(u >> 56);
+}
+
+// computeGradients, SOBEL == 1 (VV/gradient.cpp:199-307)
+__constant__ signed char c_sobel[3][3][3][3] = {
+    {{{-1, -3, -1}, {-3, -6, -3}, {-1, -3, -1}},
+     {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+     {{1, 3, 1}, {3, 6, 3}, {1, 3, 1}}},
+    {{{-1, -3, -1}, {0, 0, 0}, {1, 3, 1}},
+     {{-3, -6, -3}, {0, 0, 0}, {3, 6, 3}},
+     {{-1, -3, -1}, {0, 0, 0}, {1, 3, 1}}},
+    {{{-1, 0, 1}, {-3, 0, 3}, {-1, 0, 1}},
+     {{-3, 0, 3}, {-6, 0, 6}, {-3, 0, 3}},
+     {{-1, 0, 1}, {-3, 0, 3}, {-1, 0, 1}}}};
+
+__global__ void __launch_bounds__(kThreads) k_noise_grad_raw(const uint8_t* v, int nx, int ny, int nz,
+                                                             float sd0, float sd1, float sd2, float* g)
+{
+    long long n = (long long)nx * ny * nz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int x = (int)(q % nx), y = (int)((q / nx) % ny), z = (int)(q / ((long long)nx * ny));
+    auto vox = [&](int a, int b, int c) -> float {
+        return (float)v[((long long)c * ny + b) * nx + a];
+    };
+    float sd[3] = {sd0, sd1, sd2};
+    float* gp = g + 3 * q;
+    if (x > 0 && x < nx - 1 && y > 0 && y < ny - 1 && z > 0 && z < nz - 1) {
+        for (int d = 0; d < 3; ++d) {
+            float acc = 0.0f;
+            for (int i = -1; i < 2; ++i)
+                for (int j = -1; j < 2; ++j)
+                    for (int k = -1; k < 2; ++k)
+                        acc += (float)c_sobel[d][i + 1][j + 1][k + 1] * vox(x + i, y + j, z + k);
+            gp[d] = acc / (2.0f * sd[d]);
+        }
+    } else {
+        gp[0] = (x < 1) ? (vox(x + 1, y, z) - vox(x, y, z)) / sd[0] : (vox(x, y, z) - vox(x - 1, y, z)) / sd[0];
+        gp[1] = (y < 1) ? (vox(x, y + 1, z) - vox(x, y, z)) / sd[1] : (vox(x, y, z) - vox(x, y - 1, z)) / sd[1];
+        gp[2] = (z < 1) ? (vox(x, y, z + 1) - vox(x, y, z)) / sd[2] : (vox(x, y, z) - vox(x, y, z - 1)) / sd[2];
+    }
+}
+
+struct W27 {
+    float w[27];
+};
+
+// filterGradients (VV/gradient.cpp:377-459: window [-fw, fw-2], fw shrinking
+// at the border) followed by quantize8 (:462-478, double arithmetic)
+__global__ void __launch_bounds__(kThreads) k_noise_grad_filter(const float* g, int nx, int ny, int nz,
+                                                                W27 fw27, uint8_t* rgb)
+{
+    long long n = (long long)nx * ny * nz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int x = (int)(q % nx), y = (int)((q / nx) % ny), z = (int)(q / ((long long)nx * ny));
+    int bx = min(x, nx - x - 1), by = min(y, ny - y - 1), bz = min(z, nz - z - 1);
+    int w = min(2, min(min(bx, by), bz));
+    float f[3];
+    for (int c = 0; c < 3; ++c) {
+        float acc = 0.0f;
+        for (int k = -w; k < w - 1; ++k)
+            for (int j = -w; j < w - 1; ++j)
+                for (int i = -w; i < w - 1; ++i) {
+                    long long o = 3 * (((long long)(z + k) * ny + (y + j)) * nx + (x + i)) + c;
+                    acc += fw27.w[((w + k) * 3 + (w + j)) * 3 + (w + i)] * g[o];
+                }
+        f[c] = acc;
+    }
+    float len = sqrtf(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    if (len < 1e-5f) {
+        f[0] = f[1] = f[2] = 0.0f;
+    } else {
+        f[0] /= len;
+        f[1] /= len;
+        f[2] /= len;
+    }
+    for (int c = 0; c < 3; ++c) rgb[3 * q + c] = (uint8_t)(((double)f[c] + 1.0) / 2.0 * 255.0);
+}
+
+// raw noise (+ quantised gradients) -> REPEAT-padded texture
+__global__ void __launch_bounds__(kThreads) k_noise_tex(const uint8_t* noise, const uint8_t* rgb,
+                                                        int nx, int ny, int nz, void* tex, int nc)
+{
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    long long np = (long long)px * py * pz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    if (x == nx) x = 0;
+    if (y == ny) y = 0;
+    if (z == nz) z = 0;
+    long long i = ((long long)z * ny + y) * nx + x;
+    if (nc == 1) {
+        reinterpret_cast<uint8_t*>(tex)[q] = noise[i];
+    } else {
+        uchar4 u;
+        u.x = rgb[3 * i];
+        u.y = rgb[3 * i + 1];
+        u.z = rgb[3 * i + 2];
+        u.w = noise[i];
+        reinterpret_cast<uchar4*>(tex)[q] = u;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_noise_untex(const void* tex, int nc, int nx, int ny, int nz,
+                                                          uint8_t* out)
+{
+    long long n = (long long)nx * ny * nz;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+    long long q = ((long long)z * (ny + 1) + y) * (nx + 1) + x;
+    if (nc == 1) {
+        out[i] = reinterpret_cast<const uint8_t*>(tex)[q];
+    } else {
+        uchar4 u = reinterpret_cast<const uchar4*>(tex)[q];
+        out[4 * i] = u.x;
+        out[4 * i + 1] = u.y;
+        out[4 * i + 2] = u.z;
+        out[4 * i + 3] = u.w;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pad_clamp(const T* src, int nx, int ny, int nz, T* dst)
+{
+    const int px = nx + 2, py = ny + 2, pz = nz + 2;
+    long long np = (long long)px * py * pz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    x = min(max(x - 1, 0), nx - 1);
+    y = min(max(y - 1, 0), ny - 1);
+    z = min(max(z - 1, 0), nz - 1);
+    dst[q] = src[((long long)z * ny + y) * nx + x];
+}
+
+__global__ void __launch_bounds__(kThreads) k_pad_repeat_f32(const float* src, int nc, int nx, int ny,
+                                                             int nz, float* dst)
+{
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    long long np = (long long)px * py * pz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    if (x == nx) x = 0;
+    if (y == ny) y = 0;
+    if (z == nz) z = 0;
+    long long i = ((long long)z * ny + y) * nx + x;
+    for (int c = 0; c < nc; ++c) dst[q * nc + c] = src[i * nc + c];
+}
+
+__global__ void __launch_bounds__(kThreads) k_unpad_repeat_f32(const float* src, int nc, int nx, int ny,
+                                                               int nz, float* dst)
+{
+    long long n = (long long)nx * ny * nz;
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
+    long long q = ((long long)z * (ny + 1) + y) * (nx + 1) + x;
+    for (int c = 0; c < nc; ++c) dst[i * nc + c] = src[q * nc + c];
+}
+
+// fills plane n of each axis of a REPEAT-padded volume from plane 0
+__global__ void __launch_bounds__(kThreads) k_repeat_halo_f32(float* vol, int nc, int nx, int ny, int nz)
+{
+    const int px = nx + 1, py = ny + 1, pz = nz + 1;
+    long long np = (long long)px * py * pz;
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    if (x < nx && y < ny && z < nz) return;
+    int sx = x == nx ? 0 : x, sy = y == ny ? 0 : y, sz = z == nz ? 0 : z;
+    long long s = ((long long)sz * py + sy) * px + sx;
+    for (int c = 0; c < nc; ++c) vol[q * nc + c] = vol[s * nc + c];
+}
+
+// ---- LIC ---------------------------------------------------------------------
+
+struct LicArgs {
+    Vol<void> field;
+    Vol<void> noise;
+    Vol<uint8_t> scalar;
+    const float* w;
+    int F, B;
+    float he, scl, freq, gate_lo, gate_hi;
+    float sc0, sc1, sc2;
+    float* out;
+    int osy, osz;
+    int tx, ty, tz;      // block tile
+    int gx, gy, gz;      // tiles per axis
+    int nblocks;         // gx*gy*gz
+    int per_xcd;         // tiles handled by each of the 8 XCDs
+};
+
+template <int NC>
+struct NoiseT;
+template <>
+struct NoiseT<1> {
+    using T = uint8_t;
+};
+template <>
+struct NoiseT<4> {
+    using T = uchar4;
+};
+
+// noise sample n(x): freqSampling (inc_lic.glsl:71-90) for scalar noise,
+// freqSamplingGrad (:61-68, no frequency scale, no gate) for RGBA noise
+template <int NC, bool GATE>
+__device__ __forceinline__ float4 noise_at(const LicArgs& a, float x, float y, float z)
+{
+    float4 n = make_float4(0.f, 0.f, 0.f, 0.f);
+    if constexpr (NC == 4) {
+        const Vol<uchar4> nv{(const uchar4*)a.noise.p, {a.noise.n[0], a.noise.n[1], a.noise.n[2]},
+                             a.noise.sy, a.noise.sz};
+        Cell c = cell_repeat(nv, x, y, z);
+        float4 r = tri_u8x4(nv, c);
+        n.x = r.x * kInv255;
+        n.y = r.y * kInv255;
+        n.z = r.z * kInv255;
+        n.w = r.w * kInv255;
+    } else {
+        if constexpr (GATE) {
+            Cell cs = cell_clamp(a.scalar, x, y, z);
+            float s = tri_u8(a.scalar, cs) * kInv255;
+            if (!(s > a.gate_lo && s < a.gate_hi)) return n;
+        }
+        const Vol<uint8_t> nv{(const uint8_t*)a.noise.p, {a.noise.n[0], a.noise.n[1], a.noise.n[2]},
+                              a.noise.sy, a.noise.sz};
+        Cell c = cell_repeat(nv, x * a.freq, y * a.freq, z * a.freq);
+        n.x = tri_u8(nv, c) * kInv255;
+    }
+    return n;
+}
+
+template <typename FT>
+__device__ __forceinline__ float3 field_rgb(const LicArgs& a, float x, float y, float z)
+{
+    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                     a.field.sz};
+    Cell c = cell_clamp(fv, x, y, z);
+    return tri_rgb(fv, c);
+}
+
+template <typename FT, int NC, bool GATE>
+__global__ void __launch_bounds__(kThreads) k_lic(LicArgs a)
+{
+    // XCD-aware tile order: hardware block b runs on XCD b % 8; give each
+    // XCD a contiguous run of tiles so streamlines of neighbouring tiles
+    // share that XCD's L2.
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * a.per_xcd + (b >> 3);
+    if (tile >= a.nblocks) return;
+    const int bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / (a.gx * a.gy);
+    const int t = threadIdx.x;
+    const int x = bx * a.tx + t % a.tx;
+    const int y = by * a.ty + (t / a.tx) % a.ty;
+    const int z = bz * a.tz + t / (a.tx * a.ty);
+    const int nx = a.field.n[0], ny = a.field.n[1], nz = a.field.n[2];
+    if (x >= nx || y >= ny || z >= nz) return;
+
+    // voxel-centre seed (VV/renderer.cpp:1353-1358, lic3d_volume_fragment.glsl:4-5)
+    const float sx = ((float)x + 0.5f) / (float)nx * a.sc0;
+    const float sy = ((float)y + 0.5f) / (float)ny * a.sc1;
+    const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
+
+    const float3 s0 = field_rgb<FT>(a, sx, sy, sz);
+    float4 n = noise_at<NC, GATE>(a, sx, sy, sz);
+    const float w0 = a.w[0];
+    float4 acc;
+    acc.x = n.x * w0;
+    acc.y = n.y * w0;
+    acc.z = n.z * w0;
+    acc.w = n.w * w0;
+
+#pragma unroll 1
+    for (int dirn = 0; dirn < 2; ++dirn) {
+        const int steps = dirn == 0 ? a.B : a.F;
+        const float h = dirn == 0 ? -a.he : a.he;
+        const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
+        float px = sx, py = sy, pz = sz;
+        float3 s = s0;
+#pragma unroll 1
+        for (int i = 0; i < steps; ++i) {
+            // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector
+            const float d1x = (2.0f * s.x - 1.0f) * h;
+            const float d1y = (2.0f * s.y - 1.0f) * h;
+            const float d1z = (2.0f * s.z - 1.0f) * h;
+            const float3 s2 = field_rgb<FT>(a, px + d1x, py + d1y, pz + d1z);
+            const float d2x = (2.0f * s2.x - 1.0f) * h;
+            const float d2y = (2.0f * s2.y - 1.0f) * h;
+            const float d2z = (2.0f * s2.z - 1.0f) * h;
+            px = px + 0.5f * (d1x + d2x);
+            py = py + 0.5f * (d1y + d2y);
+            pz = pz + 0.5f * (d1z + d2z);
+            s = field_rgb<FT>(a, px, py, pz);
+            n = noise_at<NC, GATE>(a, px, py, pz);
+            const float wi = wk[i];
+            acc.x = acc.x + n.x * wi;
+            if constexpr (NC == 4) {
+                acc.y = acc.y + n.y * wi;
+                acc.z = acc.z + n.z * wi;
+                acc.w = acc.w + n.w * wi;
+            }
+        }
+    }
+    const long long o = (long long)z * a.osz + (long long)y * a.osy + x;
+    if constexpr (NC == 1) {
+        a.out[o] = acc.x * a.scl;  // lic3d_volume_fragment.glsl:15
+    } else {
+        reinterpret_cast<float4*>(a.out)[o] = make_float4(acc.x, acc.y, acc.z, acc.w * a.scl);
+    }
+}
+
+inline unsigned blocks_for(long long n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n)), dim3(kThreads), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i64(long long* p, long long v, long long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i64, dim3(blocks_for(n)), dim3(kThreads), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long long n,
+                               uint32_t* maxbits, hipStream_t s)
+{
+    unsigned g = blocks_for(n);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_field_maxlen, dim3(g), dim3(kThreads), 0, s, v0, v1, t, n, maxbits);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int d[3],
+                               const uint32_t* maxbits, void* out, int fp16, hipStream_t s)
+{
+    long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
+    if (fp16)
+        hipLaunchKernelGGL(k_field_encode<half4_t>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1,
+                           t, d[0], d[1], d[2], maxbits, (half4_t*)out);
+    else
+        hipLaunchKernelGGL(k_field_encode<float4>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1,
+                           t, d[0], d[1], d[2], maxbits, (float4*)out);
+    return hipGetLastError();
+}
+
+hipError_t launch_field_decode(const void* field, int fp16, const int d[3], float* out4, hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    if (fp16)
+        hipLaunchKernelGGL(k_field_decode<half4_t>, dim3(blocks_for(n)), dim3(kThreads), 0, s,
+                           (const half4_t*)field, d[0], d[1], d[2], (float4*)out4);
+    else
+        hipLaunchKernelGGL(k_field_decode<float4>, dim3(blocks_for(n)), dim3(kThreads), 0, s,
+                           (const float4*)field, d[0], d[1], d[2], (float4*)out4);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int d[3], uint8_t* out,
+                            hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_noise_gen, dim3(blocks_for(n)), dim3(kThreads), 0, s, seed, kind, thr, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_grad_raw(const uint8_t* noise, const int d[3], const float sd[3], float* g,
+                                 hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_noise_grad_raw, dim3(blocks_for(n)), dim3(kThreads), 0, s, noise, d[0], d[1],
+                       d[2], sd[0], sd[1], sd[2], g);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_grad_filter(const float* g, const int d[3], const float w27[27], uint8_t* rgb,
+                                    hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    W27 w;
+    for (int i = 0; i < 27; ++i) w.w[i] = w27[i];
+    hipLaunchKernelGGL(k_noise_grad_filter, dim3(blocks_for(n)), dim3(kThreads), 0, s, g, d[0], d[1],
+                       d[2], w, rgb);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_tex(const uint8_t* noise, const uint8_t* rgb, const int d[3], void* tex, int nc,
+                            hipStream_t s)
+{
+    long long np = (long long)(d[0] + 1) * (d[1] + 1) * (d[2] + 1);
+    hipLaunchKernelGGL(k_noise_tex, dim3(blocks_for(np)), dim3(kThreads), 0, s, noise, rgb, d[0], d[1],
+                       d[2], tex, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise_untex(const void* tex, int nc, const int d[3], uint8_t* out, hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_noise_untex, dim3(blocks_for(n)), dim3(kThreads), 0, s, tex, nc, d[0], d[1],
+                       d[2], out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_clamp_u8(const uint8_t* src, const int d[3], uint8_t* dst, hipStream_t s)
+{
+    long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
+    hipLaunchKernelGGL(k_pad_clamp<uint8_t>, dim3(blocks_for(np)), dim3(kThreads), 0, s, src, d[0], d[1],
+                       d[2], dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_clamp_f32(const float* src, const int d[3], float* dst, hipStream_t s)
+{
+    long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
+    hipLaunchKernelGGL(k_pad_clamp<float>, dim3(blocks_for(np)), dim3(kThreads), 0, s, src, d[0], d[1],
+                       d[2], dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_pad_repeat_f32(const float* src, int nc, const int d[3], float* dst, hipStream_t s)
+{
+    long long np = (long long)(d[0] + 1) * (d[1] + 1) * (d[2] + 1);
+    hipLaunchKernelGGL(k_pad_repeat_f32, dim3(blocks_for(np)), dim3(kThreads), 0, s, src, nc, d[0], d[1],
+                       d[2], dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpad_repeat_f32(const float* src, int nc, const int d[3], float* dst, hipStream_t s)
+{
+    long long n = (long long)d[0] * d[1] * d[2];
+    hipLaunchKernelGGL(k_unpad_repeat_f32, dim3(blocks_for(n)), dim3(kThreads), 0, s, src, nc, d[0], d[1],
+                       d[2], dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_repeat_halo_f32(float* vol, int nc, const int d[3], hipStream_t s)
+{
+    long long np = (long long)(d[0] + 1) * (d[1] + 1) * (d[2] + 1);
+    hipLaunchKernelGGL(k_repeat_halo_f32, dim3(blocks_for(np)), dim3(kThreads), 0, s, vol, nc, d[0], d[1],
+                       d[2]);
+    return hipGetLastError();
+}
+
+hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
+{
+    LicArgs a;
+    a.field = Vol<void>{L.field.p, {L.field.n[0], L.field.n[1], L.field.n[2]}, L.field.sy, L.field.sz};
+    a.noise = Vol<void>{L.noise.p, {L.noise.n[0], L.noise.n[1], L.noise.n[2]}, L.noise.sy, L.noise.sz};
+    a.scalar = Vol<uint8_t>{(const uint8_t*)L.scalar.p, {L.scalar.n[0], L.scalar.n[1], L.scalar.n[2]},
+                            L.scalar.sy, L.scalar.sz};
+    a.w = L.w;
+    a.F = L.F;
+    a.B = L.B;
+    a.he = L.he;
+    a.scl = L.scl;
+    a.freq = L.freq;
+    a.gate_lo = L.gate_lo;
+    a.gate_hi = L.gate_hi;
+    a.sc0 = L.scale[0];
+    a.sc1 = L.scale[1];
+    a.sc2 = L.scale[2];
+    a.out = (float*)L.out.p;
+    a.osy = L.out.sy;
+    a.osz = L.out.sz;
+    a.tx = L.tile[0];
+    a.ty = L.tile[1];
+    a.tz = L.tile[2];
+    if (a.tx * a.ty * a.tz != kThreads) return hipErrorInvalidValue;
+    a.gx = (L.field.n[0] + a.tx - 1) / a.tx;
+    a.gy = (L.field.n[1] + a.ty - 1) / a.ty;
+    a.gz = (L.field.n[2] + a.tz - 1) / a.tz;
+    a.nblocks = a.gx * a.gy * a.gz;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    dim3 grid((unsigned)(a.per_xcd * 8));
+    dim3 block(kThreads);
+    const bool g = L.gate != 0;
+    if (L.field_fp16) {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_lic<half4_t, 4, false>), grid, block, 0, s, a);
+        else if (g)
+            hipLaunchKernelGGL((k_lic<half4_t, 1, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_lic<half4_t, 1, false>), grid, block, 0, s, a);
+    } else {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_lic<float4, 4, false>), grid, block, 0, s, a);
+        else if (g)
+            hipLaunchKernelGGL((k_lic<float4, 1, true>), grid, block, 0, s, a);
+        else
+            hipLaunchKernelGGL((k_lic<float4, 1, false>), grid, block, 0, s, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace vv

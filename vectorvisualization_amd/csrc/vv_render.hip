@@ -1,0 +1,478 @@
+// vv_render.hip — fused front-to-back ray caster (TF lookup + illumination +
+// compositing in one kernel) and its hand-written adjoint.
+//
+//   ray setup     VV/camera.cpp:56-68, VV/renderer.cpp:126-147, 682-736,
+//                 VV/shader/raycast_lic3d_fragment.glsl:13-21
+//   sample loop   VV/shader/raycast_lic3d_fragment.glsl:35-69
+//   illumLIC      VV/shader/inc_illum.glsl:158-172
+//   illumGradient VV/shader/inc_illum.glsl:1-41 (fed by the RGBA LIC volume)
+//
+// One thread per pixel; the 256x4 RGBA transfer function and the 256-entry
+// opacity table live in LDS.  The build is -ffp-contract=off; fmaf marks
+// every fused operation of the shared formulation.
+#include "vv_device.hpp"
+#include "vv_internal.hpp"
+
+namespace vv {
+
+namespace {
+
+constexpr int kBX = 16, kBY = 16, kRThreads = kBX * kBY;
+constexpr int kSeg = 64;  // adjoint checkpoint interval (samples)
+
+struct Sample {
+    float tfu;     // transfer-function coordinate
+    float l;       // LIC intensity
+    float m;       // d rgb / d tf.rgb
+    float ao;      // opacity-table value
+    float sg;      // opacity scale at the sample
+    float araw;    // ao * tf.a * sg
+    float a;       // corrected alpha
+    float rgb[3];  // colour before premultiplication
+    float tfa;     // tf.a
+    bool aclamp;   // opacity-scale clamp active
+};
+
+struct Ray {
+    float pos[3], dir[3], V[3];
+};
+
+__device__ __forceinline__ void normalize3(const float v[3], float o[3])
+{
+    float d = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (d > 0.0f) {
+        o[0] = v[0] / d;
+        o[1] = v[1] / d;
+        o[2] = v[2] / d;
+    } else {
+        o[0] = o[1] = o[2] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ float ipow(float x, int e)
+{
+    float r = 1.0f, b = x;
+    while (e > 0) {
+        if (e & 1) r = r * b;
+        b = b * b;
+        e >>= 1;
+    }
+    return r;
+}
+
+__device__ __forceinline__ float pow_corr(float b, float c, int mode)
+{
+    switch (mode) {
+    case 0: return b;
+    case 1: return sqrtf(b);
+    case 2: return b * b;
+    default: return powf(b, c);
+    }
+}
+
+// per-pixel ray: analytic entry into [0,extent]^3 (the rasterised front
+// faces of drawCubeFaces), dir = normalize(entry - camera) * scaleVol
+__device__ __forceinline__ bool ray_setup(const ViewBlock& v, int i, int j, Ray& r)
+{
+    const float nxp = fmaf((float)i + 0.5f, v.inv_w2, -1.0f);
+    const float nyp = fmaf((float)j + 0.5f, v.inv_h2, -1.0f);
+    float d[3];
+    float tn = -INFINITY, tf = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        d[a] = fmaf(nxp, v.du[a], fmaf(nyp, v.dv[a], v.dw[a]));
+        const float c = v.cam[a], e = v.extent[a];
+        if (d[a] != 0.0f) {
+            const float inv = 1.0f / d[a];
+            const float t0 = (0.0f - c) * inv, t1 = (e - c) * inv;
+            const float lo = t0 < t1 ? t0 : t1, hi = t0 < t1 ? t1 : t0;
+            if (lo > tn) tn = lo;
+            if (hi < tf) tf = hi;
+        } else if (c < 0.0f || c > e) {
+            return false;
+        }
+    }
+    if (!(tn <= tf) || !(tn > 0.0f)) return false;
+    float p0[3], g[3], gn[3];
+    for (int a = 0; a < 3; ++a) {
+        const float q = fmaf(tn, d[a], v.cam[a]);
+        const float e = v.extent[a];
+        p0[a] = q < 0.0f ? 0.0f : (q > e ? e : q);
+        g[a] = p0[a] - v.cam[a];
+    }
+    normalize3(g, gn);
+    for (int a = 0; a < 3; ++a) {
+        r.dir[a] = gn[a] * v.scale[a];
+        r.pos[a] = p0[a] * v.scale[a];
+    }
+    const float vd[3] = {-r.dir[0], -r.dir[1], -r.dir[2]};
+    normalize3(vd, r.V);
+    return true;
+}
+
+__device__ __forceinline__ bool outside(const float p[3], const ViewBlock& v)
+{
+    return p[0] < 0.0f || p[0] > v.tex_max[0] || p[1] < 0.0f || p[1] > v.tex_max[1] ||
+           p[2] < 0.0f || p[2] > v.tex_max[2];
+}
+
+template <typename FT, int LC>
+__device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* tf, const float* op,
+                                            const Ray& r, const float pos[3], Sample& s)
+{
+    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                     a.field.sz};
+    const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
+    const float v = tri_ch(fv, cf, a.tf_ch);
+    float l4[4];
+    if constexpr (LC == 1) {
+        const Vol<float> lv{(const float*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
+                            a.lic.sz};
+        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
+        l4[0] = tri_f(lv, cl);
+    } else {
+        const Vol<float4> lv{(const float4*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
+                             a.lic.sz};
+        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
+        const float4 q = tri_f4(lv, cl);
+        l4[0] = q.x;
+        l4[1] = q.y;
+        l4[2] = q.z;
+        l4[3] = q.w;
+    }
+    s.tfu = v;
+    float t4[4];
+    for (int ch = 0; ch < 4; ++ch) t4[ch] = lut1d(tf, 256, 4, ch, v);
+    s.tfa = t4[3];
+    const float g = a.illum_scale;
+    const float l = LC == 1 ? l4[0] : l4[3];
+    s.l = l;
+    if (LC == 1 || a.illum == 0) {
+        // illumLIC: rgb = illum * tf.rgb * gradient.g
+        for (int c = 0; c < 3; ++c) s.rgb[c] = l * t4[c] * g;
+        s.m = l * g;
+    } else {
+        // illumGradient: N = normalize(-I.xyz), L = normalize(light) (scaleVolInv = 0)
+        const float nn[3] = {-l4[0], -l4[1], -l4[2]};
+        float N[3], rr[3], Rv[3];
+        normalize3(nn, N);
+        const float ldn = a.light[0] * N[0] + a.light[1] * N[1] + a.light[2] * N[2];
+        for (int c = 0; c < 3; ++c) rr[c] = fmaf(2.0f * ldn, N[c], -a.light[c]);
+        normalize3(rr, Rv);
+        float sp = clamp01(Rv[0] * r.V[0] + Rv[1] * r.V[1] + Rv[2] * r.V[2]);
+        sp = ipow(sp, a.spec_exp);
+        sp = sp * l * a.light_specular;
+        const float diff = clamp01(ldn) * a.light_diffuse * g;
+        const float kk = (diff + 0.3f) + a.light_ambient;
+        for (int c = 0; c < 3; ++c) {
+            const float col = t4[c] * l;
+            s.rgb[c] = fmaf(col, kk, sp) * g;
+        }
+        s.m = l * kk * g;
+    }
+    s.ao = lut1d(op, 256, 1, 0, l * 1.3f);
+    float al = s.ao * t4[3];
+    s.sg = 1.0f;
+    s.aclamp = false;
+    if (a.sigma.p) {
+        // per-voxel opacity scale: alpha * sigma, clamped to [0,1]
+        const Vol<float> sv{(const float*)a.sigma.p, {a.sigma.n[0], a.sigma.n[1], a.sigma.n[2]},
+                            a.sigma.sy, a.sigma.sz};
+        const Cell cs = cell_clamp(sv, pos[0], pos[1], pos[2]);
+        s.sg = tri_f(sv, cs);
+        al = al * s.sg;
+        if (al > 1.0f) {
+            al = 1.0f;
+            s.aclamp = true;
+        }
+        if (al < 0.0f) {
+            al = 0.0f;
+            s.aclamp = true;
+        }
+    }
+    s.araw = al;
+    s.a = 1.0f - pow_corr(1.0f - al, a.alpha_corr, a.corr_mode);
+}
+
+__device__ __forceinline__ void load_tables(const float* g, float* tf, float* op)
+{
+    for (int i = threadIdx.y * kBX + threadIdx.x; i < 1280; i += kRThreads) {
+        float v = g[i];
+        if (i < 1024)
+            tf[i] = v;
+        else
+            op[i - 1024] = v;
+    }
+    __syncthreads();
+}
+
+// composite one sample; returns the unclamped-out-of-range mask (bit c)
+__device__ __forceinline__ int composite(float dest[4], const Sample& s)
+{
+    const float src[4] = {s.rgb[0] * s.a, s.rgb[1] * s.a, s.rgb[2] * s.a, s.a};
+    const float t = 1.0f - dest[3];
+    int mask = 0;
+    for (int c = 0; c < 4; ++c) {
+        const float u = fmaf(t, src[c], dest[c]);
+        if (u < 0.0f || u > 1.0f) mask |= 1 << c;
+        dest[c] = clamp01(u);
+    }
+    return mask;
+}
+
+template <typename FT, int LC>
+__global__ void __launch_bounds__(kRThreads) k_render(RenderLaunch a)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    unsigned long long ns = 0;
+    if (i < a.view.width && j < a.view.height) {
+        float dest[4] = {0.f, 0.f, 0.f, 0.f};
+        Ray r;
+        if (ray_setup(a.view, i, j, r)) {
+            float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
+            for (long long k = 0; k < a.max_samples; ++k) {
+                Sample s;
+                eval_sample<FT, LC>(a, tf, op, r, pos, s);
+                composite(dest, s);
+                ++ns;
+                for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
+                if (outside(pos, a.view) || dest[3] > 0.95f) break;
+            }
+        }
+        const long long pix = (long long)j * a.view.width + i;
+        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    if (a.counter) {
+        for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
+        const int tid = threadIdx.y * kBX + threadIdx.x;
+        if ((tid & 63) == 0) red[tid >> 6] = ns;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long t = 0;
+            for (int w = 0; w < kRThreads / 64; ++w) t += red[w];
+            atomicAdd(a.counter, t);
+        }
+    }
+}
+
+// ---- adjoint -----------------------------------------------------------------
+//
+// Forward  D' = clamp(D + (1 - D.a) s, 0, 1),  s = (rgb a, a).
+// Backward G -> G masked where the clamp was active;
+//          dL/ds = (1 - D.a) G;  dL/dD = G - (G . s) e_a.
+// Phase 1 runs the ray once and stores (pos, D) every kSeg samples.
+// Phase 2 walks the segments last to first: it re-runs a segment from its
+// checkpoint, saving (pos, mask) and D per sample in a per-ray scratch
+// column, then walks it backwards, re-evaluating each sample.  Gradient
+// contributions are rounded to fixed point (fx_scale) and accumulated with
+// 64-bit integer atomics, so the sums do not depend on thread order.
+
+__device__ __forceinline__ long long to_fx(float v, double scale)
+{
+    return (long long)llrint((double)v * scale);
+}
+
+__device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float val, double scale)
+{
+    const float x = fmaf(u, 256.0f, -0.5f);
+    float fl = floorf(x);
+    const float f = x - fl;
+    fl = fminf(fmaxf(fl, -1.0f), 255.0f);
+    const int i = (int)fl;
+    const int i0 = i < 0 ? 0 : i;
+    const int i1 = i + 1 > 255 ? 255 : i + 1;
+    const long long a0 = to_fx((1.0f - f) * val, scale);
+    const long long a1 = to_fx(f * val, scale);
+    if (a0) atomicAdd((unsigned long long*)&acc[i0 * 4 + ch], (unsigned long long)a0);
+    if (a1) atomicAdd((unsigned long long*)&acc[i1 * 4 + ch], (unsigned long long)a1);
+}
+
+template <typename FT, int LC>
+__global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ long long dtf[1024];
+    const RenderLaunch& a = A.r;
+    load_tables(a.tf, tf, op);
+    const int tid = threadIdx.y * kBX + threadIdx.x;
+    for (int q = tid; q < 1024; q += kRThreads) dtf[q] = 0;
+    __syncthreads();
+
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const long long W = a.view.width, H = a.view.height;
+    const long long nrays = W * H;
+    Ray r;
+    if (i < W && j < H && ray_setup(a.view, i, j, r)) {
+        const long long pix = (long long)j * W + i;
+        float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]
+        // phase 1: forward with checkpoints
+        float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
+        float dest[4] = {0.f, 0.f, 0.f, 0.f};
+        long long n = 0;
+        for (long long k = 0; k < a.max_samples; ++k) {
+            if ((k % kSeg) == 0) {
+                const long long c = k / kSeg;
+                ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+                ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+            }
+            Sample s;
+            eval_sample<FT, LC>(a, tf, op, r, pos, s);
+            composite(dest, s);
+            ++n;
+            for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
+            if (outside(pos, a.view) || dest[3] > 0.95f) break;
+        }
+        // phase 2: segments in reverse
+        const float4 gi = reinterpret_cast<const float4*>(A.dimage)[pix];
+        float G[4] = {gi.x, gi.y, gi.z, gi.w};
+        float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;  // [k][2][ray]
+        const long long nseg = (n + kSeg - 1) / kSeg;
+        for (long long c = nseg - 1; c >= 0; --c) {
+            const float4 cp = ck[(2 * c) * nrays + pix];
+            const float4 cd = ck[(2 * c + 1) * nrays + pix];
+            float p[3] = {cp.x, cp.y, cp.z};
+            float d[4] = {cd.x, cd.y, cd.z, cd.w};
+            const long long k0 = c * kSeg;
+            const long long k1 = min(n, k0 + kSeg);
+            for (long long k = k0; k < k1; ++k) {
+                Sample s;
+                eval_sample<FT, LC>(a, tf, op, r, p, s);
+                const long long slot = (k - k0) * 2;
+                const float4 dprev = make_float4(d[0], d[1], d[2], d[3]);
+                const int mask = composite(d, s);
+                seg[slot * nrays + pix] = make_float4(p[0], p[1], p[2], __int_as_float(mask));
+                seg[(slot + 1) * nrays + pix] = dprev;
+                for (int q = 0; q < 3; ++q) p[q] = fmaf(r.dir[q], a.step, p[q]);
+            }
+            for (long long k = k1 - 1; k >= k0; --k) {
+                const long long slot = (k - k0) * 2;
+                const float4 sp = seg[slot * nrays + pix];
+                const float4 dk = seg[(slot + 1) * nrays + pix];
+                const float pk[3] = {sp.x, sp.y, sp.z};
+                const int mask = __float_as_int(sp.w);
+                Sample s;
+                eval_sample<FT, LC>(a, tf, op, r, pk, s);
+                for (int q = 0; q < 4; ++q)
+                    if (mask & (1 << q)) G[q] = 0.0f;
+                const float t = 1.0f - dk.w;
+                const float al = s.a;
+                const float src[4] = {s.rgb[0] * al, s.rgb[1] * al, s.rgb[2] * al, al};
+                float dsrc[4];
+                for (int q = 0; q < 4; ++q) dsrc[q] = t * G[q];
+                const float gs = G[0] * src[0] + G[1] * src[1] + G[2] * src[2] + G[3] * src[3];
+                G[3] = G[3] - gs;
+                float da = dsrc[3];
+                float drgb[3];
+                for (int q = 0; q < 3; ++q) {
+                    drgb[q] = dsrc[q] * al;
+                    da = da + dsrc[q] * s.rgb[q];
+                }
+                const float b = 1.0f - s.araw;
+                float dadraw;
+                if (a.corr_mode == 0)
+                    dadraw = 1.0f;
+                else if (b > 0.0f)
+                    dadraw = a.alpha_corr * powf(b, a.alpha_corr - 1.0f);
+                else
+                    dadraw = 0.0f;
+                const float draw = s.aclamp ? 0.0f : da * dadraw;
+                for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m, A.fx_scale);
+                splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, A.fx_scale);
+                if (A.dsigma_fx) {
+                    const float dsg = draw * s.ao * s.tfa;
+                    // corner voxels of the CLAMP cell over the sigma (LIC) grid
+                    const int* n3 = a.lic.n;
+                    float fx, fy, fz;
+                    const int px = tc_clamp(pk[0], n3[0], fx) - 1;
+                    const int py = tc_clamp(pk[1], n3[1], fy) - 1;
+                    const int pz = tc_clamp(pk[2], n3[2], fz) - 1;
+                    const int xs[2] = {max(px, 0), min(px + 1, n3[0] - 1)};
+                    const int ys[2] = {max(py, 0), min(py + 1, n3[1] - 1)};
+                    const int zs[2] = {max(pz, 0), min(pz + 1, n3[2] - 1)};
+                    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
+                    for (int cz = 0; cz < 2; ++cz)
+                        for (int cy = 0; cy < 2; ++cy)
+                            for (int cx = 0; cx < 2; ++cx) {
+                                const long long v = to_fx(wx[cx] * wy[cy] * wz[cz] * dsg, A.fx_scale);
+                                if (v) {
+                                    const long long o = ((long long)zs[cz] * n3[1] + ys[cy]) * n3[0] + xs[cx];
+                                    atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+                                }
+                            }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < 1024; q += kRThreads) {
+        const long long v = dtf[q];
+        if (v) atomicAdd((unsigned long long*)&A.dtf_fx[q], (unsigned long long)v);
+    }
+}
+
+__global__ void k_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)((double)src[i] * inv_scale);
+}
+
+}  // namespace
+
+hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
+{
+    dim3 block(kBX, kBY);
+    dim3 grid((R.view.width + kBX - 1) / kBX, (R.view.height + kBY - 1) / kBY);
+    if (R.field_fp16) {
+        if (R.lc == 4)
+            hipLaunchKernelGGL((k_render<half4_t, 4>), grid, block, 0, s, R);
+        else
+            hipLaunchKernelGGL((k_render<half4_t, 1>), grid, block, 0, s, R);
+    } else {
+        if (R.lc == 4)
+            hipLaunchKernelGGL((k_render<float4, 4>), grid, block, 0, s, R);
+        else
+            hipLaunchKernelGGL((k_render<float4, 1>), grid, block, 0, s, R);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
+{
+    dim3 block(kBX, kBY);
+    dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
+    if (A.r.field_fp16) {
+        if (A.r.lc == 4)
+            hipLaunchKernelGGL((k_render_adjoint<half4_t, 4>), grid, block, 0, s, A);
+        else
+            hipLaunchKernelGGL((k_render_adjoint<half4_t, 1>), grid, block, 0, s, A);
+    } else {
+        if (A.r.lc == 4)
+            hipLaunchKernelGGL((k_render_adjoint<float4, 4>), grid, block, 0, s, A);
+        else
+            hipLaunchKernelGGL((k_render_adjoint<float4, 1>), grid, block, 0, s, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
+                                 hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fixed_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n,
+                       inv_scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpad_clamp_fixed(const long long* src, const int d[3], float* dst, double inv_scale,
+                                    hipStream_t s)
+{
+    return launch_fixed_to_float(src, dst, (long long)d[0] * d[1] * d[2], inv_scale, s);
+}
+
+}  // namespace vv
