@@ -1,0 +1,259 @@
+"""Benchmark: 3D LIC volume + ray-cast frame on one MI355X.
+
+Workload (the "plain method at full size" configuration):
+  field   256^3 float32, 12 seeded Rankine vortices + uniform flow (0.1,0,0)
+  noise   dense seeded uniform uint8, 256^3
+  LIC     Gaussian filter (sigma = L/4), L = F = B = 40, step 0.005
+  render  1024^2 ray cast, sample distance 1/256, vector-length TF
+One step = one LIC-volume computation + one frame, inputs resident in HBM.
+
+Prints one JSON line.  `value` is LIC voxel-samples per second over the
+whole step (LIC + frame); the components, the FP16-storage variant and the
+forward+adjoint time are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--image", type=int, default=1024)
+    ap.add_argument("--L", type=int, default=40)
+    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip FP16 and adjoint legs")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def lic_bytes_per_voxel(F: int, B: int, field_bytes: int = 12, noise_bytes: int = 1,
+                        out_bytes: int = 4) -> int:
+    """Algorithmic bytes of one LIC voxel: (1 + 2(F+B)) trilinear field fetches
+    and (F+B+1) noise fetches of 8 texels each, plus the output."""
+    return (1 + 2 * (F + B)) * 8 * field_bytes + (F + B + 1) * 8 * noise_bytes + out_bytes
+
+
+def load_traffic(name: str):
+    """HBM bytes per LIC launch from the committed rocprof PMC summary, if any."""
+    p = ROOT / "profiles" / name
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1])
+            except ValueError:
+                return None
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import vectorvisualization_amd as vv
+    from vectorvisualization_amd import fields
+
+    n, L, W = args.n, args.L, args.image
+    dims = (n, n, n)
+    dev = torch.device("cuda:0")
+    storage = vv.STORE_FP16 if args.storage == "fp16" else vv.STORE_FP32
+
+    print(f"[bench] generating {n}^3 Rankine field on {dev}", file=sys.stderr, flush=True)
+    field = fields.rankine_vortices_torch(n, device=dev).contiguous()
+    ctx = vv.Context(0)
+    ctx.upload_field(field, dims, storage=storage)
+    ctx.generate_noise(1, vv.NOISE_UNIFORM, 0.0, dims)
+    lut, inv = vv.make_filter(vv.FILTER_GAUSSIAN, L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    p.step_size_lic = 0.005
+    p.step_size_vol = 1.0 / 256.0
+    p.gradient_scale = 3.0
+    p.tf_index = vv.TF_INDEX_MAG
+    ctx.set_lic_params(p)
+    rgba, op = fields.tf_magnitude()
+    ctx.set_tf(rgba, op)
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(W, W, cen)
+    image = torch.empty(W * W * 4, dtype=torch.float32, device=dev)
+
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    print("[bench] warmup", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        ctx.compute_lic()
+        ctx.render(cam, out=image)
+    _, ray_samples = ctx.render(cam, out=image, count_samples=True)
+    torch.cuda.synchronize()
+
+    # ---- timed region: K steps of (LIC volume + frame) ----
+    print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
+    marks = [(ev(), ev(), ev()) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b, c in marks:
+        a.record(stream)
+        ctx.compute_lic()
+        b.record(stream)
+        ctx.render(cam, out=image)
+        c.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lic_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in marks]))
+    frame_ms = float(np.mean([b.elapsed_time(c) for _, b, c in marks]))
+    step_ms = float(np.mean([a.elapsed_time(c) for a, _, c in marks]))
+    wall_ms = wall * 1000.0 / args.steps
+
+    vox = n ** 3
+    samples = vox * (2 * L + 1)
+    value = samples / (step_ms / 1000.0)
+    bpv = lic_bytes_per_voxel(L, L, 6 if storage == vv.STORE_FP16 else 12)
+    lic_bytes = bpv * vox
+    achieved = lic_bytes / (lic_ms / 1000.0) / 1e9
+    tag = f"lic_rankine{n}_gauss{L}_{args.storage}"
+    traffic = load_traffic(f"{tag}_pmc.txt")
+
+    out = {
+        "metric": "LIC voxel-samples/s + ms/frame (256^3, 1024^2 view); fwd+adjoint vs plain GPU",
+        "value": value,
+        "unit": "voxel-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage == vv.STORE_FP32 else "f16-storage/f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"LIC {n}^3 Rankine x12 + (0.1,0,0), dense uniform noise, Gaussian L={L} "
+                        f"h=0.005; ray cast {W}^2 step 1/256 vector-length TF",
+            "field": f"{n}^3 {args.storage}",
+            "image": f"{W}x{W}",
+            "steps_fwd_bwd": L,
+        },
+        "lic_ms": lic_ms,
+        "frame_ms": frame_ms,
+        "wall_ms_per_step": wall_ms,
+        "lic_voxel_samples_per_s": samples / (lic_ms / 1000.0),
+        "ray_samples_per_frame": int(ray_samples),
+        "ray_samples_per_s": ray_samples / (frame_ms / 1000.0),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_lic",
+            "algorithmic_bytes_per_voxel": bpv,
+        },
+    }
+
+    if not args.no_extras:
+        # FP16-stored field, same LIC: speedup over the FP32 plain path
+        other = vv.STORE_FP16 if storage == vv.STORE_FP32 else vv.STORE_FP32
+        ctx.upload_field(field, dims, storage=other)
+        ctx.compute_lic()
+        torch.cuda.synchronize()
+        e = [(ev(), ev()) for _ in range(3)]
+        for a, b in e:
+            a.record(stream)
+            ctx.compute_lic()
+            b.record(stream)
+        torch.cuda.synchronize()
+        other_ms = float(np.mean([a.elapsed_time(b) for a, b in e]))
+        key = "fp16_lic_ms" if other == vv.STORE_FP16 else "fp32_lic_ms"
+        out[key] = other_ms
+        fp16_ms = other_ms if other == vv.STORE_FP16 else lic_ms
+        fp32_ms = lic_ms if other == vv.STORE_FP16 else other_ms
+        out["fp16_speedup_vs_fp32_lic"] = fp32_ms / fp16_ms
+        ctx.upload_field(field, dims, storage=storage)
+        ctx.compute_lic()
+        # forward + TF/opacity adjoint of an L2 image loss (1024^2)
+        dimg = torch.randn(W * W * 4, dtype=torch.float32, device=dev) * 1e-3
+        dtf = torch.empty(1024, dtype=torch.float32, device=dev)
+        dsig = torch.empty(vox, dtype=torch.float32, device=dev)
+        ctx.render_adjoint(cam, dimg, dtf, dsig)
+        torch.cuda.synchronize()
+        a, b, c = ev(), ev(), ev()
+        a.record(stream)
+        ctx.render(cam, out=image)
+        b.record(stream)
+        ctx.render_adjoint(cam, dimg, dtf, dsig)
+        c.record(stream)
+        torch.cuda.synchronize()
+        out["adjoint_ms"] = b.elapsed_time(c)
+        out["fwd_plus_adjoint_ms"] = a.elapsed_time(c)
+        out["adjoint_over_forward"] = b.elapsed_time(c) / max(a.elapsed_time(b), 1e-9)
+
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(field.cpu().numpy(), dims, L, lut, inv, p,
+                                           args.cpu_seconds, samples / vox)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(field_np, dims, L, lut, inv, p, seconds, samples_per_voxel):
+    """The oracle's FP32 LIC on a z-slab of the same workload, all host threads."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    n = dims[0]
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
+    f4 = orc.encode_field(field_np, dims)
+    noise = orc.gen_noise(1, 1, 0.0, dims)
+    op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.005, inv_filter_area=inv,
+                    gradient_scale=3.0)
+    z0 = n // 2
+    t = time.perf_counter()
+    orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + 1, nthreads=cores)
+    one = time.perf_counter() - t
+    k = int(max(1, min(n - z0, round(seconds / max(one, 1e-3)))))
+    t = time.perf_counter()
+    orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + k, nthreads=cores)
+    dt = time.perf_counter() - t
+    vox = k * n * n
+    return {
+        "value": vox * samples_per_voxel / dt,
+        "unit": "voxel-samples/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle FP32 LIC of z-slices {z0}..{z0 + k - 1} ({vox} voxels) of the same "
+                  f"{n}^3 workload, {dt:.1f} s",
+    }
+
+
+if __name__ == "__main__":
+    main()

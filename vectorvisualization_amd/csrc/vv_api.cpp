@@ -92,6 +92,7 @@ struct vv_ctx {
     int lch = 1;
     DevBuf lic;
     DevBuf weights;
+    std::vector<float> weights_host;  // what `weights` holds
 
     // opacity scale (CLAMP layout, LIC grid)
     bool has_sigma = false;
@@ -786,9 +787,13 @@ vv_status vv_compute_lic(vv_ctx* c)
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<float> w;
     lic_weights(c, w);
-    HIPCHK(c, c->weights.ensure(sizeof(float) * w.size()));
-    HIPCHK(c, hipMemcpyAsync(c->weights.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice,
-                             c->stream));
+    if (w != c->weights_host) {
+        HIPCHK(c, c->weights.ensure(sizeof(float) * w.size()));
+        HIPCHK(c, hipMemcpyAsync(c->weights.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));  // w is a local
+        c->weights_host = w;
+    }
     const int nc = c->nch;
     HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
     LicLaunch L;

@@ -26,38 +26,56 @@ def abc_flow(n: int, A: float = np.sqrt(3.0), B: float = np.sqrt(2.0), C: float 
 
 
 def rankine_vortices(n: int, count: int = 12, seed: int = 1, background=(0.1, 0.0, 0.0),
-                     rmin: float = 0.05, rmax: float = 0.2, chunk: int = 32) -> np.ndarray:
+                     rmin: float = 0.05, rmax: float = 0.2, device="cpu"):
     """Sum of `count` Rankine vortices in the unit box plus a uniform flow.
 
     Vortex k has a seeded centre c_k in [0,1]^3, unit axis a_k, core radius
     R_k in [rmin, rmax] and strength s_k in +-[0.5, 1.5].  At distance rho
     from its axis it adds s_k * (rho/R_k if rho < R_k else R_k/rho) along
-    a_k x (p - c_k) / |a_k x (p - c_k)|.
+    a_k x (p - c_k) / |a_k x (p - c_k)|.  Returns a numpy array (computed
+    with rankine_vortices_torch on `device`).
     """
+    t = rankine_vortices_torch(n, count, seed, background, rmin, rmax, device)
+    return t.cpu().numpy()
+
+
+def rankine_vortices_torch(n: int, count: int = 12, seed: int = 1, background=(0.1, 0.0, 0.0),
+                           rmin: float = 0.05, rmax: float = 0.2, device="cpu"):
+    """rankine_vortices() as a contiguous (n, n, n, 3) float32 torch tensor on
+    `device`: seeded vortex parameters, component-wise float32 arithmetic."""
+    import torch
+
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.0, 1.0, (count, 3))
     axes = rng.normal(size=(count, 3))
     axes /= np.linalg.norm(axes, axis=1, keepdims=True)
     radii = rng.uniform(rmin, rmax, count)
     strength = rng.uniform(0.5, 1.5, count) * rng.choice([-1.0, 1.0], count)
-    t = _centres(n, 0.0, 1.0)
-    out = np.empty((n, n, n, 3), np.float32)
-    y2, x2 = np.meshgrid(t, t, indexing="ij")
-    for z0 in range(0, n, chunk):
-        zs = t[z0:z0 + chunk]
-        p = np.stack(np.broadcast_arrays(zs[:, None, None], y2[None], x2[None]), -1)[..., ::-1]
-        v = np.zeros(p.shape, np.float64)
-        v[...] = background
-        for k in range(count):
-            d = p - centres[k]
-            cr = np.cross(axes[k], d)
-            rho = np.linalg.norm(cr, axis=-1)
-            R = radii[k]
-            mag = np.where(rho < R, rho / R, R / np.maximum(rho, 1e-12))
-            with np.errstate(invalid="ignore", divide="ignore"):
-                unit = np.where(rho[..., None] > 1e-12, cr / rho[..., None], 0.0)
-            v += strength[k] * mag[..., None] * unit
-        out[z0:z0 + chunk] = v
+    t = torch.tensor(_centres(n, 0.0, 1.0), dtype=torch.float32, device=device)
+    z = t.view(n, 1, 1)
+    y = t.view(1, n, 1)
+    x = t.view(1, 1, n)
+    out = torch.empty((n, n, n, 3), dtype=torch.float32, device=device)
+    u = torch.full((n, n, n), float(background[0]), dtype=torch.float32, device=device)
+    v = torch.full((n, n, n), float(background[1]), dtype=torch.float32, device=device)
+    w = torch.full((n, n, n), float(background[2]), dtype=torch.float32, device=device)
+    for k in range(count):
+        ax, ay, az = (float(a) for a in axes[k])
+        dx, dy, dz = x - float(centres[k, 0]), y - float(centres[k, 1]), z - float(centres[k, 2])
+        cx = ay * dz - az * dy
+        cy = az * dx - ax * dz
+        cz = ax * dy - ay * dx
+        rho = torch.sqrt(cx * cx + cy * cy + cz * cz)
+        R = float(radii[k])
+        mag = torch.where(rho < R, rho / R, R / torch.clamp(rho, min=1e-12))
+        sc = float(strength[k]) * mag / torch.clamp(rho, min=1e-12)
+        sc = torch.where(rho > 1e-12, sc, torch.zeros_like(sc))
+        u += sc * cx
+        v += sc * cy
+        w += sc * cz
+    out[..., 0] = u
+    out[..., 1] = v
+    out[..., 2] = w
     return out
 
 
@@ -97,6 +115,19 @@ def tf_linear_ramp():
     r = np.arange(256, dtype=np.float32) / 255.0
     rgba = np.stack([r, r, r, r], 1).astype(np.float32).ravel()
     return rgba, r.copy()
+
+
+def tf_magnitude(alpha_lo: float = 0.02, alpha_hi: float = 0.12):
+    """"Vector-length" TF for an index = |v|/max|v|: rgb runs blue -> green
+    -> red with the magnitude, alpha ramps from alpha_lo to alpha_hi; the
+    LIC-opacity table is the linear ramp i/255."""
+    t = np.arange(256, dtype=np.float64) / 255.0
+    rgba = np.empty((256, 4), np.float64)
+    rgba[:, 0] = np.clip(2.0 * t - 0.5, 0.0, 1.0)
+    rgba[:, 1] = 1.0 - np.abs(2.0 * t - 1.0)
+    rgba[:, 2] = np.clip(1.5 - 2.0 * t, 0.0, 1.0)
+    rgba[:, 3] = alpha_lo + (alpha_hi - alpha_lo) * t
+    return rgba.astype(np.float32).ravel(), t.astype(np.float32)
 
 
 def tf_random(seed: int):
