@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP16 and adjoint legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="only 3 LIC launches and 3 copies of 256 MiB (rocprof counter runs)")
     return ap.parse_args()
 
 
@@ -102,6 +104,21 @@ def main():
 
     stream = torch.cuda.ExternalStream(ctx.stream)
     ev = lambda: torch.cuda.Event(enable_timing=True)  # noqa: E731
+
+    if args.pmc_probe:
+        # counter runs: the LIC kernel, then a plain copy of known size whose
+        # FETCH_SIZE / WRITE_SIZE calibrate the counters (256 MiB each way)
+        for _ in range(3):
+            ctx.compute_lic()
+        ctx.synchronize()
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ctx.close()
+        print("[bench] pmc probe done", file=sys.stderr, flush=True)
+        return
 
     print("[bench] warmup", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
