@@ -209,6 +209,15 @@ vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
+/* ---- launch tuning (used by the autotuner) ----------------------------
+ * keys: "lic.tile_x", "lic.tile_y", "lic.tile_z" (powers of two whose
+ * product is 256 threads per block), "lic.super_x", "lic.super_y",
+ * "lic.super_z" (tiles per super-tile: the 3D-blocked order in which tiles
+ * are handed to the GPU), "lic.cache" (0/1: per-lane trilinear cell
+ * caches).  Results do not depend on these settings. */
+vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
+vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);
+
 /* ---- helpers exposed for tests --------------------------------------- */
 /* The 24 floats + 2 ints of the per-frame ray setup (see vv_view_block in
  * the implementation): cam[3] du[3] dv[3] dw[3] inv_w2 inv_h2 extent[3]

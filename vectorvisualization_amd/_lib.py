@@ -112,6 +112,8 @@ def _load() -> C.CDLL:
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
+        "vv_set_tuning": (I, [P, C.c_char_p, I]),
+        "vv_get_tuning": (I, [P, C.c_char_p, IP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -131,6 +133,7 @@ EXPORTED = [
     "vv_default_tf_u8", "vv_upload_scalar", "vv_set_lic_params", "vv_get_lic_params",
     "vv_compute_lic", "vv_lic_info", "vv_download_lic", "vv_upload_lic", "vv_render",
     "vv_set_opacity_scale", "vv_render_adjoint", "vv_view_block", "vv_lic_weights",
+    "vv_set_tuning", "vv_get_tuning",
 ]
 
 
@@ -323,6 +326,14 @@ class Context:
         out = np.empty(1 + p.steps_forward + p.steps_backward, np.float32)
         self._chk(LIB.vv_lic_weights(self._h, out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
+
+    def set_tuning(self, key: str, value: int) -> None:
+        self._chk(LIB.vv_set_tuning(self._h, key.encode(), int(value)))
+
+    def get_tuning(self, key: str) -> int:
+        v = C.c_int()
+        self._chk(LIB.vv_get_tuning(self._h, key.encode(), C.byref(v)))
+        return v.value
 
     # -- render --
     def render(self, cam: Camera, out=None, count_samples: bool = False):

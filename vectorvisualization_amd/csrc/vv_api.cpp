@@ -94,6 +94,11 @@ struct vv_ctx {
     DevBuf weights;
     std::vector<float> weights_host;  // what `weights` holds
 
+    // launch tuning (vv_set_tuning)
+    int lic_tile[3] = {16, 4, 4};
+    int lic_super[3] = {2, 8, 8};
+    int lic_cache = 1;
+
     // opacity scale (CLAMP layout, LIC grid)
     bool has_sigma = false;
     DevBuf sigma;
@@ -784,6 +789,9 @@ vv_status vv_compute_lic(vv_ctx* c)
         return fail(c, VV_ESTATE, "gate enabled but no scalar volume uploaded");
     if (p.illum_mode == VV_ILLUM_GRADIENT && c->nch != 4)
         return fail(c, VV_ESTATE, "gradient illumination needs RGBA noise (with_gradients)");
+    if (c->lic_tile[0] * c->lic_tile[1] * c->lic_tile[2] != 256)
+        return fail(c, VV_EINVAL, "LIC tile %dx%dx%d must hold 256 threads", c->lic_tile[0],
+                    c->lic_tile[1], c->lic_tile[2]);
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<float> w;
     lic_weights(c, w);
@@ -816,9 +824,14 @@ vv_status vv_compute_lic(vv_ctx* c)
     L.freq = p.freq_scale;
     for (int a = 0; a < 3; ++a) L.scale[a] = c->scale[a];
     L.out = make_repeat_desc(c->lic.p, c->fdims);
-    L.tile[0] = 8;
-    L.tile[1] = 8;
-    L.tile[2] = 4;
+    for (int a = 0; a < 3; ++a) L.tile[a] = c->lic_tile[a];
+    for (int a = 0; a < 3; ++a) L.super_tile[a] = c->lic_super[a];
+    L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * (L.field_fp16 ? 8ull : 16ull);
+    L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
+    // the cached kernels address through 32-bit buffer offsets
+    // and 24-bit index multiplies (plane strides below 2^24 elements)
+    L.cache = c->lic_cache && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
+              (long long)L.field.sz < (1ll << 24) && (long long)L.noise.sz < (1ll << 24);
     HIPCHK(c, launch_lic(L, c->stream));
     HIPCHK(c, launch_repeat_halo_f32((float*)c->lic.p, nc, c->fdims, c->stream));
     // the opacity scale lives on the LIC grid: drop it when that grid changes
@@ -873,6 +886,45 @@ vv_status vv_upload_lic(vv_ctx* c, const float* lic, int channels, const int dim
     c->lch = channels;
     c->has_lic = true;
     c->has_sigma = false;  // opacity scale lives on the LIC grid
+    return VV_OK;
+}
+
+// ---- tuning ------------------------------------------------------------------
+vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
+{
+    if (!c) return VV_EINVAL;
+    if (!key) return fail(c, VV_EINVAL, "key is NULL");
+    const std::string k(key);
+    int* slot = nullptr;
+    if (k == "lic.tile_x") slot = &c->lic_tile[0];
+    else if (k == "lic.tile_y") slot = &c->lic_tile[1];
+    else if (k == "lic.tile_z") slot = &c->lic_tile[2];
+    else if (k == "lic.cache") slot = &c->lic_cache;
+    else if (k == "lic.super_x") slot = &c->lic_super[0];
+    else if (k == "lic.super_y") slot = &c->lic_super[1];
+    else if (k == "lic.super_z") slot = &c->lic_super[2];
+    else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
+    if (k == "lic.cache") {
+        if (value != 0 && value != 1) return fail(c, VV_EINVAL, "lic.cache must be 0 or 1");
+    } else if (value < 1 || value > 256 || (value & (value - 1))) {
+        return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
+    }
+    *slot = value;
+    return VV_OK;
+}
+
+vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
+{
+    if (!c || !key || !value) return VV_EINVAL;
+    const std::string k(key);
+    if (k == "lic.tile_x") *value = c->lic_tile[0];
+    else if (k == "lic.tile_y") *value = c->lic_tile[1];
+    else if (k == "lic.tile_z") *value = c->lic_tile[2];
+    else if (k == "lic.cache") *value = c->lic_cache;
+    else if (k == "lic.super_x") *value = c->lic_super[0];
+    else if (k == "lic.super_y") *value = c->lic_super[1];
+    else if (k == "lic.super_z") *value = c->lic_super[2];
+    else return VV_EINVAL;
     return VV_OK;
 }
 

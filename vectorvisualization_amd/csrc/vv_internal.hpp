@@ -53,6 +53,9 @@ struct LicLaunch {
     float scale[3];
     VolDesc out;       // REPEAT layout, float (nc 1) or float4 (nc 4)
     int tile[3];       // threads per block along x,y,z
+    int super_tile[3]; // tiles per super-tile along x,y,z (tile order)
+    int cache;         // 1: per-lane cell caches (scalar noise, no gate)
+    unsigned long long field_bytes, noise_bytes;  // padded buffer sizes
 };
 
 struct ViewBlock {

@@ -338,11 +338,27 @@ struct LicArgs {
     float sc0, sc1, sc2;
     float* out;
     int osy, osz;
+    unsigned field_bytes, noise_bytes;  // buffer sizes for the cached kernels
     int tx, ty, tz;      // block tile
     int gx, gy, gz;      // tiles per axis
     int nblocks;         // gx*gy*gz
     int per_xcd;         // tiles handled by each of the 8 XCDs
+    int sx, sy, sz;      // super-tile size in tiles (3D-blocked tile order)
+    int sgx, sgy;        // super-tiles per axis (x, y)
 };
+
+// logical tile t -> tile coordinates: super-tiles in raster order, tiles
+// raster order inside each super-tile, so the tiles resident on one XCD at a
+// time cover a compact 3D region whose streamline halo fits that XCD's L2
+__device__ __forceinline__ void tile_coords(const LicArgs& a, int t, int& bx, int& by, int& bz)
+{
+    const int per_super = a.sx * a.sy * a.sz;
+    const int st = t / per_super, r = t - st * per_super;
+    const int ux = st % a.sgx, uy = (st / a.sgx) % a.sgy, uz = st / (a.sgx * a.sgy);
+    bx = ux * a.sx + r % a.sx;
+    by = uy * a.sy + (r / a.sx) % a.sy;
+    bz = uz * a.sz + r / (a.sx * a.sy);
+}
 
 template <int NC>
 struct NoiseT;
@@ -393,6 +409,317 @@ __device__ __forceinline__ float3 field_rgb(const LicArgs& a, float x, float y, 
     return tri_rgb(fv, c);
 }
 
+// Per-lane cache of the last trilinear cell.  Consecutive samples of a
+// streamline are ~h_e*n voxels apart (0.38 at 256^3, h = 0.005), so the
+// corrector sample almost always lands in the predictor's cell and the next
+// predictor often in the same cell: a lane that hits skips its gathers (it
+// is masked off the loads).  For each channel the cache keeps the four x0
+// texels and the four differences (x1 - x0) of the cell, so the four x-lerps
+// of a hit are single fmas; the values and the operation order are exactly
+// those of tri(), so results do not change.
+struct FieldCache {
+    int base;
+    float v0[3][4];   // [channel][(y,z) corner]: texel at x0
+    float dx[3][4];   // texel(x1) - texel(x0)
+};
+
+struct NoiseCache {
+    int base;
+    float v0[4], dx[4];
+};
+
+__device__ __forceinline__ float tri_cached(const float* v0, const float* dx, float fx, float fy, float fz)
+{
+    const float c00 = fmaf(fx, dx[0], v0[0]);
+    const float c10 = fmaf(fx, dx[1], v0[1]);
+    const float c01 = fmaf(fx, dx[2], v0[2]);
+    const float c11 = fmaf(fx, dx[3], v0[3]);
+    const float c0 = lerpf(c00, c10, fy);
+    const float c1 = lerpf(c01, c11, fy);
+    return lerpf(c0, c1, fz);
+}
+
+// axis coordinate: floor and fraction of x = u*n - 0.5 (shared by the
+// CLAMP field cell and, on the same grid, the REPEAT noise cell)
+struct Ax {
+    float fl, f;
+};
+
+__device__ __forceinline__ Ax axis(float u, int n)
+{
+    Ax r;
+    const float x = fmaf(u, (float)n, -0.5f);
+    r.fl = floorf(x);
+    r.f = x - r.fl;
+    return r;
+}
+
+// corner index of a CLAMP-padded axis: clamp(floor, -1, n-1) + 1
+__device__ __forceinline__ unsigned clamp_idx(float fl, float nm1)
+{
+    return (unsigned)((int)__builtin_amdgcn_fmed3f(fl, -1.0f, nm1) + 1);
+}
+
+// corner index of a REPEAT-padded axis: floor mod n
+__device__ __forceinline__ unsigned repeat_idx(float fl, int n)
+{
+    int i = (int)__builtin_amdgcn_fmed3f(fl, -1.0e9f, 1.0e9f);
+    if ((unsigned)i >= (unsigned)n) {
+        i %= n;
+        if (i < 0) i += n;
+    }
+    return (unsigned)i;
+}
+
+// raw buffer resource over a device array (gfx9 dword3 0x00020000); loads
+// take 32-bit byte offsets and an offset past num_records reads 0
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename FT>
+struct FieldLoad;
+
+// float4 texels: two 12-byte loads (rgb of x0 and of x1)
+template <>
+struct FieldLoad<float4> {
+    static constexpr unsigned kShift = 4;
+    __device__ static __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned off, float t0[3],
+                                                float t1[3])
+    {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b96(r, off + 16u, 0, 0);
+        t0[0] = __int_as_float(a[0]);
+        t0[1] = __int_as_float(a[1]);
+        t0[2] = __int_as_float(a[2]);
+        t1[0] = __int_as_float(b[0]);
+        t1[1] = __int_as_float(b[1]);
+        t1[2] = __int_as_float(b[2]);
+    }
+};
+
+// half4 texels: one 16-byte load holds x0 and x1
+template <>
+struct FieldLoad<half4_t> {
+    static constexpr unsigned kShift = 3;
+    __device__ static __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned off, float t0[3],
+                                                float t1[3])
+    {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        t0[0] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[0] & 0xffffu)));
+        t0[1] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[0] >> 16)));
+        t0[2] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[1] & 0xffffu)));
+        t1[0] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[2] & 0xffffu)));
+        t1[1] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[2] >> 16)));
+        t1[2] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[3] & 0xffffu)));
+    }
+};
+
+struct CachedArgs {
+    __amdgpu_buffer_rsrc_t fr, nr;
+    float fnm1[3];      // field n-1 per axis
+    unsigned fsy, fsz;  // field element strides
+    unsigned nsy, nsz;  // noise element strides
+};
+
+template <typename FT>
+__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache& fc, const Ax& ax,
+                                                  const Ax& ay, const Ax& az)
+{
+    const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
+                          __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
+    if ((int)base != fc.base) {
+        fc.base = (int)base;
+        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float t0[3], t1[3];
+            FieldLoad<FT>::pair(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                fc.v0[ch][k] = t0[ch];
+                fc.dx[ch][k] = t1[ch] - t0[ch];
+            }
+        }
+    }
+    float3 r;
+    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
+    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
+    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
+    return r;
+}
+
+// corrector sample: the field cell and the noise cell of the same position.
+// Both fills are issued before either result is used, so the two gathers
+// overlap (one memory round trip instead of two); values and arithmetic
+// are those of field_from_axes and noise_from_axes.
+template <typename FT>
+__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache& fc,
+                                                  NoiseCache& nc, const Ax& ax, const Ax& ay,
+                                                  const Ax& az, const Ax& nx_, const Ax& ny_,
+                                                  const Ax& nz_, float& noise)
+{
+    const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
+                        __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
+    const unsigned nb = __umul24(repeat_idx(nz_.fl, a.noise.n[2]), c.nsz) +
+                        __umul24(repeat_idx(ny_.fl, a.noise.n[1]), c.nsy) + repeat_idx(nx_.fl, a.noise.n[0]);
+    const bool fmiss = (int)fb != fc.base;
+    const bool nmiss = (int)nb != nc.base;
+    unsigned short nraw[4];
+    if (fmiss) {
+        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            float t0[3], t1[3];
+            FieldLoad<FT>::pair(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                fc.v0[ch][k] = t0[ch];
+                fc.dx[ch][k] = t1[ch];  // raw x1 texel until both fills are in flight
+            }
+        }
+    }
+    if (nmiss) {
+        const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) nraw[k] = __builtin_amdgcn_raw_buffer_load_b16(c.nr, nb + off[k], 0, 0);
+    }
+    if (fmiss) {
+        fc.base = (int)fb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) fc.dx[ch][k] = fc.dx[ch][k] - fc.v0[ch][k];
+    }
+    if (nmiss) {
+        nc.base = (int)nb;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float t0 = (float)(nraw[k] & 0xffu), t1 = (float)(nraw[k] >> 8);
+            nc.v0[k] = t0;
+            nc.dx[k] = t1 - t0;
+        }
+    }
+    noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f) * kInv255;
+    float3 r;
+    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
+    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
+    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
+    return r;
+}
+
+__device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedArgs& c, NoiseCache& nc,
+                                                 const Ax& ax, const Ax& ay, const Ax& az)
+{
+    const unsigned base = __umul24(repeat_idx(az.fl, a.noise.n[2]), c.nsz) +
+                          __umul24(repeat_idx(ay.fl, a.noise.n[1]), c.nsy) + repeat_idx(ax.fl, a.noise.n[0]);
+    if ((int)base != nc.base) {
+        nc.base = (int)base;
+        const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            // two adjacent bytes (x0, x1) in one load
+            const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(c.nr, base + off[k], 0, 0);
+            const float t0 = (float)(u & 0xffu), t1 = (float)(u >> 8);
+            nc.v0[k] = t0;
+            nc.dx[k] = t1 - t0;
+        }
+    }
+    return tri_cached(nc.v0, nc.dx, ax.f, ay.f, az.f) * kInv255;
+}
+
+// LIC with the per-lane cell caches (scalar noise without gate).  SAME: the
+// noise grid equals the field grid and freq == 1, so the noise cell of the
+// corrector position reuses the corrector's floors and fractions.
+template <typename FT, bool SAME>
+__global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
+{
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * a.per_xcd + (b >> 3);
+    if (tile >= a.nblocks) return;
+    int bx, by, bz;
+    tile_coords(a, tile, bx, by, bz);
+    const int t = threadIdx.x;
+    const int x = bx * a.tx + t % a.tx;
+    const int y = by * a.ty + (t / a.tx) % a.ty;
+    const int z = bz * a.tz + t / (a.tx * a.ty);
+    const int nx = a.field.n[0], ny = a.field.n[1], nz = a.field.n[2];
+    if (x >= nx || y >= ny || z >= nz) return;
+
+    const float sx = ((float)x + 0.5f) / (float)nx * a.sc0;
+    const float sy = ((float)y + 0.5f) / (float)ny * a.sc1;
+    const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
+
+    CachedArgs c;
+    c.fr = rsrc(a.field.p, a.field_bytes);
+    c.nr = rsrc(a.noise.p, a.noise_bytes);
+    c.fnm1[0] = (float)(nx - 1);
+    c.fnm1[1] = (float)(ny - 1);
+    c.fnm1[2] = (float)(nz - 1);
+    c.fsy = (unsigned)a.field.sy;
+    c.fsz = (unsigned)a.field.sz;
+    c.nsy = (unsigned)a.noise.sy;
+    c.nsz = (unsigned)a.noise.sz;
+
+    FieldCache fc;
+    fc.base = -1;
+    NoiseCache ncache;
+    ncache.base = -1;
+    auto noise = [&](float px, float py, float pz, const Ax& ax, const Ax& ay, const Ax& az) -> float {
+        if constexpr (SAME) {
+            return noise_from_axes(a, c, ncache, ax, ay, az);
+        } else {
+            const float q0 = px * a.freq, q1 = py * a.freq, q2 = pz * a.freq;
+            return noise_from_axes(a, c, ncache, axis(q0, a.noise.n[0]), axis(q1, a.noise.n[1]),
+                                   axis(q2, a.noise.n[2]));
+        }
+    };
+    Ax ax = axis(sx, nx), ay = axis(sy, ny), az = axis(sz, nz);
+    const float3 s0 = field_from_axes<FT>(c, fc, ax, ay, az);
+    float acc = noise(sx, sy, sz, ax, ay, az) * a.w[0];
+
+#pragma unroll 1
+    for (int dirn = 0; dirn < 2; ++dirn) {
+        const int steps = dirn == 0 ? a.B : a.F;
+        const float h = dirn == 0 ? -a.he : a.he;
+        const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
+        float px = sx, py = sy, pz = sz;
+        float3 s = s0;
+#pragma unroll 1
+        for (int i = 0; i < steps; ++i) {
+            // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector.
+            // 2s is exact, so fma(2, s, -1) rounds 2s - 1 exactly once, and
+            // 0.5*(d1 + d2) is exact, so fma(0.5, d1 + d2, p) rounds once:
+            // both equal the two-operation forms bit for bit.
+            const float d1x = fmaf(2.0f, s.x, -1.0f) * h;
+            const float d1y = fmaf(2.0f, s.y, -1.0f) * h;
+            const float d1z = fmaf(2.0f, s.z, -1.0f) * h;
+            const float qx = px + d1x, qy = py + d1y, qz = pz + d1z;
+            const float3 s2 = field_from_axes<FT>(c, fc, axis(qx, nx), axis(qy, ny), axis(qz, nz));
+            const float d2x = fmaf(2.0f, s2.x, -1.0f) * h;
+            const float d2y = fmaf(2.0f, s2.y, -1.0f) * h;
+            const float d2z = fmaf(2.0f, s2.z, -1.0f) * h;
+            px = fmaf(0.5f, d1x + d2x, px);
+            py = fmaf(0.5f, d1y + d2y, py);
+            pz = fmaf(0.5f, d1z + d2z, pz);
+            ax = axis(px, nx);
+            ay = axis(py, ny);
+            az = axis(pz, nz);
+            float nval;
+            if constexpr (SAME) {
+                s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, ax, ay, az, nval);
+            } else {
+                s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, axis(px * a.freq, a.noise.n[0]),
+                                        axis(py * a.freq, a.noise.n[1]), axis(pz * a.freq, a.noise.n[2]),
+                                        nval);
+            }
+            acc = acc + nval * wk[i];
+        }
+    }
+    a.out[(long long)z * a.osz + (long long)y * a.osy + x] = acc * a.scl;
+}
+
 template <typename FT, int NC, bool GATE>
 __global__ void __launch_bounds__(kThreads) k_lic(LicArgs a)
 {
@@ -402,7 +729,8 @@ __global__ void __launch_bounds__(kThreads) k_lic(LicArgs a)
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.per_xcd + (b >> 3);
     if (tile >= a.nblocks) return;
-    const int bx = tile % a.gx, by = (tile / a.gx) % a.gy, bz = tile / (a.gx * a.gy);
+    int bx, by, bz;
+    tile_coords(a, tile, bx, by, bz);
     const int t = threadIdx.x;
     const int x = bx * a.tx + t % a.tx;
     const int y = by * a.ty + (t / a.tx) % a.ty;
@@ -618,20 +946,46 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
     a.sc1 = L.scale[1];
     a.sc2 = L.scale[2];
     a.out = (float*)L.out.p;
+    a.field_bytes = (unsigned)L.field_bytes;
+    a.noise_bytes = (unsigned)L.noise_bytes;
     a.osy = L.out.sy;
     a.osz = L.out.sz;
     a.tx = L.tile[0];
     a.ty = L.tile[1];
     a.tz = L.tile[2];
     if (a.tx * a.ty * a.tz != kThreads) return hipErrorInvalidValue;
-    a.gx = (L.field.n[0] + a.tx - 1) / a.tx;
-    a.gy = (L.field.n[1] + a.ty - 1) / a.ty;
-    a.gz = (L.field.n[2] + a.tz - 1) / a.tz;
+    a.sx = L.super_tile[0] > 0 ? L.super_tile[0] : 1;
+    a.sy = L.super_tile[1] > 0 ? L.super_tile[1] : 1;
+    a.sz = L.super_tile[2] > 0 ? L.super_tile[2] : 1;
+    // the grid of tiles is rounded up to whole super-tiles; tiles past the
+    // volume exit at the voxel bounds check
+    a.sgx = (L.field.n[0] + a.tx * a.sx - 1) / (a.tx * a.sx);
+    a.sgy = (L.field.n[1] + a.ty * a.sy - 1) / (a.ty * a.sy);
+    const int sgz = (L.field.n[2] + a.tz * a.sz - 1) / (a.tz * a.sz);
+    a.gx = a.sgx * a.sx;
+    a.gy = a.sgy * a.sy;
+    a.gz = sgz * a.sz;
     a.nblocks = a.gx * a.gy * a.gz;
     a.per_xcd = (a.nblocks + 7) / 8;
     dim3 grid((unsigned)(a.per_xcd * 8));
     dim3 block(kThreads);
     const bool g = L.gate != 0;
+    if (L.cache && L.nc == 1 && !g) {
+        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                          L.noise.n[2] == L.field.n[2];
+        if (L.field_fp16) {
+            if (same)
+                hipLaunchKernelGGL((k_lic_cached<half4_t, true>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_lic_cached<half4_t, false>), grid, block, 0, s, a);
+        } else {
+            if (same)
+                hipLaunchKernelGGL((k_lic_cached<float4, true>), grid, block, 0, s, a);
+            else
+                hipLaunchKernelGGL((k_lic_cached<float4, false>), grid, block, 0, s, a);
+        }
+        return hipGetLastError();
+    }
     if (L.field_fp16) {
         if (L.nc == 4)
             hipLaunchKernelGGL((k_lic<half4_t, 4, false>), grid, block, 0, s, a);
