@@ -214,7 +214,9 @@ vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dima
  * product is 256 threads per block), "lic.super_x", "lic.super_y",
  * "lic.super_z" (tiles per super-tile: the 3D-blocked order in which tiles
  * are handed to the GPU), "lic.cache" (0/1: per-lane trilinear cell
- * caches).  Results do not depend on these settings. */
+ * caches), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
+ * an LDS window per ray segment before the global integer atomics).
+ * Results do not depend on these settings. */
 vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
 vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);
 

@@ -70,6 +70,15 @@ class VVError(RuntimeError):
 
 
 def _load() -> C.CDLL:
+    # torch ships its own HIP runtime under the same SONAME as the system one
+    # (libamdhip64.so.7).  Loading torch first makes libvvlic bind to that
+    # runtime, so a process that also uses torch has exactly one HIP runtime;
+    # loading libvvlic first would leave torch pairing the system HIP runtime
+    # with its bundled HSA runtime, and torch's device initialisation fails.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not LIB_PATH.exists():
         raise ImportError(
             f"libvvlic.so not found at {LIB_PATH}; build it with "

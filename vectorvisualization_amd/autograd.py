@@ -1,0 +1,56 @@
+"""torch.autograd.Function over the ray caster and its hand-written adjoint.
+
+``render_tf(engine, cam, tf_rgba, sigma=None)`` returns the RGBA image
+(height, width, 4) rendered by the engine's fused ray caster from its current
+LIC volume and field.  Gradients flow to the 256x4 RGBA transfer function and
+to the per-voxel opacity scale; both are computed by ``vv_render_adjoint``
+(checkpointed, deterministic) on the GPU.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._lib import Camera, Context
+
+
+class _RenderTF(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, tf_rgba, sigma, engine: Context, cam: Camera, opacity: np.ndarray):
+        if not tf_rgba.is_cuda:
+            raise ValueError("tf_rgba must be a CUDA tensor")
+        rgba = tf_rgba.detach().to(torch.float32).contiguous().cpu().numpy()
+        engine.set_tf(rgba.ravel(), opacity)
+        if sigma is not None:
+            engine.set_opacity_scale(sigma.detach().to(torch.float32).contiguous())
+        else:
+            engine.set_opacity_scale(None)
+        img = torch.empty(cam.height * cam.width * 4, dtype=torch.float32, device=tf_rgba.device)
+        engine.render(cam, out=img)
+        engine.synchronize()
+        fctx.engine = engine
+        fctx.cam = cam
+        fctx.has_sigma = sigma is not None
+        fctx.sigma_shape = None if sigma is None else sigma.shape
+        return img.view(cam.height, cam.width, 4)
+
+    @staticmethod
+    def backward(fctx, grad_img):
+        engine, cam = fctx.engine, fctx.cam
+        g = grad_img.detach().to(torch.float32).contiguous().view(-1)
+        dtf = torch.empty(1024, dtype=torch.float32, device=g.device)
+        dsig = None
+        if fctx.has_sigma and fctx.needs_input_grad[1]:
+            dsig = torch.empty(int(np.prod(fctx.sigma_shape)), dtype=torch.float32, device=g.device)
+        torch.cuda.current_stream().synchronize()
+        engine.render_adjoint(cam, g, dtf, dsig)
+        engine.synchronize()
+        dsig_out = None if dsig is None else dsig.view(fctx.sigma_shape)
+        return dtf.view(256, 4), dsig_out, None, None, None
+
+
+def render_tf(engine: Context, cam: Camera, tf_rgba: torch.Tensor, opacity: np.ndarray,
+              sigma: torch.Tensor | None = None) -> torch.Tensor:
+    """Differentiable ray cast: image (H, W, 4) as a function of the
+    (256, 4) RGBA transfer function and the optional opacity scale."""
+    return _RenderTF.apply(tf_rgba, sigma, engine, cam, opacity)

@@ -98,6 +98,7 @@ struct vv_ctx {
     int lic_tile[3] = {16, 4, 4};
     int lic_super[3] = {2, 8, 8};
     int lic_cache = 1;
+    int adj_window = 1;
 
     // opacity scale (CLAMP layout, LIC grid)
     bool has_sigma = false;
@@ -903,9 +904,10 @@ vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
     else if (k == "lic.super_x") slot = &c->lic_super[0];
     else if (k == "lic.super_y") slot = &c->lic_super[1];
     else if (k == "lic.super_z") slot = &c->lic_super[2];
+    else if (k == "adjoint.window") slot = &c->adj_window;
     else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
-    if (k == "lic.cache") {
-        if (value != 0 && value != 1) return fail(c, VV_EINVAL, "lic.cache must be 0 or 1");
+    if (k == "lic.cache" || k == "adjoint.window") {
+        if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
     } else if (value < 1 || value > 256 || (value & (value - 1))) {
         return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
     }
@@ -924,6 +926,7 @@ vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
     else if (k == "lic.super_x") *value = c->lic_super[0];
     else if (k == "lic.super_y") *value = c->lic_super[1];
     else if (k == "lic.super_z") *value = c->lic_super[2];
+    else if (k == "adjoint.window") *value = c->adj_window;
     else return VV_EINVAL;
     return VV_OK;
 }
@@ -1043,6 +1046,7 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
     A.dtf_fx = acc;
     A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
     A.fx_scale = 4294967296.0;  // 2^32
+    A.use_window = c->adj_window;
     HIPCHK(c, launch_render_adjoint(A, c->stream));
     const size_t out_bytes = sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0));
     float* dout = nullptr;

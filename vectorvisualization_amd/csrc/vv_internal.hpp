@@ -95,6 +95,7 @@ struct AdjointLaunch {
     long long* dtf_fx;            // 256*4 fixed-point accumulators
     long long* dsigma_fx;         // sigma padded-clamp element count accumulators
     double fx_scale;              // fixed-point scale (2^k)
+    int use_window;               // accumulate dsigma in an LDS window per segment
 };
 
 // ---- launchers (vv_lic.hip / vv_render.hip) ----

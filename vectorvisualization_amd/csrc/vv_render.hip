@@ -291,30 +291,44 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
     if (a1) atomicAdd((unsigned long long*)&acc[i1 * 4 + ch], (unsigned long long)a1);
 }
 
+// dsigma accumulation window in LDS: entries (int64 fixed point), one
+// window per kSub samples of a segment
+constexpr int kWin = 6144;
+constexpr int kSub = 32;
+constexpr int kNSub = kSeg / kSub;
+
 template <typename FT, int LC>
 __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
     __shared__ long long dtf[1024];
+    __shared__ long long win[kWin];
+    __shared__ int box[kNSub][6];
+    __shared__ int seg_max;
     const RenderLaunch& a = A.r;
     load_tables(a.tf, tf, op);
     const int tid = threadIdx.y * kBX + threadIdx.x;
     for (int q = tid; q < 1024; q += kRThreads) dtf[q] = 0;
+    if (tid == 0) seg_max = 0;
     __syncthreads();
 
     const int i = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
     const long long W = a.view.width, H = a.view.height;
     const long long nrays = W * H;
+    const long long pix = (long long)j * W + i;
+    const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
+    float4* ck = reinterpret_cast<float4*>(A.ckpt);                                   // [c][2][ray]
+    float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;  // [k][2][ray]
+
+    // phase 1: forward with a checkpoint (pos, dest) every kSeg samples
     Ray r;
-    if (i < W && j < H && ray_setup(a.view, i, j, r)) {
-        const long long pix = (long long)j * W + i;
-        float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]
-        // phase 1: forward with checkpoints
+    const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
+    long long n = 0;
+    if (active) {
         float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
         float dest[4] = {0.f, 0.f, 0.f, 0.f};
-        long long n = 0;
         for (long long k = 0; k < a.max_samples; ++k) {
             if ((k % kSeg) == 0) {
                 const long long c = k / kSeg;
@@ -328,18 +342,38 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
             for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
             if (outside(pos, a.view) || dest[3] > 0.95f) break;
         }
-        // phase 2: segments in reverse
+        atomicMax(&seg_max, (int)((n + kSeg - 1) / kSeg));
+    }
+    __syncthreads();
+
+    // phase 2: segments last to first, all rays of the block in lockstep
+    float G[4] = {0.f, 0.f, 0.f, 0.f};
+    if (active) {
         const float4 gi = reinterpret_cast<const float4*>(A.dimage)[pix];
-        float G[4] = {gi.x, gi.y, gi.z, gi.w};
-        float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;  // [k][2][ray]
-        const long long nseg = (n + kSeg - 1) / kSeg;
-        for (long long c = nseg - 1; c >= 0; --c) {
+        G[0] = gi.x;
+        G[1] = gi.y;
+        G[2] = gi.z;
+        G[3] = gi.w;
+    }
+    const int nseg = (int)((n + kSeg - 1) / kSeg);
+    const int smax = seg_max;
+    for (int c = smax - 1; c >= 0; --c) {
+        const bool mine = active && c < nseg;
+        const long long k0 = (long long)c * kSeg;
+        const long long k1 = min(n, k0 + kSeg);
+        if (tid < kNSub) {
+            box[tid][0] = box[tid][1] = box[tid][2] = 0x7fffffff;
+            box[tid][3] = box[tid][4] = box[tid][5] = -1;
+        }
+        __syncthreads();
+        // (a) re-run the segment from its checkpoint into the scratch column;
+        //     collect the voxel box of each kSub-sample part
+        if (mine) {
             const float4 cp = ck[(2 * c) * nrays + pix];
             const float4 cd = ck[(2 * c + 1) * nrays + pix];
             float p[3] = {cp.x, cp.y, cp.z};
             float d[4] = {cd.x, cd.y, cd.z, cd.w};
-            const long long k0 = c * kSeg;
-            const long long k1 = min(n, k0 + kSeg);
+            int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
             for (long long k = k0; k < k1; ++k) {
                 Sample s;
                 eval_sample<FT, LC>(a, tf, op, r, p, s);
@@ -348,68 +382,114 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
                 const int mask = composite(d, s);
                 seg[slot * nrays + pix] = make_float4(p[0], p[1], p[2], __int_as_float(mask));
                 seg[(slot + 1) * nrays + pix] = dprev;
+                if (A.dsigma_fx) {
+                    for (int q = 0; q < 3; ++q) {
+                        float f;
+                        const int c0 = tc_clamp(p[q], n3[q], f) - 1;
+                        lo[q] = min(lo[q], max(c0, 0));
+                        hi[q] = max(hi[q], min(c0 + 1, n3[q] - 1));
+                    }
+                    if ((k - k0) % kSub == kSub - 1 || k == k1 - 1) {
+                        const int m = (int)((k - k0) / kSub);
+                        for (int q = 0; q < 3; ++q) {
+                            atomicMin(&box[m][q], lo[q]);
+                            atomicMax(&box[m][3 + q], hi[q]);
+                            lo[q] = 0x7fffffff;
+                            hi[q] = -1;
+                        }
+                    }
+                }
                 for (int q = 0; q < 3; ++q) p[q] = fmaf(r.dir[q], a.step, p[q]);
             }
-            for (long long k = k1 - 1; k >= k0; --k) {
-                const long long slot = (k - k0) * 2;
-                const float4 sp = seg[slot * nrays + pix];
-                const float4 dk = seg[(slot + 1) * nrays + pix];
-                const float pk[3] = {sp.x, sp.y, sp.z};
-                const int mask = __float_as_int(sp.w);
-                Sample s;
-                eval_sample<FT, LC>(a, tf, op, r, pk, s);
-                for (int q = 0; q < 4; ++q)
-                    if (mask & (1 << q)) G[q] = 0.0f;
-                const float t = 1.0f - dk.w;
-                const float al = s.a;
-                const float src[4] = {s.rgb[0] * al, s.rgb[1] * al, s.rgb[2] * al, al};
-                float dsrc[4];
-                for (int q = 0; q < 4; ++q) dsrc[q] = t * G[q];
-                const float gs = G[0] * src[0] + G[1] * src[1] + G[2] * src[2] + G[3] * src[3];
-                G[3] = G[3] - gs;
-                float da = dsrc[3];
-                float drgb[3];
-                for (int q = 0; q < 3; ++q) {
-                    drgb[q] = dsrc[q] * al;
-                    da = da + dsrc[q] * s.rgb[q];
-                }
-                const float b = 1.0f - s.araw;
-                float dadraw;
-                if (a.corr_mode == 0)
-                    dadraw = 1.0f;
-                else if (b > 0.0f)
-                    dadraw = a.alpha_corr * powf(b, a.alpha_corr - 1.0f);
-                else
-                    dadraw = 0.0f;
-                const float draw = s.aclamp ? 0.0f : da * dadraw;
-                for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m, A.fx_scale);
-                splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, A.fx_scale);
-                if (A.dsigma_fx) {
-                    const float dsg = draw * s.ao * s.tfa;
-                    // corner voxels of the CLAMP cell over the sigma (LIC) grid
-                    const int* n3 = a.lic.n;
-                    float fx, fy, fz;
-                    const int px = tc_clamp(pk[0], n3[0], fx) - 1;
-                    const int py = tc_clamp(pk[1], n3[1], fy) - 1;
-                    const int pz = tc_clamp(pk[2], n3[2], fz) - 1;
-                    const int xs[2] = {max(px, 0), min(px + 1, n3[0] - 1)};
-                    const int ys[2] = {max(py, 0), min(py + 1, n3[1] - 1)};
-                    const int zs[2] = {max(pz, 0), min(pz + 1, n3[2] - 1)};
-                    const float wx[2] = {1.0f - fx, fx}, wy[2] = {1.0f - fy, fy}, wz[2] = {1.0f - fz, fz};
-                    for (int cz = 0; cz < 2; ++cz)
-                        for (int cy = 0; cy < 2; ++cy)
-                            for (int cx = 0; cx < 2; ++cx) {
-                                const long long v = to_fx(wx[cx] * wy[cy] * wz[cz] * dsg, A.fx_scale);
-                                if (v) {
-                                    const long long o = ((long long)zs[cz] * n3[1] + ys[cy]) * n3[0] + xs[cx];
-                                    atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+        }
+        __syncthreads();
+        for (int m = kNSub - 1; m >= 0; --m) {
+            const int bx0 = box[m][0], by0 = box[m][1], bz0 = box[m][2];
+            const int wx = box[m][3] - bx0 + 1, wy = box[m][4] - by0 + 1, wz = box[m][5] - bz0 + 1;
+            const long long wvol = (box[m][3] >= 0) ? (long long)wx * wy * wz : 0;
+            const bool use_win = A.use_window && A.dsigma_fx && wvol > 0 && wvol <= kWin;
+            if (use_win)
+                for (int q = tid; q < (int)wvol; q += kRThreads) win[q] = 0;
+            __syncthreads();
+            // (b) walk this part of the segment backwards
+            const long long ka = k0 + (long long)m * kSub;
+            const long long kb = min(k1, ka + kSub);
+            if (mine) {
+                for (long long k = kb - 1; k >= ka; --k) {
+                    const long long slot = (k - k0) * 2;
+                    const float4 sp = seg[slot * nrays + pix];
+                    const float4 dk = seg[(slot + 1) * nrays + pix];
+                    const float pk[3] = {sp.x, sp.y, sp.z};
+                    const int mask = __float_as_int(sp.w);
+                    Sample s;
+                    eval_sample<FT, LC>(a, tf, op, r, pk, s);
+                    for (int q = 0; q < 4; ++q)
+                        if (mask & (1 << q)) G[q] = 0.0f;
+                    const float t = 1.0f - dk.w;
+                    const float al = s.a;
+                    const float src[4] = {s.rgb[0] * al, s.rgb[1] * al, s.rgb[2] * al, al};
+                    float dsrc[4];
+                    for (int q = 0; q < 4; ++q) dsrc[q] = t * G[q];
+                    const float gs = G[0] * src[0] + G[1] * src[1] + G[2] * src[2] + G[3] * src[3];
+                    G[3] = G[3] - gs;
+                    float da = dsrc[3];
+                    float drgb[3];
+                    for (int q = 0; q < 3; ++q) {
+                        drgb[q] = dsrc[q] * al;
+                        da = da + dsrc[q] * s.rgb[q];
+                    }
+                    const float b = 1.0f - s.araw;
+                    float dadraw;
+                    if (a.corr_mode == 0)
+                        dadraw = 1.0f;
+                    else if (b > 0.0f)
+                        dadraw = a.alpha_corr * powf(b, a.alpha_corr - 1.0f);
+                    else
+                        dadraw = 0.0f;
+                    const float draw = s.aclamp ? 0.0f : da * dadraw;
+                    for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m, A.fx_scale);
+                    splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, A.fx_scale);
+                    if (A.dsigma_fx) {
+                        const float dsg = draw * s.ao * s.tfa;
+                        float fx, fy, fz;
+                        const int px = tc_clamp(pk[0], n3[0], fx) - 1;
+                        const int py = tc_clamp(pk[1], n3[1], fy) - 1;
+                        const int pz = tc_clamp(pk[2], n3[2], fz) - 1;
+                        const int xs[2] = {max(px, 0), min(px + 1, n3[0] - 1)};
+                        const int ys[2] = {max(py, 0), min(py + 1, n3[1] - 1)};
+                        const int zs[2] = {max(pz, 0), min(pz + 1, n3[2] - 1)};
+                        const float wxs[2] = {1.0f - fx, fx}, wys[2] = {1.0f - fy, fy}, wzs[2] = {1.0f - fz, fz};
+                        for (int cz = 0; cz < 2; ++cz)
+                            for (int cy = 0; cy < 2; ++cy)
+                                for (int cx = 0; cx < 2; ++cx) {
+                                    const long long v = to_fx(wxs[cx] * wys[cy] * wzs[cz] * dsg, A.fx_scale);
+                                    if (!v) continue;
+                                    if (use_win) {
+                                        const int o = ((zs[cz] - bz0) * wy + (ys[cy] - by0)) * wx + (xs[cx] - bx0);
+                                        atomicAdd((unsigned long long*)&win[o], (unsigned long long)v);
+                                    } else {
+                                        const long long o = ((long long)zs[cz] * n3[1] + ys[cy]) * n3[0] + xs[cx];
+                                        atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+                                    }
                                 }
-                            }
+                    }
                 }
             }
+            __syncthreads();
+            // (c) flush the window: one global integer atomic per touched voxel
+            if (use_win) {
+                for (int q = tid; q < (int)wvol; q += kRThreads) {
+                    const long long v = win[q];
+                    if (v) {
+                        const int ox = q % wx, oy = (q / wx) % wy, oz = q / (wx * wy);
+                        const long long o = ((long long)(bz0 + oz) * n3[1] + (by0 + oy)) * n3[0] + (bx0 + ox);
+                        atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+                    }
+                }
+            }
+            __syncthreads();
         }
     }
-    __syncthreads();
     for (int q = tid; q < 1024; q += kRThreads) {
         const long long v = dtf[q];
         if (v) atomicAdd((unsigned long long*)&A.dtf_fx[q], (unsigned long long)v);
