@@ -193,41 +193,48 @@ def main():
     }
 
     if not args.no_extras:
-        # FP16-stored field, same LIC: speedup over the FP32 plain path
-        other = vv.STORE_FP16 if storage == vv.STORE_FP32 else vv.STORE_FP32
-        ctx.upload_field(field, dims, storage=other)
-        ctx.compute_lic()
-        torch.cuda.synchronize()
-        e = [(ev(), ev()) for _ in range(3)]
-        for a, b in e:
-            a.record(stream)
-            ctx.compute_lic()
-            b.record(stream)
-        torch.cuda.synchronize()
-        other_ms = float(np.mean([a.elapsed_time(b) for a, b in e]))
-        key = "fp16_lic_ms" if other == vv.STORE_FP16 else "fp32_lic_ms"
-        out[key] = other_ms
-        fp16_ms = other_ms if other == vv.STORE_FP16 else lic_ms
-        fp32_ms = lic_ms if other == vv.STORE_FP16 else other_ms
-        out["fp16_speedup_vs_fp32_lic"] = fp32_ms / fp16_ms
-        ctx.upload_field(field, dims, storage=storage)
-        ctx.compute_lic()
-        # forward + TF/opacity adjoint of an L2 image loss (1024^2)
+        def timed(fn, reps=3):
+            fn()
+            torch.cuda.synchronize()
+            best = None
+            for _ in range(reps):
+                a, b = ev(), ev()
+                a.record(stream)
+                fn()
+                b.record(stream)
+                torch.cuda.synchronize()
+                t = a.elapsed_time(b)
+                best = t if best is None else min(best, t)
+            return best
+
         dimg = torch.randn(W * W * 4, dtype=torch.float32, device=dev) * 1e-3
         dtf = torch.empty(1024, dtype=torch.float32, device=dev)
         dsig = torch.empty(vox, dtype=torch.float32, device=dev)
-        ctx.render_adjoint(cam, dimg, dtf, dsig)
-        torch.cuda.synchronize()
-        a, b, c = ev(), ev(), ev()
-        a.record(stream)
-        ctx.render(cam, out=image)
-        b.record(stream)
-        ctx.render_adjoint(cam, dimg, dtf, dsig)
-        c.record(stream)
-        torch.cuda.synchronize()
-        out["adjoint_ms"] = b.elapsed_time(c)
-        out["fwd_plus_adjoint_ms"] = a.elapsed_time(c)
-        out["adjoint_over_forward"] = b.elapsed_time(c) / max(a.elapsed_time(b), 1e-9)
+        legs = {}
+        for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
+            ctx.upload_field(field, dims, storage=st)
+            l_ms = timed(ctx.compute_lic)
+            f_ms = timed(lambda: ctx.render(cam, out=image))
+            a_ms = timed(lambda: ctx.render_adjoint(cam, dimg, dtf, dsig))
+            legs[name] = dict(lic_ms=l_ms, frame_ms=f_ms, adjoint_ms=a_ms,
+                              pipeline_ms=l_ms + f_ms + a_ms)
+        # the straightforward FP32 kernel: no per-lane cell caches
+        ctx.upload_field(field, dims, storage=vv.STORE_FP32)
+        ctx.set_tuning("lic.cache", 0)
+        plain_lic = timed(ctx.compute_lic)
+        ctx.set_tuning("lic.cache", 1)
+        plain = plain_lic + legs["fp32"]["frame_ms"] + legs["fp32"]["adjoint_ms"]
+        out["pipeline"] = {
+            "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms)",
+            "fp32": legs["fp32"],
+            "fp16": legs["fp16"],
+            "fp32_plain_lic_ms": plain_lic,
+            "fp16_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["fp16"]["pipeline_ms"],
+            "fp16_speedup_vs_fp32_plain_kernel": plain / legs["fp16"]["pipeline_ms"],
+        }
+        out["fwd_plus_adjoint_ms"] = legs[args.storage]["frame_ms"] + legs[args.storage]["adjoint_ms"]
+        ctx.upload_field(field, dims, storage=storage)
+        ctx.compute_lic()
 
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field.cpu().numpy(), dims, L, lut, inv, p,

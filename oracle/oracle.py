@@ -101,6 +101,8 @@ def _load() -> C.CDLL:
         "orc_lic_weights_f64": (None, [P, I, I, I, P]),
         "orc_lic_f32": (None, [P, I, I, I, P, P, I, P, P, P, P, I, C.POINTER(Params), I, I, I, P]),
         "orc_lic_f64": (None, [P, I, I, I, P, P, I, P, P, P, P, I, C.POINTER(Params), I, I, I, P]),
+        "orc_lic_box_f32": (None, [P, I, I, I, P, P, I, P, P, P, P, I, C.POINTER(Params), P, I, P]),
+        "orc_lic_box_f64": (None, [P, I, I, I, P, P, I, P, P, P, P, I, C.POINTER(Params), P, I, P]),
         "orc_raycast_f32": (C.c_long, [P, P, P, I, P, P, P, P, C.POINTER(Params), C.POINTER(View),
                                        I, P, P]),
         "orc_raycast_f64": (C.c_long, [P, P, P, I, P, P, P, P, C.POINTER(Params), C.POINTER(View),
@@ -119,6 +121,8 @@ LIB = _load()
 
 
 def _p(a: np.ndarray | None):
+    """Address of a C-contiguous array.  The caller must keep `a` referenced
+    until the C call returns (no temporaries)."""
     if a is None:
         return None
     assert a.flags["C_CONTIGUOUS"]
@@ -276,6 +280,25 @@ def lic(field4: np.ndarray, dims, noise: np.ndarray, noise_ch: int, ndims, lut: 
     return out
 
 
+def lic_box(field4: np.ndarray, dims, noise: np.ndarray, noise_ch: int, ndims, lut: np.ndarray,
+            p: Params, box, scale=(1, 1, 1), nthreads=0) -> np.ndarray:
+    """LIC of the voxels in box = (x0, x1, y0, y1, z0, z1); precision follows field4."""
+    nx, ny, nz = dims
+    f64 = field4.dtype == np.float64
+    field4 = np.ascontiguousarray(field4)
+    noise = np.ascontiguousarray(noise, np.uint8)
+    lut = np.ascontiguousarray(lut, np.uint8)
+    b = np.array(box, np.int32)
+    out = np.zeros((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) * noise_ch,
+                   np.float64 if f64 else np.float32)
+    fn = LIB.orc_lic_box_f64 if f64 else LIB.orc_lic_box_f32
+    sc = np.array(scale, np.float32)
+    nd = _i3(ndims)
+    fn(_p(field4), nx, ny, nz, _p(sc), _p(noise), noise_ch,
+       _p(nd), None, None, _p(lut), lut.size, C.byref(p), _p(b), nthreads, _p(out))
+    return out
+
+
 def raycast(field4, fdims, licv, lic_ch, ldims, tf_rgba, tf_opac, p: Params, v: View,
             sigma=None, nthreads=0, f64=None):
     if f64 is None:
@@ -289,7 +312,8 @@ def raycast(field4, fdims, licv, lic_ch, ldims, tf_rgba, tf_opac, p: Params, v: 
     img = np.zeros(v.width * v.height * 4, dt)
     ns = np.zeros(v.width * v.height, np.int32)
     fn = LIB.orc_raycast_f64 if f64 else LIB.orc_raycast_f32
-    total = fn(_p(field4), _p(_i3(fdims)), _p(licv), lic_ch, _p(_i3(ldims)), _p(tf_rgba),
+    fd, ld = _i3(fdims), _i3(ldims)  # keep the arrays alive across the call
+    total = fn(_p(field4), _p(fd), _p(licv), lic_ch, _p(ld), _p(tf_rgba),
                _p(tf_opac), _p(sigma), C.byref(p), C.byref(v), nthreads, _p(img), _p(ns))
     return img, ns, int(total)
 
@@ -302,7 +326,8 @@ def raycast_adjoint(field4, fdims, licv, lic_ch, ldims, tf_rgba, tf_opac, p: Par
     dtf = np.zeros(1024, np.float64)
     nv = int(np.prod(ldims))
     dsig = np.zeros(nv, np.float64) if want_sigma else None
-    LIB.orc_raycast_adjoint_f64(_p(field4), _p(_i3(fdims)), _p(licv), lic_ch, _p(_i3(ldims)),
+    fd, ld = _i3(fdims), _i3(ldims)  # keep the arrays alive across the call
+    LIB.orc_raycast_adjoint_f64(_p(field4), _p(fd), _p(licv), lic_ch, _p(ld),
                                 _p(tf_rgba), _p(tf_opac), _p(sigma), C.byref(p), C.byref(v),
                                 _p(dimage), _p(dtf), _p(dsig))
     return dtf, dsig

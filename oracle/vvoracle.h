@@ -122,6 +122,17 @@ void orc_lic_f64(const double* field4, int nx, int ny, int nz, const float scale
                  const uint8_t* lut, int lut_width, const orc_params* p,
                  int z0, int z1, int nthreads, double* out);
 
+void orc_lic_box_f32(const float* field4, int nx, int ny, int nz, const float scale[3],
+                     const uint8_t* noise, int noise_ch, const int ndims[3],
+                     const uint8_t* scalar, const int sdims[3],
+                     const uint8_t* lut, int lut_width, const orc_params* p,
+                     const int box[6], int nthreads, float* out);
+void orc_lic_box_f64(const double* field4, int nx, int ny, int nz, const float scale[3],
+                     const uint8_t* noise, int noise_ch, const int ndims[3],
+                     const uint8_t* scalar, const int sdims[3],
+                     const uint8_t* lut, int lut_width, const orc_params* p,
+                     const int box[6], int nthreads, double* out);
+
 long orc_raycast_f32(const float* field4, const int fdims[3],
                      const float* lic, int lic_ch, const int ldims[3],
                      const float* tf_rgba, const float* tf_opac,

@@ -260,11 +260,13 @@ static void FN(lic_voxel)(const FN(lic_ctx)* c, const R* w, const R seed[3], R o
     }
 }
 
-void FN(orc_lic)(const R* field4, int nx, int ny, int nz, const float scale[3],
-                 const uint8_t* noise, int noise_ch, const int ndims[3],
-                 const uint8_t* scalar, const int sdims[3],
-                 const uint8_t* lut, int lut_width, const orc_params* p,
-                 int z0, int z1, int nthreads, R* out)
+/* LIC of the voxels in [x0,x1) x [y0,y1) x [z0,z1); out holds the box,
+ * x fastest */
+void FN(orc_lic_box)(const R* field4, int nx, int ny, int nz, const float scale[3],
+                     const uint8_t* noise, int noise_ch, const int ndims[3],
+                     const uint8_t* scalar, const int sdims[3],
+                     const uint8_t* lut, int lut_width, const orc_params* p,
+                     const int box[6], int nthreads, R* out)
 {
     FN(lic_ctx) c;
     c.field = field4; c.fd[0] = nx; c.fd[1] = ny; c.fd[2] = nz;
@@ -276,24 +278,37 @@ void FN(orc_lic)(const R* field4, int nx, int ny, int nz, const float scale[3],
     int nw = 1 + p->steps_forward + p->steps_backward;
     R* w = (R*)malloc(sizeof(R) * nw);
     FN(orc_lic_weights)(lut, lut_width, p->steps_forward, p->steps_backward, w);
-    if (z0 < 0) z0 = 0;
-    if (z1 > nz) z1 = nz;
+    int x0 = box[0] < 0 ? 0 : box[0], x1 = box[1] > nx ? nx : box[1];
+    int y0 = box[2] < 0 ? 0 : box[2], y1 = box[3] > ny ? ny : box[3];
+    int z0 = box[4] < 0 ? 0 : box[4], z1 = box[5] > nz ? nz : box[5];
+    const int bw = x1 - x0, bh = y1 - y0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 1) collapse(2) num_threads(nthreads)
 #endif
     for (int z = z0; z < z1; ++z)
-        for (int y = 0; y < ny; ++y)
-            for (int x = 0; x < nx; ++x) {
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x) {
                 R seed[3] = {((R)x + K(0.5)) / (R)nx * (R)scale[0],
                              ((R)y + K(0.5)) / (R)ny * (R)scale[1],
                              ((R)z + K(0.5)) / (R)nz * (R)scale[2]};
                 R o[4];
                 FN(lic_voxel)(&c, w, seed, o);
-                long idx = (((long)(z - z0)) * ny + y) * nx + x;
+                long idx = (((long)(z - z0)) * bh + (y - y0)) * bw + (x - x0);
                 for (int ch = 0; ch < noise_ch; ++ch) out[idx * noise_ch + ch] = o[ch];
             }
     free(w);
+}
+
+void FN(orc_lic)(const R* field4, int nx, int ny, int nz, const float scale[3],
+                 const uint8_t* noise, int noise_ch, const int ndims[3],
+                 const uint8_t* scalar, const int sdims[3],
+                 const uint8_t* lut, int lut_width, const orc_params* p,
+                 int z0, int z1, int nthreads, R* out)
+{
+    const int box[6] = {0, nx, 0, ny, z0, z1};
+    FN(orc_lic_box)(field4, nx, ny, nz, scale, noise, noise_ch, ndims, scalar, sdims, lut,
+                    lut_width, p, box, nthreads, out);
 }
 
 /* ------------------------------------------------------------------ */
