@@ -209,13 +209,45 @@ vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
+/* ---- batching: many timesteps and views in one captured launch --------
+ * A batch belongs to a context and uses its LIC parameters, filter and
+ * transfer function at run time.  Timesteps are resident volumes (field +
+ * seeded scalar noise + LIC volume) of any size; views render one timestep
+ * each into a caller-owned device image (width*height*4 float32).
+ * vv_batch_run computes every LIC volume and every view with two persistent
+ * kernels captured once into a hipGraph and replayed on later runs; it
+ * returns without waiting.  Replaces the reference's per-frame re-encode +
+ * LIC recompute of unsteady data (VV/3DLIC.cpp:129-172). */
+typedef struct vv_batch vv_batch;
+vv_status vv_batch_create(vv_ctx* ctx, vv_batch** out);
+vv_status vv_batch_destroy(vv_batch* b);
+vv_status vv_batch_add_timestep(vv_batch* b, const float* v, const int dims[3],
+                                const float spacing[3], int storage, uint64_t noise_seed,
+                                int noise_kind, double density, int on_device);
+vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, float* image);
+vv_status vv_batch_info(const vv_batch* b, int* ntimesteps, int* nviews);
+/* times LIC tile shapes x brick (super-tile) edges per timestep size and the
+ * render tile on the first view; picks the fastest (cached per size) */
+vv_status vv_batch_autotune(vv_batch* b);
+/* tile x,y,z, brick tiles x,y,z of a timestep, then render tile x,y */
+vv_status vv_batch_get_tuning(const vv_batch* b, int timestep, int out8[8]);
+vv_status vv_batch_run(vv_batch* b);
+/* per view: milliseconds from the start of the last run to the end of that
+ * view's last pixel tile (device wall clock); waits for the run */
+vv_status vv_batch_view_latency(vv_batch* b, double* ms);
+/* total compositing samples of the last run (waits) */
+vv_status vv_batch_samples(vv_batch* b, uint64_t* total);
+vv_status vv_batch_download_lic(vv_batch* b, int timestep, float* out, int on_device);
+
 /* ---- launch tuning (used by the autotuner) ----------------------------
  * keys: "lic.tile_x", "lic.tile_y", "lic.tile_z" (powers of two whose
  * product is 256 threads per block), "lic.super_x", "lic.super_y",
  * "lic.super_z" (tiles per super-tile: the 3D-blocked order in which tiles
  * are handed to the GPU), "lic.cache" (0/1: per-lane trilinear cell
  * caches), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
- * an LDS window per ray segment before the global integer atomics).
+ * an LDS window per ray segment before the global integer atomics),
+ * "render.block_x", "render.block_y" (pixel tile of the ray caster, 64 to
+ * 256 threads in multiples of 64).
  * Results do not depend on these settings. */
 vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
 vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);

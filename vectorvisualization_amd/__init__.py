@@ -8,6 +8,7 @@ adjoint (``autograd``, imported lazily because it needs torch).
 """
 from ._lib import (  # noqa: F401
     LIB,
+    Batch,
     LIB_PATH,
     Camera,
     Context,

@@ -121,6 +121,17 @@ def _load() -> C.CDLL:
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
+        "vv_batch_create": (I, [P, C.POINTER(P)]),
+        "vv_batch_destroy": (I, [P]),
+        "vv_batch_add_timestep": (I, [P, P, IP, FP, I, C.c_uint64, I, D, I]),
+        "vv_batch_add_view": (I, [P, I, C.POINTER(Camera), P]),
+        "vv_batch_info": (I, [P, IP, IP]),
+        "vv_batch_autotune": (I, [P]),
+        "vv_batch_get_tuning": (I, [P, I, IP]),
+        "vv_batch_run": (I, [P]),
+        "vv_batch_view_latency": (I, [P, C.POINTER(C.c_double)]),
+        "vv_batch_samples": (I, [P, C.POINTER(C.c_uint64)]),
+        "vv_batch_download_lic": (I, [P, I, P, I]),
         "vv_set_tuning": (I, [P, C.c_char_p, I]),
         "vv_get_tuning": (I, [P, C.c_char_p, IP]),
     }
@@ -142,7 +153,10 @@ EXPORTED = [
     "vv_default_tf_u8", "vv_upload_scalar", "vv_set_lic_params", "vv_get_lic_params",
     "vv_compute_lic", "vv_lic_info", "vv_download_lic", "vv_upload_lic", "vv_render",
     "vv_set_opacity_scale", "vv_render_adjoint", "vv_view_block", "vv_lic_weights",
-    "vv_set_tuning", "vv_get_tuning",
+    "vv_set_tuning", "vv_get_tuning", "vv_batch_create", "vv_batch_destroy",
+    "vv_batch_add_timestep", "vv_batch_add_view", "vv_batch_info", "vv_batch_autotune",
+    "vv_batch_get_tuning", "vv_batch_run", "vv_batch_view_latency", "vv_batch_samples",
+    "vv_batch_download_lic",
 ]
 
 
@@ -380,3 +394,73 @@ class Context:
         ps = _ptr(dsigma)[0] if dsigma is not None else None
         self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
         return dtf, dsigma
+
+
+class Batch:
+    """Many timesteps and views in one captured launch (vv_batch)."""
+
+    def __init__(self, ctx: Context):
+        h = C.c_void_p()
+        ctx._chk(LIB.vv_batch_create(ctx.handle, C.byref(h)))
+        self._ctx = ctx
+        self._h = h
+        self._images = []
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            LIB.vv_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_timestep(self, v, dims, spacing=(1.0, 1.0, 1.0), storage: int = STORE_FP32,
+                     noise_seed: int = 1, noise_kind: int = NOISE_UNIFORM, density: float = 0.0) -> int:
+        p, dev = _ptr(v)
+        self._ctx._chk(LIB.vv_batch_add_timestep(self._h, p, _i3(dims), _f3(spacing), int(storage),
+                                                 C.c_uint64(noise_seed), int(noise_kind),
+                                                 float(density), dev))
+        return self.info()[0] - 1
+
+    def add_view(self, timestep: int, cam: Camera, image) -> int:
+        p, dev = _ptr(image)
+        if not dev:
+            raise ValueError("batch images must be device tensors")
+        self._ctx._chk(LIB.vv_batch_add_view(self._h, int(timestep), C.byref(cam), p))
+        self._images.append(image)
+        return self.info()[1] - 1
+
+    def info(self):
+        t, v = C.c_int(), C.c_int()
+        LIB.vv_batch_info(self._h, C.byref(t), C.byref(v))
+        return t.value, v.value
+
+    def autotune(self) -> None:
+        self._ctx._chk(LIB.vv_batch_autotune(self._h))
+
+    def tuning(self, timestep: int):
+        out = (C.c_int * 8)()
+        self._ctx._chk(LIB.vv_batch_get_tuning(self._h, int(timestep), out))
+        return list(out)
+
+    def run(self) -> None:
+        self._ctx._chk(LIB.vv_batch_run(self._h))
+
+    def view_latency_ms(self) -> np.ndarray:
+        n = self.info()[1]
+        out = (C.c_double * n)()
+        self._ctx._chk(LIB.vv_batch_view_latency(self._h, out))
+        return np.array(out)
+
+    def samples(self) -> int:
+        t = C.c_uint64()
+        self._ctx._chk(LIB.vv_batch_samples(self._h, C.byref(t)))
+        return t.value
+
+    def download_lic(self, timestep: int, dims) -> np.ndarray:
+        out = np.empty(int(np.prod(dims)), np.float32)
+        self._ctx._chk(LIB.vv_batch_download_lic(self._h, int(timestep), out.ctypes.data, 0))
+        return out

@@ -1,0 +1,563 @@
+// vv_batch.cpp — batched LIC + ray casting: several (unsteady) timesteps of
+// different sizes and many views rendered from them, run as one captured
+// hipGraph of persistent kernels:
+//
+//   k_stamp         start time of the batch (wall clock)
+//   k_fill_u32      zero the render work counter, per-view sample counters
+//                   and completion stamps (a kernel, not a memset, so the
+//                   graph can be replayed)
+//   k_lic_batch     every LIC tile of every timestep
+//   k_render_batch  every pixel tile of every view, dynamically scheduled
+//
+// The reference animates an unsteady field by re-encoding, re-uploading and
+// recomputing the LIC volume frame by frame (VV/3DLIC.cpp:129-172,
+// VV/dataset.cpp:202-210, 290-366); a batch holds all timesteps resident and
+// produces all views in one launch.  The autotuner times the LIC tile /
+// brick order per volume size and the render tile on the batch's own data.
+#include "vv_ctx.hpp"
+
+#include <algorithm>
+#include <array>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+using namespace vv_host;
+
+namespace {
+
+struct BatchStep {
+    int dims[3];
+    float extent[3], scale[3], center[3];
+    int storage;
+    DevBuf field, noise, lic;
+    int tile[3] = {16, 4, 4};
+    int super[3] = {2, 8, 8};
+};
+
+struct BatchView {
+    int step;
+    vv_camera cam;
+    float* image;
+};
+
+// (dims, storage, F, B) -> tile x,y,z + brick (super-tile) x,y,z
+using TuneKey = std::tuple<int, int, int, int, int, int>;
+std::map<TuneKey, std::array<int, 6>> g_lic_tune;
+std::mutex g_tune_mu;
+
+}  // namespace
+
+struct vv_batch {
+    vv_ctx* ctx = nullptr;
+    std::vector<BatchStep*> steps;
+    std::vector<BatchView> views;
+    DevBuf jobs, first, rviews, vfirst, ctr;
+    std::vector<unsigned char> host_jobs;
+    std::vector<int> host_first, host_vfirst;
+    std::vector<RenderLaunch> host_views;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<long long> graph_key;
+    int render_block[2] = {16, 16};
+    bool ran = false;
+};
+
+namespace {
+
+void release_graph(vv_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_key.clear();
+}
+
+// counter block layout (u32 units): [0] render work counter, [2..] per-view
+// u64 sample counters, then per-view u64 completion stamps, then u64 t0
+size_t ctr_words(size_t nviews) { return 2 + 2 * nviews + 2 * nviews + 2; }
+
+vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
+{
+    const vv_lic_params& p = c->params;
+    std::memset(&L, 0, sizeof(L));
+    L.field = make_clamp_desc(st->field.p, st->dims);
+    L.field_fp16 = st->storage == VV_STORE_FP16;
+    L.noise = make_repeat_desc(st->noise.p, st->dims);
+    L.nc = 1;
+    L.w = (const float*)c->weights.p;
+    L.F = p.steps_forward;
+    L.B = p.steps_backward;
+    L.he = p.step_size_lic * p.eye_factor;
+    L.scl = (c->inv_area / (float)(p.steps_forward + p.steps_backward)) * p.gradient_scale;
+    L.freq = p.freq_scale;
+    for (int a = 0; a < 3; ++a) L.scale[a] = st->scale[a];
+    L.out = make_repeat_desc(st->lic.p, st->dims);
+    for (int a = 0; a < 3; ++a) {
+        L.tile[a] = st->tile[a];
+        L.super_tile[a] = st->super[a];
+    }
+    L.field_bytes = (unsigned long long)clamp_elems(st->dims) * (L.field_fp16 ? 8ull : 16ull);
+    L.noise_bytes = (unsigned long long)repeat_elems(st->dims);
+    L.cache = 1;
+    if (!(L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) && (long long)L.field.sz < (1ll << 24) &&
+          (long long)L.noise.sz < (1ll << 24)))
+        return fail(c, VV_EUNSUPPORTED, "batch timestep %dx%dx%d too large for 32-bit buffer offsets",
+                    st->dims[0], st->dims[1], st->dims[2]);
+    return VV_OK;
+}
+
+vv_status build_view_launch(vv_batch* b, const BatchView& v, RenderLaunch& R)
+{
+    vv_ctx* c = b->ctx;
+    BatchStep* st = b->steps[v.step];
+    vv_status s = check_camera(c, &v.cam);
+    if (s) return s;
+    const vv_lic_params& p = c->params;
+    std::memset(&R, 0, sizeof(R));
+    R.field = make_clamp_desc(st->field.p, st->dims);
+    R.field_fp16 = st->storage == VV_STORE_FP16;
+    R.tf_ch = p.tf_index == VV_TF_INDEX_MAG ? 3 : 2;
+    R.lic = make_repeat_desc(st->lic.p, st->dims);
+    R.lc = 1;
+    R.tf = (const float*)c->tf_dev.p;
+    R.view = view_block_geom(st->extent, st->scale, st->center, &v.cam);
+    R.step = p.step_size_vol;
+    R.alpha_corr = p.step_size_vol * 128.0f;
+    R.corr_mode = R.alpha_corr == 1.0f ? 0 : (R.alpha_corr == 0.5f ? 1 : (R.alpha_corr == 2.0f ? 2 : 3));
+    R.max_samples = (long long)p.num_iterations * (long long)p.num_iterations;
+    R.illum = VV_ILLUM_LIC;
+    R.illum_scale = p.illum_scale;
+    const float l0 = v.cam.light_pos[0], l1 = v.cam.light_pos[1], l2 = v.cam.light_pos[2];
+    const float ld = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+    if (ld > 0.0f) {
+        R.light[0] = l0 / ld;
+        R.light[1] = l1 / ld;
+        R.light[2] = l2 / ld;
+    }
+    R.light_diffuse = v.cam.light_diffuse;
+    R.light_specular = v.cam.light_specular;
+    R.light_ambient = v.cam.light_ambient;
+    R.spec_exp = v.cam.spec_exponent;
+    R.image = v.image;
+    R.block[0] = b->render_block[0];
+    R.block[1] = b->render_block[1];
+    return VV_OK;
+}
+
+float time_ms(hipStream_t s, const std::function<hipError_t()>& fn, int reps, hipError_t* err)
+{
+    hipEvent_t a, e;
+    *err = hipEventCreate(&a);
+    if (*err) return 0;
+    *err = hipEventCreate(&e);
+    if (*err) {
+        (void)hipEventDestroy(a);
+        return 0;
+    }
+    float best = 1e30f;
+    for (int r = 0; r < reps && *err == hipSuccess; ++r) {
+        (void)hipEventRecord(a, s);
+        *err = fn();
+        (void)hipEventRecord(e, s);
+        if (*err == hipSuccess) *err = hipEventSynchronize(e);
+        float ms = 0;
+        if (*err == hipSuccess) (void)hipEventElapsedTime(&ms, a, e);
+        if (r > 0 || reps == 1) best = std::min(best, ms);  // first rep warms up
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(e);
+    return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+vv_status vv_batch_create(vv_ctx* ctx, vv_batch** out)
+{
+    if (!ctx || !out) return VV_EINVAL;
+    *out = nullptr;
+    vv_batch* b = new (std::nothrow) vv_batch();
+    if (!b) return fail(ctx, VV_ENOMEM, "out of host memory");
+    b->ctx = ctx;
+    *out = b;
+    return VV_OK;
+}
+
+vv_status vv_batch_destroy(vv_batch* b)
+{
+    if (!b) return VV_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    release_graph(b);
+    for (BatchStep* st : b->steps) {
+        st->field.release();
+        st->noise.release();
+        st->lic.release();
+        delete st;
+    }
+    DevBuf* bufs[] = {&b->jobs, &b->first, &b->rviews, &b->vfirst, &b->ctr};
+    for (DevBuf* d : bufs) d->release();
+    delete b;
+    return VV_OK;
+}
+
+vv_status vv_batch_info(const vv_batch* b, int* ntimesteps, int* nviews)
+{
+    if (!b) return VV_EINVAL;
+    if (ntimesteps) *ntimesteps = (int)b->steps.size();
+    if (nviews) *nviews = (int)b->views.size();
+    return VV_OK;
+}
+
+vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3], const float spacing[3],
+                                int storage, uint64_t noise_seed, int noise_kind, double density,
+                                int on_device)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (storage != VV_STORE_FP32 && storage != VV_STORE_FP16) return fail(c, VV_EINVAL, "unknown storage");
+    if (!b->steps.empty() && b->steps[0]->storage != storage)
+        return fail(c, VV_EINVAL, "all timesteps of a batch share one storage type");
+    if (noise_kind != VV_NOISE_BINARY && noise_kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
+    if (noise_kind == VV_NOISE_BINARY && !(density >= 0.0 && density <= 1.0))
+        return fail(c, VV_EINVAL, "density must be in [0,1]");
+    float sd[3] = {1, 1, 1};
+    if (spacing)
+        for (int a = 0; a < 3; ++a) {
+            if (!(spacing[a] > 0.0f) || !std::isfinite(spacing[a])) return fail(c, VV_EINVAL, "spacing must be > 0");
+            sd[a] = spacing[a];
+        }
+    HIPCHK(c, hipSetDevice(c->device));
+    const long long n = nvox(dims);
+    if (!on_device)
+        for (long long i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(v0[i])) return fail(c, VV_EINVAL, "field contains a non-finite value");
+    BatchStep* st = new (std::nothrow) BatchStep();
+    if (!st) return fail(c, VV_ENOMEM, "out of host memory");
+    auto bail = [&](vv_status s) {
+        st->field.release();
+        st->noise.release();
+        st->lic.release();
+        delete st;
+        return s;
+    };
+    const float* d0 = v0;
+    if (!on_device) {
+        vv_status s = upload(c, c->stage, v0, sizeof(float) * 3 * (size_t)n, 0);
+        if (s) return bail(s);
+        d0 = (const float*)c->stage.p;
+    }
+    const int fp16 = storage == VV_STORE_FP16;
+    hipError_t e = st->field.ensure((fp16 ? 8 : 16) * (size_t)clamp_elems(dims));
+    if (!e) e = st->noise.ensure((size_t)repeat_elems(dims));
+    if (!e) e = st->lic.ensure(sizeof(float) * (size_t)repeat_elems(dims));
+    if (!e) e = c->counter.ensure(64);
+    if (!e) e = launch_fill_u32((uint32_t*)c->counter.p, 0u, 1, c->stream);
+    if (!e) e = launch_field_maxlen(d0, d0, 0.0f, n, (uint32_t*)c->counter.p, c->stream);
+    if (!e) e = launch_field_encode(d0, d0, 0.0f, dims, (const uint32_t*)c->counter.p, st->field.p, fp16, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);  // the staging buffer is reused below
+    if (e) return bail(fail(c, e == hipErrorOutOfMemory ? VV_ENOMEM : VV_EDEVICE, "timestep upload: %s",
+                            hipGetErrorString(e)));
+    double t = std::floor((noise_kind == VV_NOISE_BINARY ? density : 0.0) * 4294967296.0);
+    t = std::min(std::max(t, 0.0), 4294967296.0);
+    e = c->stage.ensure((size_t)n);
+    if (!e) e = launch_noise_gen(noise_seed, noise_kind, (uint64_t)t, dims, (uint8_t*)c->stage.p, c->stream);
+    if (!e) e = launch_noise_tex((const uint8_t*)c->stage.p, nullptr, dims, st->noise.p, 1, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) return bail(fail(c, VV_EDEVICE, "timestep noise: %s", hipGetErrorString(e)));
+    // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
+    float vol[3], maxvol = 0.0f;
+    int maxtex = 0;
+    for (int a = 0; a < 3; ++a) {
+        st->dims[a] = dims[a];
+        vol[a] = (float)dims[a] * sd[a];
+        maxvol = std::max(maxvol, vol[a]);
+        maxtex = std::max(maxtex, dims[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        st->scale[a] = (float)maxtex / ((float)dims[a] * sd[a]);
+        st->extent[a] = (float)dims[a] * sd[a] / maxvol;
+        st->center[a] = st->extent[a] / 2.0f;
+    }
+    st->storage = storage;
+    b->steps.push_back(st);
+    release_graph(b);
+    return VV_OK;
+}
+
+vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, float* image)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (timestep < 0 || timestep >= (int)b->steps.size()) return fail(c, VV_EINVAL, "timestep %d out of range", timestep);
+    vv_status s = check_camera(c, cam);
+    if (s) return s;
+    if (!image) return fail(c, VV_EINVAL, "image is NULL (a device buffer of width*height*4 floats)");
+    BatchView v;
+    v.step = timestep;
+    v.cam = *cam;
+    v.image = image;
+    b->views.push_back(v);
+    release_graph(b);
+    return VV_OK;
+}
+
+vv_status vv_batch_run(vv_batch* b)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (b->steps.empty() || b->views.empty()) return fail(c, VV_ESTATE, "batch needs timesteps and views");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // host descriptors are rebuilt below
+    vv_status s = ensure_tf(c);
+    if (!s) s = ensure_weights(c);
+    if (s) return s;
+    const int fp16 = b->steps[0]->storage == VV_STORE_FP16;
+    const int same = c->params.freq_scale == 1.0f;
+    // LIC jobs
+    const size_t asz = lic_args_size();
+    b->host_jobs.resize(asz * b->steps.size());
+    b->host_first.assign(b->steps.size() + 1, 0);
+    for (size_t i = 0; i < b->steps.size(); ++i) {
+        LicLaunch L;
+        s = build_lic_launch(c, b->steps[i], L);
+        if (s) return s;
+        int nt = 0;
+        if (lic_args_fill(L, b->host_jobs.data() + i * asz, &nt) != hipSuccess)
+            return fail(c, VV_EINVAL, "LIC tile of timestep %zu must hold 256 threads", i);
+        b->host_first[i + 1] = b->host_first[i] + nt;
+    }
+    // views
+    const int bx = b->render_block[0], by = b->render_block[1];
+    const size_t nv = b->views.size();
+    b->host_views.resize(nv);
+    b->host_vfirst.assign(nv + 1, 0);
+    HIPCHK(c, b->ctr.ensure(sizeof(uint32_t) * ctr_words(nv)));
+    uint32_t* ctr = (uint32_t*)b->ctr.p;
+    unsigned long long* samples = (unsigned long long*)(ctr + 2);
+    unsigned long long* done = samples + nv;
+    unsigned long long* t0 = done + nv;
+    for (size_t v = 0; v < nv; ++v) {
+        s = build_view_launch(b, b->views[v], b->host_views[v]);
+        if (s) return s;
+        b->host_views[v].counter = samples + v;
+        const int tiles = ((b->views[v].cam.width + bx - 1) / bx) * ((b->views[v].cam.height + by - 1) / by);
+        b->host_vfirst[v + 1] = b->host_vfirst[v] + tiles;
+    }
+    HIPCHK(c, b->jobs.ensure(b->host_jobs.size()));
+    HIPCHK(c, b->first.ensure(sizeof(int) * b->host_first.size()));
+    HIPCHK(c, b->rviews.ensure(sizeof(RenderLaunch) * nv));
+    HIPCHK(c, b->vfirst.ensure(sizeof(int) * b->host_vfirst.size()));
+    int cus = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    const int lic_grid = ((cus * lic_batch_blocks_per_cu(fp16, same) + 7) / 8) * 8;
+    const int render_grid = cus * render_batch_blocks_per_cu(fp16, 1, bx * by);
+    const std::vector<long long> key = {(long long)b->steps.size(), (long long)nv, lic_grid, render_grid,
+                                        b->host_first.back(), b->host_vfirst.back(), bx, by, fp16, same,
+                                        (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
+                                        (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
+                                        (long long)(intptr_t)b->ctr.p};
+    // descriptors for this run (the graph's kernels read them from device memory)
+    HIPCHK(c, hipMemcpyAsync(b->jobs.p, b->host_jobs.data(), b->host_jobs.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->first.p, b->host_first.data(), sizeof(int) * b->host_first.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->rviews.p, b->host_views.data(), sizeof(RenderLaunch) * nv, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(b->vfirst.p, b->host_vfirst.data(), sizeof(int) * b->host_vfirst.size(),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!b->exec || key != b->graph_key) {
+        release_graph(b);
+        LicBatchLaunch LB;
+        LB.jobs = b->jobs.p;
+        LB.first = (const int*)b->first.p;
+        LB.njobs = (int)b->steps.size();
+        LB.total = b->host_first.back();
+        LB.grid = lic_grid;
+        LB.fp16 = fp16;
+        LB.same = same;
+        RenderBatchLaunch RB;
+        RB.views = (const RenderLaunch*)b->rviews.p;
+        RB.first = (const int*)b->vfirst.p;
+        RB.nviews = (int)nv;
+        RB.total = b->host_vfirst.back();
+        RB.counter = ctr;
+        RB.done = done;
+        RB.block[0] = bx;
+        RB.block[1] = by;
+        RB.grid = render_grid;
+        RB.fp16 = fp16;
+        RB.lc = 1;
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = launch_stamp(t0, c->stream);
+        if (!e) e = launch_fill_u32(ctr, 0u, (long long)(2 + 4 * nv), c->stream);
+        if (!e) e = launch_lic_batch(LB, c->stream);
+        if (!e) e = launch_render_batch(RB, c->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e || e2) {
+            if (g) (void)hipGraphDestroy(g);
+            return fail(c, VV_EDEVICE, "graph capture: %s", hipGetErrorString(e ? e : e2));
+        }
+        b->graph = g;
+        HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_key = key;
+    }
+    HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+    b->ran = true;
+    return VV_OK;
+}
+
+vv_status vv_batch_view_latency(vv_batch* b, double* ms)
+{
+    if (!b || !ms) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (!b->ran) return fail(c, VV_ESTATE, "batch has not run");
+    const size_t nv = b->views.size();
+    std::vector<unsigned long long> h(2 * nv + 1);
+    const uint32_t* ctr = (const uint32_t*)b->ctr.p;
+    HIPCHK(c, hipMemcpyAsync(h.data(), ctr + 2 + 2 * nv, sizeof(unsigned long long) * (nv + 1),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int khz = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    if (khz <= 0) return fail(c, VV_EUNSUPPORTED, "no wall-clock rate");
+    const unsigned long long t0 = h[nv];
+    for (size_t v = 0; v < nv; ++v) ms[v] = (double)(h[v] - t0) / (double)khz;
+    return VV_OK;
+}
+
+vv_status vv_batch_samples(vv_batch* b, uint64_t* total)
+{
+    if (!b || !total) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (!b->ran) return fail(c, VV_ESTATE, "batch has not run");
+    const size_t nv = b->views.size();
+    std::vector<unsigned long long> h(nv);
+    HIPCHK(c, hipMemcpyAsync(h.data(), (const uint32_t*)b->ctr.p + 2, sizeof(unsigned long long) * nv,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint64_t t = 0;
+    for (unsigned long long x : h) t += x;
+    *total = t;
+    return VV_OK;
+}
+
+vv_status vv_batch_download_lic(vv_batch* b, int timestep, float* out, int on_device)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (timestep < 0 || timestep >= (int)b->steps.size()) return fail(c, VV_EINVAL, "timestep out of range");
+    if (!out) return fail(c, VV_EINVAL, "out is NULL");
+    BatchStep* st = b->steps[timestep];
+    const size_t bytes = sizeof(float) * (size_t)nvox(st->dims);
+    float* dst = out;
+    if (!on_device) {
+        HIPCHK(c, c->stage.ensure(bytes));
+        dst = (float*)c->stage.p;
+    }
+    HIPCHK(c, launch_unpad_repeat_f32((const float*)st->lic.p, 1, st->dims, dst, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return VV_OK;
+}
+
+vv_status vv_batch_autotune(vv_batch* b)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (b->steps.empty()) return fail(c, VV_ESTATE, "batch has no timesteps");
+    HIPCHK(c, hipSetDevice(c->device));
+    vv_status s = ensure_tf(c);
+    if (!s) s = ensure_weights(c);
+    if (s) return s;
+    // LIC: tile shape (256 threads) x brick edge in voxels (super-tile)
+    static const int tiles[][3] = {{16, 4, 4}, {16, 8, 2}, {16, 16, 1}, {32, 4, 2}, {8, 8, 4}};
+    static const int bricks[] = {16, 32, 64};
+    for (BatchStep* st : b->steps) {
+        const TuneKey key{st->dims[0], st->dims[1], st->dims[2], st->storage, c->params.steps_forward,
+                          c->params.steps_backward};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            auto it = g_lic_tune.find(key);
+            if (it != g_lic_tune.end()) {
+                for (int a = 0; a < 3; ++a) {
+                    st->tile[a] = it->second[a];
+                    st->super[a] = it->second[3 + a];
+                }
+                continue;
+            }
+        }
+        float best = 1e30f;
+        std::array<int, 6> pick{16, 4, 4, 2, 8, 8};
+        for (const auto& t : tiles)
+            for (int edge : bricks) {
+                for (int a = 0; a < 3; ++a) {
+                    st->tile[a] = t[a];
+                    st->super[a] = std::max(1, edge / t[a]);
+                }
+                LicLaunch L;
+                s = build_lic_launch(c, st, L);
+                if (s) return s;
+                hipError_t e = hipSuccess;
+                const float ms = time_ms(c->stream, [&] { return launch_lic(L, c->stream); }, 3, &e);
+                if (e) return fail(c, VV_EDEVICE, "autotune LIC: %s", hipGetErrorString(e));
+                if (ms < best) {
+                    best = ms;
+                    pick = {t[0], t[1], t[2], st->super[0], st->super[1], st->super[2]};
+                }
+            }
+        for (int a = 0; a < 3; ++a) {
+            st->tile[a] = pick[a];
+            st->super[a] = pick[3 + a];
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_lic_tune[key] = pick;
+    }
+    // render tile, on the first view
+    if (!b->views.empty()) {
+        static const int blocks[][2] = {{16, 16}, {32, 8}, {8, 8}, {64, 4}, {16, 4}, {32, 4}};
+        float best = 1e30f;
+        int pick[2] = {16, 16};
+        for (const auto& bl : blocks) {
+            b->render_block[0] = bl[0];
+            b->render_block[1] = bl[1];
+            RenderLaunch R;
+            s = build_view_launch(b, b->views[0], R);
+            if (s) return s;
+            hipError_t e = hipSuccess;
+            const float ms = time_ms(c->stream, [&] { return launch_render(R, c->stream); }, 3, &e);
+            if (e) return fail(c, VV_EDEVICE, "autotune render: %s", hipGetErrorString(e));
+            if (ms < best) {
+                best = ms;
+                pick[0] = bl[0];
+                pick[1] = bl[1];
+            }
+        }
+        b->render_block[0] = pick[0];
+        b->render_block[1] = pick[1];
+    }
+    release_graph(b);
+    return VV_OK;
+}
+
+vv_status vv_batch_get_tuning(const vv_batch* b, int timestep, int out8[8])
+{
+    if (!b || !out8 || timestep < 0 || timestep >= (int)b->steps.size()) return VV_EINVAL;
+    const BatchStep* st = b->steps[timestep];
+    for (int a = 0; a < 3; ++a) {
+        out8[a] = st->tile[a];
+        out8[3 + a] = st->super[a];
+    }
+    out8[6] = b->render_block[0];
+    out8[7] = b->render_block[1];
+    return VV_OK;
+}
+
+}  // extern "C"

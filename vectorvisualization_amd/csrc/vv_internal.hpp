@@ -85,6 +85,26 @@ struct RenderLaunch {
     int spec_exp;
     float* image;
     unsigned long long* counter;  // total samples (may be null)
+    int block[2];                 // pixel tile (threads), 0: 16x16
+};
+
+struct RenderBatchLaunch {
+    const RenderLaunch* views;    // device array, one per view
+    const int* first;             // device: first work item of each view
+    int nviews, total;            // views, work items (pixel tiles)
+    unsigned* counter;            // device work counter, zero before launch
+    unsigned long long* done;     // device: completion stamp per view
+    int block[2];
+    int grid;                     // persistent blocks
+    int fp16, lc;                 // shared by all views
+};
+
+struct LicBatchLaunch {
+    const void* jobs;             // device array of per-volume kernel arguments
+    const int* first;             // device: first tile of each volume
+    int njobs, total;
+    int grid;
+    int fp16, same;               // shared by all volumes
 };
 
 struct AdjointLaunch {
@@ -125,6 +145,15 @@ hipError_t launch_unpad_repeat_f32(const float* src, int nc, const int dims[3], 
 hipError_t launch_repeat_halo_f32(float* vol, int nc, const int dims[3], hipStream_t s);
 hipError_t launch_lic(const LicLaunch& L, hipStream_t s);
 hipError_t launch_render(const RenderLaunch& R, hipStream_t s);
+hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s);
+hipError_t launch_stamp(unsigned long long* t0, hipStream_t s);
+// fills the per-volume kernel arguments of a LIC batch (host memory, size
+// returned by lic_args_size()), tiles per volume in *ntiles
+size_t lic_args_size();
+hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
+hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
+int lic_batch_blocks_per_cu(int fp16, int same);
+int render_batch_blocks_per_cu(int fp16, int lc, int threads);
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s);
 hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
                                  hipStream_t s);

@@ -13,6 +13,8 @@
 #include "vv_device.hpp"
 #include "vv_internal.hpp"
 
+#include <cstring>
+
 namespace vv {
 
 namespace {
@@ -409,6 +411,28 @@ __device__ __forceinline__ float3 field_rgb(const LicArgs& a, float x, float y, 
     return tri_rgb(fv, c);
 }
 
+// Stores one LIC voxel into the REPEAT-padded output and into the copies
+// of plane 0 at index n along each axis (the wrap halo), so no separate
+// halo pass is needed.
+template <typename T>
+__device__ __forceinline__ void store_repeat(T* out, int osy, int osz, int x, int y, int z, int nx, int ny,
+                                             int nz, T v)
+{
+    const long long o = (long long)z * osz + (long long)y * osy + x;
+    out[o] = v;
+    const bool hx = x == 0, hy = y == 0, hz = z == 0;
+    if (hx | hy | hz) {
+        const long long ox = nx, oy = (long long)ny * osy, oz = (long long)nz * osz;
+        if (hx) out[o + ox] = v;
+        if (hy) out[o + oy] = v;
+        if (hz) out[o + oz] = v;
+        if (hx & hy) out[o + ox + oy] = v;
+        if (hx & hz) out[o + ox + oz] = v;
+        if (hy & hz) out[o + oy + oz] = v;
+        if (hx & hy & hz) out[o + ox + oy + oz] = v;
+    }
+}
+
 // Per-lane cache of the last trilinear cell.  Consecutive samples of a
 // streamline are ~h_e*n voxels apart (0.38 at 256^3, h = 0.005), so the
 // corrector sample almost always lands in the predictor's cell and the next
@@ -633,11 +657,8 @@ __device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedA
 // noise grid equals the field grid and freq == 1, so the noise cell of the
 // corrector position reuses the corrector's floors and fractions.
 template <typename FT, bool SAME>
-__global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
+__device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
 {
-    const int b = blockIdx.x;
-    const int tile = (b & 7) * a.per_xcd + (b >> 3);
-    if (tile >= a.nblocks) return;
     int bx, by, bz;
     tile_coords(a, tile, bx, by, bz);
     const int t = threadIdx.x;
@@ -717,7 +738,34 @@ __global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
             acc = acc + nval * wk[i];
         }
     }
-    a.out[(long long)z * a.osz + (long long)y * a.osy + x] = acc * a.scl;
+    store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
+}
+
+template <typename FT, bool SAME>
+__global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
+{
+    const int b = blockIdx.x;
+    const int tile = (b & 7) * a.per_xcd + (b >> 3);
+    if (tile >= a.nblocks) return;
+    lic_cached_tile<FT, SAME>(a, tile);
+}
+
+// Persistent LIC over the tiles of several volumes (one LicArgs per volume,
+// same storage and noise grid relation).  Tile i of the concatenated list
+// goes to block i mod gridDim.x; the XCD order of each volume's tiles is the
+// one of k_lic_cached, so every XCD works on a contiguous part.
+template <typename FT, bool SAME>
+__global__ void __launch_bounds__(kThreads) k_lic_batch(const LicArgs* jobs, const int* first, int njobs,
+                                                        int total)
+{
+    for (int item = blockIdx.x; item < total; item += gridDim.x) {
+        int j = 0;
+        while (j + 1 < njobs && item >= first[j + 1]) ++j;
+        const int i = item - first[j];
+        const LicArgs& a = jobs[j];
+        const int tile = (i & 7) * a.per_xcd + (i >> 3);
+        if (tile < a.nblocks) lic_cached_tile<FT, SAME>(a, tile);
+    }
 }
 
 template <typename FT, int NC, bool GATE>
@@ -783,11 +831,11 @@ __global__ void __launch_bounds__(kThreads) k_lic(LicArgs a)
             }
         }
     }
-    const long long o = (long long)z * a.osz + (long long)y * a.osy + x;
     if constexpr (NC == 1) {
-        a.out[o] = acc.x * a.scl;  // lic3d_volume_fragment.glsl:15
+        store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc.x * a.scl);  // lic3d_volume_fragment.glsl:15
     } else {
-        reinterpret_cast<float4*>(a.out)[o] = make_float4(acc.x, acc.y, acc.z, acc.w * a.scl);
+        store_repeat<float4>(reinterpret_cast<float4*>(a.out), a.osy, a.osz, x, y, z, nx, ny, nz,
+                             make_float4(acc.x, acc.y, acc.z, acc.w * a.scl));
     }
 }
 
@@ -1002,6 +1050,88 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
             hipLaunchKernelGGL((k_lic<float4, 1, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+}  // namespace vv
+
+namespace vv {
+
+size_t lic_args_size() { return sizeof(LicArgs); }
+
+// per-volume kernel arguments of the cached kernel, as launch_lic builds them
+hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles)
+{
+    LicArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.field = Vol<void>{L.field.p, {L.field.n[0], L.field.n[1], L.field.n[2]}, L.field.sy, L.field.sz};
+    a.noise = Vol<void>{L.noise.p, {L.noise.n[0], L.noise.n[1], L.noise.n[2]}, L.noise.sy, L.noise.sz};
+    a.w = L.w;
+    a.F = L.F;
+    a.B = L.B;
+    a.he = L.he;
+    a.scl = L.scl;
+    a.freq = L.freq;
+    a.sc0 = L.scale[0];
+    a.sc1 = L.scale[1];
+    a.sc2 = L.scale[2];
+    a.out = (float*)L.out.p;
+    a.osy = L.out.sy;
+    a.osz = L.out.sz;
+    a.field_bytes = (unsigned)L.field_bytes;
+    a.noise_bytes = (unsigned)L.noise_bytes;
+    a.tx = L.tile[0];
+    a.ty = L.tile[1];
+    a.tz = L.tile[2];
+    if (a.tx * a.ty * a.tz != kThreads) return hipErrorInvalidValue;
+    a.sx = L.super_tile[0] > 0 ? L.super_tile[0] : 1;
+    a.sy = L.super_tile[1] > 0 ? L.super_tile[1] : 1;
+    a.sz = L.super_tile[2] > 0 ? L.super_tile[2] : 1;
+    a.sgx = (L.field.n[0] + a.tx * a.sx - 1) / (a.tx * a.sx);
+    a.sgy = (L.field.n[1] + a.ty * a.sy - 1) / (a.ty * a.sy);
+    const int sgz = (L.field.n[2] + a.tz * a.sz - 1) / (a.tz * a.sz);
+    a.gx = a.sgx * a.sx;
+    a.gy = a.sgy * a.sy;
+    a.gz = sgz * a.sz;
+    a.nblocks = a.gx * a.gy * a.gz;
+    a.per_xcd = (a.nblocks + 7) / 8;
+    std::memcpy(host_args, &a, sizeof(a));
+    *ntiles = a.per_xcd * 8;
+    return hipSuccess;
+}
+
+hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s)
+{
+    if (B.njobs < 1 || B.grid < 1) return hipErrorInvalidValue;
+    const LicArgs* jobs = (const LicArgs*)B.jobs;
+    if (B.fp16) {
+        if (B.same)
+            hipLaunchKernelGGL((k_lic_batch<half4_t, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
+                               B.njobs, B.total);
+        else
+            hipLaunchKernelGGL((k_lic_batch<half4_t, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
+                               B.njobs, B.total);
+    } else {
+        if (B.same)
+            hipLaunchKernelGGL((k_lic_batch<float4, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
+                               B.njobs, B.total);
+        else
+            hipLaunchKernelGGL((k_lic_batch<float4, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
+                               B.njobs, B.total);
+    }
+    return hipGetLastError();
+}
+
+int lic_batch_blocks_per_cu(int fp16, int same)
+{
+    int n = 0;
+    hipError_t e;
+    if (fp16)
+        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<half4_t, true>, kThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<half4_t, false>, kThreads, 0);
+    else
+        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<float4, true>, kThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<float4, false>, kThreads, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace vv

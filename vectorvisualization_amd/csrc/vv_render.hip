@@ -195,7 +195,8 @@ __device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* 
 
 __device__ __forceinline__ void load_tables(const float* g, float* tf, float* op)
 {
-    for (int i = threadIdx.y * kBX + threadIdx.x; i < 1280; i += kRThreads) {
+    const int nt = blockDim.x * blockDim.y;
+    for (int i = threadIdx.y * blockDim.x + threadIdx.x; i < 1280; i += nt) {
         float v = g[i];
         if (i < 1024)
             tf[i] = v;
@@ -219,6 +220,47 @@ __device__ __forceinline__ int composite(float dest[4], const Sample& s)
     return mask;
 }
 
+// one pixel: ray setup, sample loop, image store; returns the sample count
+template <typename FT, int LC>
+__device__ __forceinline__ unsigned render_pixel(const RenderLaunch& a, const float* tf, const float* op,
+                                                 int i, int j)
+{
+    unsigned ns = 0;
+    float dest[4] = {0.f, 0.f, 0.f, 0.f};
+    Ray r;
+    if (ray_setup(a.view, i, j, r)) {
+        float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
+        for (long long k = 0; k < a.max_samples; ++k) {
+            Sample s;
+            eval_sample<FT, LC>(a, tf, op, r, pos, s);
+            composite(dest, s);
+            ++ns;
+            for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
+            if (outside(pos, a.view) || dest[3] > 0.95f) break;
+        }
+    }
+    const long long pix = (long long)j * a.view.width + i;
+    reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    return ns;
+}
+
+// block-wide sum of per-thread sample counts into *counter
+__device__ __forceinline__ void add_samples(unsigned long long* counter, unsigned long long ns,
+                                            unsigned long long* red)
+{
+    for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int nw = (blockDim.x * blockDim.y + 63) / 64;
+    if ((tid & 63) == 0) red[tid >> 6] = ns;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < nw; ++w) t += red[w];
+        atomicAdd(counter, t);
+    }
+}
+
+// blockDim = pixel tile (any shape of at most 256 threads, a multiple of 64)
 template <typename FT, int LC>
 __global__ void __launch_bounds__(kRThreads) k_render(RenderLaunch a)
 {
@@ -226,38 +268,54 @@ __global__ void __launch_bounds__(kRThreads) k_render(RenderLaunch a)
     __shared__ float op[256];
     __shared__ unsigned long long red[kRThreads / 64];
     load_tables(a.tf, tf, op);
-    const int i = blockIdx.x * kBX + threadIdx.x;
-    const int j = blockIdx.y * kBY + threadIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     unsigned long long ns = 0;
-    if (i < a.view.width && j < a.view.height) {
-        float dest[4] = {0.f, 0.f, 0.f, 0.f};
-        Ray r;
-        if (ray_setup(a.view, i, j, r)) {
-            float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
-            for (long long k = 0; k < a.max_samples; ++k) {
-                Sample s;
-                eval_sample<FT, LC>(a, tf, op, r, pos, s);
-                composite(dest, s);
-                ++ns;
-                for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
-                if (outside(pos, a.view) || dest[3] > 0.95f) break;
-            }
-        }
-        const long long pix = (long long)j * a.view.width + i;
-        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
-    }
-    if (a.counter) {
-        for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
-        const int tid = threadIdx.y * kBX + threadIdx.x;
-        if ((tid & 63) == 0) red[tid >> 6] = ns;
+    if (i < a.view.width && j < a.view.height) ns = render_pixel<FT, LC>(a, tf, op, i, j);
+    if (a.counter) add_samples(a.counter, ns, red);
+}
+
+// Persistent ray caster over the pixel tiles of many views.  Blocks take
+// work items (view, tile) from a counter (items of different views cost
+// different amounts); the end of each view's last tile is recorded as the
+// largest wall-clock stamp written by that view's tiles.
+template <typename FT, int LC>
+__global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* views, const int* first,
+                                                           int nviews, int total, unsigned* counter,
+                                                           unsigned long long* done)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    __shared__ int item_s;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    int cur = -1;
+    for (;;) {
+        if (tid == 0) item_s = (int)atomicAdd(counter, 1u);
         __syncthreads();
-        if (tid == 0) {
-            unsigned long long t = 0;
-            for (int w = 0; w < kRThreads / 64; ++w) t += red[w];
-            atomicAdd(a.counter, t);
+        const int item = item_s;
+        __syncthreads();
+        if (item >= total) break;
+        int v = 0;
+        while (v + 1 < nviews && item >= first[v + 1]) ++v;
+        const RenderLaunch& a = views[v];
+        if (v != cur) {
+            load_tables(a.tf, tf, op);
+            cur = v;
         }
+        const int t = item - first[v];
+        const int tx = (a.view.width + blockDim.x - 1) / blockDim.x;
+        const int i = (t % tx) * blockDim.x + threadIdx.x;
+        const int j = (t / tx) * blockDim.y + threadIdx.y;
+        unsigned long long ns = 0;
+        if (i < a.view.width && j < a.view.height) ns = render_pixel<FT, LC>(a, tf, op, i, j);
+        if (a.counter) add_samples(a.counter, ns, red);
+        __syncthreads();
+        if (tid == 0) atomicMax(&done[v], (unsigned long long)wall_clock64());
     }
 }
+
+__global__ void k_stamp(unsigned long long* t0) { *t0 = (unsigned long long)wall_clock64(); }
 
 // ---- adjoint -----------------------------------------------------------------
 //
@@ -506,8 +564,10 @@ __global__ void k_fixed_to_float(const long long* src, float* dst, long long n, 
 
 hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
 {
-    dim3 block(kBX, kBY);
-    dim3 grid((R.view.width + kBX - 1) / kBX, (R.view.height + kBY - 1) / kBY);
+    const int bx = R.block[0] > 0 ? R.block[0] : kBX, by = R.block[1] > 0 ? R.block[1] : kBY;
+    if (bx * by > kRThreads || (bx * by) % 64) return hipErrorInvalidValue;
+    dim3 block(bx, by);
+    dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
     if (R.field_fp16) {
         if (R.lc == 4)
             hipLaunchKernelGGL((k_render<half4_t, 4>), grid, block, 0, s, R);
@@ -540,6 +600,36 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s)
+{
+    const int bx = B.block[0], by = B.block[1];
+    if (bx * by > kRThreads || (bx * by) % 64 || B.nviews < 1) return hipErrorInvalidValue;
+    dim3 block(bx, by);
+    dim3 grid((unsigned)B.grid);
+    if (B.fp16) {
+        if (B.lc == 4)
+            hipLaunchKernelGGL((k_render_batch<half4_t, 4>), grid, block, 0, s, B.views, B.first, B.nviews,
+                               B.total, B.counter, B.done);
+        else
+            hipLaunchKernelGGL((k_render_batch<half4_t, 1>), grid, block, 0, s, B.views, B.first, B.nviews,
+                               B.total, B.counter, B.done);
+    } else {
+        if (B.lc == 4)
+            hipLaunchKernelGGL((k_render_batch<float4, 4>), grid, block, 0, s, B.views, B.first, B.nviews,
+                               B.total, B.counter, B.done);
+        else
+            hipLaunchKernelGGL((k_render_batch<float4, 1>), grid, block, 0, s, B.views, B.first, B.nviews,
+                               B.total, B.counter, B.done);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_stamp(unsigned long long* t0, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, s, t0);
+    return hipGetLastError();
+}
+
 hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
                                  hipStream_t s)
 {
@@ -553,6 +643,23 @@ hipError_t launch_unpad_clamp_fixed(const long long* src, const int d[3], float*
                                     hipStream_t s)
 {
     return launch_fixed_to_float(src, dst, (long long)d[0] * d[1] * d[2], inv_scale, s);
+}
+
+}  // namespace vv
+
+namespace vv {
+
+int render_batch_blocks_per_cu(int fp16, int lc, int threads)
+{
+    int n = 0;
+    hipError_t e;
+    if (fp16)
+        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<half4_t, 4>, threads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<half4_t, 1>, threads, 0);
+    else
+        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<float4, 4>, threads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<float4, 1>, threads, 0);
+    return (e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace vv

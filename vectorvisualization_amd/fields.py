@@ -136,3 +136,25 @@ def tf_random(seed: int):
     rgba = rng.uniform(0.0, 1.0, (256, 4)).astype(np.float32).ravel()
     op = (np.arange(256, dtype=np.float32) / 255.0)
     return rgba, op
+
+
+def orbit_camera(width: int, height: int, center, yaw: float, pitch: float = 0.0, dist: float = 4.0):
+    """Camera on an orbit around `center`: rotation by `yaw` about the y axis
+    followed by `pitch` about the x axis (radians), at distance `dist`; the
+    light keeps the default offset along the view axis."""
+    from ._lib import default_camera
+
+    cam = default_camera(width, height, center)
+    qy = np.array([0.0, np.sin(0.5 * yaw), 0.0, np.cos(0.5 * yaw)])
+    qx = np.array([np.sin(0.5 * pitch), 0.0, 0.0, np.cos(0.5 * pitch)])
+    # Hamilton product qy * qx, (x, y, z, w) order
+    x1, y1, z1, w1 = qy
+    x2, y2, z2, w2 = qx
+    q = (w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+         w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+         w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2,
+         w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2)
+    for i in range(4):
+        cam.quat[i] = float(q[i])
+    cam.dist = float(dist)
+    return cam
