@@ -38,6 +38,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP16 and adjoint legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched timesteps x views leg")
+    ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 FP16-storage leg")
+    ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only 3 LIC launches and 3 copies of 256 MiB (rocprof counter runs)")
     return ap.parse_args()
@@ -187,7 +190,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_lic",
+            "kernel": "k_lic_cached<half4_t>" if storage == vv.STORE_FP16 else "k_lic_cached<float4>",
             "algorithmic_bytes_per_voxel": bpv,
         },
     }
@@ -236,11 +239,209 @@ def main():
         ctx.upload_field(field, dims, storage=storage)
         ctx.compute_lic()
 
+    if not args.no_batch:
+        out["batch"] = batch_leg(args, dev)
+
+    if not args.no_precision:
+        out["precision"] = precision_leg(dev)
+
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field.cpu().numpy(), dims, L, lut, inv, p,
                                            args.cpu_seconds, samples / vox)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def batch_leg(args, dev, seed=7, nsteps=8, nviews=64, L=30):
+    """8 timesteps of the rotating 3D double gyre with sizes drawn from
+    {128,192,256,384}^3, 64 views with square resolutions uniform in
+    256..1024 on seeded orbit poses, triangle filter L = 30: every LIC volume
+    and every view in one captured launch (autotuned).  Compared with the
+    same work issued call by call through the single-volume API."""
+    import numpy as np
+    import torch
+
+    import vectorvisualization_amd as vv
+    from vectorvisualization_amd import fields
+
+    rng = np.random.default_rng(seed)
+    sizes = [int(x) for x in rng.choice([128, 192, 256, 384], size=nsteps)]
+    views = []
+    for _ in range(nviews):
+        w = int(rng.integers(256, 1025))
+        views.append((int(rng.integers(nsteps)), w, float(rng.uniform(0, 2 * np.pi)),
+                      float(rng.uniform(-0.6, 0.6))))
+    print(f"[bench] batch: timesteps {sizes}, {nviews} views", file=sys.stderr, flush=True)
+    storage = vv.STORE_FP16 if args.storage == "fp16" else vv.STORE_FP32
+    ctx = vv.Context(0)
+    lut, inv = vv.make_filter(vv.FILTER_TRIANGLE, L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    p.step_size_lic = 0.005
+    p.step_size_vol = 1.0 / 256.0
+    p.gradient_scale = 3.0
+    p.tf_index = vv.TF_INDEX_MAG
+    ctx.set_lic_params(p)
+    ctx.set_tf(*fields.tf_magnitude())
+    b = vv.Batch(ctx)
+    for i, n in enumerate(sizes):
+        f = fields.double_gyre_3d_torch(n, 1.25 * i, device=dev)
+        b.add_timestep(f, (n, n, n), storage=storage, noise_seed=100 + i, noise_kind=vv.NOISE_UNIFORM)
+        del f
+    images = []
+    for t, w, yaw, pitch in views:
+        img = torch.empty(w * w * 4, dtype=torch.float32, device=dev)
+        b.add_view(t, fields.orbit_camera(w, w, (0.5, 0.5, 0.5), yaw, pitch), img)
+        images.append(img)
+    t = time.perf_counter()
+    b.autotune()
+    tune_s = time.perf_counter() - t
+    b.run()
+    ctx.synchronize()
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    run_ms, lat = [], []
+    for _ in range(max(1, args.batch_reps)):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        b.run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        run_ms.append(e0.elapsed_time(e1))
+        lat.append(b.view_latency_ms())
+    lat = np.concatenate(lat)
+    ray = b.samples()
+    lic = sum(n ** 3 * (2 * L + 1) for n in sizes)
+    batch_ms = float(np.median(run_ms))
+    tuning = {f"{n}^3": b.tuning(i)[:6] for i, n in enumerate(sizes)}
+    render_tile = b.tuning(0)[6:]
+    b.close()
+
+    # the same work call by call: per timestep one vv_compute_lic and its views' vv_render
+    seq_ms = 0.0
+    for i, n in enumerate(sizes):
+        f = fields.double_gyre_3d_torch(n, 1.25 * i, device=dev)
+        ctx.upload_field(f, (n, n, n), storage=storage)
+        ctx.generate_noise(100 + i, vv.NOISE_UNIFORM, 0.0, (n, n, n))
+        del f
+        mine = [(k, v) for k, v in enumerate(views) if v[0] == i]
+
+        def work():
+            ctx.compute_lic()
+            for k, (_, w, yaw, pitch) in mine:
+                ctx.render(fields.orbit_camera(w, w, (0.5, 0.5, 0.5), yaw, pitch), out=images[k])
+
+        work()
+        ctx.synchronize()
+        t = time.perf_counter()
+        work()
+        ctx.synchronize()
+        seq_ms += (time.perf_counter() - t) * 1000.0
+    ctx.close()
+    return {
+        "workload": f"{nsteps} double-gyre timesteps {sizes}, {nviews} views 256^2..1024^2, "
+                    f"triangle L={L}, {args.storage}",
+        "batch_ms": batch_ms,
+        "view_latency_ms_p50": float(np.percentile(lat, 50)),
+        "view_latency_ms_p99": float(np.percentile(lat, 99)),
+        "lic_voxel_samples": int(lic),
+        "ray_samples": int(ray),
+        "aggregate_samples_per_s": (lic + ray) / (batch_ms / 1000.0),
+        "call_by_call_ms": seq_ms,
+        "speedup_vs_call_by_call": seq_ms / batch_ms,
+        "autotune_s": tune_s,
+        "lic_tile_and_brick": tuning,
+        "render_tile": render_tile,
+    }
+
+
+def precision_leg(dev, n=512, L=60, box=64, W=1024):
+    """512^3 vortex field stored as 3xFP16 (noise uint8, 4% sparse), cos^2
+    filter L = 60, FP32 integration: LIC volume and 1024^2 image against the
+    FP32-stored run (whole volume / image), and the LIC of a central 64^3
+    subvolume against the FP64 oracle."""
+    import numpy as np
+    import torch
+
+    import vectorvisualization_amd as vv
+    from vectorvisualization_amd import fields
+    from oracle import oracle as orc
+
+    print(f"[bench] precision leg: {n}^3 fp16 vs fp32 vs FP64 oracle ({box}^3)", file=sys.stderr, flush=True)
+    dims = (n, n, n)
+    field = fields.rankine_vortices_torch(n, device=dev).contiguous()
+    ctx = vv.Context(0)
+    lut, inv = vv.make_filter(vv.FILTER_COS2, L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    p.step_size_lic = 0.005
+    p.step_size_vol = 1.0 / 256.0
+    p.gradient_scale = 3.0
+    p.tf_index = vv.TF_INDEX_MAG
+    ctx.set_lic_params(p)
+    ctx.set_tf(*fields.tf_magnitude())
+    ctx.generate_noise(1, vv.NOISE_BINARY, 0.04, dims)
+    stream = torch.cuda.ExternalStream(ctx.stream)
+    res = {}
+    for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
+        ctx.upload_field(field, dims, storage=st)
+        ctx.compute_lic()
+        ctx.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.compute_lic()
+        e1.record(stream)
+        lic = torch.empty(n ** 3, dtype=torch.float32, device=dev)
+        ctx.download_lic(lic)
+        cen = ctx.field_geometry()[2]
+        img = torch.empty(W * W * 4, dtype=torch.float32, device=dev)
+        ctx.render(vv.default_camera(W, W, cen), out=img)
+        torch.cuda.synchronize()
+        res[name] = dict(lic=lic, img=img, lic_ms=e0.elapsed_time(e1))
+    ctx.close()
+    a, b = res["fp16"], res["fp32"]
+    scale = float(b["lic"].abs().max())
+    dl = (a["lic"] - b["lic"]).double().abs()
+    di = (a["img"] - b["img"]).double().abs()
+    out = {
+        "workload": f"{n}^3 Rankine x12, field 3xFP16 ({n ** 3 * 6 / 1e6:.0f} MB) vs FP32, 4% binary noise, "
+                    f"cos^2 L={L} h=0.005; {W}^2 frame",
+        "lic_ms_fp32": b["lic_ms"],
+        "lic_ms_fp16": a["lic_ms"],
+        "fp16_vs_fp32": {
+            "lic_max_rel": float(dl.max()) / scale,
+            "lic_rms_rel": float(dl.pow(2).mean().sqrt()) / scale,
+            "image_max_abs": float(di.max()),
+            "image_rms": float(di.pow(2).mean().sqrt()),
+        },
+    }
+    # FP64 oracle on a central box: the whole field is encoded in double
+    host = field.cpu().numpy()
+    del field
+    f4 = orc.encode_field(host, dims, f64=True)
+    del host
+    noise = orc.gen_noise(1, vv.NOISE_BINARY, 0.04, dims)
+    op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.005, inv_filter_area=inv,
+                    gradient_scale=3.0)
+    b0 = n // 2 - box // 2
+    try:
+        cores = max(1, min(len(os.sched_getaffinity(0)), 16))
+    except AttributeError:
+        cores = 1
+    t = time.perf_counter()
+    ref = orc.lic_box(f4, dims, noise, 1, dims, lut, op, (b0, b0 + box) * 3, nthreads=cores)
+    oracle_s = time.perf_counter() - t
+    del f4
+    ref = ref.reshape(box, box, box)
+    rscale = float(np.max(np.abs(ref)))
+    for name in ("fp32", "fp16"):
+        sub = res[name]["lic"].view(n, n, n)[b0:b0 + box, b0:b0 + box, b0:b0 + box].cpu().numpy()
+        d = np.abs(sub.astype(np.float64) - ref)
+        out[f"{name}_vs_fp64_{box}^3"] = {"lic_max_rel": float(d.max()) / rscale,
+                                         "lic_rms_rel": float(np.sqrt(np.mean(d ** 2))) / rscale}
+    out["oracle_box_s"] = oracle_s
+    return out
 
 
 def cpu_baseline(field_np, dims, L, lut, inv, p, seconds, samples_per_voxel):

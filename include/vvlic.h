@@ -24,7 +24,10 @@
  *   - Pointer arguments marked "host or device" take a memory-kind flag
  *     `on_device` (0 = host memory, 1 = device memory of the context's GPU).
  *   - All device work is ordered on the context's stream (vv_stream());
- *     calls that return data to host memory synchronise that stream.
+ *     calls that return data to host memory synchronise that stream.  Device
+ *     inputs are read, and device outputs written, on that stream: a caller
+ *     producing or consuming them on another stream orders the two streams
+ *     itself (hipEventRecord + hipStreamWaitEvent), before and after the call.
  *   - A context is not thread-safe; use one context per host thread.
  */
 #ifndef VVLIC_H

@@ -181,6 +181,44 @@ def _ptr(a) -> tuple[int, int]:
     raise TypeError(f"unsupported buffer type {type(a)}")
 
 
+def _is_cuda(a) -> bool:
+    return a is not None and getattr(a, "is_cuda", False)
+
+
+class _StreamOrder:
+    """Orders the context's stream after the caller's current torch stream on
+    entry, and the caller's stream after the context's stream on exit, when
+    any argument is a CUDA tensor (device events; the host does not wait).
+    Inputs produced by torch are then complete before the library reads
+    them, and torch work queued after the call sees the library's outputs."""
+
+    def __init__(self, ctx: "Context", arrays):
+        self._ctx = ctx
+        self._on = any(_is_cuda(a) for a in arrays)
+
+    def __enter__(self):
+        if self._on:
+            import torch
+
+            self._cur = torch.cuda.current_stream(self._ctx.device)
+            self._ext = self._ctx._torch_stream()
+            self._on = self._cur.cuda_stream != self._ext.cuda_stream
+            if self._on:
+                ev = torch.cuda.Event()
+                ev.record(self._cur)
+                self._ext.wait_event(ev)
+        return self
+
+    def __exit__(self, *exc):
+        if self._on:
+            import torch
+
+            ev = torch.cuda.Event()
+            ev.record(self._ext)
+            self._cur.wait_event(ev)
+        return False
+
+
 def default_lic_params() -> LicParams:
     p = LicParams()
     LIB.vv_default_lic_params(C.byref(p))
@@ -220,6 +258,21 @@ class Context:
         if st:
             raise VVError(st, LIB.vv_last_error(None).decode())
         self._h = h
+        self.device = int(device)
+        self._ext = None
+
+    def _torch_stream(self):
+        """The context's stream as a torch stream object."""
+        if self._ext is None:
+            import torch
+
+            self._ext = torch.cuda.ExternalStream(self.stream, device=self.device)
+        return self._ext
+
+    def ordered(self, *arrays) -> _StreamOrder:
+        """Context manager ordering this context's stream against the current
+        torch stream around work that touches the given tensors."""
+        return _StreamOrder(self, arrays)
 
     # -- plumbing --
     def _chk(self, st: int) -> None:
@@ -262,8 +315,9 @@ class Context:
                      storage: int = STORE_FP32) -> None:
         p0, dev = _ptr(v0)
         p1 = _ptr(v1)[0] if v1 is not None else None
-        self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
-                                      int(storage), dev))
+        with self.ordered(v0, v1):
+            self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
+                                          int(storage), dev))
 
     def field_geometry(self):
         e, s, c = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
@@ -278,7 +332,8 @@ class Context:
     def upload_noise(self, noise, dims, with_gradients: bool = False, grad_rgb=None) -> None:
         p, dev = _ptr(noise)
         g = _ptr(grad_rgb)[0] if grad_rgb is not None else None
-        self._chk(LIB.vv_upload_noise(self._h, p, _i3(dims), int(bool(with_gradients)), g, dev))
+        with self.ordered(noise, grad_rgb):
+            self._chk(LIB.vv_upload_noise(self._h, p, _i3(dims), int(bool(with_gradients)), g, dev))
 
     def generate_noise(self, seed: int, kind: int, density: float, dims,
                        with_gradients: bool = False) -> None:
@@ -312,7 +367,8 @@ class Context:
 
     def upload_scalar(self, scalar, dims) -> None:
         p, dev = _ptr(scalar)
-        self._chk(LIB.vv_upload_scalar(self._h, p, _i3(dims), dev))
+        with self.ordered(scalar):
+            self._chk(LIB.vv_upload_scalar(self._h, p, _i3(dims), dev))
 
     def set_lic_params(self, p: LicParams) -> None:
         self._chk(LIB.vv_set_lic_params(self._h, C.byref(p)))
@@ -337,12 +393,14 @@ class Context:
         if out is None:
             out = np.empty(int(np.prod(d)) * ch, np.float32)
         p, dev = _ptr(out)
-        self._chk(LIB.vv_download_lic(self._h, p, dev))
+        with self.ordered(out):
+            self._chk(LIB.vv_download_lic(self._h, p, dev))
         return out
 
     def upload_lic(self, lic, channels: int, dims) -> None:
         p, dev = _ptr(lic)
-        self._chk(LIB.vv_upload_lic(self._h, p, int(channels), _i3(dims), dev))
+        with self.ordered(lic):
+            self._chk(LIB.vv_upload_lic(self._h, p, int(channels), _i3(dims), dev))
 
     def lic_weights(self) -> np.ndarray:
         p = self.get_lic_params()
@@ -364,8 +422,9 @@ class Context:
             out = np.empty(cam.width * cam.height * 4, np.float32)
         p, dev = _ptr(out)
         ns = C.c_uint64()
-        self._chk(LIB.vv_render(self._h, C.byref(cam), p, dev,
-                                C.byref(ns) if count_samples else None))
+        with self.ordered(out):
+            self._chk(LIB.vv_render(self._h, C.byref(cam), p, dev,
+                                    C.byref(ns) if count_samples else None))
         return (out, ns.value) if count_samples else out
 
     def view_block(self, cam: Camera) -> np.ndarray:
@@ -378,7 +437,8 @@ class Context:
             self._chk(LIB.vv_set_opacity_scale(self._h, None, 0))
             return
         p, dev = _ptr(sigma)
-        self._chk(LIB.vv_set_opacity_scale(self._h, p, dev))
+        with self.ordered(sigma):
+            self._chk(LIB.vv_set_opacity_scale(self._h, p, dev))
 
     def render_adjoint(self, cam: Camera, dimage, dtf=None, dsigma=None, want_sigma: bool = True):
         host = isinstance(dimage, np.ndarray)
@@ -392,7 +452,8 @@ class Context:
         pi, dev = _ptr(dimage)
         pt = _ptr(dtf)[0]
         ps = _ptr(dsigma)[0] if dsigma is not None else None
-        self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
+        with self.ordered(dimage, dtf, dsigma):
+            self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
         return dtf, dsigma
 
 
@@ -420,9 +481,10 @@ class Batch:
     def add_timestep(self, v, dims, spacing=(1.0, 1.0, 1.0), storage: int = STORE_FP32,
                      noise_seed: int = 1, noise_kind: int = NOISE_UNIFORM, density: float = 0.0) -> int:
         p, dev = _ptr(v)
-        self._ctx._chk(LIB.vv_batch_add_timestep(self._h, p, _i3(dims), _f3(spacing), int(storage),
-                                                 C.c_uint64(noise_seed), int(noise_kind),
-                                                 float(density), dev))
+        with self._ctx.ordered(v):
+            self._ctx._chk(LIB.vv_batch_add_timestep(self._h, p, _i3(dims), _f3(spacing),
+                                                     int(storage), C.c_uint64(noise_seed),
+                                                     int(noise_kind), float(density), dev))
         return self.info()[0] - 1
 
     def add_view(self, timestep: int, cam: Camera, image) -> int:
@@ -447,7 +509,8 @@ class Batch:
         return list(out)
 
     def run(self) -> None:
-        self._ctx._chk(LIB.vv_batch_run(self._h))
+        with self._ctx.ordered(*self._images):
+            self._ctx._chk(LIB.vv_batch_run(self._h))
 
     def view_latency_ms(self) -> np.ndarray:
         n = self.info()[1]

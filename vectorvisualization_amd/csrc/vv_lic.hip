@@ -495,12 +495,17 @@ __device__ __forceinline__ unsigned repeat_idx(float fl, int n)
     return (unsigned)i;
 }
 
-// raw buffer resource over a device array (gfx9 dword3 0x00020000); loads
-// take 32-bit byte offsets and an offset past num_records reads 0
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void* p, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
-}
+// Texel loads at a 32-bit unsigned byte offset from a block-uniform base:
+// base + zext(off) compiles to global_load ... voffset, s[base] (a VGPR
+// offset of 32 bits, the 64-bit base in SGPRs), so arrays up to 4 GiB need
+// no 64-bit address arithmetic per lane.  (Raw buffer loads, which also take
+// a 32-bit VGPR offset, returned zeros for offsets of 2 GiB and more.)
+struct F3 {
+    float x, y, z;
+};
+// global address space: the batch kernel reads its array pointers from a
+// descriptor in memory, where the compiler cannot infer it (flat loads)
+typedef const __attribute__((address_space(1))) char* gchar;
 
 template <typename FT>
 struct FieldLoad;
@@ -509,17 +514,17 @@ struct FieldLoad;
 template <>
 struct FieldLoad<float4> {
     static constexpr unsigned kShift = 4;
-    __device__ static __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned off, float t0[3],
-                                                float t1[3])
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
     {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b96(r, off, 0, 0);
-        const auto b = __builtin_amdgcn_raw_buffer_load_b96(r, off + 16u, 0, 0);
-        t0[0] = __int_as_float(a[0]);
-        t0[1] = __int_as_float(a[1]);
-        t0[2] = __int_as_float(a[2]);
-        t1[0] = __int_as_float(b[0]);
-        t1[1] = __int_as_float(b[1]);
-        t1[2] = __int_as_float(b[2]);
+        gchar g = (gchar)base;
+        const F3 a = *(const __attribute__((address_space(1))) F3*)(g + off);
+        const F3 b = *(const __attribute__((address_space(1))) F3*)(g + off + 16u);
+        t0[0] = a.x;
+        t0[1] = a.y;
+        t0[2] = a.z;
+        t1[0] = b.x;
+        t1[1] = b.y;
+        t1[2] = b.z;
     }
 };
 
@@ -527,21 +532,30 @@ struct FieldLoad<float4> {
 template <>
 struct FieldLoad<half4_t> {
     static constexpr unsigned kShift = 3;
-    __device__ static __forceinline__ void pair(__amdgpu_buffer_rsrc_t r, unsigned off, float t0[3],
-                                                float t1[3])
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
     {
-        const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-        t0[0] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[0] & 0xffffu)));
-        t0[1] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[0] >> 16)));
-        t0[2] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[1] & 0xffffu)));
-        t1[0] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[2] & 0xffffu)));
-        t1[1] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[2] >> 16)));
-        t1[2] = __half2float(__ushort_as_half((unsigned short)((unsigned)a[3] & 0xffffu)));
+        gchar g = (gchar)base;
+        const uint4 a = *(const __attribute__((address_space(1))) uint4*)(g + off);
+        t0[0] = __half2float(__ushort_as_half((unsigned short)(a.x & 0xffffu)));
+        t0[1] = __half2float(__ushort_as_half((unsigned short)(a.x >> 16)));
+        t0[2] = __half2float(__ushort_as_half((unsigned short)(a.y & 0xffffu)));
+        t1[0] = __half2float(__ushort_as_half((unsigned short)(a.z & 0xffffu)));
+        t1[1] = __half2float(__ushort_as_half((unsigned short)(a.z >> 16)));
+        t1[2] = __half2float(__ushort_as_half((unsigned short)(a.w & 0xffffu)));
     }
 };
 
+// two adjacent noise bytes (x0, x1) in one 16-bit load; the address may be odd
+typedef unsigned short __attribute__((aligned(1))) u16_unaligned;
+__device__ __forceinline__ unsigned noise_pair(const unsigned char* base, unsigned off)
+{
+    gchar g = (gchar)(const char*)base;
+    return *(const __attribute__((address_space(1))) u16_unaligned*)(g + off);
+}
+
 struct CachedArgs {
-    __amdgpu_buffer_rsrc_t fr, nr;
+    const char* fr;           // field texels
+    const unsigned char* nr;  // noise bytes
     float fnm1[3];      // field n-1 per axis
     unsigned fsy, fsz;  // field element strides
     unsigned nsy, nsz;  // noise element strides
@@ -607,7 +621,7 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
     if (nmiss) {
         const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) nraw[k] = __builtin_amdgcn_raw_buffer_load_b16(c.nr, nb + off[k], 0, 0);
+        for (int k = 0; k < 4; ++k) nraw[k] = (unsigned short)noise_pair(c.nr, nb + off[k]);
     }
     if (fmiss) {
         fc.base = (int)fb;
@@ -644,7 +658,7 @@ __device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedA
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             // two adjacent bytes (x0, x1) in one load
-            const unsigned short u = __builtin_amdgcn_raw_buffer_load_b16(c.nr, base + off[k], 0, 0);
+            const unsigned u = noise_pair(c.nr, base + off[k]);
             const float t0 = (float)(u & 0xffu), t1 = (float)(u >> 8);
             nc.v0[k] = t0;
             nc.dx[k] = t1 - t0;
@@ -673,8 +687,8 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
 
     CachedArgs c;
-    c.fr = rsrc(a.field.p, a.field_bytes);
-    c.nr = rsrc(a.noise.p, a.noise_bytes);
+    c.fr = (const char*)a.field.p;
+    c.nr = (const unsigned char*)a.noise.p;
     c.fnm1[0] = (float)(nx - 1);
     c.fnm1[1] = (float)(ny - 1);
     c.fnm1[2] = (float)(nz - 1);
@@ -755,12 +769,13 @@ __global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
 // goes to block i mod gridDim.x; the XCD order of each volume's tiles is the
 // one of k_lic_cached, so every XCD works on a contiguous part.
 template <typename FT, bool SAME>
-__global__ void __launch_bounds__(kThreads) k_lic_batch(const LicArgs* jobs, const int* first, int njobs,
-                                                        int total)
+__global__ void __launch_bounds__(kThreads) k_lic_batch(const LicArgs* __restrict__ jobs,
+                                                        const int* __restrict__ first, int njobs, int total)
 {
     for (int item = blockIdx.x; item < total; item += gridDim.x) {
         int j = 0;
         while (j + 1 < njobs && item >= first[j + 1]) ++j;
+        j = __builtin_amdgcn_readfirstlane(j);  // block-uniform: descriptor loads go to SGPRs
         const int i = item - first[j];
         const LicArgs& a = jobs[j];
         const int tile = (i & 7) * a.per_xcd + (i >> 3);

@@ -110,6 +110,36 @@ def double_gyre_3d(n: int, t: float, A: float = 0.1, eps: float = 0.25,
     return out
 
 
+def double_gyre_3d_torch(n: int, t: float, A: float = 0.1, eps: float = 0.25,
+                         omega: float = 2.0 * np.pi / 10.0, rot: float = 0.5, device="cuda"):
+    """double_gyre_3d evaluated with torch on `device` (float64 math, float32
+    result of shape (n, n, n, 3)), for volumes too large to build quickly on
+    the host."""
+    import torch
+
+    c = (torch.arange(n, dtype=torch.float64, device=device) + 0.5) / n
+    z = c.view(n, 1, 1)
+    y = c.view(1, n, 1)
+    x = c.view(1, 1, n)
+    th = rot * t
+    ct, st = float(np.cos(th)), float(np.sin(th))
+    xr = ct * (x - 0.5) + st * (y - 0.5) + 0.5
+    yr = -st * (x - 0.5) + ct * (y - 0.5) + 0.5
+    X = 2.0 * xr
+    a = eps * float(np.sin(omega * t))
+    b = 1.0 - 2.0 * a
+    f = a * X * X + b * X
+    dfdx = 2.0 * a * X + b
+    u = -np.pi * A * torch.sin(np.pi * f) * torch.cos(np.pi * yr)
+    v = np.pi * A * torch.cos(np.pi * f) * torch.sin(np.pi * yr) * dfdx
+    w = 0.25 * A * np.pi * torch.sin(2.0 * np.pi * z) * torch.cos(np.pi * yr)
+    out = torch.empty((n, n, n, 3), dtype=torch.float32, device=device)
+    out[..., 0] = ct * u - st * v
+    out[..., 1] = st * u + ct * v
+    out[..., 2] = w.expand(n, n, n)
+    return out
+
+
 def tf_linear_ramp():
     """256-entry RGBA + opacity ramp in the LIC value (rgb = a = opacity = i/255)."""
     r = np.arange(256, dtype=np.float32) / 255.0
