@@ -48,7 +48,8 @@ typedef enum vv_status {
     VV_ENOMEM = 2,        /* device or host allocation failed */
     VV_EDEVICE = 3,       /* HIP runtime error */
     VV_EUNSUPPORTED = 4,  /* valid request this build does not implement */
-    VV_ESTATE = 5         /* required input (field, noise, TF, ...) not loaded */
+    VV_ESTATE = 5,        /* required input (field, noise, TF, ...) not loaded */
+    VV_EIO = 6            /* file missing, unreadable or malformed */
 } vv_status;
 
 typedef enum vv_storage {
@@ -228,6 +229,10 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v, const int dims[3],
                                 const float spacing[3], int storage, uint64_t noise_seed,
                                 int noise_kind, double density, int on_device);
 vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, float* image);
+/* re-encode time step `timestep` from new vectors (same dims; v1 != NULL lerps v0 + lerp_t
+ * (v1 - v0)), so an unsteady series advances without rebuilding the batch or its graph */
+vv_status vv_batch_set_field(vv_batch* b, int timestep, const float* v0, const float* v1,
+                             float lerp_t, int on_device);
 vv_status vv_batch_info(const vv_batch* b, int* ntimesteps, int* nviews);
 /* times LIC tile shapes x brick (super-tile) edges per timestep size and the
  * render tile on the first view; picks the fastest (cached per size) */
@@ -241,6 +246,89 @@ vv_status vv_batch_view_latency(vv_batch* b, double* ms);
 /* total compositing samples of the last run (waits) */
 vv_status vv_batch_samples(vv_batch* b, uint64_t* total);
 vv_status vv_batch_download_lic(vv_batch* b, int timestep, float* out, int on_device);
+
+/* ---- the reference's file formats -------------------------------------
+ * Functions without a context report errors through vv_last_error(NULL). */
+typedef enum vv_raw_type {
+    VV_RAW_NONE = 0,
+    VV_RAW_UCHAR = 1,
+    VV_RAW_USHORT = 2,
+    VV_RAW_FLOAT = 3
+} vv_raw_type;
+
+/* a parsed .dat descriptor (DatFile, VV/reader.h:38-77) */
+typedef struct vv_dat_info {
+    char raw_pattern[1024]; /* ObjectFileName: printf pattern of the time step, resolved
+                               against the .dat file's directory when not found as given */
+    int dims[3];            /* Resolution */
+    float spacing[3];       /* SliceThickness (default 1 1 1) */
+    int raw_type;           /* Format: vv_raw_type */
+    int components;         /* Format's trailing count: FLOAT3 -> 3 (default 1) */
+    int t_begin, t_end;     /* TimeDependent (default 0 0) */
+} vv_dat_info;
+
+/* DatFile::parseDatFile (VV/reader.cpp:81-258): also checks every time step's RAW file exists */
+vv_status vv_dat_parse(const char* dat_path, vv_dat_info* info);
+/* DatFile::readRawData (VV/reader.cpp:261-297): raw bytes of one time step */
+vv_status vv_dat_read(const vv_dat_info* info, int timestep, void* out, size_t out_bytes);
+/* one time step of a 3-component data set as float32 (UCHAR components c -> c - 128) */
+vv_status vv_dat_read_field(const vv_dat_info* info, int timestep, float* out3);
+/* VectorDataSet::loadData + loadTimeStep + createTextureIterp (VV/dataset.cpp:97-176,
+ * 195-210, 290-366): reads `timestep` (and `next_timestep` when lerp_t != 0) and uploads
+ * the field lerped by lerp_t, as vv_upload_field */
+vv_status vv_load_field_dat(vv_ctx* ctx, const char* dat_path, int timestep, int next_timestep,
+                            float lerp_t, int storage);
+/* VolumeDataSet::loadData + createTexture (VV/dataset.cpp:836-1000) for the scalar gate:
+ * UCHAR as is, FLOAT as UNORM8 round(clamp(v,0,1)*255) */
+vv_status vv_load_scalar_dat(vv_ctx* ctx, const char* dat_path);
+/* noise file: 3 x int32 sizes, then the bytes (NoiseDataSet::loadRawData,
+ * VV/dataset.cpp:1347-1389).  out == NULL reads the sizes only. */
+vv_status vv_noise_file_read(const char* path, int dims[3], uint8_t* out, size_t out_bytes);
+vv_status vv_noise_file_write(const char* path, const uint8_t* noise, const int dims[3]);
+/* NoiseDataSet::loadData + createTexture (VV/dataset.cpp:1124-1344): with gradients, the
+ * 3-byte-per-voxel cache "<path>.grd" is read when present and valid, otherwise the
+ * gradients are computed on the GPU and, with use_gradient_cache, written there
+ * (loadGradients / saveGradients, VV/gradient.cpp:112-187) */
+vv_status vv_load_noise_file(vv_ctx* ctx, const char* path, int with_gradients,
+                             int use_gradient_cache);
+/* pngRead (VV/imageUtils.cpp:19-96): 8-bit gray, gray+alpha, RGB or RGBA, rows in file
+ * order.  out == NULL reads the header only. */
+vv_status vv_png_read(const char* path, uint8_t* out, size_t out_bytes, int* width, int* height,
+                      int* channels);
+/* pngWrite (VV/imageUtils.cpp:99-123); flip_rows writes the last row first */
+vv_status vv_png_write(const char* path, const uint8_t* pixels, int width, int height, int channels,
+                       int flip_rows);
+/* a rendered image (RGBA float32, bottom row first) as an 8-bit RGBA PNG, the
+ * screenshot path of the reference (UNORM8 conversion, rows flipped) */
+vv_status vv_image_write_png(const char* path, const float* rgba, int width, int height);
+/* LICFilter::loadData (VV/dataset.cpp:1420-1467): first row, first channel, centred in
+ * the next power of two; invFilterArea computed (vv_set_filter with inv_area 0) */
+vv_status vv_load_filter_png(vv_ctx* ctx, const char* path);
+/* TransferEdit::loadTF / saveTF (VV/transferEdit.cpp:113-334) on the 256x5 byte table of
+ * vv_set_tf_u8: "<stem>_rgba.png" and "<stem>_alpha.png".  Loading keeps the reference's
+ * order: the alpha/opacity file is read only when the RGBA file cannot be; channels a
+ * file does not hold keep their values in tf5. */
+vv_status vv_tf_load_png(const char* name, uint8_t tf5[1280]);
+vv_status vv_tf_save_png(const char* name, const uint8_t tf5[1280]);
+
+/* ---- unsteady fields: the animation loop -------------------------------
+ * All time steps of a series stay on the GPU as raw vectors.  Frame k (k = 0, 1, ...)
+ * encodes into the context's field the lerp of steps (p mod n) and (p+1 mod n),
+ * p = k / interp_size, at t = (k mod interp_size) / interp_size: the sequence of the
+ * reference's init + idle loop (VV/3DLIC.cpp:700-708, :129-172; checkInterpolateStage,
+ * VV/dataset.cpp:202-210; InterpSize 10 there).  The caller then runs vv_compute_lic and
+ * vv_render as for a static field. */
+typedef struct vv_anim vv_anim;
+vv_status vv_anim_create(vv_ctx* ctx, const float* const* steps, int nsteps, const int dims[3],
+                         const float spacing[3], int interp_size, int storage, int on_device,
+                         vv_anim** out);
+/* all time steps of a time-dependent DAT series (TimeDependent: begin end) */
+vv_status vv_anim_load_dat(vv_ctx* ctx, const char* dat_path, int interp_size, int storage,
+                           vv_anim** out);
+/* encode the next frame; reports the series index of its first step and its lerp factor */
+vv_status vv_anim_next(vv_anim* a, int* step, float* lerp_t);
+vv_status vv_anim_seek(vv_anim* a, long long frame);
+vv_status vv_anim_destroy(vv_anim* a);
 
 /* ---- launch tuning (used by the autotuner) ----------------------------
  * keys: "lic.tile_x", "lic.tile_y", "lic.tile_z" (powers of two whose

@@ -8,6 +8,7 @@ adjoint (``autograd``, imported lazily because it needs torch).
 """
 from ._lib import (  # noqa: F401
     LIB,
+    Anim,
     Batch,
     LIB_PATH,
     Camera,

@@ -16,7 +16,7 @@ LIB_PATH = Path(os.environ.get("VVLIC_LIB", _PKG / "lib" / "libvvlic.so"))
 
 VV_OK, VV_EINVAL, VV_ENOMEM, VV_EDEVICE, VV_EUNSUPPORTED, VV_ESTATE = range(6)
 STATUS_NAMES = {0: "VV_OK", 1: "VV_EINVAL", 2: "VV_ENOMEM", 3: "VV_EDEVICE",
-                4: "VV_EUNSUPPORTED", 5: "VV_ESTATE"}
+                4: "VV_EUNSUPPORTED", 5: "VV_ESTATE", 6: "VV_EIO"}
 STORE_FP32, STORE_FP16 = 0, 1
 FILTER_BOX, FILTER_TRIANGLE, FILTER_COS2, FILTER_GAUSSIAN = range(4)
 NOISE_BINARY, NOISE_UNIFORM = 0, 1
@@ -67,6 +67,24 @@ class VVError(RuntimeError):
     def __init__(self, status: int, msg: str):
         super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
         self.status = status
+
+
+RAW_NONE, RAW_UCHAR, RAW_USHORT, RAW_FLOAT = range(4)
+_RAW_DTYPE = {RAW_UCHAR: np.uint8, RAW_USHORT: np.uint16, RAW_FLOAT: np.float32}
+
+
+class DatInfo(C.Structure):
+    """vv_dat_info: a parsed .dat descriptor (DatFile, VV/reader.h:38-77)."""
+
+    _fields_ = [
+        ("raw_pattern", C.c_char * 1024),
+        ("dims", C.c_int * 3),
+        ("spacing", C.c_float * 3),
+        ("raw_type", C.c_int),
+        ("components", C.c_int),
+        ("t_begin", C.c_int),
+        ("t_end", C.c_int),
+    ]
 
 
 def _load() -> C.CDLL:
@@ -132,6 +150,26 @@ def _load() -> C.CDLL:
         "vv_batch_view_latency": (I, [P, C.POINTER(C.c_double)]),
         "vv_batch_samples": (I, [P, C.POINTER(C.c_uint64)]),
         "vv_batch_download_lic": (I, [P, I, P, I]),
+        "vv_dat_parse": (I, [C.c_char_p, C.POINTER(DatInfo)]),
+        "vv_dat_read": (I, [C.POINTER(DatInfo), I, P, C.c_size_t]),
+        "vv_dat_read_field": (I, [C.POINTER(DatInfo), I, FP]),
+        "vv_load_field_dat": (I, [P, C.c_char_p, I, I, F, I]),
+        "vv_load_scalar_dat": (I, [P, C.c_char_p]),
+        "vv_noise_file_read": (I, [C.c_char_p, IP, u8p, C.c_size_t]),
+        "vv_noise_file_write": (I, [C.c_char_p, u8p, IP]),
+        "vv_load_noise_file": (I, [P, C.c_char_p, I, I]),
+        "vv_png_read": (I, [C.c_char_p, u8p, C.c_size_t, IP, IP, IP]),
+        "vv_png_write": (I, [C.c_char_p, u8p, I, I, I, I]),
+        "vv_image_write_png": (I, [C.c_char_p, FP, I, I]),
+        "vv_load_filter_png": (I, [P, C.c_char_p]),
+        "vv_tf_load_png": (I, [C.c_char_p, u8p]),
+        "vv_tf_save_png": (I, [C.c_char_p, u8p]),
+        "vv_batch_set_field": (I, [P, I, P, P, F, I]),
+        "vv_anim_create": (I, [P, C.POINTER(P), I, IP, FP, I, I, I, C.POINTER(P)]),
+        "vv_anim_load_dat": (I, [P, C.c_char_p, I, I, C.POINTER(P)]),
+        "vv_anim_next": (I, [P, IP, FP]),
+        "vv_anim_seek": (I, [P, C.c_longlong]),
+        "vv_anim_destroy": (I, [P]),
         "vv_set_tuning": (I, [P, C.c_char_p, I]),
         "vv_get_tuning": (I, [P, C.c_char_p, IP]),
     }
@@ -156,7 +194,11 @@ EXPORTED = [
     "vv_set_tuning", "vv_get_tuning", "vv_batch_create", "vv_batch_destroy",
     "vv_batch_add_timestep", "vv_batch_add_view", "vv_batch_info", "vv_batch_autotune",
     "vv_batch_get_tuning", "vv_batch_run", "vv_batch_view_latency", "vv_batch_samples",
-    "vv_batch_download_lic",
+    "vv_batch_download_lic", "vv_dat_parse", "vv_dat_read", "vv_dat_read_field",
+    "vv_load_field_dat", "vv_load_scalar_dat", "vv_noise_file_read", "vv_noise_file_write",
+    "vv_load_noise_file", "vv_png_read", "vv_png_write", "vv_image_write_png",
+    "vv_load_filter_png", "vv_tf_load_png", "vv_tf_save_png", "vv_batch_set_field",
+    "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
 ]
 
 
@@ -249,6 +291,88 @@ def default_tf_u8() -> np.ndarray:
     return t
 
 
+def _io_chk(st: int) -> None:
+    if st:
+        raise VVError(st, LIB.vv_last_error(None).decode(errors="replace"))
+
+
+def _path(p) -> bytes:
+    return os.fsencode(p)
+
+
+def dat_parse(path) -> DatInfo:
+    """Parse a .dat descriptor; every time step's RAW file must exist."""
+    info = DatInfo()
+    _io_chk(LIB.vv_dat_parse(_path(path), C.byref(info)))
+    return info
+
+
+def dat_read(info: DatInfo, timestep: int | None = None) -> np.ndarray:
+    """Raw voxels of one time step, shape (z, y, x[, components])."""
+    t = info.t_begin if timestep is None else int(timestep)
+    shape = (info.dims[2], info.dims[1], info.dims[0]) + ((info.components,) if info.components > 1 else ())
+    out = np.empty(shape, _RAW_DTYPE[info.raw_type])
+    _io_chk(LIB.vv_dat_read(C.byref(info), t, out.ctypes.data, out.nbytes))
+    return out
+
+
+def dat_read_field(info: DatInfo, timestep: int | None = None) -> np.ndarray:
+    """One time step of a vector data set as float32 (z, y, x, 3)."""
+    t = info.t_begin if timestep is None else int(timestep)
+    out = np.empty((info.dims[2], info.dims[1], info.dims[0], 3), np.float32)
+    _io_chk(LIB.vv_dat_read_field(C.byref(info), t, out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out
+
+
+def noise_file_read(path):
+    d = (C.c_int * 3)()
+    _io_chk(LIB.vv_noise_file_read(_path(path), d, None, 0))
+    out = np.empty(d[0] * d[1] * d[2], np.uint8)
+    _io_chk(LIB.vv_noise_file_read(_path(path), d, out.ctypes.data, out.nbytes))
+    return out, tuple(d)
+
+
+def noise_file_write(path, noise: np.ndarray, dims) -> None:
+    noise = np.ascontiguousarray(noise, np.uint8)
+    _io_chk(LIB.vv_noise_file_write(_path(path), noise.ctypes.data, _i3(dims)))
+
+
+def png_read(path) -> np.ndarray:
+    """8-bit PNG as (height, width, channels), rows in file order."""
+    w, h, ch = C.c_int(), C.c_int(), C.c_int()
+    _io_chk(LIB.vv_png_read(_path(path), None, 0, C.byref(w), C.byref(h), C.byref(ch)))
+    out = np.empty((h.value, w.value, ch.value), np.uint8)
+    _io_chk(LIB.vv_png_read(_path(path), out.ctypes.data, out.nbytes, C.byref(w), C.byref(h),
+                            C.byref(ch)))
+    return out
+
+
+def png_write(path, pixels: np.ndarray, flip_rows: bool = False) -> None:
+    """pixels: (height, width[, channels]) uint8."""
+    px = np.ascontiguousarray(pixels, np.uint8)
+    ch = 1 if px.ndim == 2 else px.shape[2]
+    _io_chk(LIB.vv_png_write(_path(path), px.ctypes.data, px.shape[1], px.shape[0], ch, int(flip_rows)))
+
+
+def image_write_png(path, rgba, width: int, height: int) -> None:
+    """A rendered image (width*height RGBA float32, bottom row first) as PNG."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    _io_chk(LIB.vv_image_write_png(_path(path), rgba.ctypes.data_as(C.POINTER(C.c_float)), int(width),
+                                   int(height)))
+
+
+def tf_load_png(name, tf5: np.ndarray | None = None) -> np.ndarray:
+    """TransferEdit::loadTF on a 256x5 byte table (starting from tf5, or the default)."""
+    t = default_tf_u8() if tf5 is None else np.array(tf5, np.uint8).ravel().copy()
+    _io_chk(LIB.vv_tf_load_png(_path(name), t.ctypes.data))
+    return t
+
+
+def tf_save_png(name, tf5: np.ndarray) -> None:
+    t = np.ascontiguousarray(tf5, np.uint8).ravel()
+    _io_chk(LIB.vv_tf_save_png(_path(name), t.ctypes.data))
+
+
 class Context:
     """One engine context on one GPU (vv_ctx)."""
 
@@ -256,7 +380,7 @@ class Context:
         h = C.c_void_p()
         st = LIB.vv_create(C.byref(h), int(device))
         if st:
-            raise VVError(st, LIB.vv_last_error(None).decode())
+            raise VVError(st, LIB.vv_last_error(None).decode(errors="replace"))
         self._h = h
         self.device = int(device)
         self._ext = None
@@ -277,7 +401,7 @@ class Context:
     # -- plumbing --
     def _chk(self, st: int) -> None:
         if st:
-            raise VVError(st, LIB.vv_last_error(self._h).decode())
+            raise VVError(st, LIB.vv_last_error(self._h).decode(errors="replace"))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -318,6 +442,23 @@ class Context:
         with self.ordered(v0, v1):
             self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
                                           int(storage), dev))
+
+    def load_field_dat(self, path, timestep: int | None = None, next_timestep: int | None = None,
+                       lerp_t: float = 0.0, storage: int = STORE_FP32) -> None:
+        info = dat_parse(path)
+        t = info.t_begin if timestep is None else int(timestep)
+        n = t if next_timestep is None else int(next_timestep)
+        self._chk(LIB.vv_load_field_dat(self._h, _path(path), t, n, float(lerp_t), int(storage)))
+
+    def load_scalar_dat(self, path) -> None:
+        self._chk(LIB.vv_load_scalar_dat(self._h, _path(path)))
+
+    def load_noise_file(self, path, with_gradients: bool = False, use_gradient_cache: bool = True) -> None:
+        self._chk(LIB.vv_load_noise_file(self._h, _path(path), int(bool(with_gradients)),
+                                         int(bool(use_gradient_cache))))
+
+    def load_filter_png(self, path) -> None:
+        self._chk(LIB.vv_load_filter_png(self._h, _path(path)))
 
     def field_geometry(self):
         e, s, c = (C.c_float * 3)(), (C.c_float * 3)(), (C.c_float * 3)()
@@ -523,7 +664,55 @@ class Batch:
         self._ctx._chk(LIB.vv_batch_samples(self._h, C.byref(t)))
         return t.value
 
+    def set_field(self, timestep: int, v0, v1=None, lerp_t: float = 0.0) -> None:
+        p0, dev = _ptr(v0)
+        p1 = _ptr(v1)[0] if v1 is not None else None
+        with self._ctx.ordered(v0, v1):
+            self._ctx._chk(LIB.vv_batch_set_field(self._h, int(timestep), p0, p1, float(lerp_t), dev))
+
     def download_lic(self, timestep: int, dims) -> np.ndarray:
         out = np.empty(int(np.prod(dims)), np.float32)
         self._ctx._chk(LIB.vv_batch_download_lic(self._h, int(timestep), out.ctypes.data, 0))
         return out
+
+
+class Anim:
+    """An unsteady series resident on the GPU; next() encodes the next frame's
+    lerped field into the context (vv_anim)."""
+
+    def __init__(self, ctx: Context, steps=None, dims=None, spacing=(1.0, 1.0, 1.0), interp_size: int = 10,
+                 storage: int = STORE_FP32, dat_path=None):
+        h = C.c_void_p()
+        self._ctx = ctx
+        if dat_path is not None:
+            ctx._chk(LIB.vv_anim_load_dat(ctx.handle, _path(dat_path), int(interp_size), int(storage),
+                                          C.byref(h)))
+        else:
+            ptrs = [_ptr(s) for s in steps]
+            devs = {d for _, d in ptrs}
+            if len(devs) != 1:
+                raise ValueError("all time steps must be host arrays or all device tensors")
+            arr = (C.c_void_p * len(ptrs))(*[p for p, _ in ptrs])
+            with ctx.ordered(*steps):
+                ctx._chk(LIB.vv_anim_create(ctx.handle, arr, len(ptrs), _i3(dims), _f3(spacing),
+                                            int(interp_size), int(storage), devs.pop(), C.byref(h)))
+        self._h = h
+
+    def next(self):
+        s, t = C.c_int(), C.c_float()
+        self._ctx._chk(LIB.vv_anim_next(self._h, C.byref(s), C.byref(t)))
+        return s.value, t.value
+
+    def seek(self, frame: int) -> None:
+        self._ctx._chk(LIB.vv_anim_seek(self._h, int(frame)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            LIB.vv_anim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

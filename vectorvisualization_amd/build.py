@@ -43,7 +43,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-SOURCES = ["vv_lic.hip", "vv_render.hip", "vv_api.cpp", "vv_batch.cpp"]
+SOURCES = ["vv_lic.hip", "vv_render.hip", "vv_api.cpp", "vv_batch.cpp", "vv_io.cpp", "vv_anim.cpp"]
 
 
 def _compile(src: str) -> Path:
@@ -65,7 +65,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, SOURCES))
     LIB_DIR.mkdir(parents=True, exist_ok=True)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

@@ -291,6 +291,42 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     return VV_OK;
 }
 
+vv_status vv_batch_set_field(vv_batch* b, int timestep, const float* v0, const float* v1, float lerp_t,
+                             int on_device)
+{
+    if (!b) return VV_EINVAL;
+    vv_ctx* c = b->ctx;
+    if (timestep < 0 || timestep >= (int)b->steps.size()) return fail(c, VV_EINVAL, "timestep %d out of range", timestep);
+    if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
+    if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
+    BatchStep* st = b->steps[timestep];
+    const long long n = nvox(st->dims);
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // a running batch may read this field
+    const float* d0 = v0;
+    const float* d1 = v1 ? v1 : v0;
+    if (!on_device) {
+        for (long long i = 0; i < 3 * n; ++i)
+            if (!std::isfinite(v0[i]) || (v1 && !std::isfinite(v1[i])))
+                return fail(c, VV_EINVAL, "field contains a non-finite value");
+        const size_t bytes = sizeof(float) * 3 * (size_t)n;
+        vv_status s = upload(c, c->stage, v0, bytes, 0);
+        if (!s && v1) s = upload(c, c->stage2, v1, bytes, 0);
+        if (s) return s;
+        d0 = (const float*)c->stage.p;
+        d1 = v1 ? (const float*)c->stage2.p : d0;
+    }
+    const float t = v1 ? lerp_t : 0.0f;
+    hipError_t e = c->counter.ensure(64);
+    if (!e) e = launch_fill_u32((uint32_t*)c->counter.p, 0u, 1, c->stream);
+    if (!e) e = launch_field_maxlen(d0, d1, t, n, (uint32_t*)c->counter.p, c->stream);
+    if (!e) e = launch_field_encode(d0, d1, t, st->dims, (const uint32_t*)c->counter.p, st->field.p,
+                                    st->storage == VV_STORE_FP16, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (e) return fail(c, VV_EDEVICE, "timestep encode: %s", hipGetErrorString(e));
+    return VV_OK;
+}
+
 vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, float* image)
 {
     if (!b) return VV_EINVAL;

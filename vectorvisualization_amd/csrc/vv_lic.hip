@@ -500,12 +500,11 @@ __device__ __forceinline__ unsigned repeat_idx(float fl, int n)
 // offset of 32 bits, the 64-bit base in SGPRs), so arrays up to 4 GiB need
 // no 64-bit address arithmetic per lane.  (Raw buffer loads, which also take
 // a 32-bit VGPR offset, returned zeros for offsets of 2 GiB and more.)
-struct F3 {
-    float x, y, z;
-};
 // global address space: the batch kernel reads its array pointers from a
 // descriptor in memory, where the compiler cannot infer it (flat loads)
 typedef const __attribute__((address_space(1))) char* gchar;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) unsigned gu32;
 
 template <typename FT>
 struct FieldLoad;
@@ -516,15 +515,13 @@ struct FieldLoad<float4> {
     static constexpr unsigned kShift = 4;
     __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
     {
-        gchar g = (gchar)base;
-        const F3 a = *(const __attribute__((address_space(1))) F3*)(g + off);
-        const F3 b = *(const __attribute__((address_space(1))) F3*)(g + off + 16u);
-        t0[0] = a.x;
-        t0[1] = a.y;
-        t0[2] = a.z;
-        t1[0] = b.x;
-        t1[1] = b.y;
-        t1[2] = b.z;
+        const gfloat* q = (const gfloat*)((gchar)base + off);
+        t0[0] = q[0];
+        t0[1] = q[1];
+        t0[2] = q[2];
+        t1[0] = q[4];
+        t1[1] = q[5];
+        t1[2] = q[6];
     }
 };
 
@@ -534,8 +531,12 @@ struct FieldLoad<half4_t> {
     static constexpr unsigned kShift = 3;
     __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
     {
-        gchar g = (gchar)base;
-        const uint4 a = *(const __attribute__((address_space(1))) uint4*)(g + off);
+        const gu32* q = (const gu32*)((gchar)base + off);
+        uint4 a;
+        a.x = q[0];
+        a.y = q[1];
+        a.z = q[2];
+        a.w = q[3];
         t0[0] = __half2float(__ushort_as_half((unsigned short)(a.x & 0xffffu)));
         t0[1] = __half2float(__ushort_as_half((unsigned short)(a.x >> 16)));
         t0[2] = __half2float(__ushort_as_half((unsigned short)(a.y & 0xffffu)));
