@@ -71,8 +71,17 @@ typedef enum vv_noise_kind {
 
 typedef enum vv_illum {
     VV_ILLUM_LIC = 0,      /* illumLIC, VV/shader/inc_illum.glsl:158-172 */
-    VV_ILLUM_GRADIENT = 1  /* illumGradient, VV/shader/inc_illum.glsl:1-41 */
+    VV_ILLUM_GRADIENT = 1, /* illumGradient, VV/shader/inc_illum.glsl:1-41 */
+    VV_ILLUM_MALLO = 2,    /* illumMallo, VV/shader/inc_illum.glsl:46-113 */
+    VV_ILLUM_ZOECKLER = 3  /* illumZoeckler, VV/shader/inc_illum.glsl:117-154 */
 } vv_illum;
+
+/* Material of the line-illumination tables (LineMat, VV/illumination.h:40-62). */
+typedef struct vv_line_material {
+    float light_color[4], ambient[4], diffuse[4], specular[4];
+    float spec_exp;   /* 40 */
+    float diff_exp;   /* 2 */
+} vv_line_material;
 
 typedef enum vv_tf_index {
     VV_TF_INDEX_B = 0,    /* field .b, as both reference ray casters do */
@@ -246,6 +255,20 @@ vv_status vv_batch_view_latency(vv_batch* b, double* ms);
 /* total compositing samples of the last run (waits) */
 vv_status vv_batch_samples(vv_batch* b, uint64_t* total);
 vv_status vv_batch_download_lic(vv_batch* b, int timestep, float* out, int on_device);
+
+/* ---- line illumination (illum_mode VV_ILLUM_MALLO / VV_ILLUM_ZOECKLER) -----
+ * Both modes shade with 256x256 tables built from a line material: Zoeckler et al.
+ * (diffuse, specular) over (L.T, V.T), Mallo et al. diffuse and specular (Simpson rule,
+ * 20 intervals), computed in double and rounded to FP16 as the reference's float
+ * textures hold them (Illumination::createIllumTextures, VV/illumination.cpp:46-320;
+ * VV/3DLIC.cpp:735).  The tangent is the field direction at the sample. */
+void vv_default_line_material(vv_line_material* m);
+vv_status vv_set_line_material(vv_ctx* ctx, const vv_line_material* m);
+vv_status vv_get_line_material(const vv_ctx* ctx, vv_line_material* m);
+/* the tables of a material (host): zoeckler 256*256*2, mallo_diff and mallo_spec
+ * 256*256*3 floats, row y column x at y*256 + x */
+vv_status vv_make_illum_tables(const vv_line_material* m, float* zoeckler, float* mallo_diff,
+                               float* mallo_spec);
 
 /* ---- the reference's file formats -------------------------------------
  * Functions without a context report errors through vv_last_error(NULL). */

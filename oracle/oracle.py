@@ -49,7 +49,42 @@ class Params(C.Structure):
         ("light_specular", C.c_float),
         ("light_ambient", C.c_float),
         ("spec_exponent", C.c_int),
+        ("lut_zoeckler", C.c_void_p),
+        ("lut_mallo_diff", C.c_void_p),
+        ("lut_mallo_spec", C.c_void_p),
     ]
+
+
+class LineMat(C.Structure):
+    """orc_line_mat (LineMat, VV/illumination.h:40-62)."""
+
+    _fields_ = [
+        ("light_color", C.c_float * 4),
+        ("ambient", C.c_float * 4),
+        ("diffuse", C.c_float * 4),
+        ("specular", C.c_float * 4),
+        ("spec_exp", C.c_float),
+        ("diff_exp", C.c_float),
+    ]
+
+
+def line_mat(**kw) -> LineMat:
+    """The reference's LineMat defaults, overridden by keyword arguments."""
+    m = LineMat()
+    for i in range(4):
+        m.light_color[i] = 1.0
+        m.ambient[i] = 0.1
+        m.diffuse[i] = 0.5
+        m.specular[i] = 0.8
+    m.spec_exp = 40.0
+    m.diff_exp = 2.0
+    for k, v in kw.items():
+        if isinstance(getattr(m, k), (int, float)):
+            setattr(m, k, v)
+        else:
+            for i in range(4):
+                getattr(m, k)[i] = v[i]
+    return m
 
 
 class View(C.Structure):
@@ -82,6 +117,8 @@ def _load() -> C.CDLL:
     L = C.c_long
     sig = {
         "orc_f32_to_f16": (C.c_uint16, [F]),
+        "orc_illum_luts": (None, [C.POINTER(LineMat), P, P, P]),
+        "orc_mallo_spec_term": (D, [D, D, D]),
         "orc_f16_to_f32": (F, [C.c_uint16]),
         "orc_f32_to_f16_array": (None, [P, P, L]),
         "orc_f16_to_f32_array": (None, [P, P, L]),
@@ -331,3 +368,25 @@ def raycast_adjoint(field4, fdims, licv, lic_ch, ldims, tf_rgba, tf_opac, p: Par
                                 _p(tf_rgba), _p(tf_opac), _p(sigma), C.byref(p), C.byref(v),
                                 _p(dimage), _p(dtf), _p(dsig))
     return dtf, dsig
+
+
+def illum_luts(mat: LineMat | None = None):
+    """(zoeckler 256x256x2, mallo_diff 256x256x3, mallo_spec 256x256x3) float32."""
+    m = line_mat() if mat is None else mat
+    z = np.empty(256 * 256 * 2, np.float32)
+    md = np.empty(256 * 256 * 3, np.float32)
+    ms = np.empty(256 * 256 * 3, np.float32)
+    LIB.orc_illum_luts(C.byref(m), _p(z), _p(md), _p(ms))
+    return z, md, ms
+
+
+def attach_luts(p: Params, luts) -> None:
+    """Point p at illumination tables (the caller keeps `luts` alive)."""
+    z, md, ms = luts
+    p.lut_zoeckler = z.ctypes.data
+    p.lut_mallo_diff = md.ctypes.data
+    p.lut_mallo_spec = ms.ctypes.data
+
+
+def mallo_spec_term(alpha: float, beta: float, n: float) -> float:
+    return LIB.orc_mallo_spec_term(alpha, beta, n)

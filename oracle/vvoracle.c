@@ -379,6 +379,72 @@ void orc_light_position(const double center[3], const double cam_pos[3],
 /* ------------------------------------------------------------------ */
 /* Precision-dependent part, compiled twice.                          */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Line-illumination tables (VV/illumination.cpp:96-390).  The          */
+/* expressions keep the reference's types: material entries are float,  */
+/* so ambient*lightColor is a float product; the rest is double.        */
+/* ------------------------------------------------------------------ */
+static double mallo_integrand(double beta, double n, double theta)
+{
+    double y = cos(theta - beta);
+    if (y < 0.0) y = 0.0;
+    return pow(y, n) * (cos(theta) / 2.0);
+}
+
+/* composite Simpson rule over [alpha - pi/2, pi/2] with 2m = 20 intervals,
+ * summed in the reference's order: pairs 2 f(x_i) + 4 f(x_i+1) for even i,
+ * then + f(b) - f(a) (f(a) entered the loop with weight 2), times h/3 */
+double orc_mallo_spec_term(double alpha, double beta, double n)
+{
+    const double a = alpha - M_PI / 2.0, b = M_PI / 2.0;
+    const int m = 10;
+    const double h = (b - a) / (2.0 * m);
+    double sum = 0.0;
+    for (int i = 0; i < 2 * m; i += 2) {
+        sum += 2.0 * mallo_integrand(beta, n, a + i * h);
+        sum += 4.0 * mallo_integrand(beta, n, a + (i + 1) * h);
+    }
+    sum += mallo_integrand(beta, n, b);
+    sum -= mallo_integrand(beta, n, a);
+    return sum * (h / 3.0);
+}
+
+static float half_round(double v) { return orc_f16_to_f32(orc_f32_to_f16((float)v)); }
+static double clamp01d(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
+
+void orc_illum_luts(const orc_line_mat* m, float* zoeckler, float* mallo_diff, float* mallo_spec)
+{
+    const int W = 256, H = 256;
+    const double irx = 1.0 / (double)(W - 1), iry = 1.0 / (double)(H - 1);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            const long i = (long)y * W + x;
+            /* Zoeckler: texel (x, y) <-> (L.T, V.T) = (2x/(W-1) - 1, 2y/(H-1) - 1) */
+            const double lt = 2.0 * ((double)x * irx) - 1.0;
+            const double vt = 2.0 * ((double)y * iry) - 1.0;
+            double diffuse = pow(sqrt(1.0 - lt * lt), (double)m->diff_exp);
+            double dp = lt * vt - sqrt(1.0 - lt * lt) * sqrt(1.0 - vt * vt);
+            dp = dp < -1.0 ? -1.0 : (dp > 1.0 ? 1.0 : dp);
+            const float amb = m->ambient[0] * m->light_color[0];
+            double dterm = amb + diffuse * m->diffuse[0] * m->light_color[0];
+            double sterm = pow(fabs(dp), (double)m->spec_exp) * m->specular[0] * m->light_color[0];
+            dterm = clamp01d(dterm * 0.5);
+            sterm = clamp01d(sterm * 0.5);
+            zoeckler[2 * i] = half_round(dterm);
+            zoeckler[2 * i + 1] = half_round((double)((float)sterm * 0.9f));
+            /* Mallo: texel centres, alpha = acos(2s-1), beta = acos(2t-1), L.T = 2t-1 */
+            const double s = ((double)x + 0.5) / W, t = ((double)y + 0.5) / H;
+            const double alpha = acos(2.0 * s - 1.0), beta = acos(2.0 * t - 1.0);
+            const double l2 = 2.0 * t - 1.0;
+            const double fd = sqrt(1.0 - l2 * l2) * (sin(alpha) + (M_PI - alpha) * cos(alpha)) * 0.25;
+            const double fs = 3.5 * orc_mallo_spec_term(alpha, beta, (double)m->spec_exp);
+            for (int c = 0; c < 3; ++c) {
+                mallo_diff[3 * i + c] = half_round(clamp01d(fd * m->diffuse[c] * m->light_color[c]));
+                mallo_spec[3 * i + c] = half_round(clamp01d(fs * m->specular[c] * m->light_color[c]));
+            }
+        }
+}
+
 #define ORC_DOUBLE 0
 #include "vvoracle_impl.h"
 #undef ORC_DOUBLE

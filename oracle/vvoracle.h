@@ -56,7 +56,25 @@ typedef struct orc_params {
     float light[3];         /* gl_LightSource[0].position.xyz (eye space) */
     float light_diffuse, light_specular, light_ambient;
     int   spec_exponent;    /* GL_SPOT_EXPONENT = 40 */
+    /* line-illumination tables (orc_illum_luts), read by illum_mode 2 and 3 */
+    const float* lut_zoeckler;   /* 256x256 x (diffuse, specular) */
+    const float* lut_mallo_diff; /* 256x256 x rgb */
+    const float* lut_mallo_spec; /* 256x256 x rgb */
 } orc_params;
+
+/* Line material of the illumination tables (LineMat, VV/illumination.h:40-62). */
+typedef struct orc_line_mat {
+    float light_color[4], ambient[4], diffuse[4], specular[4];
+    float spec_exp, diff_exp;
+} orc_line_mat;
+
+/* Illumination textures: Zoeckler et al. (VV/illumination.cpp:96-183) and
+ * Mallo et al. (:188-320, Simpson rule :348-390), computed in double and
+ * rounded to FP16 as the reference's RGBA16F / LUMINANCE_ALPHA16F
+ * textures store them.  Row y, column x at index y*256 + x. */
+void orc_illum_luts(const orc_line_mat* m, float* zoeckler /* 256*256*2 */,
+                    float* mallo_diff /* 256*256*3 */, float* mallo_spec /* 256*256*3 */);
+double orc_mallo_spec_term(double alpha, double beta, double n);
 
 /* Per-frame view block: every float the per-pixel ray setup consumes.
  * Computed in double from the camera and rounded once to float. */

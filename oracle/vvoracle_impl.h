@@ -17,6 +17,8 @@
 #define SQRT sqrt
 #define FLOOR floor
 #define POW pow
+#define FMIN fmin
+#define FMAX fmax
 #else
 #define R float
 #define FN(n) n##_f32
@@ -24,6 +26,8 @@
 #define SQRT sqrtf
 #define FLOOR floorf
 #define POW powf
+#define FMIN fminf
+#define FMAX fmaxf
 #endif
 
 #define K(x) ((R)(x))
@@ -97,6 +101,22 @@ static inline R FN(lut1d)(const R* t, int n, int nc, int ch, R u)
     int i = FN(tcoord)(u, n, &f);
     int i0 = FN(icl)(i, n), i1 = FN(icl)(i + 1, n);
     return FN(lerp)(t[i0 * nc + ch], t[i1 * nc + ch], f);
+}
+
+/* 256x256 illumination table, LINEAR + CLAMP_TO_EDGE.  The coordinate is
+ * first limited to [0,1]: no change for finite values (the edge texels
+ * already saturate there), and +-inf / NaN map to an edge deterministically
+ * (fmax(NaN, 0) = 0). */
+static inline R FN(lut2d)(const float* t, int nc, int ch, R u, R v)
+{
+    u = FMIN(FMAX(u, K(0)), K(1));
+    v = FMIN(FMAX(v, K(0)), K(1));
+    R fx, fy;
+    int i = FN(tcoord)(u, 256, &fx), j = FN(tcoord)(v, 256, &fy);
+    int i0 = FN(icl)(i, 256), i1 = FN(icl)(i + 1, 256), j0 = FN(icl)(j, 256), j1 = FN(icl)(j + 1, 256);
+    R a = FN(lerp)((R)t[(j0 * 256 + i0) * nc + ch], (R)t[(j0 * 256 + i1) * nc + ch], fx);
+    R b = FN(lerp)((R)t[(j1 * 256 + i0) * nc + ch], (R)t[(j1 * 256 + i1) * nc + ch], fx);
+    return FN(lerp)(a, b, fy);
 }
 
 /* byte LUT variant (filter kernel): blend bytes, then scale by 1/255 */
@@ -390,7 +410,7 @@ typedef struct {
 typedef struct {
     R tfu;      /* TF coordinate */
     R opu;      /* opacity-table coordinate (1.3*l) */
-    R m;        /* d rgb / d tf.rgb */
+    R m[3];     /* d rgb / d tf.rgb, per channel */
     R spec;     /* additive rgb term (illumGradient) */
     R ao;       /* opacity table value */
     R sg;       /* opacity scale at the sample (1 without sigma) */
@@ -414,9 +434,70 @@ static void FN(rc_eval)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], FN(
     for (int ch = 0; ch < 4; ++ch) s->tf[ch] = FN(lut1d)(c->tf, 256, 4, ch, v);
     R g = (R)p->illum_scale;
     R l = c->lc == 1 ? l4[0] : l4[3];
-    if (p->illum_mode == 0 || c->lc == 1) {
+    if (p->illum_mode == 2 || p->illum_mode == 3) {
+        /* line illumination with the field direction as tangent
+         * (vectorData.xyz; tangent = normalize(2 t - 1)) */
+        FN(cell) cf;
+        FN(cell_clamp)(pos, c->fd, &cf);
+        R tg[3], T[3];
+        for (int a = 0; a < 3; ++a) {
+            R x;
+            ORC_TRILIN(R, c->field, 4, a, c->fd, cf, x);
+            tg[a] = FMA(K(2), x, K(-1));
+        }
+        FN(normalize3)(tg, T);
+        const R* L = c->light;
+        if (p->illum_mode == 3) {
+            /* illumZoeckler (inc_illum.glsl:117-154).  The table is
+             * LUMINANCE_ALPHA, so .rg reads (L, L): both shading terms are the
+             * luminance (diffuse) channel. */
+            R V[3];
+            FN(normalize3)(dir, V);
+            R u = FMA(K(0.5), L[0] * T[0] + L[1] * T[1] + L[2] * T[2], K(0.5));
+            R w = FMA(K(0.5), V[0] * T[0] + V[1] * T[1] + V[2] * T[2], K(0.5));
+            R lum = FN(lut2d)(p->lut_zoeckler, 2, 0, u, w);
+            R shr = lum * g;
+            R sp = K(0.9) * lum;
+            for (int a = 0; a < 3; ++a) {
+                s->rgb[a] = (s->tf[a] * l) * shr + sp;
+                s->m[a] = l * shr;
+            }
+            s->spec = sp;
+        } else {
+            /* illumMallo (inc_illum.glsl:46-113) */
+            R vd[3] = {-dir[0], -dir[1], -dir[2]}, V[3], bn[3], Bv[3], N[3], hv[3], Hh[3];
+            FN(normalize3)(vd, V);
+            bn[0] = T[1] * V[2] - T[2] * V[1];
+            bn[1] = T[2] * V[0] - T[0] * V[2];
+            bn[2] = T[0] * V[1] - T[1] * V[0];
+            FN(normalize3)(bn, Bv);
+            N[0] = Bv[1] * T[2] - Bv[2] * T[1];
+            N[1] = Bv[2] * T[0] - Bv[0] * T[2];
+            N[2] = Bv[0] * T[1] - Bv[1] * T[0];
+            for (int a = 0; a < 3; ++a) hv[a] = V[a] + L[a];
+            FN(normalize3)(hv, Hh);
+            R lx = L[0] * N[0] + L[1] * N[1] + L[2] * N[2];
+            R ly = L[0] * T[0] + L[1] * T[1] + L[2] * T[2];
+            R lz = Hh[0] * N[0] + Hh[1] * N[1] + Hh[2] * N[2];
+            R lw = Hh[0] * T[0] + Hh[1] * T[1] + Hh[2] * T[2];
+            R ql = SQRT(FMAX(K(1) - ly * ly, K(0))), qh = SQRT(FMAX(K(1) - lw * lw, K(0)));
+            R tx = K(1) / ql, ty = K(1) / qh;
+            R ud = FMA(K(0.5), lx * tx, K(0.5));
+            R vdc = FMA(K(0.5), ly, K(0.5));
+            R vs = FMA(K(0.5), lz * ty, K(0.5));
+            /* pow(tmp.y, -spotExponent) = (1 - (H.T)^2)^(n/2) */
+            R att = FN(ipow)(qh, p->spec_exponent);
+            for (int a = 0; a < 3; ++a) {
+                R dif = FN(lut2d)(p->lut_mallo_diff, 3, a, ud, vdc) * g;
+                R spc = FN(clamp01)(FN(lut2d)(p->lut_mallo_spec, 3, a, ud, vs) * att);
+                s->rgb[a] = (s->tf[a] * l) * dif + spc;
+                s->m[a] = l * dif;
+            }
+            s->spec = K(0);
+        }
+    } else if (p->illum_mode == 0 || c->lc == 1) {
         for (int a = 0; a < 3; ++a) s->rgb[a] = l * s->tf[a] * g;
-        s->m = l * g;
+        s->m[0] = s->m[1] = s->m[2] = l * g;
         s->spec = K(0);
     } else {
         R nn[3] = {-l4[0], -l4[1], -l4[2]}, N[3], V[3], Rv[3], rr[3];
@@ -436,7 +517,7 @@ static void FN(rc_eval)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], FN(
             R col = s->tf[a] * l;
             s->rgb[a] = FMA(col, kk, sp) * g;
         }
-        s->m = l * kk * g;
+        s->m[0] = s->m[1] = s->m[2] = l * kk * g;
         s->spec = sp * g;
     }
     s->opu = l * K(1.3);
@@ -600,7 +681,7 @@ void orc_raycast_adjoint_f64(const double* field4, const int fdims[3],
                 else if (b > 0.0) dadraw = cc * pow(b, cc - 1.0);
                 else dadraw = 0.0;
                 double draw = s->aclamp ? 0.0 : da * dadraw;
-                for (int q = 0; q < 3; ++q) orc_splat_lin(dtf, 256, 4, q, s->tfu, drgb[q] * s->m);
+                for (int q = 0; q < 3; ++q) orc_splat_lin(dtf, 256, 4, q, s->tfu, drgb[q] * s->m[q]);
                 orc_splat_lin(dtf, 256, 4, 3, s->tfu, draw * s->ao * s->sg);
                 if (dsigma) {
                     double dsg = draw * s->ao * s->tf[3];
@@ -625,6 +706,8 @@ void orc_raycast_adjoint_f64(const double* field4, const int fdims[3],
 #undef SQRT
 #undef FLOOR
 #undef POW
+#undef FMIN
+#undef FMAX
 #undef K
 #undef ORC_IDX
 #undef ORC_TRILIN

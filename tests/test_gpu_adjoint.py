@@ -16,11 +16,21 @@ from workloads import lic_setup
 pytestmark = pytest.mark.gpu
 
 
+_LUTS = None
+
+
 def _scene(ctx, n=16, grad=False, step=1.0 / 128.0, W=48, H=40, quat=(0.2, -0.1, 0.05, 0.97),
-           sigma_seed=None, tf_seed=3, op_scale=1.0):
+           sigma_seed=None, tf_seed=3, op_scale=1.0, illum=None):
+    global _LUTS
     c = lic_setup(ctx, n=n, kind="rankine", grad=grad, L=6, density=0.3)
     p = c["params"]
     p.step_size_vol = step
+    if illum is not None:
+        p.illum_mode = illum
+        c["op"].illum_mode = illum
+        if _LUTS is None:
+            _LUTS = orc.illum_luts()
+        orc.attach_luts(c["op"], _LUTS)
     ctx.set_lic_params(p)
     ctx.compute_lic()
     ext, sc, cen = ctx.field_geometry()
@@ -43,11 +53,14 @@ def _scene(ctx, n=16, grad=False, step=1.0 / 128.0, W=48, H=40, quat=(0.2, -0.1,
     return c, cam, rgba, op, sigma, oparams, view
 
 
-@pytest.mark.parametrize("grad,step,with_sigma", [(False, 1.0 / 128.0, True), (True, 1.0 / 256.0, True),
-                                                  (False, 1.0 / 256.0, False)])
-def test_adjoint_matches_oracle(ctx, grad, step, with_sigma):
+@pytest.mark.parametrize("grad,step,with_sigma,illum", [(False, 1.0 / 128.0, True, None),
+                                                        (True, 1.0 / 256.0, True, None),
+                                                        (False, 1.0 / 256.0, False, None),
+                                                        (False, 1.0 / 128.0, True, vv.ILLUM_MALLO),
+                                                        (True, 1.0 / 256.0, True, vv.ILLUM_ZOECKLER)])
+def test_adjoint_matches_oracle(ctx, grad, step, with_sigma, illum):
     c, cam, rgba, op, sigma, oparams, view = _scene(ctx, grad=grad, step=step,
-                                                    sigma_seed=7 if with_sigma else None)
+                                                    sigma_seed=7 if with_sigma else None, illum=illum)
     W, H = cam.width, cam.height
     dimg = np.random.default_rng(11).normal(size=W * H * 4).astype(np.float32)
     dtf, dsig = ctx.render_adjoint(cam, dimg)

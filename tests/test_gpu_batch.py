@@ -177,3 +177,15 @@ def test_batch_errors(ctx):
     with pytest.raises(ValueError):
         b.add_view(0, vv.default_camera(8, 8), np.zeros(256, np.float32))
     b.close()
+
+
+def test_batch_line_illumination_matches_single(ctx):
+    _setup(ctx)
+    p = ctx.get_lic_params()
+    p.illum_mode = vv.ILLUM_ZOECKLER
+    ctx.set_lic_params(p)
+    lics, centers, imgs, total = _single(ctx, vv.STORE_FP32)
+    b, outs = _batch(ctx, vv.STORE_FP32, centers)
+    b.run()
+    _check(b, outs, lics, imgs, total)
+    b.close()

@@ -160,16 +160,21 @@ def test_lic_deterministic(ctx):
 
 # ---- a7-a10: ray cast ----------------------------------------------------
 def _render_case(ctx, c, w, h, tf_index=vv.TF_INDEX_B, step=1.0 / 128.0, quat=(0, 0, 0, 1),
-                 tf=None, sigma=None, illum_scale=1.0):
+                 tf=None, sigma=None, illum_scale=1.0, illum=None, light=None, luts=None):
     p = c["params"]
     p.tf_index = tf_index
     p.step_size_vol = step
     p.illum_scale = illum_scale
+    if illum is not None:
+        p.illum_mode = illum
     ctx.set_lic_params(p)
     ext, sc, cen = ctx.field_geometry()
     cam = vv.default_camera(w, h, cen)
     for i in range(4):
         cam.quat[i] = quat[i]
+    if light is not None:
+        for i in range(3):
+            cam.light_pos[i] = light[i]
     if tf is None:
         rgba, op = vv.fields.tf_linear_ramp()
     else:
@@ -190,9 +195,42 @@ def _render_case(ctx, c, w, h, tf_index=vv.TF_INDEX_B, step=1.0 / 128.0, quat=(0
     oparams.illum_scale = illum_scale
     lp = list(cam.light_pos)
     oparams.light[0], oparams.light[1], oparams.light[2] = lp
+    if illum is not None:
+        oparams.illum_mode = illum
+    if luts is not None:
+        orc.attach_luts(oparams, luts)
     ref, ns, total = orc.raycast(c["f4"], c["dims"], licv, lch, ldims, rgba, op, oparams, view,
                                  sigma=sigma)
     return img, ref, nsamp, total
+
+
+@pytest.mark.parametrize("illum", [vv.ILLUM_MALLO, vv.ILLUM_ZOECKLER])
+@pytest.mark.parametrize("grad", [False, True])
+def test_render_line_illumination(ctx, illum, grad):
+    c = lic_setup(ctx, n=24, kind="rankine", L=8, grad=grad, density=0.3)
+    ctx.compute_lic()
+    luts = orc.illum_luts()
+    img, ref, nsamp, total = _render_case(ctx, c, 64, 56, quat=(0.2, 0.3, 0.1, 0.927), illum=illum,
+                                          light=(0.9, 1.4, 2.2), luts=luts, tf_index=vv.TF_INDEX_MAG,
+                                          illum_scale=1.3)
+    _report(f"render-illum{illum}", img, ref)
+    assert nsamp == total
+    assert np.max(ref) > 0.01
+    assert rel_err(img, ref) <= TOL
+
+
+def test_render_line_material(ctx):
+    c = lic_setup(ctx, n=20, kind="abc", L=6)
+    ctx.compute_lic()
+    m = vv.default_line_material()
+    m.diffuse[0], m.diffuse[2] = 0.9, 0.2
+    m.spec_exp = 8.0
+    ctx.set_line_material(m)
+    om = orc.line_mat(diffuse=(0.9, 0.5, 0.2, 0.5), spec_exp=8.0)
+    luts = orc.illum_luts(om)
+    img, ref, _, _ = _render_case(ctx, c, 48, 48, illum=vv.ILLUM_MALLO, luts=luts)
+    assert rel_err(img, ref) <= TOL
+    assert ctx.get_line_material().spec_exp == 8.0
 
 
 @pytest.mark.parametrize("tf_index", [vv.TF_INDEX_B, vv.TF_INDEX_MAG])

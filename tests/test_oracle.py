@@ -430,7 +430,8 @@ def test_raycast_f32_close_to_f64():
 
 
 # ---- adjoint ----------------------------------------------------------------
-@pytest.mark.parametrize("lic_ch,illum,step", [(1, 0, 1.0 / 128.0), (4, 1, 1.0 / 256.0)])
+@pytest.mark.parametrize("lic_ch,illum,step", [(1, 0, 1.0 / 128.0), (4, 1, 1.0 / 256.0),
+                                              (1, 2, 1.0 / 128.0), (4, 3, 1.0 / 200.0)])
 def test_adjoint_matches_central_differences(lic_ch, illum, step):
     n = 8
     rng = np.random.default_rng(6)
@@ -440,7 +441,9 @@ def test_adjoint_matches_central_differences(lic_ch, illum, step):
     rgba = rng.uniform(0.05, 0.9, 1024)
     op = rng.uniform(0.01, 0.08, 256)
     sigma = rng.uniform(0.5, 1.5, n ** 3)
-    p = orc.params(step_size_vol=step, illum_mode=illum)
+    p = orc.params(step_size_vol=step, illum_mode=illum, light=(0.3, 0.8, 1.2))
+    luts = orc.illum_luts()
+    orc.attach_luts(p, luts)
     view = orc.view_setup(14, 12, quat=(0.2, -0.1, 0.05, 0.97))
     W = rng.normal(size=14 * 12 * 4)
 

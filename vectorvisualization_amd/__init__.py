@@ -29,6 +29,10 @@ from ._lib import (  # noqa: F401
     NOISE_UNIFORM,
     ILLUM_GRADIENT,
     ILLUM_LIC,
+    ILLUM_MALLO,
+    ILLUM_ZOECKLER,
+    LineMaterial,
+    default_line_material,
     TF_INDEX_B,
     TF_INDEX_MAG,
 )

@@ -69,7 +69,21 @@ class VVError(RuntimeError):
         self.status = status
 
 
+ILLUM_MALLO, ILLUM_ZOECKLER = 2, 3
 RAW_NONE, RAW_UCHAR, RAW_USHORT, RAW_FLOAT = range(4)
+
+
+class LineMaterial(C.Structure):
+    """vv_line_material (LineMat, VV/illumination.h:40-62)."""
+
+    _fields_ = [
+        ("light_color", C.c_float * 4),
+        ("ambient", C.c_float * 4),
+        ("diffuse", C.c_float * 4),
+        ("specular", C.c_float * 4),
+        ("spec_exp", C.c_float),
+        ("diff_exp", C.c_float),
+    ]
 _RAW_DTYPE = {RAW_UCHAR: np.uint8, RAW_USHORT: np.uint16, RAW_FLOAT: np.float32}
 
 
@@ -170,6 +184,10 @@ def _load() -> C.CDLL:
         "vv_anim_next": (I, [P, IP, FP]),
         "vv_anim_seek": (I, [P, C.c_longlong]),
         "vv_anim_destroy": (I, [P]),
+        "vv_default_line_material": (None, [C.POINTER(LineMaterial)]),
+        "vv_set_line_material": (I, [P, C.POINTER(LineMaterial)]),
+        "vv_get_line_material": (I, [P, C.POINTER(LineMaterial)]),
+        "vv_make_illum_tables": (I, [C.POINTER(LineMaterial), FP, FP, FP]),
         "vv_set_tuning": (I, [P, C.c_char_p, I]),
         "vv_get_tuning": (I, [P, C.c_char_p, IP]),
     }
@@ -199,6 +217,8 @@ EXPORTED = [
     "vv_load_noise_file", "vv_png_read", "vv_png_write", "vv_image_write_png",
     "vv_load_filter_png", "vv_tf_load_png", "vv_tf_save_png", "vv_batch_set_field",
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
+    "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
+    "vv_make_illum_tables",
 ]
 
 
@@ -289,6 +309,24 @@ def default_tf_u8() -> np.ndarray:
     t = np.zeros(256 * 5, np.uint8)
     LIB.vv_default_tf_u8(t.ctypes.data)
     return t
+
+
+def default_line_material() -> LineMaterial:
+    m = LineMaterial()
+    LIB.vv_default_line_material(C.byref(m))
+    return m
+
+
+def make_illum_tables(mat: LineMaterial | None = None):
+    """(zoeckler 256x256x2, mallo_diff 256x256x3, mallo_spec 256x256x3) float32."""
+    m = default_line_material() if mat is None else mat
+    z = np.empty(256 * 256 * 2, np.float32)
+    md = np.empty(256 * 256 * 3, np.float32)
+    ms = np.empty(256 * 256 * 3, np.float32)
+    fp = C.POINTER(C.c_float)
+    _io_chk(LIB.vv_make_illum_tables(C.byref(m), z.ctypes.data_as(fp), md.ctypes.data_as(fp),
+                                     ms.ctypes.data_as(fp)))
+    return z, md, ms
 
 
 def _io_chk(st: int) -> None:
@@ -442,6 +480,14 @@ class Context:
         with self.ordered(v0, v1):
             self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
                                           int(storage), dev))
+
+    def set_line_material(self, m: LineMaterial) -> None:
+        self._chk(LIB.vv_set_line_material(self._h, C.byref(m)))
+
+    def get_line_material(self) -> LineMaterial:
+        m = LineMaterial()
+        self._chk(LIB.vv_get_line_material(self._h, C.byref(m)))
+        return m
 
     def load_field_dat(self, path, timestep: int | None = None, next_timestep: int | None = None,
                        lerp_t: float = 0.0, storage: int = STORE_FP32) -> None:

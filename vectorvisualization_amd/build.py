@@ -43,7 +43,7 @@ DEVICE = [
     "-munsafe-fp-atomics",
 ]
 
-SOURCES = ["vv_lic.hip", "vv_render.hip", "vv_api.cpp", "vv_batch.cpp", "vv_io.cpp", "vv_anim.cpp"]
+SOURCES = ["vv_lic.hip", "vv_render.hip", "vv_api.cpp", "vv_batch.cpp", "vv_io.cpp", "vv_anim.cpp", "vv_illum.cpp"]
 
 
 def _compile(src: str) -> Path:

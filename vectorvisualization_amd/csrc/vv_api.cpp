@@ -37,6 +37,7 @@ vv_status vv_create(vv_ctx** out, int device)
         return fail(nullptr, VV_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     default_params(&c->params);
+    vv_default_line_material(&c->line_mat);
     uint8_t t5[256 * 5];
     default_tf5(t5);
     tf5_to_float(t5, c->tf);

@@ -128,7 +128,14 @@ vv_status build_view_launch(vv_batch* b, const BatchView& v, RenderLaunch& R)
     R.alpha_corr = p.step_size_vol * 128.0f;
     R.corr_mode = R.alpha_corr == 1.0f ? 0 : (R.alpha_corr == 0.5f ? 1 : (R.alpha_corr == 2.0f ? 2 : 3));
     R.max_samples = (long long)p.num_iterations * (long long)p.num_iterations;
-    R.illum = VV_ILLUM_LIC;
+    // batch LIC volumes are scalar: illumGradient falls back to illumLIC as
+    // for a scalar volume in vv_render; the line modes need their tables
+    R.illum = p.illum_mode;
+    if (p.illum_mode >= VV_ILLUM_MALLO) {
+        vv_status st = ensure_illum_luts(c);
+        if (st) return st;
+        illum_lut_ptrs(c, R);
+    }
     R.illum_scale = p.illum_scale;
     const float l0 = v.cam.light_pos[0], l1 = v.cam.light_pos[1], l2 = v.cam.light_pos[2];
     const float ld = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);

@@ -100,6 +100,11 @@ struct vv_ctx {
     int adj_window = 1;
     int render_block[2] = {16, 16};
 
+    // line-illumination tables (built on first use of illum_mode 2 / 3)
+    vv_line_material line_mat;
+    bool illum_dirty = true;
+    DevBuf illum_luts;  // zoeckler (2 ch), mallo diffuse (3 ch), mallo specular (3 ch)
+
     // opacity scale (CLAMP layout, LIC grid)
     bool has_sigma = false;
     DevBuf sigma;
@@ -315,7 +320,7 @@ inline vv_status check_params(vv_ctx* c, const vv_lic_params* p)
                         p->eye_factor, p->gate_lo, p->gate_hi};
     for (float f : fs)
         if (!std::isfinite(f)) return fail(c, VV_EINVAL, "non-finite parameter");
-    if (p->illum_mode != VV_ILLUM_LIC && p->illum_mode != VV_ILLUM_GRADIENT)
+    if (p->illum_mode < VV_ILLUM_LIC || p->illum_mode > VV_ILLUM_ZOECKLER)
         return fail(c, VV_EINVAL, "illum_mode invalid");
     if (p->tf_index != VV_TF_INDEX_B && p->tf_index != VV_TF_INDEX_MAG)
         return fail(c, VV_EINVAL, "tf_index invalid");
@@ -346,6 +351,17 @@ inline vv_status ensure_weights(vv_ctx* c)
     return VV_OK;
 }
 
+// the line-illumination tables of c->line_mat, on the device
+vv_status ensure_illum_luts(vv_ctx* c);
+
+inline void illum_lut_ptrs(vv_ctx* c, RenderLaunch& R)
+{
+    const float* base = (const float*)c->illum_luts.p;
+    R.lut_zoeckler = base;
+    R.lut_mallo_diff = base + 256 * 256 * 2;
+    R.lut_mallo_spec = base + 256 * 256 * 5;
+}
+
 inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunch& R)
 {
     if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
@@ -370,6 +386,11 @@ inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunc
     R.corr_mode = R.alpha_corr == 1.0f ? 0 : (R.alpha_corr == 0.5f ? 1 : (R.alpha_corr == 2.0f ? 2 : 3));
     R.max_samples = (long long)p.num_iterations * (long long)p.num_iterations;
     R.illum = p.illum_mode;
+    if (p.illum_mode >= VV_ILLUM_MALLO) {
+        st = ensure_illum_luts(c);
+        if (st) return st;
+        illum_lut_ptrs(c, R);
+    }
     R.illum_scale = p.illum_scale;
     // L = normalize(gl_LightSource[0].position.xyz) (scaleVolInv is never uploaded)
     const float l0 = cam->light_pos[0], l1 = cam->light_pos[1], l2 = cam->light_pos[2];

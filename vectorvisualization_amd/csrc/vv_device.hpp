@@ -220,6 +220,25 @@ __device__ __forceinline__ float lut1d(const float* t, int n, int nc, int ch, fl
     return lerpf(t[i0 * nc + ch], t[i1 * nc + ch], f);
 }
 
+// 2D LINEAR + CLAMP_TO_EDGE lookup in a 256x256 table (row v, column u) with
+// nc channels.  The coordinate is first limited to [0,1]: no change for
+// finite values (the edge texels saturate there), and +-inf / NaN go to an
+// edge deterministically (fmaxf(NaN, 0) = 0).
+__device__ __forceinline__ float lut2d(const float* t, int nc, int ch, float u, float v)
+{
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    v = fminf(fmaxf(v, 0.0f), 1.0f);
+    const float x = fmaf(u, 256.0f, -0.5f), y = fmaf(v, 256.0f, -0.5f);
+    const float flx = floorf(x), fly = floorf(y);
+    const float fx = x - flx, fy = y - fly;
+    const int i = (int)flx, j = (int)fly;  // -1 .. 255
+    const int i0 = i < 0 ? 0 : i, i1 = i + 1 > 255 ? 255 : i + 1;
+    const int j0 = j < 0 ? 0 : j, j1 = j + 1 > 255 ? 255 : j + 1;
+    const float a = lerpf(t[(j0 * 256 + i0) * nc + ch], t[(j0 * 256 + i1) * nc + ch], fx);
+    const float b = lerpf(t[(j1 * 256 + i0) * nc + ch], t[(j1 * 256 + i1) * nc + ch], fx);
+    return lerpf(a, b, fy);
+}
+
 constexpr float kInv255 = (float)(1.0 / 255.0);
 
 }  // namespace vv

@@ -78,7 +78,10 @@ struct RenderLaunch {
     float alpha_corr;
     int corr_mode;     // 0: c==1, 1: sqrt, 2: square, 3: powf
     long long max_samples;
-    int illum;         // 0 illumLIC, 1 illumGradient
+    int illum;         // 0 illumLIC, 1 illumGradient, 2 illumMallo, 3 illumZoeckler
+    const float* lut_zoeckler;    // 256x256 x 2 (device), illum 3
+    const float* lut_mallo_diff;  // 256x256 x 3 (device), illum 2
+    const float* lut_mallo_spec;  // 256x256 x 3 (device), illum 2
     float illum_scale;
     float light[3];    // normalised light direction
     float light_diffuse, light_specular, light_ambient;

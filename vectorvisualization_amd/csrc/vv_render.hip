@@ -23,7 +23,7 @@ constexpr int kSeg = 64;  // adjoint checkpoint interval (samples)
 struct Sample {
     float tfu;     // transfer-function coordinate
     float l;       // LIC intensity
-    float m;       // d rgb / d tf.rgb
+    float m[3];    // d rgb / d tf.rgb, per channel
     float ao;      // opacity-table value
     float sg;      // opacity scale at the sample
     float araw;    // ao * tf.a * sg
@@ -146,10 +146,64 @@ __device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* 
     const float g = a.illum_scale;
     const float l = LC == 1 ? l4[0] : l4[3];
     s.l = l;
-    if (LC == 1 || a.illum == 0) {
+    if (a.illum >= 2) {
+        // line illumination with the field direction as tangent
+        // (vectorData.xyz; tangent = normalize(2 t - 1))
+        const float3 fr = tri_rgb(fv, cf);
+        const float tg[3] = {fmaf(2.0f, fr.x, -1.0f), fmaf(2.0f, fr.y, -1.0f), fmaf(2.0f, fr.z, -1.0f)};
+        float T[3];
+        normalize3(tg, T);
+        const float* L = a.light;
+        if (a.illum == 3) {
+            // illumZoeckler (inc_illum.glsl:117-154); the table is LUMINANCE_ALPHA,
+            // so .rg reads (L, L): both shading terms are the luminance channel
+            float V[3];
+            normalize3(r.dir, V);
+            const float u = fmaf(0.5f, L[0] * T[0] + L[1] * T[1] + L[2] * T[2], 0.5f);
+            const float w = fmaf(0.5f, V[0] * T[0] + V[1] * T[1] + V[2] * T[2], 0.5f);
+            const float lum = lut2d(a.lut_zoeckler, 2, 0, u, w);
+            const float shr = lum * g;
+            const float sp = 0.9f * lum;
+            for (int c = 0; c < 3; ++c) {
+                s.rgb[c] = (t4[c] * l) * shr + sp;
+                s.m[c] = l * shr;
+            }
+        } else {
+            // illumMallo (inc_illum.glsl:46-113)
+            const float vd[3] = {-r.dir[0], -r.dir[1], -r.dir[2]};
+            float V[3], bn[3], Bv[3], N[3], hv[3], H[3];
+            normalize3(vd, V);
+            bn[0] = T[1] * V[2] - T[2] * V[1];
+            bn[1] = T[2] * V[0] - T[0] * V[2];
+            bn[2] = T[0] * V[1] - T[1] * V[0];
+            normalize3(bn, Bv);
+            N[0] = Bv[1] * T[2] - Bv[2] * T[1];
+            N[1] = Bv[2] * T[0] - Bv[0] * T[2];
+            N[2] = Bv[0] * T[1] - Bv[1] * T[0];
+            for (int c = 0; c < 3; ++c) hv[c] = V[c] + L[c];
+            normalize3(hv, H);
+            const float lx = L[0] * N[0] + L[1] * N[1] + L[2] * N[2];
+            const float ly = L[0] * T[0] + L[1] * T[1] + L[2] * T[2];
+            const float lz = H[0] * N[0] + H[1] * N[1] + H[2] * N[2];
+            const float lw = H[0] * T[0] + H[1] * T[1] + H[2] * T[2];
+            const float ql = sqrtf(fmaxf(1.0f - ly * ly, 0.0f)), qh = sqrtf(fmaxf(1.0f - lw * lw, 0.0f));
+            const float tx = 1.0f / ql, ty = 1.0f / qh;
+            const float ud = fmaf(0.5f, lx * tx, 0.5f);
+            const float vdc = fmaf(0.5f, ly, 0.5f);
+            const float vs = fmaf(0.5f, lz * ty, 0.5f);
+            // pow(tmp.y, -spotExponent) = (1 - (H.T)^2)^(n/2)
+            const float att = ipow(qh, a.spec_exp);
+            for (int c = 0; c < 3; ++c) {
+                const float dif = lut2d(a.lut_mallo_diff, 3, c, ud, vdc) * g;
+                const float spc = clamp01(lut2d(a.lut_mallo_spec, 3, c, ud, vs) * att);
+                s.rgb[c] = (t4[c] * l) * dif + spc;
+                s.m[c] = l * dif;
+            }
+        }
+    } else if (LC == 1 || a.illum == 0) {
         // illumLIC: rgb = illum * tf.rgb * gradient.g
         for (int c = 0; c < 3; ++c) s.rgb[c] = l * t4[c] * g;
-        s.m = l * g;
+        s.m[0] = s.m[1] = s.m[2] = l * g;
     } else {
         // illumGradient: N = normalize(-I.xyz), L = normalize(light) (scaleVolInv = 0)
         const float nn[3] = {-l4[0], -l4[1], -l4[2]};
@@ -167,7 +221,7 @@ __device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* 
             const float col = t4[c] * l;
             s.rgb[c] = fmaf(col, kk, sp) * g;
         }
-        s.m = l * kk * g;
+        s.m[0] = s.m[1] = s.m[2] = l * kk * g;
     }
     s.ao = lut1d(op, 256, 1, 0, l * 1.3f);
     float al = s.ao * t4[3];
@@ -505,7 +559,7 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
                     else
                         dadraw = 0.0f;
                     const float draw = s.aclamp ? 0.0f : da * dadraw;
-                    for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m, A.fx_scale);
+                    for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m[q], A.fx_scale);
                     splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, A.fx_scale);
                     if (A.dsigma_fx) {
                         const float dsg = draw * s.ao * s.tfa;
