@@ -210,6 +210,15 @@ vv_status vv_upload_lic(vv_ctx* ctx, const float* lic, int channels, const int d
 vv_status vv_render(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
                     uint64_t* samples_out);
 
+/* Renderer::raycastVolume with lic3d_fragment.glsl:5-99 (VOLIC_RAYCAST, the reference's
+ * default mode, VV/renderer.cpp:34,184-185): no LIC volume; every ray sample computes the
+ * LIC of its position on the fly, then shades and composites it as vv_render does.  A ray
+ * stops when it leaves the box or when the sample's own alpha exceeds 0.95 (src.a, :91),
+ * not the accumulated alpha.  Needs a field and noise; uses the LIC parameters, filter,
+ * TF, illumination mode and (when set) the opacity scale. */
+vv_status vv_render_fused(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
+                          uint64_t* samples);
+
 /* ---- differentiation ---------------------------------------------------
  * Per-voxel opacity scale sigma (N float32 at the LIC-volume resolution,
  * host or device) multiplies each sample's alpha before opacity

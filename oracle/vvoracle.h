@@ -162,6 +162,25 @@ long orc_raycast_f64(const double* field4, const int fdims[3],
                      const double* sigma, const orc_params* p, const orc_view* v,
                      int nthreads, double* image, int* nsamples);
 
+/* Fused ray caster (lic3d_fragment.glsl:5-99): the LIC of every sample's
+ * position computed on the fly; sigma (optional) on a grid of sigma_dims. */
+long orc_raycast_fused_f32(const float* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const float* tf_rgba, const float* tf_opac,
+                           const float* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, float* image, int* nsamples);
+long orc_raycast_fused_f64(const double* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const double* tf_rgba, const double* tf_opac,
+                           const double* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, double* image, int* nsamples);
+
 /* Reverse-mode derivative of orc_raycast_* with respect to tf_rgba (256x4)
  * and the per-voxel opacity scale sigma (ldims voxels), for a loss whose
  * image gradient is dimage (width*height*4).  Stores every ray's samples,

@@ -421,15 +421,14 @@ typedef struct {
     int aclamp; /* the opacity-scale clamp was active */
 } FN(rc_sample);
 
-static void FN(rc_eval)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], FN(rc_sample)* s)
+/* shading and opacity of one sample from its LIC value(s) l4 (c->lc channels) */
+static void FN(rc_shade)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], const R l4[4], FN(rc_sample)* s)
 {
     const orc_params* p = c->p;
     FN(cell) cl;
-    R v, l4[4];
+    R v;
     FN(cell_clamp)(pos, c->fd, &cl);
     ORC_TRILIN(R, c->field, 4, p->tf_index ? 3 : 2, c->fd, cl, v);
-    FN(cell_repeat)(pos, c->ld, &cl);
-    for (int ch = 0; ch < c->lc; ++ch) ORC_TRILIN(R, c->lic, c->lc, ch, c->ld, cl, l4[ch]);
     s->tfu = v;
     for (int ch = 0; ch < 4; ++ch) s->tf[ch] = FN(lut1d)(c->tf, 256, 4, ch, v);
     R g = (R)p->illum_scale;
@@ -537,6 +536,16 @@ static void FN(rc_eval)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], FN(
     s->a = K(1) - FN(pow_corr)(K(1) - a, (R)p->alpha_correction);
 }
 
+/* one sample of the LIC-volume ray caster: LIC value(s) from the volume */
+static void FN(rc_eval)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], FN(rc_sample)* s)
+{
+    FN(cell) cl;
+    R l4[4] = {0, 0, 0, 0};
+    FN(cell_repeat)(pos, c->ld, &cl);
+    for (int ch = 0; ch < c->lc; ++ch) ORC_TRILIN(R, c->lic, c->lc, ch, c->ld, cl, l4[ch]);
+    FN(rc_shade)(c, pos, dir, l4, s);
+}
+
 static inline int FN(outside)(const R pos[3], const orc_view* v)
 {
     for (int a = 0; a < 3; ++a)
@@ -584,6 +593,68 @@ long FN(orc_raycast)(const R* field4, const int fdims[3],
             if (nsamples) nsamples[pix] = ns;
             total += ns;
         }
+    return total;
+}
+
+/* ------------------------------------------------------------------ */
+/* Fused ray caster: lic3d_fragment.glsl:5-99.  Every sample computes   */
+/* the LIC of its position (computeLIC, scaled as the LIC volume is),   */
+/* then shades it as rc_shade; a ray stops when it leaves the box or    */
+/* when the sample's own alpha exceeds 0.95 (src.a, :91).               */
+/* ------------------------------------------------------------------ */
+long FN(orc_raycast_fused)(const R* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const R* tf_rgba, const R* tf_opac,
+                           const R* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, R* image, int* nsamples)
+{
+    FN(lic_ctx) lc;
+    lc.field = field4;
+    for (int a = 0; a < 3; ++a) { lc.fd[a] = fdims[a]; lc.nd[a] = ndims[a]; lc.sd[a] = sdims ? sdims[a] : 1; }
+    lc.noise = noise; lc.nc = noise_ch; lc.scalar = scalar; lc.p = p;
+    const int nw = 1 + p->steps_forward + p->steps_backward;
+    R* w = (R*)malloc(sizeof(R) * nw);
+    FN(orc_lic_weights)(lut, lut_width, p->steps_forward, p->steps_backward, w);
+    FN(rc_ctx) c;
+    c.field = field4; c.lic = NULL; c.lc = noise_ch;
+    for (int a = 0; a < 3; ++a) { c.fd[a] = fdims[a]; c.ld[a] = sigma_dims ? sigma_dims[a] : fdims[a]; }
+    c.tf = tf_rgba; c.op = tf_opac; c.sigma = sigma; c.p = p;
+    R lraw[3] = {(R)p->light[0], (R)p->light[1], (R)p->light[2]};
+    FN(normalize3)(lraw, c.light);
+    const long maxs = (long)p->num_iterations * (long)p->num_iterations;
+    const R step = (R)p->step_size_vol;
+    long total = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total) num_threads(nthreads)
+#endif
+    for (int j = 0; j < v->height; ++j)
+        for (int i = 0; i < v->width; ++i) {
+            R pos[3], dir[3], dest[4] = {0, 0, 0, 0};
+            long pix = (long)j * v->width + i;
+            int ns = 0;
+            if (FN(ray_setup)(v, i, j, pos, dir)) {
+                for (long k = 0; k < maxs; ++k) {
+                    R l4[4];
+                    FN(lic_voxel)(&lc, w, pos, l4);
+                    FN(rc_sample) s;
+                    FN(rc_shade)(&c, pos, dir, l4, &s);
+                    R src[4] = {s.rgb[0] * s.a, s.rgb[1] * s.a, s.rgb[2] * s.a, s.a};
+                    R t = K(1) - dest[3];
+                    for (int a = 0; a < 4; ++a) dest[a] = FN(clamp01)(FMA(t, src[a], dest[a]));
+                    ++ns;
+                    for (int a = 0; a < 3; ++a) pos[a] = FMA(dir[a], step, pos[a]);
+                    if (FN(outside)(pos, v) || s.a > K(0.95)) break;
+                }
+            }
+            for (int a = 0; a < 4; ++a) image[4 * pix + a] = dest[a];
+            if (nsamples) nsamples[pix] = ns;
+            total += ns;
+        }
+    free(w);
     return total;
 }
 

@@ -1,0 +1,94 @@
+"""Fused on-the-fly LIC ray caster (lic3d_fragment.glsl) on the GPU against
+the oracle's fused caster (which the CPU tests tie to the LIC-volume caster
+and to its own termination rule).  Tolerance: 1e-5 relative in FP32, sample
+counts exact."""
+import numpy as np
+import pytest
+
+import vectorvisualization_amd as vv
+from oracle import oracle as orc
+
+from conftest import rel_err
+from workloads import lic_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _fused_case(ctx, c, w, h, quat=(0.15, 0.25, 0.05, 0.95), illum=None, luts=None, tf_index=vv.TF_INDEX_MAG,
+                sigma=None, step=1.0 / 64.0):
+    p = c["params"]
+    p.tf_index = tf_index
+    p.step_size_vol = step
+    if illum is not None:
+        p.illum_mode = illum
+    ctx.set_lic_params(p)
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(w, h, cen)
+    for i in range(4):
+        cam.quat[i] = quat[i]
+    rgba, op = vv.fields.tf_magnitude(0.05, 0.4)
+    ctx.set_tf(rgba, op)
+    if sigma is not None:
+        ctx.set_opacity_scale(sigma)
+    img, nsamp = ctx.render_fused(cam, count_samples=True)
+    view = orc.view_setup(w, h, quat=[float(np.float32(q)) for q in quat], center=cen, extent=ext, scale=sc)
+    op_ = c["op"]
+    op_.tf_index = tf_index
+    op_.step_size_vol = step
+    op_.alpha_correction = float(np.float32(step) * np.float32(128.0))
+    op_.light[0], op_.light[1], op_.light[2] = list(cam.light_pos)
+    if illum is not None:
+        op_.illum_mode = illum
+    if luts is not None:
+        orc.attach_luts(op_, luts)
+    ref, ns, total = orc.raycast_fused(c["f4"], c["dims"], c["noise"], c["nch"], c["dims"], c["lut"], rgba, op,
+                                       op_, view, scalar=c["scalar"],
+                                       sdims=c["dims"] if c["scalar"] is not None else None,
+                                       sigma=sigma, sigma_dims=c["dims"] if sigma is not None else None)
+    print(f"fused: rel_err {rel_err(img, ref):.2e} samples {nsamp}", flush=True)
+    return img, ref, nsamp, total
+
+
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+def test_fused_cached_path(ctx, storage):
+    c = lic_setup(ctx, n=16, kind="rankine", L=6, noise_kind=vv.NOISE_UNIFORM, storage=storage,
+                  gradient_scale=3.0)
+    img, ref, nsamp, total = _fused_case(ctx, c, 40, 32)
+    assert nsamp == total and np.max(ref) > 0.01
+    assert rel_err(img, ref) <= 1e-5
+
+
+def test_fused_generic_paths(ctx):
+    # gate (scalar volume) and freq != 1 take the generic per-sample LIC
+    n = 14
+    gate = np.random.default_rng(2).integers(0, 256, n ** 3, dtype=np.uint8)
+    c = lic_setup(ctx, n=n, kind="abc", L=5, freq=1.7, gate=gate, gradient_scale=3.0)
+    img, ref, nsamp, total = _fused_case(ctx, c, 36, 30)
+    assert nsamp == total
+    assert rel_err(img, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("illum", [vv.ILLUM_GRADIENT, vv.ILLUM_MALLO, vv.ILLUM_ZOECKLER])
+def test_fused_illumination(ctx, illum):
+    grad = illum == vv.ILLUM_GRADIENT
+    c = lic_setup(ctx, n=14, kind="rankine", L=5, grad=grad, density=0.3, gradient_scale=3.0)
+    luts = orc.illum_luts()
+    img, ref, nsamp, total = _fused_case(ctx, c, 32, 28, illum=illum, luts=luts)
+    assert nsamp == total and np.max(ref) > 0.01
+    assert rel_err(img, ref) <= 1e-5
+
+
+def test_fused_with_opacity_scale(ctx):
+    n = 12
+    c = lic_setup(ctx, n=n, kind="rankine", L=4, noise_kind=vv.NOISE_UNIFORM, gradient_scale=3.0)
+    ctx.compute_lic()  # the opacity scale lives on the LIC grid
+    sigma = np.random.default_rng(5).uniform(0.3, 2.5, n ** 3).astype(np.float32)
+    img, ref, nsamp, total = _fused_case(ctx, c, 30, 30, sigma=sigma)
+    assert nsamp == total
+    assert rel_err(img, ref) <= 1e-5
+
+
+def test_fused_needs_noise(ctx):
+    ctx.upload_field(np.ones((8, 8, 8, 3), np.float32), (8, 8, 8))
+    with pytest.raises(vv.VVError):
+        ctx.render_fused(vv.default_camera(8, 8))

@@ -467,3 +467,50 @@ def test_adjoint_matches_central_differences(lic_ch, illum, step):
         e[idx] = hh
         fd = (loss(rgba, sigma + e) - loss(rgba, sigma - e)) / (2 * hh)
         assert abs(fd - dsg[idx]) <= 1e-6 * max(1.0, abs(fd)), (idx, fd, dsg[idx])
+
+
+# ---- fused ray caster ---------------------------------------------------------
+def test_fused_equals_volume_caster_for_constant_lic():
+    # constant noise makes the LIC constant everywhere: the on-the-fly LIC of
+    # every sample then equals the trilinear LIC volume, and with low opacity
+    # neither caster terminates early, so the images agree
+    n = 10
+    rng = np.random.default_rng(8)
+    v = rng.normal(size=(n, n, n, 3)).astype(np.float32)
+    f4 = orc.encode_field(v, (n, n, n))
+    noise = np.full(n ** 3, 200, np.uint8)
+    lut, inv = orc.make_filter(1, 6)
+    p = orc.params(steps_forward=6, steps_backward=6, inv_filter_area=inv, gradient_scale=1.0)
+    lic = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p)
+    assert np.ptp(lic) < 1e-5 * np.max(lic)
+    t = np.arange(256) / 255.0
+    rgba = np.stack([t, 1 - t, 0.5 + 0 * t, 0.01 + 0.01 * t], 1).astype(np.float32).ravel()
+    op = t.astype(np.float32)
+    view = orc.view_setup(24, 20, quat=(0.1, 0.2, 0.0, 0.97))
+    a, na, _ = orc.raycast(f4, (n,) * 3, lic, 1, (n,) * 3, rgba, op, p, view)
+    b, nb, _ = orc.raycast_fused(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    assert np.max(a) > 0
+    np.testing.assert_array_equal(na, nb)
+    assert np.max(np.abs(a - b)) <= 1e-5 * np.max(a)
+
+
+def test_fused_terminates_on_sample_alpha():
+    # a constant per-sample alpha of 0.5: the volume caster stops when the
+    # accumulated alpha 1 - 0.5^k exceeds 0.95 (k = 5); the fused caster
+    # tests the sample's own alpha, never above 0.95, and runs to the exit
+    n = 8
+    f4 = orc.encode_field(np.ones((n, n, n, 3), np.float32), (n, n, n))
+    noise = np.full(n ** 3, 255, np.uint8)
+    lut = np.full(256, 255, np.uint8)
+    p = orc.params(steps_forward=2, steps_backward=2, inv_filter_area=0.5, gradient_scale=1.0)
+    licv = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p)
+    rgba = np.tile(np.array([1, 1, 1, 0.5], np.float32), 256)
+    op = np.ones(256, np.float32)  # opacity table 1 at any LIC value
+    view = orc.view_setup(16, 16)
+    _, n_vol, _ = orc.raycast(f4, (n,) * 3, licv, 1, (n,) * 3, rgba, op, p, view)
+    _, n_fused, _ = orc.raycast_fused(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    _, n_full, _ = orc.raycast(f4, (n,) * 3, licv, 1, (n,) * 3, rgba * np.tile([1, 1, 1, 0], 256), op,
+                               p, view)
+    hit = n_full > 0
+    assert np.all(n_vol[hit] == np.minimum(5, n_full[hit]))
+    np.testing.assert_array_equal(n_fused, n_full)

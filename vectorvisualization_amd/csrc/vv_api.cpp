@@ -425,9 +425,12 @@ vv_status vv_lic_weights(const vv_ctx* c, float* out)
 }
 
 // ---- LIC ---------------------------------------------------------------------
-vv_status vv_compute_lic(vv_ctx* c)
+namespace {
+
+// the LIC launch of the context's field, noise, filter and parameters
+// (updateLICVolume + setRenderVolParams); out: the LIC volume, or none
+vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
 {
-    if (!c) return VV_EINVAL;
     if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
     if (!c->has_noise) return fail(c, VV_ESTATE, "no noise uploaded");
     const vv_lic_params& p = c->params;
@@ -442,8 +445,6 @@ vv_status vv_compute_lic(vv_ctx* c)
     vv_status wst = ensure_weights(c);
     if (wst) return wst;
     const int nc = c->nch;
-    HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
-    LicLaunch L;
     std::memset(&L, 0, sizeof(L));
     L.field = make_clamp_desc(c->field.p, c->fdims);
     L.field_fp16 = c->storage == VV_STORE_FP16;
@@ -462,15 +463,30 @@ vv_status vv_compute_lic(vv_ctx* c)
     L.scl = (c->inv_area / (float)(p.steps_forward + p.steps_backward)) * p.gradient_scale;
     L.freq = p.freq_scale;
     for (int a = 0; a < 3; ++a) L.scale[a] = c->scale[a];
-    L.out = make_repeat_desc(c->lic.p, c->fdims);
+    L.out = make_repeat_desc(out, c->fdims);
     for (int a = 0; a < 3; ++a) L.tile[a] = c->lic_tile[a];
     for (int a = 0; a < 3; ++a) L.super_tile[a] = c->lic_super[a];
     L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * (L.field_fp16 ? 8ull : 16ull);
     L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
-    // the cached kernels address through 32-bit buffer offsets
+    // the cached kernels address through 32-bit byte offsets
     // and 24-bit index multiplies (plane strides below 2^24 elements)
     L.cache = c->lic_cache && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
               (long long)L.field.sz < (1ll << 24) && (long long)L.noise.sz < (1ll << 24);
+    return VV_OK;
+}
+
+}  // namespace
+
+vv_status vv_compute_lic(vv_ctx* c)
+{
+    if (!c) return VV_EINVAL;
+    if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
+    LicLaunch L;
+    vv_status st = fill_lic_launch(c, L, nullptr);
+    if (st) return st;
+    const int nc = c->nch;
+    HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
+    L.out = make_repeat_desc(c->lic.p, c->fdims);
     HIPCHK(c, launch_lic(L, c->stream));  // also writes the REPEAT halo planes
     // the opacity scale lives on the LIC grid: drop it when that grid changes
     if (c->ldims[0] != c->fdims[0] || c->ldims[1] != c->fdims[1] || c->ldims[2] != c->fdims[2])
@@ -612,6 +628,42 @@ vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device
         HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
     }
     HIPCHK(c, launch_render(R, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (samples_out) {
+        unsigned long long t = 0;
+        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *samples_out = t;
+    } else if (!on_device) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return VV_OK;
+}
+
+vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
+{
+    if (!c) return VV_EINVAL;
+    if (!image) return fail(c, VV_EINVAL, "image is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    LicLaunch L;
+    vv_status st = fill_lic_launch(c, L, nullptr);
+    if (st) return st;
+    RenderLaunch R;
+    st = fill_render_launch(c, cam, R, false);
+    if (st) return st;
+    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+    float* dst = image;
+    if (!on_device) {
+        HIPCHK(c, c->image.ensure(bytes));
+        dst = (float*)c->image.p;
+    }
+    R.image = dst;
+    if (samples_out) {
+        HIPCHK(c, c->counter.ensure(64));
+        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+    }
+    HIPCHK(c, launch_render_fused(R, L, c->stream));
     if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
     if (samples_out) {
         unsigned long long t = 0;

@@ -362,10 +362,12 @@ inline void illum_lut_ptrs(vv_ctx* c, RenderLaunch& R)
     R.lut_mallo_spec = base + 256 * 256 * 5;
 }
 
-inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunch& R)
+// need_lic: the LIC-volume ray caster; without it (the fused caster) the
+// shading channels are those of the noise
+inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunch& R, bool need_lic = true)
 {
     if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
-    if (!c->has_lic) return fail(c, VV_ESTATE, "no LIC volume (call vv_compute_lic or vv_upload_lic)");
+    if (need_lic && !c->has_lic) return fail(c, VV_ESTATE, "no LIC volume (call vv_compute_lic or vv_upload_lic)");
     vv_status st = check_camera(c, cam);
     if (st) return st;
     st = ensure_tf(c);
@@ -375,8 +377,12 @@ inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunc
     R.field = make_clamp_desc(c->field.p, c->fdims);
     R.field_fp16 = c->storage == VV_STORE_FP16;
     R.tf_ch = p.tf_index == VV_TF_INDEX_MAG ? 3 : 2;
-    R.lic = make_repeat_desc(c->lic.p, c->ldims);
-    R.lc = c->lch;
+    if (need_lic) {
+        R.lic = make_repeat_desc(c->lic.p, c->ldims);
+        R.lc = c->lch;
+    } else {
+        R.lc = c->nch;
+    }
     if (c->has_sigma) R.sigma = make_clamp_desc(c->sigma.p, c->ldims);
     R.tf = (const float*)c->tf_dev.p;
     R.view = view_block(c, cam);

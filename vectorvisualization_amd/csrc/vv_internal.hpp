@@ -157,6 +157,9 @@ hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
 int lic_batch_blocks_per_cu(int fp16, int same);
 int render_batch_blocks_per_cu(int fp16, int lc, int threads);
+// fused on-the-fly LIC ray caster: R as for launch_render (its lic fields
+// unused), L as for launch_lic (its out fields unused)
+hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s);
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s);
 hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
                                  hipStream_t s);

@@ -329,348 +329,8 @@ __global__ void __launch_bounds__(kThreads) k_repeat_halo_f32(float* vol, int nc
 }
 
 // ---- LIC ---------------------------------------------------------------------
+#include "vv_lic_dev.hpp"
 
-struct LicArgs {
-    Vol<void> field;
-    Vol<void> noise;
-    Vol<uint8_t> scalar;
-    const float* w;
-    int F, B;
-    float he, scl, freq, gate_lo, gate_hi;
-    float sc0, sc1, sc2;
-    float* out;
-    int osy, osz;
-    unsigned field_bytes, noise_bytes;  // buffer sizes for the cached kernels
-    int tx, ty, tz;      // block tile
-    int gx, gy, gz;      // tiles per axis
-    int nblocks;         // gx*gy*gz
-    int per_xcd;         // tiles handled by each of the 8 XCDs
-    int sx, sy, sz;      // super-tile size in tiles (3D-blocked tile order)
-    int sgx, sgy;        // super-tiles per axis (x, y)
-};
-
-// logical tile t -> tile coordinates: super-tiles in raster order, tiles
-// raster order inside each super-tile, so the tiles resident on one XCD at a
-// time cover a compact 3D region whose streamline halo fits that XCD's L2
-__device__ __forceinline__ void tile_coords(const LicArgs& a, int t, int& bx, int& by, int& bz)
-{
-    const int per_super = a.sx * a.sy * a.sz;
-    const int st = t / per_super, r = t - st * per_super;
-    const int ux = st % a.sgx, uy = (st / a.sgx) % a.sgy, uz = st / (a.sgx * a.sgy);
-    bx = ux * a.sx + r % a.sx;
-    by = uy * a.sy + (r / a.sx) % a.sy;
-    bz = uz * a.sz + r / (a.sx * a.sy);
-}
-
-template <int NC>
-struct NoiseT;
-template <>
-struct NoiseT<1> {
-    using T = uint8_t;
-};
-template <>
-struct NoiseT<4> {
-    using T = uchar4;
-};
-
-// noise sample n(x): freqSampling (inc_lic.glsl:71-90) for scalar noise,
-// freqSamplingGrad (:61-68, no frequency scale, no gate) for RGBA noise
-template <int NC, bool GATE>
-__device__ __forceinline__ float4 noise_at(const LicArgs& a, float x, float y, float z)
-{
-    float4 n = make_float4(0.f, 0.f, 0.f, 0.f);
-    if constexpr (NC == 4) {
-        const Vol<uchar4> nv{(const uchar4*)a.noise.p, {a.noise.n[0], a.noise.n[1], a.noise.n[2]},
-                             a.noise.sy, a.noise.sz};
-        Cell c = cell_repeat(nv, x, y, z);
-        float4 r = tri_u8x4(nv, c);
-        n.x = r.x * kInv255;
-        n.y = r.y * kInv255;
-        n.z = r.z * kInv255;
-        n.w = r.w * kInv255;
-    } else {
-        if constexpr (GATE) {
-            Cell cs = cell_clamp(a.scalar, x, y, z);
-            float s = tri_u8(a.scalar, cs) * kInv255;
-            if (!(s > a.gate_lo && s < a.gate_hi)) return n;
-        }
-        const Vol<uint8_t> nv{(const uint8_t*)a.noise.p, {a.noise.n[0], a.noise.n[1], a.noise.n[2]},
-                              a.noise.sy, a.noise.sz};
-        Cell c = cell_repeat(nv, x * a.freq, y * a.freq, z * a.freq);
-        n.x = tri_u8(nv, c) * kInv255;
-    }
-    return n;
-}
-
-template <typename FT>
-__device__ __forceinline__ float3 field_rgb(const LicArgs& a, float x, float y, float z)
-{
-    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
-                     a.field.sz};
-    Cell c = cell_clamp(fv, x, y, z);
-    return tri_rgb(fv, c);
-}
-
-// Stores one LIC voxel into the REPEAT-padded output and into the copies
-// of plane 0 at index n along each axis (the wrap halo), so no separate
-// halo pass is needed.
-template <typename T>
-__device__ __forceinline__ void store_repeat(T* out, int osy, int osz, int x, int y, int z, int nx, int ny,
-                                             int nz, T v)
-{
-    const long long o = (long long)z * osz + (long long)y * osy + x;
-    out[o] = v;
-    const bool hx = x == 0, hy = y == 0, hz = z == 0;
-    if (hx | hy | hz) {
-        const long long ox = nx, oy = (long long)ny * osy, oz = (long long)nz * osz;
-        if (hx) out[o + ox] = v;
-        if (hy) out[o + oy] = v;
-        if (hz) out[o + oz] = v;
-        if (hx & hy) out[o + ox + oy] = v;
-        if (hx & hz) out[o + ox + oz] = v;
-        if (hy & hz) out[o + oy + oz] = v;
-        if (hx & hy & hz) out[o + ox + oy + oz] = v;
-    }
-}
-
-// Per-lane cache of the last trilinear cell.  Consecutive samples of a
-// streamline are ~h_e*n voxels apart (0.38 at 256^3, h = 0.005), so the
-// corrector sample almost always lands in the predictor's cell and the next
-// predictor often in the same cell: a lane that hits skips its gathers (it
-// is masked off the loads).  For each channel the cache keeps the four x0
-// texels and the four differences (x1 - x0) of the cell, so the four x-lerps
-// of a hit are single fmas; the values and the operation order are exactly
-// those of tri(), so results do not change.
-struct FieldCache {
-    int base;
-    float v0[3][4];   // [channel][(y,z) corner]: texel at x0
-    float dx[3][4];   // texel(x1) - texel(x0)
-};
-
-struct NoiseCache {
-    int base;
-    float v0[4], dx[4];
-};
-
-__device__ __forceinline__ float tri_cached(const float* v0, const float* dx, float fx, float fy, float fz)
-{
-    const float c00 = fmaf(fx, dx[0], v0[0]);
-    const float c10 = fmaf(fx, dx[1], v0[1]);
-    const float c01 = fmaf(fx, dx[2], v0[2]);
-    const float c11 = fmaf(fx, dx[3], v0[3]);
-    const float c0 = lerpf(c00, c10, fy);
-    const float c1 = lerpf(c01, c11, fy);
-    return lerpf(c0, c1, fz);
-}
-
-// axis coordinate: floor and fraction of x = u*n - 0.5 (shared by the
-// CLAMP field cell and, on the same grid, the REPEAT noise cell)
-struct Ax {
-    float fl, f;
-};
-
-__device__ __forceinline__ Ax axis(float u, int n)
-{
-    Ax r;
-    const float x = fmaf(u, (float)n, -0.5f);
-    r.fl = floorf(x);
-    r.f = x - r.fl;
-    return r;
-}
-
-// corner index of a CLAMP-padded axis: clamp(floor, -1, n-1) + 1
-__device__ __forceinline__ unsigned clamp_idx(float fl, float nm1)
-{
-    return (unsigned)((int)__builtin_amdgcn_fmed3f(fl, -1.0f, nm1) + 1);
-}
-
-// corner index of a REPEAT-padded axis: floor mod n
-__device__ __forceinline__ unsigned repeat_idx(float fl, int n)
-{
-    int i = (int)__builtin_amdgcn_fmed3f(fl, -1.0e9f, 1.0e9f);
-    if ((unsigned)i >= (unsigned)n) {
-        i %= n;
-        if (i < 0) i += n;
-    }
-    return (unsigned)i;
-}
-
-// Texel loads at a 32-bit unsigned byte offset from a block-uniform base:
-// base + zext(off) compiles to global_load ... voffset, s[base] (a VGPR
-// offset of 32 bits, the 64-bit base in SGPRs), so arrays up to 4 GiB need
-// no 64-bit address arithmetic per lane.  (Raw buffer loads, which also take
-// a 32-bit VGPR offset, returned zeros for offsets of 2 GiB and more.)
-// global address space: the batch kernel reads its array pointers from a
-// descriptor in memory, where the compiler cannot infer it (flat loads)
-typedef const __attribute__((address_space(1))) char* gchar;
-typedef __attribute__((address_space(1))) float gfloat;
-typedef __attribute__((address_space(1))) unsigned gu32;
-
-template <typename FT>
-struct FieldLoad;
-
-// float4 texels: two 12-byte loads (rgb of x0 and of x1)
-template <>
-struct FieldLoad<float4> {
-    static constexpr unsigned kShift = 4;
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
-    {
-        const gfloat* q = (const gfloat*)((gchar)base + off);
-        t0[0] = q[0];
-        t0[1] = q[1];
-        t0[2] = q[2];
-        t1[0] = q[4];
-        t1[1] = q[5];
-        t1[2] = q[6];
-    }
-};
-
-// half4 texels: one 16-byte load holds x0 and x1
-template <>
-struct FieldLoad<half4_t> {
-    static constexpr unsigned kShift = 3;
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
-    {
-        const gu32* q = (const gu32*)((gchar)base + off);
-        uint4 a;
-        a.x = q[0];
-        a.y = q[1];
-        a.z = q[2];
-        a.w = q[3];
-        t0[0] = __half2float(__ushort_as_half((unsigned short)(a.x & 0xffffu)));
-        t0[1] = __half2float(__ushort_as_half((unsigned short)(a.x >> 16)));
-        t0[2] = __half2float(__ushort_as_half((unsigned short)(a.y & 0xffffu)));
-        t1[0] = __half2float(__ushort_as_half((unsigned short)(a.z & 0xffffu)));
-        t1[1] = __half2float(__ushort_as_half((unsigned short)(a.z >> 16)));
-        t1[2] = __half2float(__ushort_as_half((unsigned short)(a.w & 0xffffu)));
-    }
-};
-
-// two adjacent noise bytes (x0, x1) in one 16-bit load; the address may be odd
-typedef unsigned short __attribute__((aligned(1))) u16_unaligned;
-__device__ __forceinline__ unsigned noise_pair(const unsigned char* base, unsigned off)
-{
-    gchar g = (gchar)(const char*)base;
-    return *(const __attribute__((address_space(1))) u16_unaligned*)(g + off);
-}
-
-struct CachedArgs {
-    const char* fr;           // field texels
-    const unsigned char* nr;  // noise bytes
-    float fnm1[3];      // field n-1 per axis
-    unsigned fsy, fsz;  // field element strides
-    unsigned nsy, nsz;  // noise element strides
-};
-
-template <typename FT>
-__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache& fc, const Ax& ax,
-                                                  const Ax& ay, const Ax& az)
-{
-    const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
-                          __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
-    if ((int)base != fc.base) {
-        fc.base = (int)base;
-        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float t0[3], t1[3];
-            FieldLoad<FT>::pair(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                fc.v0[ch][k] = t0[ch];
-                fc.dx[ch][k] = t1[ch] - t0[ch];
-            }
-        }
-    }
-    float3 r;
-    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
-    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
-    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
-    return r;
-}
-
-// corrector sample: the field cell and the noise cell of the same position.
-// Both fills are issued before either result is used, so the two gathers
-// overlap (one memory round trip instead of two); values and arithmetic
-// are those of field_from_axes and noise_from_axes.
-template <typename FT>
-__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache& fc,
-                                                  NoiseCache& nc, const Ax& ax, const Ax& ay,
-                                                  const Ax& az, const Ax& nx_, const Ax& ny_,
-                                                  const Ax& nz_, float& noise)
-{
-    const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
-                        __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
-    const unsigned nb = __umul24(repeat_idx(nz_.fl, a.noise.n[2]), c.nsz) +
-                        __umul24(repeat_idx(ny_.fl, a.noise.n[1]), c.nsy) + repeat_idx(nx_.fl, a.noise.n[0]);
-    const bool fmiss = (int)fb != fc.base;
-    const bool nmiss = (int)nb != nc.base;
-    unsigned short nraw[4];
-    if (fmiss) {
-        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float t0[3], t1[3];
-            FieldLoad<FT>::pair(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                fc.v0[ch][k] = t0[ch];
-                fc.dx[ch][k] = t1[ch];  // raw x1 texel until both fills are in flight
-            }
-        }
-    }
-    if (nmiss) {
-        const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) nraw[k] = (unsigned short)noise_pair(c.nr, nb + off[k]);
-    }
-    if (fmiss) {
-        fc.base = (int)fb;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) fc.dx[ch][k] = fc.dx[ch][k] - fc.v0[ch][k];
-    }
-    if (nmiss) {
-        nc.base = (int)nb;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float t0 = (float)(nraw[k] & 0xffu), t1 = (float)(nraw[k] >> 8);
-            nc.v0[k] = t0;
-            nc.dx[k] = t1 - t0;
-        }
-    }
-    noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f) * kInv255;
-    float3 r;
-    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
-    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
-    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
-    return r;
-}
-
-__device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedArgs& c, NoiseCache& nc,
-                                                 const Ax& ax, const Ax& ay, const Ax& az)
-{
-    const unsigned base = __umul24(repeat_idx(az.fl, a.noise.n[2]), c.nsz) +
-                          __umul24(repeat_idx(ay.fl, a.noise.n[1]), c.nsy) + repeat_idx(ax.fl, a.noise.n[0]);
-    if ((int)base != nc.base) {
-        nc.base = (int)base;
-        const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            // two adjacent bytes (x0, x1) in one load
-            const unsigned u = noise_pair(c.nr, base + off[k]);
-            const float t0 = (float)(u & 0xffu), t1 = (float)(u >> 8);
-            nc.v0[k] = t0;
-            nc.dx[k] = t1 - t0;
-        }
-    }
-    return tri_cached(nc.v0, nc.dx, ax.f, ay.f, az.f) * kInv255;
-}
-
-// LIC with the per-lane cell caches (scalar noise without gate).  SAME: the
-// noise grid equals the field grid and freq == 1, so the noise cell of the
-// corrector position reuses the corrector's floors and fractions.
 template <typename FT, bool SAME>
 __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
 {
@@ -683,76 +343,17 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     const int nx = a.field.n[0], ny = a.field.n[1], nz = a.field.n[2];
     if (x >= nx || y >= ny || z >= nz) return;
 
+    // voxel-centre seed (VV/renderer.cpp:1353-1358, lic3d_volume_fragment.glsl:4-5)
     const float sx = ((float)x + 0.5f) / (float)nx * a.sc0;
     const float sy = ((float)y + 0.5f) / (float)ny * a.sc1;
     const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
 
-    CachedArgs c;
-    c.fr = (const char*)a.field.p;
-    c.nr = (const unsigned char*)a.noise.p;
-    c.fnm1[0] = (float)(nx - 1);
-    c.fnm1[1] = (float)(ny - 1);
-    c.fnm1[2] = (float)(nz - 1);
-    c.fsy = (unsigned)a.field.sy;
-    c.fsz = (unsigned)a.field.sz;
-    c.nsy = (unsigned)a.noise.sy;
-    c.nsz = (unsigned)a.noise.sz;
-
+    const CachedArgs c = cached_args(a);
     FieldCache fc;
     fc.base = -1;
     NoiseCache ncache;
     ncache.base = -1;
-    auto noise = [&](float px, float py, float pz, const Ax& ax, const Ax& ay, const Ax& az) -> float {
-        if constexpr (SAME) {
-            return noise_from_axes(a, c, ncache, ax, ay, az);
-        } else {
-            const float q0 = px * a.freq, q1 = py * a.freq, q2 = pz * a.freq;
-            return noise_from_axes(a, c, ncache, axis(q0, a.noise.n[0]), axis(q1, a.noise.n[1]),
-                                   axis(q2, a.noise.n[2]));
-        }
-    };
-    Ax ax = axis(sx, nx), ay = axis(sy, ny), az = axis(sz, nz);
-    const float3 s0 = field_from_axes<FT>(c, fc, ax, ay, az);
-    float acc = noise(sx, sy, sz, ax, ay, az) * a.w[0];
-
-#pragma unroll 1
-    for (int dirn = 0; dirn < 2; ++dirn) {
-        const int steps = dirn == 0 ? a.B : a.F;
-        const float h = dirn == 0 ? -a.he : a.he;
-        const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
-        float px = sx, py = sy, pz = sz;
-        float3 s = s0;
-#pragma unroll 1
-        for (int i = 0; i < steps; ++i) {
-            // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector.
-            // 2s is exact, so fma(2, s, -1) rounds 2s - 1 exactly once, and
-            // 0.5*(d1 + d2) is exact, so fma(0.5, d1 + d2, p) rounds once:
-            // both equal the two-operation forms bit for bit.
-            const float d1x = fmaf(2.0f, s.x, -1.0f) * h;
-            const float d1y = fmaf(2.0f, s.y, -1.0f) * h;
-            const float d1z = fmaf(2.0f, s.z, -1.0f) * h;
-            const float qx = px + d1x, qy = py + d1y, qz = pz + d1z;
-            const float3 s2 = field_from_axes<FT>(c, fc, axis(qx, nx), axis(qy, ny), axis(qz, nz));
-            const float d2x = fmaf(2.0f, s2.x, -1.0f) * h;
-            const float d2y = fmaf(2.0f, s2.y, -1.0f) * h;
-            const float d2z = fmaf(2.0f, s2.z, -1.0f) * h;
-            px = fmaf(0.5f, d1x + d2x, px);
-            py = fmaf(0.5f, d1y + d2y, py);
-            pz = fmaf(0.5f, d1z + d2z, pz);
-            ax = axis(px, nx);
-            ay = axis(py, ny);
-            az = axis(pz, nz);
-            float nval;
-            if constexpr (SAME) {
-                s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, ax, ay, az, nval);
-            } else {
-                s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, axis(px * a.freq, a.noise.n[0]),
-                                        axis(py * a.freq, a.noise.n[1]), axis(pz * a.freq, a.noise.n[2]),
-                                        nval);
-            }
-            acc = acc + nval * wk[i];
-        }
-    }
+    const float acc = lic_point_cached<FT, SAME>(a, c, fc, ncache, sx, sy, sz);
     store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
 }
 
@@ -807,46 +408,7 @@ __global__ void __launch_bounds__(kThreads) k_lic(LicArgs a)
     const float sy = ((float)y + 0.5f) / (float)ny * a.sc1;
     const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
 
-    const float3 s0 = field_rgb<FT>(a, sx, sy, sz);
-    float4 n = noise_at<NC, GATE>(a, sx, sy, sz);
-    const float w0 = a.w[0];
-    float4 acc;
-    acc.x = n.x * w0;
-    acc.y = n.y * w0;
-    acc.z = n.z * w0;
-    acc.w = n.w * w0;
-
-#pragma unroll 1
-    for (int dirn = 0; dirn < 2; ++dirn) {
-        const int steps = dirn == 0 ? a.B : a.F;
-        const float h = dirn == 0 ? -a.he : a.he;
-        const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
-        float px = sx, py = sy, pz = sz;
-        float3 s = s0;
-#pragma unroll 1
-        for (int i = 0; i < steps; ++i) {
-            // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector
-            const float d1x = (2.0f * s.x - 1.0f) * h;
-            const float d1y = (2.0f * s.y - 1.0f) * h;
-            const float d1z = (2.0f * s.z - 1.0f) * h;
-            const float3 s2 = field_rgb<FT>(a, px + d1x, py + d1y, pz + d1z);
-            const float d2x = (2.0f * s2.x - 1.0f) * h;
-            const float d2y = (2.0f * s2.y - 1.0f) * h;
-            const float d2z = (2.0f * s2.z - 1.0f) * h;
-            px = px + 0.5f * (d1x + d2x);
-            py = py + 0.5f * (d1y + d2y);
-            pz = pz + 0.5f * (d1z + d2z);
-            s = field_rgb<FT>(a, px, py, pz);
-            n = noise_at<NC, GATE>(a, px, py, pz);
-            const float wi = wk[i];
-            acc.x = acc.x + n.x * wi;
-            if constexpr (NC == 4) {
-                acc.y = acc.y + n.y * wi;
-                acc.z = acc.z + n.z * wi;
-                acc.w = acc.w + n.w * wi;
-            }
-        }
-    }
+    const float4 acc = lic_point<FT, NC, GATE>(a, sx, sy, sz);
     if constexpr (NC == 1) {
         store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc.x * a.scl);  // lic3d_volume_fragment.glsl:15
     } else {
@@ -991,9 +553,12 @@ hipError_t launch_repeat_halo_f32(float* vol, int nc, const int d[3], hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
+namespace {
+
+// the kernel argument block of a LIC launch (volume, weights, tile order)
+hipError_t make_lic_args(const LicLaunch& L, LicArgs& a)
 {
-    LicArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.field = Vol<void>{L.field.p, {L.field.n[0], L.field.n[1], L.field.n[2]}, L.field.sy, L.field.sz};
     a.noise = Vol<void>{L.noise.p, {L.noise.n[0], L.noise.n[1], L.noise.n[2]}, L.noise.sy, L.noise.sz};
     a.scalar = Vol<uint8_t>{(const uint8_t*)L.scalar.p, {L.scalar.n[0], L.scalar.n[1], L.scalar.n[2]},
@@ -1031,6 +596,16 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
     a.gz = sgz * a.sz;
     a.nblocks = a.gx * a.gy * a.gz;
     a.per_xcd = (a.nblocks + 7) / 8;
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
+{
+    LicArgs a;
+    const hipError_t e = make_lic_args(L, a);
+    if (e != hipSuccess) return e;
     dim3 grid((unsigned)(a.per_xcd * 8));
     dim3 block(kThreads);
     const bool g = L.gate != 0;
@@ -1078,38 +653,8 @@ size_t lic_args_size() { return sizeof(LicArgs); }
 hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles)
 {
     LicArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.field = Vol<void>{L.field.p, {L.field.n[0], L.field.n[1], L.field.n[2]}, L.field.sy, L.field.sz};
-    a.noise = Vol<void>{L.noise.p, {L.noise.n[0], L.noise.n[1], L.noise.n[2]}, L.noise.sy, L.noise.sz};
-    a.w = L.w;
-    a.F = L.F;
-    a.B = L.B;
-    a.he = L.he;
-    a.scl = L.scl;
-    a.freq = L.freq;
-    a.sc0 = L.scale[0];
-    a.sc1 = L.scale[1];
-    a.sc2 = L.scale[2];
-    a.out = (float*)L.out.p;
-    a.osy = L.out.sy;
-    a.osz = L.out.sz;
-    a.field_bytes = (unsigned)L.field_bytes;
-    a.noise_bytes = (unsigned)L.noise_bytes;
-    a.tx = L.tile[0];
-    a.ty = L.tile[1];
-    a.tz = L.tile[2];
-    if (a.tx * a.ty * a.tz != kThreads) return hipErrorInvalidValue;
-    a.sx = L.super_tile[0] > 0 ? L.super_tile[0] : 1;
-    a.sy = L.super_tile[1] > 0 ? L.super_tile[1] : 1;
-    a.sz = L.super_tile[2] > 0 ? L.super_tile[2] : 1;
-    a.sgx = (L.field.n[0] + a.tx * a.sx - 1) / (a.tx * a.sx);
-    a.sgy = (L.field.n[1] + a.ty * a.sy - 1) / (a.ty * a.sy);
-    const int sgz = (L.field.n[2] + a.tz * a.sz - 1) / (a.tz * a.sz);
-    a.gx = a.sgx * a.sx;
-    a.gy = a.sgy * a.sy;
-    a.gz = sgz * a.sz;
-    a.nblocks = a.gx * a.gy * a.gz;
-    a.per_xcd = (a.nblocks + 7) / 8;
+    const hipError_t e = make_lic_args(L, a);
+    if (e != hipSuccess) return e;
     std::memcpy(host_args, &a, sizeof(a));
     *ntiles = a.per_xcd * 8;
     return hipSuccess;

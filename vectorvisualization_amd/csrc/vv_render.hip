@@ -17,6 +17,8 @@ namespace vv {
 
 namespace {
 
+#include "vv_lic_dev.hpp"
+
 constexpr int kBX = 16, kBY = 16, kRThreads = kBX * kBY;
 constexpr int kSeg = 64;  // adjoint checkpoint interval (samples)
 
@@ -115,30 +117,15 @@ __device__ __forceinline__ bool outside(const float p[3], const ViewBlock& v)
            p[2] < 0.0f || p[2] > v.tex_max[2];
 }
 
+// Shading and opacity of one sample from its LIC value(s) l4 (LC channels):
+// TF lookup, illumination mode, opacity table, opacity scale and the alpha
+// correction.  fv/cf: the field and its cell at pos.
 template <typename FT, int LC>
-__device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* tf, const float* op,
-                                            const Ray& r, const float pos[3], Sample& s)
+__device__ __forceinline__ void shade_sample(const RenderLaunch& a, const float* tf, const float* op,
+                                             const Ray& r, const float pos[3], const Vol<FT>& fv, const Cell& cf,
+                                             const float l4[4], Sample& s)
 {
-    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
-                     a.field.sz};
-    const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
     const float v = tri_ch(fv, cf, a.tf_ch);
-    float l4[4];
-    if constexpr (LC == 1) {
-        const Vol<float> lv{(const float*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
-                            a.lic.sz};
-        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
-        l4[0] = tri_f(lv, cl);
-    } else {
-        const Vol<float4> lv{(const float4*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
-                             a.lic.sz};
-        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
-        const float4 q = tri_f4(lv, cl);
-        l4[0] = q.x;
-        l4[1] = q.y;
-        l4[2] = q.z;
-        l4[3] = q.w;
-    }
     s.tfu = v;
     float t4[4];
     for (int ch = 0; ch < 4; ++ch) t4[ch] = lut1d(tf, 256, 4, ch, v);
@@ -245,6 +232,34 @@ __device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* 
     }
     s.araw = al;
     s.a = 1.0f - pow_corr(1.0f - al, a.alpha_corr, a.corr_mode);
+}
+
+// One sample of the LIC-volume ray caster: LIC value(s) trilinear from the
+// volume, then shade_sample.
+template <typename FT, int LC>
+__device__ __forceinline__ void eval_sample(const RenderLaunch& a, const float* tf, const float* op,
+                                            const Ray& r, const float pos[3], Sample& s)
+{
+    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                     a.field.sz};
+    const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
+    float l4[4];
+    if constexpr (LC == 1) {
+        const Vol<float> lv{(const float*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
+                            a.lic.sz};
+        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
+        l4[0] = tri_f(lv, cl);
+    } else {
+        const Vol<float4> lv{(const float4*)a.lic.p, {a.lic.n[0], a.lic.n[1], a.lic.n[2]}, a.lic.sy,
+                             a.lic.sz};
+        const Cell cl = cell_repeat(lv, pos[0], pos[1], pos[2]);
+        const float4 q = tri_f4(lv, cl);
+        l4[0] = q.x;
+        l4[1] = q.y;
+        l4[2] = q.z;
+        l4[3] = q.w;
+    }
+    shade_sample<FT, LC>(a, tf, op, r, pos, fv, cf, l4, s);
 }
 
 __device__ __forceinline__ void load_tables(const float* g, float* tf, float* op)
@@ -614,6 +629,65 @@ __global__ void k_fixed_to_float(const long long* src, float* dst, long long n, 
     if (i < n) dst[i] = (float)((double)src[i] * inv_scale);
 }
 
+// ---- fused on-the-fly LIC ray caster --------------------------------------
+//
+// lic3d_fragment.glsl:5-99 (the reference's default VOLIC_RAYCAST mode): no
+// LIC volume; every ray sample computes the LIC integral of its own position
+// (computeLIC, scaled by licKernel.b * gradient.r), shades it and composites
+// it.  A ray stops when it leaves the box or when the sample's own alpha
+// exceeds 0.95 (src.a, :91; the LIC-volume caster tests the accumulated
+// dest.a).  MODE 0: any noise / gate; 1: per-lane cell caches (scalar noise,
+// no gate), which persist along the ray; 2: the same with the noise grid
+// equal to the field grid at freq 1.
+template <typename FT, int NC, bool GATE, int MODE>
+__global__ void __launch_bounds__(kRThreads) k_render_fused(RenderLaunch a, LicArgs L)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    unsigned long long ns = 0;
+    if (i < a.view.width && j < a.view.height) {
+        float dest[4] = {0.f, 0.f, 0.f, 0.f};
+        Ray r;
+        if (ray_setup(a.view, i, j, r)) {
+            const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                             a.field.sz};
+            const CachedArgs c = cached_args(L);
+            FieldCache fc;
+            fc.base = -1;
+            NoiseCache nc;
+            nc.base = -1;
+            float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
+            for (long long k = 0; k < a.max_samples; ++k) {
+                float l4[4];
+                if constexpr (MODE > 0) {
+                    const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
+                    l4[0] = acc * L.scl;
+                } else {
+                    const float4 acc = lic_point<FT, NC, GATE>(L, pos[0], pos[1], pos[2]);
+                    l4[0] = NC == 1 ? acc.x * L.scl : acc.x;
+                    l4[1] = acc.y;
+                    l4[2] = acc.z;
+                    l4[3] = acc.w * L.scl;
+                }
+                const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
+                Sample s;
+                shade_sample<FT, NC>(a, tf, op, r, pos, fv, cf, l4, s);
+                composite(dest, s);
+                ++ns;
+                for (int q = 0; q < 3; ++q) pos[q] = fmaf(r.dir[q], a.step, pos[q]);
+                if (outside(pos, a.view) || s.a > 0.95f) break;
+            }
+        }
+        const long long pix = (long long)j * a.view.width + i;
+        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    if (a.counter) add_samples(a.counter, ns, red);
+}
+
 }  // namespace
 
 hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
@@ -632,6 +706,51 @@ hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
             hipLaunchKernelGGL((k_render<float4, 4>), grid, block, 0, s, R);
         else
             hipLaunchKernelGGL((k_render<float4, 1>), grid, block, 0, s, R);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s)
+{
+    const int bx = R.block[0] > 0 ? R.block[0] : kBX, by = R.block[1] > 0 ? R.block[1] : kBY;
+    if (bx * by > kRThreads || (bx * by) % 64) return hipErrorInvalidValue;
+    LicArgs A;
+    int nt = 0;
+    const hipError_t e = lic_args_fill(L, &A, &nt);
+    if (e != hipSuccess) return e;
+    dim3 block(bx, by);
+    dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
+    const bool g = L.gate != 0;
+    if (L.cache && L.nc == 1 && !g) {
+        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                          L.noise.n[2] == L.field.n[2];
+        if (R.field_fp16) {
+            if (same)
+                hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 2>), grid, block, 0, s, R, A);
+            else
+                hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 1>), grid, block, 0, s, R, A);
+        } else {
+            if (same)
+                hipLaunchKernelGGL((k_render_fused<float4, 1, false, 2>), grid, block, 0, s, R, A);
+            else
+                hipLaunchKernelGGL((k_render_fused<float4, 1, false, 1>), grid, block, 0, s, R, A);
+        }
+        return hipGetLastError();
+    }
+    if (R.field_fp16) {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_render_fused<half4_t, 4, false, 0>), grid, block, 0, s, R, A);
+        else if (g)
+            hipLaunchKernelGGL((k_render_fused<half4_t, 1, true, 0>), grid, block, 0, s, R, A);
+        else
+            hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 0>), grid, block, 0, s, R, A);
+    } else {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_render_fused<float4, 4, false, 0>), grid, block, 0, s, R, A);
+        else if (g)
+            hipLaunchKernelGGL((k_render_fused<float4, 1, true, 0>), grid, block, 0, s, R, A);
+        else
+            hipLaunchKernelGGL((k_render_fused<float4, 1, false, 0>), grid, block, 0, s, R, A);
     }
     return hipGetLastError();
 }
