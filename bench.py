@@ -236,6 +236,23 @@ def main():
             "fp16_speedup_vs_fp32_plain_kernel": plain / legs["fp16"]["pipeline_ms"],
         }
         out["fwd_plus_adjoint_ms"] = legs[args.storage]["frame_ms"] + legs[args.storage]["adjoint_ms"]
+        # the fused caster (lic3d_fragment.glsl): LIC of every ray sample on the fly
+        fused = {}
+        for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
+            ctx.upload_field(field, dims, storage=st)
+            for w in (256, W):
+                fcam = vv.default_camera(w, w, cen)
+                fimg = torch.empty(w * w * 4, dtype=torch.float32, device=dev)
+                _, fns = ctx.render_fused(fcam, out=fimg, count_samples=True)
+                f_ms = timed(lambda: ctx.render_fused(fcam, out=fimg))
+                fused[f"{name}_{w}"] = dict(ms=f_ms, ray_samples=int(fns),
+                                            lic_samples_per_s=fns * (2 * L + 1) / (f_ms / 1000.0))
+        out["fused"] = {
+            "what": "on-the-fly LIC ray caster: every ray sample integrates its own streamline "
+                    "(no LIC volume), same field/filter/TF, square views",
+            **fused,
+            "lic_volume_plus_frame_ms_fp32": legs["fp32"]["lic_ms"] + legs["fp32"]["frame_ms"],
+        }
         ctx.upload_field(field, dims, storage=storage)
         ctx.compute_lic()
 
