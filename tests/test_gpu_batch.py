@@ -189,3 +189,24 @@ def test_batch_line_illumination_matches_single(ctx):
     b.run()
     _check(b, outs, lics, imgs, total)
     b.close()
+
+
+def test_batch_views_finish_with_their_timestep(ctx):
+    # time steps run smallest first, each followed by its views: the views of
+    # the small time step finish before those of the large one
+    _setup(ctx)
+    b = vv.Batch(ctx)
+    b.add_timestep(_field(2, (48, 48, 48)), (48, 48, 48), noise_seed=3)
+    b.add_timestep(_field(0, (16, 16, 16)), (16, 16, 16), noise_seed=4)
+    outs = []
+    for t in (0, 1, 0, 1):
+        img = torch.zeros(64 * 64 * 4, dtype=torch.float32, device="cuda")
+        b.add_view(t, fields.orbit_camera(64, 64, (0.5, 0.5, 0.5), 0.3 * t, 0.1), img)
+        outs.append(img)
+    b.run()
+    lat = b.view_latency_ms()
+    assert max(lat[1], lat[3]) < min(lat[0], lat[2])
+    with pytest.raises(vv.VVError):
+        b.add_view(0, fields.orbit_camera(8, 8, (0.5, 0.5, 0.5), 0.0), torch.zeros(256, device="cuda"))
+        b.view_latency_ms()  # the batch changed since its last run
+    b.close()

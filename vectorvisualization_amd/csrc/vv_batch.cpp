@@ -55,8 +55,13 @@ struct vv_batch {
     std::vector<BatchView> views;
     DevBuf jobs, first, rviews, vfirst, ctr;
     std::vector<unsigned char> host_jobs;
-    std::vector<int> host_first, host_vfirst;
-    std::vector<RenderLaunch> host_views;
+    std::vector<int> host_first;   // per time step: 0, tiles
+    std::vector<int> host_vfirst;  // per group: tile offsets of its views (0 first)
+    std::vector<RenderLaunch> host_views;  // views grouped by time step, in run order
+    std::vector<int> order;        // time steps in run order (fewest voxels first)
+    std::vector<int> slot;         // user view -> position in host_views
+    std::vector<int> gview, gfirst;  // per group: first view slot, first host_vfirst entry
+    int ngroups = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<long long> graph_key;
@@ -75,9 +80,11 @@ void release_graph(vv_batch* b)
     b->graph_key.clear();
 }
 
-// counter block layout (u32 units): [0] render work counter, [2..] per-view
-// u64 sample counters, then per-view u64 completion stamps, then u64 t0
-size_t ctr_words(size_t nviews) { return 2 + 2 * nviews + 2 * nviews + 2; }
+// counter block layout (u32 units): one render work counter per time step
+// (rounded up to an even count), per-view u64 sample counters, per-view u64
+// completion stamps, then u64 t0
+size_t ctr_head(size_t nsteps) { return (nsteps + 1) & ~(size_t)1; }
+size_t ctr_words(size_t nsteps, size_t nviews) { return ctr_head(nsteps) + 4 * nviews + 2; }
 
 vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
 {
@@ -295,6 +302,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     st->storage = storage;
     b->steps.push_back(st);
     release_graph(b);
+    b->ran = false;  // the last run's counters no longer match the batch
     return VV_OK;
 }
 
@@ -348,9 +356,14 @@ vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, flo
     v.image = image;
     b->views.push_back(v);
     release_graph(b);
+    b->ran = false;
     return VV_OK;
 }
 
+// Runs the time steps one after the other, each LIC volume followed by the
+// views that show it, fewest voxels first: a view is done as soon as its own
+// time step is, so per-view latency grows with the work before it instead of
+// all views waiting for every LIC volume.
 vv_status vv_batch_run(vv_batch* b)
 {
     if (!b) return VV_EINVAL;
@@ -363,36 +376,59 @@ vv_status vv_batch_run(vv_batch* b)
     if (s) return s;
     const int fp16 = b->steps[0]->storage == VV_STORE_FP16;
     const int same = c->params.freq_scale == 1.0f;
-    // LIC jobs
+    const size_t nsteps = b->steps.size(), nv = b->views.size();
+    // LIC jobs, one per time step
     const size_t asz = lic_args_size();
-    b->host_jobs.resize(asz * b->steps.size());
-    b->host_first.assign(b->steps.size() + 1, 0);
-    for (size_t i = 0; i < b->steps.size(); ++i) {
+    b->host_jobs.resize(asz * nsteps);
+    b->host_first.assign(2 * nsteps, 0);
+    for (size_t i = 0; i < nsteps; ++i) {
         LicLaunch L;
         s = build_lic_launch(c, b->steps[i], L);
         if (s) return s;
         int nt = 0;
         if (lic_args_fill(L, b->host_jobs.data() + i * asz, &nt) != hipSuccess)
             return fail(c, VV_EINVAL, "LIC tile of timestep %zu must hold 256 threads", i);
-        b->host_first[i + 1] = b->host_first[i] + nt;
+        b->host_first[2 * i + 1] = nt;
     }
-    // views
+    b->order.resize(nsteps);
+    for (size_t i = 0; i < nsteps; ++i) b->order[i] = (int)i;
+    std::stable_sort(b->order.begin(), b->order.end(),
+                     [&](int x, int y) { return nvox(b->steps[x]->dims) < nvox(b->steps[y]->dims); });
+    // views grouped by time step in run order
     const int bx = b->render_block[0], by = b->render_block[1];
-    const size_t nv = b->views.size();
-    b->host_views.resize(nv);
-    b->host_vfirst.assign(nv + 1, 0);
-    HIPCHK(c, b->ctr.ensure(sizeof(uint32_t) * ctr_words(nv)));
+    const size_t head = ctr_head(nsteps);
+    HIPCHK(c, b->ctr.ensure(sizeof(uint32_t) * ctr_words(nsteps, nv)));
     uint32_t* ctr = (uint32_t*)b->ctr.p;
-    unsigned long long* samples = (unsigned long long*)(ctr + 2);
+    unsigned long long* samples = (unsigned long long*)(ctr + head);
     unsigned long long* done = samples + nv;
     unsigned long long* t0 = done + nv;
-    for (size_t v = 0; v < nv; ++v) {
-        s = build_view_launch(b, b->views[v], b->host_views[v]);
-        if (s) return s;
-        b->host_views[v].counter = samples + v;
-        const int tiles = ((b->views[v].cam.width + bx - 1) / bx) * ((b->views[v].cam.height + by - 1) / by);
-        b->host_vfirst[v + 1] = b->host_vfirst[v] + tiles;
+    b->host_views.resize(nv);
+    b->slot.assign(nv, 0);
+    b->gview.assign(nsteps + 1, 0);
+    b->gfirst.assign(nsteps, 0);
+    b->host_vfirst.clear();
+    std::vector<int> gtiles(nsteps, 0);
+    size_t k = 0;
+    for (size_t g = 0; g < nsteps; ++g) {
+        const int t = b->order[g];
+        b->gview[g] = (int)k;
+        b->gfirst[g] = (int)b->host_vfirst.size();
+        b->host_vfirst.push_back(0);
+        for (size_t v = 0; v < nv; ++v) {
+            if (b->views[v].step != t) continue;
+            s = build_view_launch(b, b->views[v], b->host_views[k]);
+            if (s) return s;
+            b->host_views[k].counter = samples + k;
+            b->slot[v] = (int)k;
+            const int tiles =
+                ((b->views[v].cam.width + bx - 1) / bx) * ((b->views[v].cam.height + by - 1) / by);
+            b->host_vfirst.push_back(b->host_vfirst.back() + tiles);
+            ++k;
+        }
+        gtiles[g] = b->host_vfirst.back();
     }
+    b->gview[nsteps] = (int)k;
+    b->ngroups = (int)nsteps;
     HIPCHK(c, b->jobs.ensure(b->host_jobs.size()));
     HIPCHK(c, b->first.ensure(sizeof(int) * b->host_first.size()));
     HIPCHK(c, b->rviews.ensure(sizeof(RenderLaunch) * nv));
@@ -401,11 +437,16 @@ vv_status vv_batch_run(vv_batch* b)
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
     const int lic_grid = ((cus * lic_batch_blocks_per_cu(fp16, same) + 7) / 8) * 8;
     const int render_grid = cus * render_batch_blocks_per_cu(fp16, 1, bx * by);
-    const std::vector<long long> key = {(long long)b->steps.size(), (long long)nv, lic_grid, render_grid,
-                                        b->host_first.back(), b->host_vfirst.back(), bx, by, fp16, same,
-                                        (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
-                                        (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
-                                        (long long)(intptr_t)b->ctr.p};
+    std::vector<long long> key = {(long long)nsteps, (long long)nv, lic_grid, render_grid, bx, by, fp16, same,
+                                  (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
+                                  (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
+                                  (long long)(intptr_t)b->ctr.p};
+    for (size_t g = 0; g < nsteps; ++g) {
+        key.push_back(b->order[g]);
+        key.push_back(b->gview[g + 1] - b->gview[g]);
+        key.push_back(gtiles[g]);
+        key.push_back(b->host_first[2 * b->order[g] + 1]);
+    }
     // descriptors for this run (the graph's kernels read them from device memory)
     HIPCHK(c, hipMemcpyAsync(b->jobs.p, b->host_jobs.data(), b->host_jobs.size(), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(b->first.p, b->host_first.data(), sizeof(int) * b->host_first.size(),
@@ -417,38 +458,43 @@ vv_status vv_batch_run(vv_batch* b)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (!b->exec || key != b->graph_key) {
         release_graph(b);
-        LicBatchLaunch LB;
-        LB.jobs = b->jobs.p;
-        LB.first = (const int*)b->first.p;
-        LB.njobs = (int)b->steps.size();
-        LB.total = b->host_first.back();
-        LB.grid = lic_grid;
-        LB.fp16 = fp16;
-        LB.same = same;
-        RenderBatchLaunch RB;
-        RB.views = (const RenderLaunch*)b->rviews.p;
-        RB.first = (const int*)b->vfirst.p;
-        RB.nviews = (int)nv;
-        RB.total = b->host_vfirst.back();
-        RB.counter = ctr;
-        RB.done = done;
-        RB.block[0] = bx;
-        RB.block[1] = by;
-        RB.grid = render_grid;
-        RB.fp16 = fp16;
-        RB.lc = 1;
         HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e = launch_stamp(t0, c->stream);
-        if (!e) e = launch_fill_u32(ctr, 0u, (long long)(2 + 4 * nv), c->stream);
-        if (!e) e = launch_lic_batch(LB, c->stream);
-        if (!e) e = launch_render_batch(RB, c->stream);
-        hipGraph_t g = nullptr;
-        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (!e) e = launch_fill_u32(ctr, 0u, (long long)(head + 4 * nv), c->stream);
+        for (size_t g = 0; g < nsteps && !e; ++g) {
+            const int t = b->order[g];
+            LicBatchLaunch LB;
+            LB.jobs = (const char*)b->jobs.p + (size_t)t * asz;
+            LB.first = (const int*)b->first.p + 2 * t;
+            LB.njobs = 1;
+            LB.total = b->host_first[2 * t + 1];
+            LB.grid = lic_grid;
+            LB.fp16 = fp16;
+            LB.same = same;
+            e = launch_lic_batch(LB, c->stream);
+            const int nvg = b->gview[g + 1] - b->gview[g];
+            if (e || nvg == 0) continue;
+            RenderBatchLaunch RB;
+            RB.views = (const RenderLaunch*)b->rviews.p + b->gview[g];
+            RB.first = (const int*)b->vfirst.p + b->gfirst[g];
+            RB.nviews = nvg;
+            RB.total = gtiles[g];
+            RB.counter = ctr + g;
+            RB.done = done + b->gview[g];
+            RB.block[0] = bx;
+            RB.block[1] = by;
+            RB.grid = render_grid;
+            RB.fp16 = fp16;
+            RB.lc = 1;
+            e = launch_render_batch(RB, c->stream);
+        }
+        hipGraph_t gr = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
         if (e || e2) {
-            if (g) (void)hipGraphDestroy(g);
+            if (gr) (void)hipGraphDestroy(gr);
             return fail(c, VV_EDEVICE, "graph capture: %s", hipGetErrorString(e ? e : e2));
         }
-        b->graph = g;
+        b->graph = gr;
         HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
         b->graph_key = key;
     }
@@ -463,16 +509,16 @@ vv_status vv_batch_view_latency(vv_batch* b, double* ms)
     vv_ctx* c = b->ctx;
     if (!b->ran) return fail(c, VV_ESTATE, "batch has not run");
     const size_t nv = b->views.size();
-    std::vector<unsigned long long> h(2 * nv + 1);
+    std::vector<unsigned long long> h(nv + 1);
     const uint32_t* ctr = (const uint32_t*)b->ctr.p;
-    HIPCHK(c, hipMemcpyAsync(h.data(), ctr + 2 + 2 * nv, sizeof(unsigned long long) * (nv + 1),
-                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data(), ctr + ctr_head(b->steps.size()) + 2 * nv,
+                             sizeof(unsigned long long) * (nv + 1), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     int khz = 0;
     HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
     if (khz <= 0) return fail(c, VV_EUNSUPPORTED, "no wall-clock rate");
     const unsigned long long t0 = h[nv];
-    for (size_t v = 0; v < nv; ++v) ms[v] = (double)(h[v] - t0) / (double)khz;
+    for (size_t v = 0; v < nv; ++v) ms[v] = (double)(h[b->slot[v]] - t0) / (double)khz;
     return VV_OK;
 }
 
@@ -483,8 +529,8 @@ vv_status vv_batch_samples(vv_batch* b, uint64_t* total)
     if (!b->ran) return fail(c, VV_ESTATE, "batch has not run");
     const size_t nv = b->views.size();
     std::vector<unsigned long long> h(nv);
-    HIPCHK(c, hipMemcpyAsync(h.data(), (const uint32_t*)b->ctr.p + 2, sizeof(unsigned long long) * nv,
-                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h.data(), (const uint32_t*)b->ctr.p + ctr_head(b->steps.size()),
+                             sizeof(unsigned long long) * nv, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     uint64_t t = 0;
     for (unsigned long long x : h) t += x;
