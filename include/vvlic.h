@@ -219,6 +219,16 @@ vv_status vv_render(vv_ctx* ctx, const vv_camera* cam, float* image, int on_devi
 vv_status vv_render_fused(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
                           uint64_t* samples);
 
+/* Renderer::sliceVolume with lic3d_slicing_fragment.glsl (the render-target path): view-
+ * aligned slices (ViewSlicing, VV/slicing.cpp:42-281) front to back, D / stepSizeVol + 1
+ * of them over the box's depth range D.  A pixel's sample on a slice is where its ray meets
+ * the slice plane inside the box.  It is skipped when the accumulated alpha has reached
+ * 0.95 or its TF alpha is at most 0.05 (no LIC computed); otherwise the LIC of its position
+ * is computed, shaded and composited, and the result rounded to FP16 (the RGBA16F render
+ * target) before the next slice.  samples: the composited samples. */
+vv_status vv_render_slices(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
+                           uint64_t* samples);
+
 /* ---- differentiation ---------------------------------------------------
  * Per-voxel opacity scale sigma (N float32 at the LIC-volume resolution,
  * host or device) multiplies each sample's alpha before opacity

@@ -181,6 +181,25 @@ long orc_raycast_fused_f64(const double* field4, const int fdims[3],
                            const orc_params* p, const orc_view* v,
                            int nthreads, double* image, int* nsamples);
 
+/* View-aligned slicing (Renderer::sliceVolume, lic3d_slicing_fragment.glsl):
+ * the LIC of every slice sample on the fly, FP16 render target. */
+long orc_render_slices_f32(const float* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const float* tf_rgba, const float* tf_opac,
+                           const float* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, float* image, int* nsamples);
+long orc_render_slices_f64(const double* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const double* tf_rgba, const double* tf_opac,
+                           const double* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, double* image, int* nsamples);
+
 /* Reverse-mode derivative of orc_raycast_* with respect to tf_rgba (256x4)
  * and the per-voxel opacity scale sigma (ldims voxels), for a loss whose
  * image gradient is dimage (width*height*4).  Stores every ray's samples,

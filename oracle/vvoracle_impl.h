@@ -658,6 +658,107 @@ long FN(orc_raycast_fused)(const R* field4, const int fdims[3],
     return total;
 }
 
+/* ------------------------------------------------------------------ */
+/* View-aligned slicing: Renderer::sliceVolume + ViewSlicing          */
+/* (VV/slicing.cpp:42-281) with lic3d_slicing_fragment.glsl.  Slice k  */
+/* is the plane v.(p - centre) = d_k, d_k = -D/2 + (k + 1/2) D/n; the   */
+/* pixel's sample is where its ray meets the plane inside the box.      */
+/* Skipped when dest.a >= 0.95 or tf.a <= 0.05; dest is stored in an    */
+/* RGBA16F target, i.e. rounded to FP16 after every composited slice.  */
+/* ------------------------------------------------------------------ */
+static inline R FN(half_rn)(R x) { return (R)orc_f16_to_f32(orc_f32_to_f16((float)x)); }
+
+long FN(orc_render_slices)(const R* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const R* tf_rgba, const R* tf_opac,
+                           const R* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, R* image, int* nsamples)
+{
+    /* slicing setup (VV/slicing.cpp:42-100), in float as the reference */
+    float sv[3], half[3], ctr[3];
+    const float inv = 1.0f / sqrtf(v->dw[0] * v->dw[0] + v->dw[1] * v->dw[1] + v->dw[2] * v->dw[2]);
+    for (int a = 0; a < 3; ++a) {
+        sv[a] = v->dw[a] * inv;
+        if (fabsf(sv[a]) < 1.0e-8f) sv[a] = 0.0f;
+        half[a] = 0.5f * v->extent[a];
+        ctr[a] = v->extent[a] / 2.0f;
+    }
+    const float D = 2.0f * (fabsf(sv[0]) * half[0] + fabsf(sv[1]) * half[1] + fabsf(sv[2]) * half[2]);
+    const int nsl = (int)(D / p->step_size_vol) + 1;
+    FN(lic_ctx) lc;
+    lc.field = field4;
+    for (int a = 0; a < 3; ++a) { lc.fd[a] = fdims[a]; lc.nd[a] = ndims[a]; lc.sd[a] = sdims ? sdims[a] : 1; }
+    lc.noise = noise; lc.nc = noise_ch; lc.scalar = scalar; lc.p = p;
+    const int nw = 1 + p->steps_forward + p->steps_backward;
+    R* w = (R*)malloc(sizeof(R) * nw);
+    FN(orc_lic_weights)(lut, lut_width, p->steps_forward, p->steps_backward, w);
+    FN(rc_ctx) c;
+    c.field = field4; c.lic = NULL; c.lc = noise_ch;
+    for (int a = 0; a < 3; ++a) { c.fd[a] = fdims[a]; c.ld[a] = sigma_dims ? sigma_dims[a] : fdims[a]; }
+    c.tf = tf_rgba; c.op = tf_opac; c.sigma = sigma; c.p = p;
+    R lraw[3] = {(R)p->light[0], (R)p->light[1], (R)p->light[2]};
+    FN(normalize3)(lraw, c.light);
+    long total = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total) num_threads(nthreads)
+#endif
+    for (int j = 0; j < v->height; ++j)
+        for (int i = 0; i < v->width; ++i) {
+            const long pix = (long)j * v->width + i;
+            const R nxp = FMA((R)i + K(0.5), (R)v->inv_w2, K(-1));
+            const R nyp = FMA((R)j + K(0.5), (R)v->inv_h2, K(-1));
+            R d[3], rc[3], dest[4] = {0, 0, 0, 0};
+            for (int a = 0; a < 3; ++a) {
+                d[a] = FMA(nxp, (R)v->du[a], FMA(nyp, (R)v->dv[a], (R)v->dw[a]));
+                rc[a] = (R)v->cam[a] - (R)ctr[a];
+            }
+            const R vdir = (R)sv[0] * d[0] + (R)sv[1] * d[1] + (R)sv[2] * d[2];
+            const R vcam = (R)sv[0] * rc[0] + (R)sv[1] * rc[1] + (R)sv[2] * rc[2];
+            int ns = 0;
+            for (int k = 0; k < nsl && vdir != K(0); ++k) {
+                if (!(dest[3] < K(0.95))) break;
+                const float dkf = -0.5f * D + ((float)k + 0.5f) * D / (float)nsl;
+                const R sk = ((R)dkf - vcam) / vdir;
+                if (!(sk > K(0))) continue;
+                R q[3], g[3], gn[3], pos[3], dir[3];
+                int in = 1;
+                for (int a = 0; a < 3; ++a) {
+                    q[a] = FMA(sk, d[a], (R)v->cam[a]);
+                    in = in && q[a] >= K(0) && q[a] <= (R)v->extent[a];
+                }
+                if (!in) continue;
+                for (int a = 0; a < 3; ++a) g[a] = q[a] - (R)v->cam[a];
+                FN(normalize3)(g, gn);
+                for (int a = 0; a < 3; ++a) {
+                    dir[a] = gn[a] * (R)v->scale[a];
+                    pos[a] = q[a] * (R)v->scale[a];
+                }
+                FN(cell) cl;
+                R tfu;
+                FN(cell_clamp)(pos, fdims, &cl);
+                ORC_TRILIN(R, field4, 4, p->tf_index ? 3 : 2, fdims, cl, tfu);
+                if (!(FN(lut1d)(tf_rgba, 256, 4, 3, tfu) > K(0.05))) continue;
+                R l4[4];
+                FN(lic_voxel)(&lc, w, pos, l4);
+                FN(rc_sample) s;
+                FN(rc_shade)(&c, pos, dir, l4, &s);
+                R src[4] = {s.rgb[0] * s.a, s.rgb[1] * s.a, s.rgb[2] * s.a, s.a};
+                R t = K(1) - dest[3];
+                for (int a = 0; a < 4; ++a) dest[a] = FN(half_rn)(FN(clamp01)(FMA(t, src[a], dest[a])));
+                ++ns;
+            }
+            for (int a = 0; a < 4; ++a) image[4 * pix + a] = dest[a];
+            if (nsamples) nsamples[pix] = ns;
+            total += ns;
+        }
+    free(w);
+    return total;
+}
+
 #if ORC_DOUBLE
 /* ------------------------------------------------------------------ */
 /* Adjoint of orc_raycast_f64 w.r.t. tf_rgba and sigma.                */

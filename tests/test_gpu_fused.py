@@ -92,3 +92,51 @@ def test_fused_needs_noise(ctx):
     ctx.upload_field(np.ones((8, 8, 8, 3), np.float32), (8, 8, 8))
     with pytest.raises(vv.VVError):
         ctx.render_fused(vv.default_camera(8, 8))
+
+
+# ---- view-aligned slicing ---------------------------------------------------------
+def _slices_case(ctx, c, w, h, quat=(0.1, 0.3, -0.05, 0.95), illum=None, luts=None, step=1.0 / 64.0):
+    p = c["params"]
+    p.tf_index = vv.TF_INDEX_MAG
+    p.step_size_vol = step
+    if illum is not None:
+        p.illum_mode = illum
+    ctx.set_lic_params(p)
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(w, h, cen)
+    for i in range(4):
+        cam.quat[i] = quat[i]
+    rgba, op = vv.fields.tf_magnitude(0.02, 0.5)
+    ctx.set_tf(rgba, op)
+    img, nsamp = ctx.render_slices(cam, count_samples=True)
+    view = orc.view_setup(w, h, quat=[float(np.float32(q)) for q in quat], center=cen, extent=ext, scale=sc)
+    op_ = c["op"]
+    op_.tf_index = vv.TF_INDEX_MAG
+    op_.step_size_vol = step
+    op_.alpha_correction = float(np.float32(step) * np.float32(128.0))
+    op_.light[0], op_.light[1], op_.light[2] = list(cam.light_pos)
+    if illum is not None:
+        op_.illum_mode = illum
+    if luts is not None:
+        orc.attach_luts(op_, luts)
+    ref, ns, total = orc.render_slices(c["f4"], c["dims"], c["noise"], c["nch"], c["dims"], c["lut"], rgba, op,
+                                       op_, view, scalar=c["scalar"],
+                                       sdims=c["dims"] if c["scalar"] is not None else None)
+    print(f"slices: rel_err {rel_err(img, ref):.2e} samples {nsamp}", flush=True)
+    return img, ref, nsamp, total
+
+
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+def test_slices_cached_path(ctx, storage):
+    c = lic_setup(ctx, n=16, kind="rankine", L=6, noise_kind=vv.NOISE_UNIFORM, storage=storage,
+                  gradient_scale=3.0)
+    img, ref, nsamp, total = _slices_case(ctx, c, 40, 32)
+    assert nsamp == total and total > 0 and np.max(ref) > 0.01
+    assert rel_err(img, ref) <= 1e-5
+
+
+def test_slices_generic_and_line_illumination(ctx):
+    c = lic_setup(ctx, n=14, kind="abc", L=5, freq=1.5, gradient_scale=3.0)
+    img, ref, nsamp, total = _slices_case(ctx, c, 36, 30, illum=vv.ILLUM_MALLO, luts=orc.illum_luts())
+    assert nsamp == total and total > 0
+    assert rel_err(img, ref) <= 1e-5

@@ -514,3 +514,45 @@ def test_fused_terminates_on_sample_alpha():
     hit = n_full > 0
     assert np.all(n_vol[hit] == np.minimum(5, n_full[hit]))
     np.testing.assert_array_equal(n_fused, n_full)
+
+
+# ---- view-aligned slicing ---------------------------------------------------------
+def _slicing_case(alpha):
+    n = 8
+    f4 = orc.encode_field(np.ones((n, n, n, 3), np.float32), (n, n, n))
+    noise = np.full(n ** 3, 255, np.uint8)
+    lut = np.full(256, 255, np.uint8)
+    p = orc.params(steps_forward=2, steps_backward=2, inv_filter_area=0.5, gradient_scale=1.0,
+                   step_size_vol=1.0 / 128.0)
+    rgba = np.tile(np.array([1, 1, 1, alpha], np.float32), 256)
+    op = np.ones(256, np.float32)
+    view = orc.view_setup(15, 15)  # identity rotation: the slices are z planes
+    return f4, noise, lut, p, rgba, op, view, n
+
+
+def test_slicing_counts_and_fp16_target():
+    # n = D / step + 1 = 129 z planes cross the central ray; with a per-sample
+    # alpha of 0.01 all are composited
+    f4, noise, lut, p, rgba, op, view, n = _slicing_case(0.06)
+    op = np.full(256, 0.01 / 0.06, np.float32)
+    img, ns, _ = orc.render_slices(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    assert ns[7 * 15 + 7] == 129
+    # the render target holds halves
+    assert np.array_equal(img, img.astype(np.float16).astype(np.float32))
+
+
+def test_slicing_stops_at_accumulated_alpha():
+    # per-sample alpha 0.5: dest.a = 0.5, 0.75, 0.875, 0.9375, 0.96875, then
+    # every later slice is skipped (dest.a >= 0.95)
+    f4, noise, lut, p, rgba, op, view, n = _slicing_case(0.5)
+    img, ns, _ = orc.render_slices(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    c = 7 * 15 + 7
+    assert ns[c] == 5
+    assert img[4 * c + 3] == np.float32(0.96875)
+
+
+def test_slicing_skips_low_tf_alpha():
+    # TF alpha at most 0.05: no sample is computed
+    f4, noise, lut, p, rgba, op, view, n = _slicing_case(0.05)
+    img, ns, total = orc.render_slices(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    assert total == 0 and np.all(img == 0)

@@ -150,6 +150,7 @@ def _load() -> C.CDLL:
         "vv_upload_lic": (I, [P, P, I, IP, I]),
         "vv_render": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_render_fused": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
+        "vv_render_slices": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_set_opacity_scale": (I, [P, P, I]),
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
@@ -219,7 +220,7 @@ EXPORTED = [
     "vv_load_filter_png", "vv_tf_load_png", "vv_tf_save_png", "vv_batch_set_field",
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
-    "vv_make_illum_tables", "vv_render_fused",
+    "vv_make_illum_tables", "vv_render_fused", "vv_render_slices",
 ]
 
 
@@ -624,6 +625,17 @@ class Context:
         with self.ordered(out):
             self._chk(LIB.vv_render_fused(self._h, C.byref(cam), p, dev,
                                           C.byref(ns) if count_samples else None))
+        return (out, ns.value) if count_samples else out
+
+    def render_slices(self, cam: Camera, out=None, count_samples: bool = False):
+        """View-aligned slicing with on-the-fly LIC (FP16 render target)."""
+        if out is None:
+            out = np.empty(cam.width * cam.height * 4, np.float32)
+        p, dev = _ptr(out)
+        ns = C.c_uint64()
+        with self.ordered(out):
+            self._chk(LIB.vv_render_slices(self._h, C.byref(cam), p, dev,
+                                           C.byref(ns) if count_samples else None))
         return (out, ns.value) if count_samples else out
 
     def view_block(self, cam: Camera) -> np.ndarray:

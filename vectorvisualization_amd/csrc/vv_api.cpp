@@ -676,6 +676,59 @@ vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_
     return VV_OK;
 }
 
+vv_status vv_render_slices(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
+{
+    if (!c) return VV_EINVAL;
+    if (!image) return fail(c, VV_EINVAL, "image is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    LicLaunch L;
+    vv_status st = fill_lic_launch(c, L, nullptr);
+    if (st) return st;
+    RenderLaunch R;
+    st = fill_render_launch(c, cam, R, false);
+    if (st) return st;
+    // ViewSlicing::setupSlicing (VV/slicing.cpp:42-100): the view vector, the
+    // depth range D of the box along it and n = D / stepSizeVol + 1 slices
+    SliceSetup S;
+    float inv = 1.0f / std::sqrt(R.view.dw[0] * R.view.dw[0] + R.view.dw[1] * R.view.dw[1] +
+                                 R.view.dw[2] * R.view.dw[2]);
+    float half[3], dv = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        S.v[a] = R.view.dw[a] * inv;
+        if (std::fabs(S.v[a]) < 1.0e-8f) S.v[a] = 0.0f;  // VS_EPS (VV/slicing.cpp:129-134)
+        S.center[a] = c->center[a];
+        half[a] = 0.5f * c->extent[a];
+    }
+    dv = std::fabs(S.v[0]) * half[0] + std::fabs(S.v[1]) * half[1] + std::fabs(S.v[2]) * half[2];
+    S.D = 2.0f * dv;
+    const float nslices = S.D / c->params.step_size_vol;
+    if (!(nslices < 1.0e6f)) return fail(c, VV_EINVAL, "too many slices (%g)", (double)nslices);
+    S.n = (int)nslices + 1;
+    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+    float* dst = image;
+    if (!on_device) {
+        HIPCHK(c, c->image.ensure(bytes));
+        dst = (float*)c->image.p;
+    }
+    R.image = dst;
+    if (samples_out) {
+        HIPCHK(c, c->counter.ensure(64));
+        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+    }
+    HIPCHK(c, launch_render_slices(R, L, S, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (samples_out) {
+        unsigned long long t = 0;
+        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *samples_out = t;
+    } else if (!on_device) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return VV_OK;
+}
+
 // ---- differentiation -----------------------------------------------------------
 vv_status vv_set_opacity_scale(vv_ctx* c, const float* sigma, int on_device)
 {

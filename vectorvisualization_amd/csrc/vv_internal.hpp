@@ -157,6 +157,17 @@ hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
 int lic_batch_blocks_per_cu(int fp16, int same);
 int render_batch_blocks_per_cu(int fp16, int lc, int threads);
+// view-aligned slicing (ViewSlicing): planes v.(p - center) = d_i with
+// d_i = -D/2 + (i + 1/2) D/n, i = 0..n-1, front to back
+struct SliceSetup {
+    float v[3];       // unit view vector in object space (components below 1e-8 zeroed)
+    float center[3];  // volume centre (extent / 2)
+    float D;          // depth range
+    int n;            // number of slices
+};
+
+hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, hipStream_t s);
+
 // fused on-the-fly LIC ray caster: R as for launch_render (its lic fields
 // unused), L as for launch_lic (its out fields unused)
 hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s);

@@ -688,6 +688,93 @@ __global__ void __launch_bounds__(kRThreads) k_render_fused(RenderLaunch a, LicA
     if (a.counter) add_samples(a.counter, ns, red);
 }
 
+// ---- view-aligned slicing -----------------------------------------------------
+//
+// Renderer::sliceVolume with lic3d_slicing_fragment.glsl (the render-target
+// path): the view-aligned slices of ViewSlicing (VV/slicing.cpp:42-281) are
+// drawn front to back; a pixel's fragment on slice i sits where the pixel's
+// ray meets the slice plane inside the box.  A fragment is skipped when the
+// accumulated alpha has reached 0.95 or when the sample's TF alpha is at most
+// 0.05 (no LIC is computed there); otherwise it runs the LIC of its position,
+// shades and composites it, and the result is stored in the RGBA16F render
+// target, i.e. rounded to FP16, before the next slice reads it.
+__device__ __forceinline__ float half_rn(float x) { return __half2float(__float2half_rn(x)); }
+
+template <typename FT, int NC, bool GATE, int MODE>
+__global__ void __launch_bounds__(kRThreads) k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    unsigned long long ns = 0;
+    if (i < a.view.width && j < a.view.height) {
+        const ViewBlock& vb = a.view;
+        const float nxp = fmaf((float)i + 0.5f, vb.inv_w2, -1.0f);
+        const float nyp = fmaf((float)j + 0.5f, vb.inv_h2, -1.0f);
+        float d[3], rc[3];
+        for (int q = 0; q < 3; ++q) {
+            d[q] = fmaf(nxp, vb.du[q], fmaf(nyp, vb.dv[q], vb.dw[q]));
+            rc[q] = vb.cam[q] - S.center[q];
+        }
+        const float vdir = S.v[0] * d[0] + S.v[1] * d[1] + S.v[2] * d[2];
+        const float vcam = S.v[0] * rc[0] + S.v[1] * rc[1] + S.v[2] * rc[2];
+        const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                         a.field.sz};
+        const CachedArgs c = cached_args(L);
+        FieldCache fc;
+        fc.base = -1;
+        NoiseCache nc;
+        nc.base = -1;
+        float dest[4] = {0.f, 0.f, 0.f, 0.f};
+        for (int k = 0; k < S.n && vdir != 0.0f; ++k) {
+            if (!(dest[3] < 0.95f)) break;
+            const float dk = -0.5f * S.D + ((float)k + 0.5f) * S.D / (float)S.n;
+            const float sk = (dk - vcam) / vdir;
+            if (!(sk > 0.0f)) continue;
+            float p[3];
+            bool in = true;
+            for (int q = 0; q < 3; ++q) {
+                p[q] = fmaf(sk, d[q], vb.cam[q]);
+                in = in && p[q] >= 0.0f && p[q] <= vb.extent[q];
+            }
+            if (!in) continue;
+            Ray r;
+            float g[3], gn[3], pos[3];
+            for (int q = 0; q < 3; ++q) g[q] = p[q] - vb.cam[q];
+            normalize3(g, gn);
+            for (int q = 0; q < 3; ++q) {
+                r.dir[q] = gn[q] * vb.scale[q];
+                pos[q] = p[q] * vb.scale[q];
+            }
+            const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
+            const float tfu = tri_ch(fv, cf, a.tf_ch);
+            if (!(lut1d(tf, 256, 4, 3, tfu) > 0.05f)) continue;
+            float l4[4];
+            if constexpr (MODE > 0) {
+                const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
+                l4[0] = acc * L.scl;
+            } else {
+                const float4 acc = lic_point<FT, NC, GATE>(L, pos[0], pos[1], pos[2]);
+                l4[0] = NC == 1 ? acc.x * L.scl : acc.x;
+                l4[1] = acc.y;
+                l4[2] = acc.z;
+                l4[3] = acc.w * L.scl;
+            }
+            Sample s;
+            shade_sample<FT, NC>(a, tf, op, r, pos, fv, cf, l4, s);
+            composite(dest, s);
+            for (int q = 0; q < 4; ++q) dest[q] = half_rn(dest[q]);
+            ++ns;
+        }
+        const long long pix = (long long)j * a.view.width + i;
+        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    if (a.counter) add_samples(a.counter, ns, red);
+}
+
 }  // namespace
 
 hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
@@ -751,6 +838,51 @@ hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStr
             hipLaunchKernelGGL((k_render_fused<float4, 1, true, 0>), grid, block, 0, s, R, A);
         else
             hipLaunchKernelGGL((k_render_fused<float4, 1, false, 0>), grid, block, 0, s, R, A);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, hipStream_t s)
+{
+    const int bx = R.block[0] > 0 ? R.block[0] : kBX, by = R.block[1] > 0 ? R.block[1] : kBY;
+    if (bx * by > kRThreads || (bx * by) % 64) return hipErrorInvalidValue;
+    LicArgs A;
+    int nt = 0;
+    const hipError_t e = lic_args_fill(L, &A, &nt);
+    if (e != hipSuccess) return e;
+    dim3 block(bx, by);
+    dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
+    const bool g = L.gate != 0;
+    if (L.cache && L.nc == 1 && !g) {
+        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                          L.noise.n[2] == L.field.n[2];
+        if (R.field_fp16) {
+            if (same)
+                hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 2>), grid, block, 0, s, R, A, S);
+            else
+                hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 1>), grid, block, 0, s, R, A, S);
+        } else {
+            if (same)
+                hipLaunchKernelGGL((k_render_slices<float4, 1, false, 2>), grid, block, 0, s, R, A, S);
+            else
+                hipLaunchKernelGGL((k_render_slices<float4, 1, false, 1>), grid, block, 0, s, R, A, S);
+        }
+        return hipGetLastError();
+    }
+    if (R.field_fp16) {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_render_slices<half4_t, 4, false, 0>), grid, block, 0, s, R, A, S);
+        else if (g)
+            hipLaunchKernelGGL((k_render_slices<half4_t, 1, true, 0>), grid, block, 0, s, R, A, S);
+        else
+            hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 0>), grid, block, 0, s, R, A, S);
+    } else {
+        if (L.nc == 4)
+            hipLaunchKernelGGL((k_render_slices<float4, 4, false, 0>), grid, block, 0, s, R, A, S);
+        else if (g)
+            hipLaunchKernelGGL((k_render_slices<float4, 1, true, 0>), grid, block, 0, s, R, A, S);
+        else
+            hipLaunchKernelGGL((k_render_slices<float4, 1, false, 0>), grid, block, 0, s, R, A, S);
     }
     return hipGetLastError();
 }
