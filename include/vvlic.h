@@ -380,7 +380,9 @@ vv_status vv_anim_destroy(vv_anim* a);
  * caches), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
  * an LDS window per ray segment before the global integer atomics),
  * "render.block_x", "render.block_y" (pixel tile of the ray caster, 64 to
- * 256 threads in multiples of 64).
+ * 256 threads in multiples of 64), "render.ray_group" (lanes per ray of
+ * vv_render_fused / vv_render_slices, a power of two up to 64; 0 = enough
+ * lanes for about 2^21 in total; the image does not depend on it).
  * Results do not depend on these settings. */
 vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
 vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);

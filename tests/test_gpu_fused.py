@@ -140,3 +140,41 @@ def test_slices_generic_and_line_illumination(ctx):
     img, ref, nsamp, total = _slices_case(ctx, c, 36, 30, illum=vv.ILLUM_MALLO, luts=orc.illum_luts())
     assert nsamp == total and total > 0
     assert rel_err(img, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("slices", [False, True])
+def test_ray_groups_give_identical_images(ctx, slices):
+    # every lanes-per-ray setting must reproduce the one-lane caster bit for
+    # bit, including rays that stop early (an opaque TF)
+    c = lic_setup(ctx, n=16, kind="rankine", L=5, noise_kind=vv.NOISE_UNIFORM, gradient_scale=3.0)
+    p = c["params"]
+    p.tf_index = vv.TF_INDEX_MAG
+    p.step_size_vol = 1.0 / 96.0
+    ctx.set_lic_params(p)
+    cen = ctx.field_geometry()[2]
+    cam = vv.default_camera(37, 29, cen)
+    cam.quat[0], cam.quat[1], cam.quat[3] = 0.2, 0.1, 0.97
+    for alpha_hi in (0.3, 1.0):
+        ctx.set_tf(*vv.fields.tf_magnitude(0.1, alpha_hi))
+        imgs = {}
+        for G in (1, 2, 4, 8, 16, 32, 64):
+            ctx.set_tuning("render.ray_group", G)
+            render = ctx.render_slices if slices else ctx.render_fused
+            imgs[G] = render(cam, count_samples=True)
+        ctx.set_tuning("render.ray_group", 0)
+        ref, nref = imgs[1]
+        assert nref > 0
+        for G, (img, n) in imgs.items():
+            np.testing.assert_array_equal(img, ref, err_msg=f"G={G}")
+            assert n == nref, (G, n, nref)
+
+
+@pytest.mark.parametrize("group", [1, 0])
+def test_slices_gradient_illumination(ctx, group):
+    c = lic_setup(ctx, n=14, kind="rankine", L=5, grad=True, density=0.3, gradient_scale=3.0)
+    ctx.set_tuning("render.ray_group", group)
+    img, ref, nsamp, total = _slices_case(ctx, c, 34, 30, illum=vv.ILLUM_GRADIENT)
+    ctx.set_tuning("render.ray_group", 0)
+    assert nsamp == total and total > 0
+    assert np.all(np.isfinite(img))
+    assert rel_err(img, ref) <= 1e-5

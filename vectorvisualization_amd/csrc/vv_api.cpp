@@ -560,9 +560,13 @@ vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
     else if (k == "adjoint.window") slot = &c->adj_window;
     else if (k == "render.block_x") slot = &c->render_block[0];
     else if (k == "render.block_y") slot = &c->render_block[1];
+    else if (k == "render.ray_group") slot = &c->ray_group;
     else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
     if (k == "lic.cache" || k == "adjoint.window") {
         if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
+    } else if (k == "render.ray_group") {
+        if (value < 0 || value > 64 || (value & (value - 1)))
+            return fail(c, VV_EINVAL, "render.ray_group must be 0 (automatic) or a power of two up to 64");
     } else if (value < 1 || value > 256 || (value & (value - 1))) {
         return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
     }
@@ -584,6 +588,7 @@ vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
     else if (k == "adjoint.window") *value = c->adj_window;
     else if (k == "render.block_x") *value = c->render_block[0];
     else if (k == "render.block_y") *value = c->render_block[1];
+    else if (k == "render.ray_group") *value = c->ray_group;
     else return VV_EINVAL;
     return VV_OK;
 }
@@ -640,6 +645,18 @@ vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device
     return VV_OK;
 }
 
+// lanes per ray of the on-the-fly casters: the tuning value, or enough to
+// put about 2^21 lanes to work (measured on MI355X at 256^3, L = 40: 2 lanes
+// per ray at 1024^2, 8 at 512^2, 32 at 256^2, 64 below)
+static int ray_group_for(const vv_ctx* c, const vv_camera* cam)
+{
+    if (c->ray_group > 0) return c->ray_group;
+    const long long rays = (long long)cam->width * cam->height;
+    int G = 1;
+    while (G < 64 && rays * G < (1ll << 21)) G *= 2;
+    return G;
+}
+
 vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
     if (!c) return VV_EINVAL;
@@ -663,7 +680,14 @@ vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_
         R.counter = (unsigned long long*)((char*)c->counter.p + 8);
         HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
     }
-    HIPCHK(c, launch_render_fused(R, L, c->stream));
+    const int G = ray_group_for(c, cam);
+    if (G > 1) {
+        SliceSetup S;
+        std::memset(&S, 0, sizeof(S));
+        HIPCHK(c, launch_render_group(R, L, S, 0, G, c->stream));
+    } else {
+        HIPCHK(c, launch_render_fused(R, L, c->stream));
+    }
     if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
     if (samples_out) {
         unsigned long long t = 0;
@@ -716,7 +740,11 @@ vv_status vv_render_slices(vv_ctx* c, const vv_camera* cam, float* image, int on
         R.counter = (unsigned long long*)((char*)c->counter.p + 8);
         HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
     }
-    HIPCHK(c, launch_render_slices(R, L, S, c->stream));
+    const int G = ray_group_for(c, cam);
+    if (G > 1)
+        HIPCHK(c, launch_render_group(R, L, S, 1, G, c->stream));
+    else
+        HIPCHK(c, launch_render_slices(R, L, S, c->stream));
     if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
     if (samples_out) {
         unsigned long long t = 0;

@@ -99,6 +99,7 @@ struct vv_ctx {
     int lic_cache = 1;
     int adj_window = 1;
     int render_block[2] = {16, 16};
+    int ray_group = 0;  // lanes per ray of the on-the-fly casters, 0: automatic
 
     // line-illumination tables (built on first use of illum_mode 2 / 3)
     vv_line_material line_mat;

@@ -168,6 +168,11 @@ struct SliceSetup {
 
 hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, hipStream_t s);
 
+// the fused caster (slices = 0) or slicing (slices = 1) with G lanes per ray
+// (G a power of two, 1..64); identical images, more parallelism for small views
+hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, int slices, int G,
+                               hipStream_t s);
+
 // fused on-the-fly LIC ray caster: R as for launch_render (its lic fields
 // unused), L as for launch_lic (its out fields unused)
 hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s);

@@ -640,7 +640,8 @@ __global__ void k_fixed_to_float(const long long* src, float* dst, long long n, 
 // no gate), which persist along the ray; 2: the same with the noise grid
 // equal to the field grid at freq 1.
 template <typename FT, int NC, bool GATE, int MODE>
-__global__ void __launch_bounds__(kRThreads) k_render_fused(RenderLaunch a, LicArgs L)
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_fused(RenderLaunch a, LicArgs L)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
@@ -701,7 +702,8 @@ __global__ void __launch_bounds__(kRThreads) k_render_fused(RenderLaunch a, LicA
 __device__ __forceinline__ float half_rn(float x) { return __half2float(__float2half_rn(x)); }
 
 template <typename FT, int NC, bool GATE, int MODE>
-__global__ void __launch_bounds__(kRThreads) k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
@@ -749,6 +751,8 @@ __global__ void __launch_bounds__(kRThreads) k_render_slices(RenderLaunch a, Lic
                 r.dir[q] = gn[q] * vb.scale[q];
                 pos[q] = p[q] * vb.scale[q];
             }
+            const float vneg[3] = {-r.dir[0], -r.dir[1], -r.dir[2]};
+            normalize3(vneg, r.V);
             const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
             const float tfu = tri_ch(fv, cf, a.tf_ch);
             if (!(lut1d(tf, 256, 4, 3, tfu) > 0.05f)) continue;
@@ -773,6 +777,192 @@ __global__ void __launch_bounds__(kRThreads) k_render_slices(RenderLaunch a, Lic
         reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
     }
     if (a.counter) add_samples(a.counter, ns, red);
+}
+
+// ---- ray groups: several lanes per ray for the on-the-fly casters ----------------
+//
+// With one lane per pixel a small view has too few lanes to hide the serial
+// chain of samples (each a full LIC integral).  Here G lanes (a power of two,
+// at most 64) share a ray: in every round lane g computes sample base + g,
+// then the group composites the G results in sample order, each lane keeping
+// the same running colour, so the compositing arithmetic and order are those
+// of the one-lane casters and the image is identical.  Work past the end of a
+// ray (termination inside a round) is at most G - 1 samples.
+//   SLICES == false  fused caster: sample k at the k-th step along the ray
+//                    (each lane steps from the round's first position to its
+//                    own, with the same fma sequence), stop after a sample
+//                    whose alpha exceeds 0.95 or when the next step leaves
+//                    the box
+//   SLICES == true   view-aligned slicing: sample k on slice k; skipped when
+//                    the ray misses the slice in the box or the TF alpha is
+//                    at most 0.05; stop once dest.a >= 0.95; dest rounded to
+//                    FP16 after every composited sample
+template <typename FT, int NC, bool GATE, int MODE, bool SLICES>
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int tid = threadIdx.x;
+    const int sub = tid & (G - 1);
+    const int gl0 = (tid & 63) & ~(G - 1);  // first lane of the group in the wave
+    // the block's 256/G rays form a TW x TH pixel tile (TW = TH or 2 TH), so
+    // each wave's rays are neighbours and their streamlines share cache lines
+    const int W = a.view.width, H = a.view.height;
+    const int nr = kRThreads / G;
+    const int lg = 31 - __builtin_clz(nr);
+    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
+    const int tiles_x = (W + TW - 1) / TW;
+    const int r_in = tid / G;
+    const int i = (int)(blockIdx.x % tiles_x) * TW + (r_in & (TW - 1));
+    const int j = (int)(blockIdx.x / tiles_x) * TH + r_in / TW;
+    const bool has_pix = i < W && j < H;
+    const ViewBlock& vb = a.view;
+    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy, a.field.sz};
+    const CachedArgs c = cached_args(L);
+    FieldCache fc;
+    fc.base = -1;
+    NoiseCache nc;
+    nc.base = -1;
+    float dest[4] = {0.f, 0.f, 0.f, 0.f};
+    unsigned long long ns = 0;
+    Ray r;
+    bool alive;
+    // ray policy state
+    float base[3] = {0.f, 0.f, 0.f};
+    // slice policy state
+    float d[3] = {0.f, 0.f, 0.f}, vdir = 0.f, vcam = 0.f;
+    if constexpr (SLICES) {
+        const float nxp = fmaf((float)i + 0.5f, vb.inv_w2, -1.0f);
+        const float nyp = fmaf((float)j + 0.5f, vb.inv_h2, -1.0f);
+        float rc[3];
+        for (int q = 0; q < 3; ++q) {
+            d[q] = fmaf(nxp, vb.du[q], fmaf(nyp, vb.dv[q], vb.dw[q]));
+            rc[q] = vb.cam[q] - S.center[q];
+        }
+        vdir = S.v[0] * d[0] + S.v[1] * d[1] + S.v[2] * d[2];
+        vcam = S.v[0] * rc[0] + S.v[1] * rc[1] + S.v[2] * rc[2];
+        alive = has_pix && vdir != 0.0f;
+    } else {
+        alive = has_pix && ray_setup(vb, i, j, r);
+        for (int q = 0; q < 3; ++q) base[q] = r.pos[q];
+    }
+    long long bk = 0;
+    while (alive) {
+        const long long k = bk + sub;
+        bool ok;
+        float src[4] = {0.f, 0.f, 0.f, 0.f};
+        float nxt[3] = {0.f, 0.f, 0.f};
+        bool nxt_out = true;
+        float pos[3];
+        Ray rs;
+        if constexpr (SLICES) {
+            ok = k < S.n;
+            if (ok) {
+                const float dk = -0.5f * S.D + ((float)k + 0.5f) * S.D / (float)S.n;
+                const float sk = (dk - vcam) / vdir;
+                ok = sk > 0.0f;
+                float p[3];
+                for (int q = 0; q < 3; ++q) {
+                    p[q] = fmaf(sk, d[q], vb.cam[q]);
+                    ok = ok && p[q] >= 0.0f && p[q] <= vb.extent[q];
+                }
+                if (ok) {
+                    float g[3], gn[3];
+                    for (int q = 0; q < 3; ++q) g[q] = p[q] - vb.cam[q];
+                    normalize3(g, gn);
+                    for (int q = 0; q < 3; ++q) {
+                        rs.dir[q] = gn[q] * vb.scale[q];
+                        pos[q] = p[q] * vb.scale[q];
+                    }
+                    const float vneg[3] = {-rs.dir[0], -rs.dir[1], -rs.dir[2]};
+                    normalize3(vneg, rs.V);
+                }
+            }
+        } else {
+            ok = k < a.max_samples;
+            for (int q = 0; q < 3; ++q) pos[q] = base[q];
+            for (int t = 0; t < sub; ++t) {
+                for (int q = 0; q < 3; ++q) pos[q] = fmaf(r.dir[q], a.step, pos[q]);
+                if (outside(pos, vb)) ok = false;
+            }
+            for (int q = 0; q < 3; ++q) {
+                nxt[q] = fmaf(r.dir[q], a.step, pos[q]);
+                rs.dir[q] = r.dir[q];
+                rs.V[q] = r.V[q];
+            }
+            nxt_out = !ok || outside(nxt, vb);
+        }
+        if (ok) {
+            const Cell cf = cell_clamp(fv, pos[0], pos[1], pos[2]);
+            bool visible = true;
+            if constexpr (SLICES) visible = lut1d(tf, 256, 4, 3, tri_ch(fv, cf, a.tf_ch)) > 0.05f;
+            if (visible) {
+                float l4[4];
+                if constexpr (MODE > 0) {
+                    const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
+                    l4[0] = acc * L.scl;
+                } else {
+                    const float4 acc = lic_point<FT, NC, GATE>(L, pos[0], pos[1], pos[2]);
+                    l4[0] = NC == 1 ? acc.x * L.scl : acc.x;
+                    l4[1] = acc.y;
+                    l4[2] = acc.z;
+                    l4[3] = acc.w * L.scl;
+                }
+                Sample sm;
+                shade_sample<FT, NC>(a, tf, op, rs, pos, fv, cf, l4, sm);
+                for (int q = 0; q < 3; ++q) src[q] = sm.rgb[q] * sm.a;
+                src[3] = sm.a;
+            } else {
+                ok = false;  // slicing: nothing composited for this slice
+            }
+        }
+        // composite the group's G samples in order (every lane of the group alike)
+        for (int g = 0; g < G; ++g) {
+            const int sl = gl0 + g;
+            if constexpr (SLICES) {
+                if (bk + g >= S.n || !(dest[3] < 0.95f)) {
+                    alive = false;
+                    break;
+                }
+                if (__shfl((int)ok, sl) == 0) continue;
+            } else {
+                if (__shfl((int)ok, sl) == 0) {
+                    alive = false;
+                    break;
+                }
+            }
+            float sv[4];
+            for (int q = 0; q < 4; ++q) sv[q] = __shfl(src[q], sl);
+            const float t = 1.0f - dest[3];
+            for (int q = 0; q < 4; ++q) {
+                dest[q] = clamp01(fmaf(t, sv[q], dest[q]));
+                if constexpr (SLICES) dest[q] = half_rn(dest[q]);
+            }
+            ++ns;
+            if constexpr (!SLICES) {
+                if (sv[3] > 0.95f) {
+                    alive = false;
+                    break;
+                }
+            }
+        }
+        if constexpr (!SLICES) {
+            if (alive) {
+                const int last = gl0 + G - 1;
+                for (int q = 0; q < 3; ++q) base[q] = __shfl(nxt[q], last);
+                if (__shfl((int)nxt_out, last)) alive = false;
+            }
+        }
+        bk += G;
+    }
+    if (has_pix && sub == 0) {
+        const long long pix = (long long)j * W + i;
+        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    if (a.counter) add_samples(a.counter, sub == 0 ? ns : 0ull, red);
 }
 
 }  // namespace
@@ -884,6 +1074,59 @@ hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const
         else
             hipLaunchKernelGGL((k_render_slices<float4, 1, false, 0>), grid, block, 0, s, R, A, S);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, int slices, int G,
+                               hipStream_t s)
+{
+    if (G < 1 || G > 64 || (G & (G - 1))) return hipErrorInvalidValue;
+    LicArgs A;
+    int nt = 0;
+    const hipError_t e = lic_args_fill(L, &A, &nt);
+    if (e != hipSuccess) return e;
+    const int nr = kRThreads / G;
+    int lg = 0;
+    while ((1 << (lg + 1)) <= nr) ++lg;
+    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
+    const long long nblocks = (long long)((R.view.width + TW - 1) / TW) * ((R.view.height + TH - 1) / TH);
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)nblocks), block(kRThreads);
+    const bool g = L.gate != 0;
+    const bool cached = L.cache && L.nc == 1 && !g;
+    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                      L.noise.n[2] == L.field.n[2];
+#define VV_GROUP_LAUNCH(FT, NC, GATE, MODE)                                                                 \
+    do {                                                                                                   \
+        if (slices)                                                                                        \
+            hipLaunchKernelGGL((k_render_group<FT, NC, GATE, MODE, true>), grid, block, 0, s, R, A, S, G);  \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_render_group<FT, NC, GATE, MODE, false>), grid, block, 0, s, R, A, S, G); \
+    } while (0)
+    if (R.field_fp16) {
+        if (cached && same)
+            VV_GROUP_LAUNCH(half4_t, 1, false, 2);
+        else if (cached)
+            VV_GROUP_LAUNCH(half4_t, 1, false, 1);
+        else if (L.nc == 4)
+            VV_GROUP_LAUNCH(half4_t, 4, false, 0);
+        else if (g)
+            VV_GROUP_LAUNCH(half4_t, 1, true, 0);
+        else
+            VV_GROUP_LAUNCH(half4_t, 1, false, 0);
+    } else {
+        if (cached && same)
+            VV_GROUP_LAUNCH(float4, 1, false, 2);
+        else if (cached)
+            VV_GROUP_LAUNCH(float4, 1, false, 1);
+        else if (L.nc == 4)
+            VV_GROUP_LAUNCH(float4, 4, false, 0);
+        else if (g)
+            VV_GROUP_LAUNCH(float4, 1, true, 0);
+        else
+            VV_GROUP_LAUNCH(float4, 1, false, 0);
+    }
+#undef VV_GROUP_LAUNCH
     return hipGetLastError();
 }
 
