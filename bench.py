@@ -53,18 +53,26 @@ def lic_bytes_per_voxel(F: int, B: int, field_bytes: int = 12, noise_bytes: int 
     return (1 + 2 * (F + B)) * 8 * field_bytes + (F + B + 1) * 8 * noise_bytes + out_bytes
 
 
-def load_traffic(name: str):
-    """HBM bytes per LIC launch from the committed rocprof PMC summary, if any."""
+def _profile_value(name: str, key: str):
+    """A number from a committed rocprof summary line '<key>: <value>', if any."""
     p = ROOT / "profiles" / name
     if not p.exists():
         return None
     for line in p.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             try:
-                return float(line.split(":", 1)[1])
-            except ValueError:
+                return float(line.split(":", 1)[1].split()[0])
+            except (ValueError, IndexError):
                 return None
     return None
+
+
+def load_traffic(name: str):
+    """HBM bytes per LIC launch from the committed rocprof PMC summary, if any."""
+    return _profile_value(name, "traffic_bytes_per_launch")
+
+
+MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
 
 
 def main():
@@ -195,6 +203,14 @@ def main():
         },
     }
 
+    # the bound that applies: L1 cache-line accesses per clock per CU (at most
+    # about one), from the committed counter summary of the same kernel
+    l1 = _profile_value(f"{tag}_pmc.txt", "TCP_TOTAL_CACHE_ACCESSES per launch")
+    if l1 is not None:
+        rate = l1 / (lic_ms / 1000.0) / (MI355X_CUS * MI355X_CLOCK_HZ)
+        out["roofline"]["l1_gather"] = {"accesses_per_launch": l1, "per_clock_per_cu": rate, "peak": 1.0,
+                                        "frac": rate, "hbm_traffic_gbs": (traffic or 0) / (lic_ms / 1000.0) / 1e9}
+
     if not args.no_extras:
         def timed(fn, reps=3):
             fn()
@@ -247,9 +263,13 @@ def main():
                 f_ms = timed(lambda: ctx.render_fused(fcam, out=fimg))
                 fused[f"{name}_{w}"] = dict(ms=f_ms, ray_samples=int(fns),
                                             lic_samples_per_s=fns * (2 * L + 1) / (f_ms / 1000.0))
+                _, sns = ctx.render_slices(fcam, out=fimg, count_samples=True)
+                s_ms = timed(lambda: ctx.render_slices(fcam, out=fimg))
+                fused[f"slices_{name}_{w}"] = dict(ms=s_ms, samples=int(sns))
         out["fused"] = {
-            "what": "on-the-fly LIC ray caster: every ray sample integrates its own streamline "
-                    "(no LIC volume), same field/filter/TF, square views",
+            "what": "on-the-fly LIC casters (fused ray caster and view-aligned slicing): every "
+                    "sample integrates its own streamline (no LIC volume), same field/filter/TF, "
+                    "square views; lanes per ray chosen automatically",
             **fused,
             "lic_volume_plus_frame_ms_fp32": legs["fp32"]["lic_ms"] + legs["fp32"]["frame_ms"],
         }
@@ -258,6 +278,11 @@ def main():
 
     if not args.no_batch:
         out["batch"] = batch_leg(args, dev)
+        if args.storage == "fp32":
+            fp16_args = argparse.Namespace(**{**vars(args), "storage": "fp16"})
+            b16 = batch_leg(fp16_args, dev)
+            out["batch"]["fp16"] = {k: b16[k] for k in ("batch_ms", "view_latency_ms_p50", "view_latency_ms_p99",
+                                                        "aggregate_samples_per_s")}
 
     if not args.no_precision:
         out["precision"] = precision_leg(dev)
@@ -478,7 +503,7 @@ def cpu_baseline(field_np, dims, L, lut, inv, p, seconds, samples_per_voxel):
     noise = orc.gen_noise(1, 1, 0.0, dims)
     op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.005, inv_filter_area=inv,
                     gradient_scale=3.0)
-    z0 = n // 2
+    z0 = 0
     t = time.perf_counter()
     orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + 1, nthreads=cores)
     one = time.perf_counter() - t
