@@ -230,7 +230,7 @@ def main():
         dtf = torch.empty(1024, dtype=torch.float32, device=dev)
         dsig = torch.empty(vox, dtype=torch.float32, device=dev)
         legs = {}
-        for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
+        for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16), ("unorm8", vv.STORE_UNORM8)):
             ctx.upload_field(field, dims, storage=st)
             l_ms = timed(ctx.compute_lic)
             f_ms = timed(lambda: ctx.render(cam, out=image))
@@ -247,9 +247,11 @@ def main():
             "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms)",
             "fp32": legs["fp32"],
             "fp16": legs["fp16"],
+            "unorm8": legs["unorm8"],
             "fp32_plain_lic_ms": plain_lic,
             "fp16_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["fp16"]["pipeline_ms"],
             "fp16_speedup_vs_fp32_plain_kernel": plain / legs["fp16"]["pipeline_ms"],
+            "unorm8_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["unorm8"]["pipeline_ms"],
         }
         out["fwd_plus_adjoint_ms"] = legs[args.storage]["frame_ms"] + legs[args.storage]["adjoint_ms"]
         # the fused caster (lic3d_fragment.glsl): LIC of every ray sample on the fly
@@ -426,7 +428,7 @@ def precision_leg(dev, n=512, L=60, box=64, W=1024):
     ctx.generate_noise(1, vv.NOISE_BINARY, 0.04, dims)
     stream = torch.cuda.ExternalStream(ctx.stream)
     res = {}
-    for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
+    for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16), ("unorm8", vv.STORE_UNORM8)):
         ctx.upload_field(field, dims, storage=st)
         ctx.compute_lic()
         ctx.synchronize()
@@ -442,22 +444,25 @@ def precision_leg(dev, n=512, L=60, box=64, W=1024):
         torch.cuda.synchronize()
         res[name] = dict(lic=lic, img=img, lic_ms=e0.elapsed_time(e1))
     ctx.close()
-    a, b = res["fp16"], res["fp32"]
+    b = res["fp32"]
     scale = float(b["lic"].abs().max())
-    dl = (a["lic"] - b["lic"]).double().abs()
-    di = (a["img"] - b["img"]).double().abs()
     out = {
         "workload": f"{n}^3 Rankine x12, field 3xFP16 ({n ** 3 * 6 / 1e6:.0f} MB) vs FP32, 4% binary noise, "
-                    f"cos^2 L={L} h=0.005; {W}^2 frame",
+                    f"cos^2 L={L} h=0.005; {W}^2 frame; also the UNORM8 field (4 B per voxel)",
         "lic_ms_fp32": b["lic_ms"],
-        "lic_ms_fp16": a["lic_ms"],
-        "fp16_vs_fp32": {
+        "lic_ms_fp16": res["fp16"]["lic_ms"],
+        "lic_ms_unorm8": res["unorm8"]["lic_ms"],
+    }
+    for name in ("fp16", "unorm8"):
+        a = res[name]
+        dl = (a["lic"] - b["lic"]).double().abs()
+        di = (a["img"] - b["img"]).double().abs()
+        out[f"{name}_vs_fp32"] = {
             "lic_max_rel": float(dl.max()) / scale,
             "lic_rms_rel": float(dl.pow(2).mean().sqrt()) / scale,
             "image_max_abs": float(di.max()),
             "image_rms": float(di.pow(2).mean().sqrt()),
-        },
-    }
+        }
     # FP64 oracle on a central box: the whole field is encoded in double
     host = field.cpu().numpy()
     del field
@@ -477,7 +482,7 @@ def precision_leg(dev, n=512, L=60, box=64, W=1024):
     del f4
     ref = ref.reshape(box, box, box)
     rscale = float(np.max(np.abs(ref)))
-    for name in ("fp32", "fp16"):
+    for name in ("fp32", "fp16", "unorm8"):
         sub = res[name]["lic"].view(n, n, n)[b0:b0 + box, b0:b0 + box, b0:b0 + box].cpu().numpy()
         d = np.abs(sub.astype(np.float64) - ref)
         out[f"{name}_vs_fp64_{box}^3"] = {"lic_max_rel": float(d.max()) / rscale,

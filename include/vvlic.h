@@ -54,7 +54,10 @@ typedef enum vv_status {
 
 typedef enum vv_storage {
     VV_STORE_FP32 = 0,    /* 4 x float32 per voxel: encoded direction + |v|/max */
-    VV_STORE_FP16 = 1     /* 4 x float16 per voxel, FP32 arithmetic */
+    VV_STORE_FP16 = 1,    /* 4 x float16 per voxel, FP32 arithmetic */
+    VV_STORE_UNORM8 = 2   /* 4 x UNORM8 per voxel, the reference's GL_RGBA field
+                           * (VectorDataSet::fillTexDataCharInterp,
+                           * VV/dataset.cpp:704-823), FP32 arithmetic */
 } vv_storage;
 
 typedef enum vv_filter_kind {
@@ -142,7 +145,11 @@ vv_status vv_default_camera(vv_camera* cam, int width, int height, const float c
 /* ---- a1: vector field (VectorDataSet::loadData + createTextureIterp) ----
  * v0, v1: 3*N float32 (host or device); v1 may be NULL (no time lerp).
  * v = v0 + lerp_t*(v1 - v0) is encoded on the GPU as
- * rgb = 0.5 v/|v| + 0.5, a = |v|/max|v| and kept in `storage`.
+ * rgb = 0.5 v/|v| + 0.5, a = |v|/max|v| and kept in `storage`
+ * (fillTexDataFloatInterp, VV/dataset.cpp:533-635).  VV_STORE_UNORM8 keeps
+ * the bytes of fillTexDataCharInterp instead: rgb = (unsigned char)
+ * (v/|v|*127 + 128), 128 for |v| < 1e-5, a = (unsigned char)(|v|/max*255),
+ * sampled as byte/255.
  * spacing = DAT slice distances; extent/scale/center follow
  * VV/dataset.cpp:144-176. */
 vv_status vv_upload_field(vv_ctx* ctx, const float* v0, const float* v1, float lerp_t,
@@ -151,7 +158,8 @@ vv_status vv_upload_field(vv_ctx* ctx, const float* v0, const float* v1, float l
 /* extent[3], scale[3], center[3] of the loaded field (VolumeData). */
 vv_status vv_field_geometry(const vv_ctx* ctx, float extent[3], float scale[3],
                             float center[3]);
-/* Encoded field, 4*N values (float32, also for FP16 storage). */
+/* Encoded field, 4*N values (float32 for every storage; UNORM8 bytes b
+ * come back as b * (1/255)). */
 vv_status vv_download_field(vv_ctx* ctx, float* out4, int on_device);
 
 /* ---- a2/a3: noise (NoiseDataSet) --------------------------------------

@@ -52,6 +52,7 @@ class Params(C.Structure):
         ("lut_zoeckler", C.c_void_p),
         ("lut_mallo_diff", C.c_void_p),
         ("lut_mallo_spec", C.c_void_p),
+        ("field_unorm8", C.c_int),
     ]
 
 
@@ -133,6 +134,7 @@ def _load() -> C.CDLL:
         "orc_view_setup": (None, [D, I, I, D, P, P, P, P, P, C.POINTER(View)]),
         "orc_light_position": (None, [P, P, P, D, P, P]),
         "orc_encode_field_f32": (None, [P, P, F, I, I, I, P]),
+        "orc_encode_field_u8": (None, [P, P, F, I, I, I, P]),
         "orc_encode_field_f64": (None, [P, P, D, I, I, I, P]),
         "orc_lic_weights_f32": (None, [P, I, I, I, P]),
         "orc_lic_weights_f64": (None, [P, I, I, I, P]),
@@ -293,6 +295,24 @@ def encode_field(v0: np.ndarray, dims, v1=None, t=0.0, f64=False) -> np.ndarray:
     fn = LIB.orc_encode_field_f64 if f64 else LIB.orc_encode_field_f32
     fn(_p(v0), _p(v1), t, nx, ny, nz, _p(out))
     return out
+
+
+def encode_field_u8(v0: np.ndarray, dims, v1=None, t=0.0) -> np.ndarray:
+    """UNORM8 field bytes (4 per voxel): fillTexDataCharInterp.  Pass
+    ``field_bytes_as_float(...)`` with ``params(field_unorm8=1)`` to the
+    sampling functions."""
+    nx, ny, nz = dims
+    v0 = np.ascontiguousarray(v0, np.float32).ravel()
+    v1 = None if v1 is None else np.ascontiguousarray(v1, np.float32).ravel()
+    out = np.empty(4 * nx * ny * nz, np.uint8)
+    LIB.orc_encode_field_u8(_p(v0), _p(v1), t, nx, ny, nz, _p(out))
+    return out
+
+
+def field_bytes_as_float(b: np.ndarray, f64=False) -> np.ndarray:
+    """byte values of a UNORM8 field as the float array the sampling
+    functions take (they scale blended values by 1/255 when field_unorm8)"""
+    return np.ascontiguousarray(b, np.uint8).astype(np.float64 if f64 else np.float32)
 
 
 def lic_weights(lut: np.ndarray, F: int, B: int, f64=False) -> np.ndarray:

@@ -19,6 +19,10 @@
  *   orc_make_filter                  box inv-area 0.5 (VV/dataset.cpp:1412),
  *                                    closed-form triangle area, symmetry
  *   orc_encode_field_*               closed form, zero vector -> (.5,.5,.5,0)
+ *   orc_encode_field_u8              axis directions -> bytes 255/1, zero
+ *                                    vector -> 128, byte-boundary vectors
+ *                                    (v/|v| = (k-128)/127 exactly), max |v|
+ *                                    -> 255, all-zero field -> magnitude 0
  *   orc_lic_*                        constant noise closed form, zero noise,
  *                                    uniform field straight streamline with
  *                                    displacement n*h_e, numpy brute force on
@@ -108,6 +112,50 @@ void orc_f32_to_f16_array(const float* in, uint16_t* out, long n)
 void orc_f16_to_f32_array(const uint16_t* in, float* out, long n)
 {
     for (long i = 0; i < n; ++i) out[i] = orc_f16_to_f32(in[i]);
+}
+
+/* ------------------------------------------------------------------ */
+/* UNORM8 field encode: fillTexDataCharInterp, float source branch     */
+/* (VV/dataset.cpp:704-823, :748-770 and :776-782).  The time lerp is  */
+/* the float path's; direction byte = (unsigned char)(q*127.0 + 128.0) */
+/* with q = v/|v| a float quotient and the rest in double, as written  */
+/* (floor(127q + 128), except that the double sum rounds to 128 for    */
+/* -2^-47 <= 127q < 0); 128 when |v| < EPS (1e-5f).  The               */
+/* magnitude byte is (unsigned char)(|v|/max*255) in float, clamped to */
+/* [0,255].  A field with max |v| = 0 (0/0 in the reference) gets      */
+/* magnitude 0.  GL samples the bytes as UNORM8: value = byte/255.     */
+/* ------------------------------------------------------------------ */
+void orc_encode_field_u8(const float* v0, const float* v1, float t,
+                         int nx, int ny, int nz, uint8_t* out)
+{
+    const long n = (long)nx * ny * nz;
+    float* mag = (float*)malloc((size_t)(n > 0 ? n : 1) * sizeof(float));
+    float maxlen = -1.0f;
+    if (!v1) v1 = v0;
+    for (long i = 0; i < n; ++i) {
+        float tv[3];
+        for (int c = 0; c < 3; ++c) {
+            const float a = v0[3 * i + c], b = v1[3 * i + c];
+            tv[c] = a + t * (b - a);
+        }
+        float len = sqrtf(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
+        if (len < 1e-5f) {
+            len = 0.0f;
+            out[4 * i + 0] = out[4 * i + 1] = out[4 * i + 2] = 128;
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                const float q = tv[c] / len;
+                out[4 * i + c] = (uint8_t)((double)q * 127.0 + 128.0);
+            }
+        }
+        mag[i] = len;
+        if (len > maxlen) maxlen = len;
+    }
+    for (long i = 0; i < n; ++i) {
+        float m = maxlen > 0.0f ? mag[i] / maxlen * 255.0f : 0.0f;
+        out[4 * i + 3] = (uint8_t)(m > 255.0f ? 255.0f : (m < 0.0f ? 0.0f : m));
+    }
+    free(mag);
 }
 
 /* ------------------------------------------------------------------ */

@@ -60,6 +60,9 @@ typedef struct orc_params {
     const float* lut_zoeckler;   /* 256x256 x (diffuse, specular) */
     const float* lut_mallo_diff; /* 256x256 x rgb */
     const float* lut_mallo_spec; /* 256x256 x rgb */
+    /* 1: the field array holds UNORM8 byte values (0..255, orc_encode_field_u8);
+     * blended field values are scaled by 1/255 like every UNORM8 texture */
+    int   field_unorm8;
 } orc_params;
 
 /* Line material of the illumination tables (LineMat, VV/illumination.h:40-62). */
@@ -121,6 +124,10 @@ void orc_light_position(const double center[3], const double cam_pos[3],
 /* ---- floating point work, two precisions ---- */
 void orc_encode_field_f32(const float* v0, const float* v1, float t,
                           int nx, int ny, int nz, float* out4);
+/* UNORM8 field (fillTexDataCharInterp, VV/dataset.cpp:704-823): 4 bytes per
+ * voxel, direction bytes and |v|/max as a byte; see vvoracle.c */
+void orc_encode_field_u8(const float* v0, const float* v1, float t,
+                         int nx, int ny, int nz, uint8_t* out);
 void orc_encode_field_f64(const float* v0, const float* v1, double t,
                           int nx, int ny, int nz, double* out4);
 

@@ -94,6 +94,14 @@ static inline void FN(cell_repeat)(const R p[3], const int d[3], FN(cell)* c)
         (out) = FN(lerp)(c0, c1, (c).fz);                                            \
     } while (0)
 
+/* channel ch of the 4-channel field at cell c: a UNORM8 field holds byte
+ * values and its blend is scaled by 1/255 */
+#define ORC_FIELD_TRILIN(p, arr, ch, d, c, out)                                   \
+    do {                                                                             \
+        ORC_TRILIN(R, arr, 4, ch, d, c, out);                                        \
+        if ((p)->field_unorm8) (out) = (out) * K(1.0 / 255.0);                       \
+    } while (0)
+
 /* 1D linear lookup with CLAMP_TO_EDGE over n entries of an nc-channel table */
 static inline R FN(lut1d)(const R* t, int n, int nc, int ch, R u)
 {
@@ -205,9 +213,9 @@ static inline void FN(field_rgb)(const FN(lic_ctx)* c, const R x[3], R s[3])
 {
     FN(cell) cl;
     FN(cell_clamp)(x, c->fd, &cl);
-    ORC_TRILIN(R, c->field, 4, 0, c->fd, cl, s[0]);
-    ORC_TRILIN(R, c->field, 4, 1, c->fd, cl, s[1]);
-    ORC_TRILIN(R, c->field, 4, 2, c->fd, cl, s[2]);
+    ORC_FIELD_TRILIN(c->p, c->field, 0, c->fd, cl, s[0]);
+    ORC_FIELD_TRILIN(c->p, c->field, 1, c->fd, cl, s[1]);
+    ORC_FIELD_TRILIN(c->p, c->field, 2, c->fd, cl, s[2]);
 }
 
 /* freqSampling (inc_lic.glsl:71-90) / freqSamplingGrad (inc_lic.glsl:61-68) */
@@ -428,7 +436,7 @@ static void FN(rc_shade)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], co
     FN(cell) cl;
     R v;
     FN(cell_clamp)(pos, c->fd, &cl);
-    ORC_TRILIN(R, c->field, 4, p->tf_index ? 3 : 2, c->fd, cl, v);
+    ORC_FIELD_TRILIN(p, c->field, p->tf_index ? 3 : 2, c->fd, cl, v);
     s->tfu = v;
     for (int ch = 0; ch < 4; ++ch) s->tf[ch] = FN(lut1d)(c->tf, 256, 4, ch, v);
     R g = (R)p->illum_scale;
@@ -441,7 +449,7 @@ static void FN(rc_shade)(const FN(rc_ctx)* c, const R pos[3], const R dir[3], co
         R tg[3], T[3];
         for (int a = 0; a < 3; ++a) {
             R x;
-            ORC_TRILIN(R, c->field, 4, a, c->fd, cf, x);
+            ORC_FIELD_TRILIN(p, c->field, a, c->fd, cf, x);
             tg[a] = FMA(K(2), x, K(-1));
         }
         FN(normalize3)(tg, T);
@@ -740,7 +748,7 @@ long FN(orc_render_slices)(const R* field4, const int fdims[3],
                 FN(cell) cl;
                 R tfu;
                 FN(cell_clamp)(pos, fdims, &cl);
-                ORC_TRILIN(R, field4, 4, p->tf_index ? 3 : 2, fdims, cl, tfu);
+                ORC_FIELD_TRILIN(p, field4, p->tf_index ? 3 : 2, fdims, cl, tfu);
                 if (!(FN(lut1d)(tf_rgba, 256, 4, 3, tfu) > K(0.05))) continue;
                 R l4[4];
                 FN(lic_voxel)(&lc, w, pos, l4);

@@ -20,9 +20,9 @@ _LUTS = None
 
 
 def _scene(ctx, n=16, grad=False, step=1.0 / 128.0, W=48, H=40, quat=(0.2, -0.1, 0.05, 0.97),
-           sigma_seed=None, tf_seed=3, op_scale=1.0, illum=None):
+           sigma_seed=None, tf_seed=3, op_scale=1.0, illum=None, storage=vv.STORE_FP32):
     global _LUTS
-    c = lic_setup(ctx, n=n, kind="rankine", grad=grad, L=6, density=0.3)
+    c = lic_setup(ctx, n=n, kind="rankine", grad=grad, L=6, density=0.3, storage=storage)
     p = c["params"]
     p.step_size_vol = step
     if illum is not None:
@@ -53,14 +53,17 @@ def _scene(ctx, n=16, grad=False, step=1.0 / 128.0, W=48, H=40, quat=(0.2, -0.1,
     return c, cam, rgba, op, sigma, oparams, view
 
 
-@pytest.mark.parametrize("grad,step,with_sigma,illum", [(False, 1.0 / 128.0, True, None),
-                                                        (True, 1.0 / 256.0, True, None),
-                                                        (False, 1.0 / 256.0, False, None),
-                                                        (False, 1.0 / 128.0, True, vv.ILLUM_MALLO),
-                                                        (True, 1.0 / 256.0, True, vv.ILLUM_ZOECKLER)])
-def test_adjoint_matches_oracle(ctx, grad, step, with_sigma, illum):
+@pytest.mark.parametrize("grad,step,with_sigma,illum,storage", [
+    (False, 1.0 / 128.0, True, None, vv.STORE_FP32),
+    (True, 1.0 / 256.0, True, None, vv.STORE_FP32),
+    (False, 1.0 / 256.0, False, None, vv.STORE_FP32),
+    (False, 1.0 / 128.0, True, vv.ILLUM_MALLO, vv.STORE_FP32),
+    (True, 1.0 / 256.0, True, vv.ILLUM_ZOECKLER, vv.STORE_FP32),
+    (False, 1.0 / 128.0, True, None, vv.STORE_UNORM8)])
+def test_adjoint_matches_oracle(ctx, grad, step, with_sigma, illum, storage):
     c, cam, rgba, op, sigma, oparams, view = _scene(ctx, grad=grad, step=step,
-                                                    sigma_seed=7 if with_sigma else None, illum=illum)
+                                                    sigma_seed=7 if with_sigma else None, illum=illum,
+                                                    storage=storage)
     W, H = cam.width, cam.height
     dimg = np.random.default_rng(11).normal(size=W * H * 4).astype(np.float32)
     dtf, dsig = ctx.render_adjoint(cam, dimg)

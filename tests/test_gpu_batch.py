@@ -97,7 +97,7 @@ def _check(b, outs, lics, imgs, total):
     assert b.samples() == total
 
 
-@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8])
 def test_batch_matches_single(ctx, storage):
     _setup(ctx)
     lics, centers, imgs, total = _single(ctx, storage)

@@ -49,7 +49,7 @@ def _fused_case(ctx, c, w, h, quat=(0.15, 0.25, 0.05, 0.95), illum=None, luts=No
     return img, ref, nsamp, total
 
 
-@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8])
 def test_fused_cached_path(ctx, storage):
     c = lic_setup(ctx, n=16, kind="rankine", L=6, noise_kind=vv.NOISE_UNIFORM, storage=storage,
                   gradient_scale=3.0)
@@ -126,7 +126,7 @@ def _slices_case(ctx, c, w, h, quat=(0.1, 0.3, -0.05, 0.95), illum=None, luts=No
     return img, ref, nsamp, total
 
 
-@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8])
 def test_slices_cached_path(ctx, storage):
     c = lic_setup(ctx, n=16, kind="rankine", L=6, noise_kind=vv.NOISE_UNIFORM, storage=storage,
                   gradient_scale=3.0)

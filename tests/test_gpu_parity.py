@@ -42,6 +42,28 @@ def test_field_encode_fp16_bitexact(ctx):
     np.testing.assert_array_equal(got, ref)
 
 
+def test_field_encode_unorm8_bitexact(ctx):
+    n = 18
+    v = field("rankine", n).copy()
+    v1 = field("abc", n)
+    v[0, 0, :4] = 0.0                                     # zero vectors -> 128
+    v[1, 2, 3] = (-1e-9, 1.0, 0.0)                        # 127q+128 just below 128
+    v[1, 2, 4] = (-1e-40, 1.0, 0.0)                       # subnormal component: 128
+    v[1, 2, 5] = (3.0, 4.0, 0.0)
+    v[1, 2, 6] = (-2.0 ** -53, 1.0, 0.0)                  # 127
+    v[1, 2, 7] = (-2.0 ** -54, 1.0, 0.0)                  # 128 (the double sum rounds up)
+    for (a, b, t) in ((v, None, 0.0), (v, v1, 0.3)):
+        ctx.upload_field(a, (n, n, n), v1=b, lerp_t=t, storage=vv.STORE_UNORM8)
+        got = ctx.download_field((n, n, n))
+        ref = orc.encode_field_u8(a, (n, n, n), v1=b, t=t)
+        # bytes come back as b * (1/255) in float
+        np.testing.assert_array_equal(got, ref.astype(np.float32) * np.float32(1.0 / 255.0))
+        np.testing.assert_array_equal(np.rint(got * 255.0).astype(np.uint8), ref)
+    ctx.upload_field(np.zeros((n, n, n, 3), np.float32), (n, n, n), storage=vv.STORE_UNORM8)
+    z = np.rint(ctx.download_field((n, n, n)) * 255.0).reshape(-1, 4)
+    assert np.all(z == [128, 128, 128, 0])
+
+
 def test_field_zero_vector(ctx):
     n = 4
     v = np.zeros((n, n, n, 3), np.float32)
@@ -102,6 +124,10 @@ LIC_CASES = [
     dict(n=16, kind="abc", filt=vv.FILTER_COS2, L=12, h=0.04, freq=2.0),
     dict(n=16, kind="rankine", grad=True, L=10, density=0.3),
     dict(n=20, kind="rankine", storage=vv.STORE_FP16, filt=vv.FILTER_COS2, L=15, h=0.005),
+    dict(n=20, kind="rankine", storage=vv.STORE_UNORM8, noise_kind=vv.NOISE_UNIFORM, filt=vv.FILTER_GAUSSIAN,
+         L=15, h=0.005),
+    dict(n=16, kind="abc", storage=vv.STORE_UNORM8, filt=vv.FILTER_TRIANGLE, L=10, h=0.02, freq=1.5),
+    dict(n=14, kind="gyre", storage=vv.STORE_UNORM8, grad=True, L=8, density=0.3),
 ]
 
 
@@ -241,6 +267,23 @@ def test_render_illum_lic(ctx, tf_index):
     _report("render", img, ref)
     assert nsamp == total
     assert np.max(ref) > 0
+    assert rel_err(img, ref) <= TOL
+
+
+@pytest.mark.parametrize("tf_index,illum,grad", [(vv.TF_INDEX_B, vv.ILLUM_LIC, False),
+                                                (vv.TF_INDEX_MAG, vv.ILLUM_GRADIENT, True),
+                                                (vv.TF_INDEX_MAG, vv.ILLUM_MALLO, False)])
+def test_render_unorm8_field(ctx, tf_index, illum, grad):
+    # the field is read by the ray caster for the TF index and the line tangent
+    c = lic_setup(ctx, n=20, kind="rankine", L=6, grad=grad, density=0.3, storage=vv.STORE_UNORM8,
+                  gradient_scale=3.0)
+    ctx.compute_lic()
+    np.testing.assert_allclose(ctx.download_lic(), oracle_lic(c), rtol=0, atol=TOL * np.max(np.abs(oracle_lic(c))))
+    img, ref, nsamp, total = _render_case(ctx, c, 48, 40, quat=(0.2, 0.3, 0.1, 0.927), tf_index=tf_index,
+                                          illum=illum, luts=orc.illum_luts(), light=(0.9, 1.4, 2.2))
+    _report(f"render-u8-{tf_index}-{illum}", img, ref)
+    assert nsamp == total
+    assert np.max(ref) > 0.01
     assert rel_err(img, ref) <= TOL
 
 

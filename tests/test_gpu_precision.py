@@ -9,7 +9,12 @@ discontinuity: a ray whose opacity ends within rounding distance of 0.95
 takes one sample more or less in FP32 than in FP64, whatever the storage.
 The image bound (max abs error <= 1e-3 for FP16 storage) therefore applies
 to pixels whose termination index agrees with the FP64 run; the fraction of
-pixels where it differs is reported and bounded separately.  The GPU result
+pixels where it differs is reported and bounded separately.
+
+UNORM8 storage (the reference's GL_RGBA field) quantises directions to
+1/127 and |v|/max to 1/255: the LIC values stay within ~4e-3 of FP64, but
+the vector-length TF index moves by up to half a byte, so image errors
+reach ~1e-2 and the termination index differs on ~6% of pixels.  The GPU result
 itself is checked against the oracle's FP32 path on the same stored field.
 """
 import numpy as np
@@ -20,6 +25,7 @@ from vectorvisualization_amd import fields
 from oracle import oracle as orc
 
 from conftest import rel_err
+from workloads import oracle_field
 
 pytestmark = pytest.mark.gpu
 
@@ -60,17 +66,18 @@ def _gpu(ctx, c, storage):
     return lic, img
 
 
-@pytest.mark.parametrize("storage,lic_tol,img_tol", [(vv.STORE_FP32, 1e-4, 5e-4),
-                                                     (vv.STORE_FP16, 2e-2, 1e-3)])
-def test_error_against_fp64_oracle(ctx, case, storage, lic_tol, img_tol):
+@pytest.mark.parametrize("storage,lic_tol,img_tol,flip_tol", [(vv.STORE_FP32, 1e-4, 5e-4, 0.005),
+                                                              (vv.STORE_FP16, 2e-2, 1e-3, 0.005),
+                                                              (vv.STORE_UNORM8, 1e-2, 2e-2, 0.1)])
+def test_error_against_fp64_oracle(ctx, case, storage, lic_tol, img_tol, flip_tol):
     c = case
     lic, img = _gpu(ctx, c, storage)
     # the oracle's FP32 path on the same stored field
-    f4s = orc.encode_field(c["v"], c["dims"])
-    if storage == vv.STORE_FP16:
-        f4s = orc.f16_to_f32(orc.f32_to_f16(f4s))
-    lic_s = orc.lic(f4s, c["dims"], c["noise"], 1, c["dims"], c["lut"], c["p"])
-    img_s, ns_s, _ = orc.raycast(f4s, c["dims"], lic_s, 1, c["dims"], c["rgba"], c["op"], c["p"],
+    f4s = oracle_field(c["v"], c["dims"], storage)
+    ps = orc.Params.from_buffer_copy(c["p"])
+    ps.field_unorm8 = 1 if storage == vv.STORE_UNORM8 else 0
+    lic_s = orc.lic(f4s, c["dims"], c["noise"], 1, c["dims"], c["lut"], ps)
+    img_s, ns_s, _ = orc.raycast(f4s, c["dims"], lic_s, 1, c["dims"], c["rgba"], c["op"], ps,
                                  c["view"])
     assert rel_err(lic, lic_s) <= 1e-5
     assert rel_err(img, img_s) <= 1e-5
@@ -81,7 +88,7 @@ def test_error_against_fp64_oracle(ctx, case, storage, lic_tol, img_tol):
     di = np.abs(img.astype(np.float64) - c["img64"]).reshape(-1, 4)
     di_same = di[same]
     flip = 1.0 - same.mean()
-    name = "fp16" if storage == vv.STORE_FP16 else "fp32"
+    name = {vv.STORE_FP32: "fp32", vv.STORE_FP16: "fp16", vv.STORE_UNORM8: "unorm8"}[storage]
     print(f"{name}: LIC max rel {dl.max() / scale:.3e} rms rel {np.sqrt(np.mean(dl ** 2)) / scale:.3e}; "
           f"image max abs {di_same.max():.3e} rms {np.sqrt(np.mean(di_same ** 2)):.3e} "
           f"(termination index differs on {flip * 100:.3f}% of pixels; all-pixel max {di.max():.3e})",
@@ -89,4 +96,4 @@ def test_error_against_fp64_oracle(ctx, case, storage, lic_tol, img_tol):
     assert np.max(c["img64"]) > 0.05
     assert dl.max() / scale <= lic_tol
     assert di_same.max() <= img_tol
-    assert flip <= 0.005
+    assert flip <= flip_tol

@@ -204,6 +204,83 @@ def test_encode_closed_form_and_zero_vector():
     assert np.all(got32[0, 0, 0] == [0.5, 0.5, 0.5, 0.0])
 
 
+def test_encode_u8_known_answers():
+    # fillTexDataCharInterp: byte = (unsigned char)(v/|v|*127.0 + 128.0), truncated
+    v = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0], [3, 4, 0], [-3, -4, 0],
+                  [-1e-6, 1, 0], [-1e-9, 1, 0], [0, 0, 0.5],
+                  [-2.0 ** -53, 1, 0], [-2.0 ** -54, 1, 0], [-1e-40, 1, 0]], np.float32)
+    got = orc.encode_field_u8(v, (11, 1, 1)).reshape(11, 4)
+    # directions: 0.6f*127+128 = 204.2, 0.8f*127+128 = 229.6 -> 229 (not rounded
+    # to 230), -0.6f -> 51.8 -> 51, -0.8f -> 26.4 -> 26; a component just below
+    # zero gives 127 even where float arithmetic would round 127q+128 to 128,
+    # but 128 once 127|q| <= 2^-47, where the reference's double sum rounds
+    # to 128.0 (127 * 2^-53 > 2^-47 > 127 * 2^-54; 1e-40 is subnormal)
+    np.testing.assert_array_equal(got[:, :3], [[255, 128, 128], [1, 128, 128], [128, 128, 128],
+                                              [204, 229, 128], [51, 26, 128], [127, 255, 128],
+                                              [127, 255, 128], [128, 128, 255],
+                                              [127, 255, 128], [128, 255, 128], [128, 255, 128]])
+    # magnitudes: |v|/5 * 255 truncated: 1 -> 51, 5 -> 255, 0 -> 0, 0.5 -> 25.5 -> 25
+    np.testing.assert_array_equal(got[:, 3], [51, 51, 0, 255, 255, 51, 51, 25, 51, 51, 51])
+
+
+def test_encode_u8_lerp_and_degenerate_fields():
+    v0 = np.array([[1, 0, 0], [0, 0, 2]], np.float32)
+    v1 = np.array([[0, 1, 0], [0, 0, 2]], np.float32)
+    got = orc.encode_field_u8(v0, (2, 1, 1), v1=v1, t=0.5).reshape(2, 4)
+    # (0.5, 0.5, 0): q = 0.70710677 -> 127q + 128 = 217.8 -> 217; |v| = 0.7071 of max 2
+    np.testing.assert_array_equal(got[0], [217, 217, 128, 90])
+    np.testing.assert_array_equal(got[1], [128, 128, 255, 255])
+    # an all-zero field: rgb 128, magnitude 0 (0/0 in the reference)
+    z = orc.encode_field_u8(np.zeros((5, 3), np.float32), (5, 1, 1)).reshape(5, 4)
+    assert np.all(z == [128, 128, 128, 0])
+
+
+def test_encode_u8_within_one_byte_of_the_direction():
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=(4000, 3)).astype(np.float32)
+    b = orc.encode_field_u8(v, (4000, 1, 1)).reshape(-1, 4).astype(np.float64)
+    q = v / np.linalg.norm(v.astype(np.float64), axis=1, keepdims=True)
+    e = b[:, :3] - (127.0 * q + 128.0)   # truncation: in (-1, 0]
+    assert np.all(e <= 1e-6) and np.all(e > -1.0 - 1e-6)
+    m = np.linalg.norm(v.astype(np.float64), axis=1)
+    em = b[:, 3] - 255.0 * m / m.max()
+    assert np.all(em <= 1e-4) and np.all(em > -1.0 - 1e-4)
+
+
+def test_unorm8_sampling_equals_prescaled_field():
+    # sampling bytes with field_unorm8 (blend, then x 1/255) and sampling the
+    # pre-scaled field b/255 differ only by rounding, in every use of the
+    # field: LIC directions and the ray caster's TF index (.b and .a) and
+    # line-illumination tangents
+    n, L = 10, 6
+    dims = (n, n, n)
+    v = np.random.default_rng(12).normal(size=(n, n, n, 3)).astype(np.float32)
+    b = orc.encode_field_u8(v, dims)
+    fb = orc.field_bytes_as_float(b, f64=True)
+    fs = b.astype(np.float64) / 255.0
+    noise = orc.gen_noise(3, 1, 0.0, dims)
+    lut, inv = orc.make_filter(0, L)
+    pu = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.02, inv_filter_area=inv,
+                    field_unorm8=1, gradient_scale=3.0)
+    pf = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.02, inv_filter_area=inv,
+                    gradient_scale=3.0)
+    a = orc.lic(fb, dims, noise, 1, dims, lut, pu)
+    c = orc.lic(fs, dims, noise, 1, dims, lut, pf)
+    np.testing.assert_allclose(a, c, rtol=1e-12, atol=1e-12)
+    assert np.max(np.abs(a - orc.lic(fb, dims, noise, 1, dims, lut, pf))) > 0.1  # the flag matters
+    rgba, op = np.random.default_rng(13).uniform(0, 1, 1024), np.linspace(0, 1, 256)
+    luts = orc.illum_luts()
+    view = orc.view_setup(24, 24, quat=(0.2, 0.3, 0.1, 0.927))
+    for tf_index, illum in ((0, 0), (1, 0), (0, 2), (1, 3)):
+        for q in (pu, pf):
+            q.tf_index, q.illum_mode = tf_index, illum
+            orc.attach_luts(q, luts)
+        iu, nu, _ = orc.raycast(fb, dims, a, 1, dims, rgba, op, pu, view, f64=True)
+        ifl, nf, _ = orc.raycast(fs, dims, a, 1, dims, rgba, op, pf, view, f64=True)
+        np.testing.assert_array_equal(nu, nf)
+        np.testing.assert_allclose(iu, ifl, rtol=0, atol=1e-12)
+
+
 # ---- LIC -------------------------------------------------------------------
 
 

@@ -35,6 +35,26 @@ def oracle_noise(seed: int, kind: int, density: float, dims, with_gradients: boo
     return out.ravel()
 
 
+def oracle_field(v: np.ndarray, dims, storage: int, v1=None, t=0.0) -> np.ndarray:
+    """The field array the oracle samples for a storage type: encoded floats
+    (FP32), the same rounded to halves (FP16), or UNORM8 byte values (to be
+    sampled with params(field_unorm8=1))."""
+    if storage == vv.STORE_UNORM8:
+        return orc.field_bytes_as_float(orc.encode_field_u8(v, dims, v1=v1, t=t))
+    f4 = orc.encode_field(v, dims, v1=v1, t=t)
+    if storage == vv.STORE_FP16:
+        f4 = orc.f16_to_f32(orc.f32_to_f16(f4))
+    return f4
+
+
+def float_field_params(op):
+    """copy of oracle params for an unquantised float field (the FP64 error
+    reference of a UNORM8 run)"""
+    q = orc.Params.from_buffer_copy(op)
+    q.field_unorm8 = 0
+    return q
+
+
 def lic_setup(ctx: vv.Context, *, n=32, kind="abc", noise_kind=vv.NOISE_BINARY, density=0.05,
               grad=False, filt=vv.FILTER_BOX, L=20, h=0.01, storage=vv.STORE_FP32, seed=1,
               gradient_scale=30.0, freq=1.0, gate=None):
@@ -58,21 +78,22 @@ def lic_setup(ctx: vv.Context, *, n=32, kind="abc", noise_kind=vv.NOISE_BINARY, 
         ctx.upload_scalar(scalar, dims)
         p.gate_enable = 1
     ctx.set_lic_params(p)
-    f4 = orc.encode_field(v, dims)
-    if storage == vv.STORE_FP16:
-        f4 = orc.f16_to_f32(orc.f32_to_f16(f4))
+    f4 = oracle_field(v, dims, storage)
     noise = oracle_noise(seed, noise_kind, density, dims, grad)
     op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=h, inv_filter_area=inv,
                     gradient_scale=gradient_scale, freq_scale=freq,
-                    illum_mode=1 if grad else 0, gate_enable=1 if gate is not None else 0)
+                    illum_mode=1 if grad else 0, gate_enable=1 if gate is not None else 0,
+                    field_unorm8=1 if storage == vv.STORE_UNORM8 else 0)
     return dict(v=v, dims=dims, f4=f4, noise=noise, nch=4 if grad else 1, lut=lut, inv=inv,
                 op=op, params=p, scalar=scalar)
 
 
 def oracle_lic(case, f64=False, **kw):
-    f4 = case["f4"].astype(np.float64) if f64 else case["f4"]
+    f4 = case["f4"]
+    op = case["op"]
     if f64:
         f4 = orc.encode_field(case["v"], case["dims"], f64=True)
+        op = float_field_params(op)
     return orc.lic(f4, case["dims"], case["noise"], case["nch"], case["dims"], case["lut"],
-                   case["op"], scalar=case["scalar"],
+                   op, scalar=case["scalar"],
                    sdims=case["dims"] if case["scalar"] is not None else None, **kw)

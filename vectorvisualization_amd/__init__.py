@@ -21,6 +21,7 @@ from ._lib import (  # noqa: F401
     make_filter,
     STORE_FP16,
     STORE_FP32,
+    STORE_UNORM8,
     FILTER_BOX,
     FILTER_COS2,
     FILTER_GAUSSIAN,

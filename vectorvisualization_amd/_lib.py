@@ -17,7 +17,7 @@ LIB_PATH = Path(os.environ.get("VVLIC_LIB", _PKG / "lib" / "libvvlic.so"))
 VV_OK, VV_EINVAL, VV_ENOMEM, VV_EDEVICE, VV_EUNSUPPORTED, VV_ESTATE = range(6)
 STATUS_NAMES = {0: "VV_OK", 1: "VV_EINVAL", 2: "VV_ENOMEM", 3: "VV_EDEVICE",
                 4: "VV_EUNSUPPORTED", 5: "VV_ESTATE", 6: "VV_EIO"}
-STORE_FP32, STORE_FP16 = 0, 1
+STORE_FP32, STORE_FP16, STORE_UNORM8 = 0, 1, 2
 FILTER_BOX, FILTER_TRIANGLE, FILTER_COS2, FILTER_GAUSSIAN = range(4)
 NOISE_BINARY, NOISE_UNIFORM = 0, 1
 ILLUM_LIC, ILLUM_GRADIENT = 0, 1

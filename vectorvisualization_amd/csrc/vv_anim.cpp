@@ -35,7 +35,7 @@ vv_status anim_alloc(vv_ctx* c, int nsteps, const int dims[3], const float spaci
     if (nsteps < 1) return fail(c, VV_EINVAL, "a series needs at least one time step");
     if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
     if (interp < 1) return fail(c, VV_EINVAL, "interp_size must be >= 1");
-    if (storage != VV_STORE_FP32 && storage != VV_STORE_FP16) return fail(c, VV_EINVAL, "unknown storage");
+    if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage");
     vv_anim* a = new (std::nothrow) vv_anim();
     if (!a) return fail(c, VV_ENOMEM, "out of host memory");
     a->ctx = c;

@@ -107,8 +107,7 @@ vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float ler
     if (!c) return VV_EINVAL;
     if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
     if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
-    if (storage != VV_STORE_FP32 && storage != VV_STORE_FP16)
-        return fail(c, VV_EINVAL, "unknown storage %d", storage);
+    if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage %d", storage);
     if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
     float sd[3] = {1, 1, 1};
     if (spacing)
@@ -136,14 +135,12 @@ vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float ler
             d1 = (const float*)c->stage2.p;
         }
     }
-    const int fp16 = storage == VV_STORE_FP16;
-    const size_t elem = fp16 ? 8 : 16;
-    HIPCHK(c, c->field.ensure(elem * (size_t)clamp_elems(dims)));
+    HIPCHK(c, c->field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims)));
     HIPCHK(c, c->counter.ensure(64));
     uint32_t* maxbits = (uint32_t*)c->counter.p;
     HIPCHK(c, launch_fill_u32(maxbits, 0u, 1, c->stream));
     HIPCHK(c, launch_field_maxlen(d0, d1, lerp_t, n, maxbits, c->stream));
-    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, fp16, c->stream));
+    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, storage, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
     float vol[3], maxvol = 0.0f;
@@ -187,7 +184,7 @@ vv_status vv_download_field(vv_ctx* c, float* out4, int on_device)
         HIPCHK(c, c->stage.ensure(bytes));
         dst = (float*)c->stage.p;
     }
-    HIPCHK(c, launch_field_decode(c->field.p, c->storage == VV_STORE_FP16, c->fdims, dst, c->stream));
+    HIPCHK(c, launch_field_decode(c->field.p, c->storage, c->fdims, dst, c->stream));
     if (!on_device) HIPCHK(c, hipMemcpyAsync(out4, dst, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return VV_OK;
@@ -447,7 +444,7 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     const int nc = c->nch;
     std::memset(&L, 0, sizeof(L));
     L.field = make_clamp_desc(c->field.p, c->fdims);
-    L.field_fp16 = c->storage == VV_STORE_FP16;
+    L.field_type = c->storage;
     L.noise = make_repeat_desc(c->noise.p, c->ndims);
     L.nc = nc;
     if (c->has_scalar) L.scalar = make_clamp_desc(c->scalar.p, c->sdims);
@@ -466,7 +463,7 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     L.out = make_repeat_desc(out, c->fdims);
     for (int a = 0; a < 3; ++a) L.tile[a] = c->lic_tile[a];
     for (int a = 0; a < 3; ++a) L.super_tile[a] = c->lic_super[a];
-    L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * (L.field_fp16 ? 8ull : 16ull);
+    L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * field_texel_bytes(L.field_type);
     L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
     // the cached kernels address through 32-bit byte offsets
     // and 24-bit index multiplies (plane strides below 2^24 elements)

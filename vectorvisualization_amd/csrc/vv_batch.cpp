@@ -91,7 +91,7 @@ vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
     const vv_lic_params& p = c->params;
     std::memset(&L, 0, sizeof(L));
     L.field = make_clamp_desc(st->field.p, st->dims);
-    L.field_fp16 = st->storage == VV_STORE_FP16;
+    L.field_type = st->storage;
     L.noise = make_repeat_desc(st->noise.p, st->dims);
     L.nc = 1;
     L.w = (const float*)c->weights.p;
@@ -106,7 +106,7 @@ vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
         L.tile[a] = st->tile[a];
         L.super_tile[a] = st->super[a];
     }
-    L.field_bytes = (unsigned long long)clamp_elems(st->dims) * (L.field_fp16 ? 8ull : 16ull);
+    L.field_bytes = (unsigned long long)clamp_elems(st->dims) * field_texel_bytes(L.field_type);
     L.noise_bytes = (unsigned long long)repeat_elems(st->dims);
     L.cache = 1;
     if (!(L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) && (long long)L.field.sz < (1ll << 24) &&
@@ -125,7 +125,7 @@ vv_status build_view_launch(vv_batch* b, const BatchView& v, RenderLaunch& R)
     const vv_lic_params& p = c->params;
     std::memset(&R, 0, sizeof(R));
     R.field = make_clamp_desc(st->field.p, st->dims);
-    R.field_fp16 = st->storage == VV_STORE_FP16;
+    R.field_type = st->storage;
     R.tf_ch = p.tf_index == VV_TF_INDEX_MAG ? 3 : 2;
     R.lic = make_repeat_desc(st->lic.p, st->dims);
     R.lc = 1;
@@ -235,7 +235,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     vv_ctx* c = b->ctx;
     if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
     if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
-    if (storage != VV_STORE_FP32 && storage != VV_STORE_FP16) return fail(c, VV_EINVAL, "unknown storage");
+    if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage");
     if (!b->steps.empty() && b->steps[0]->storage != storage)
         return fail(c, VV_EINVAL, "all timesteps of a batch share one storage type");
     if (noise_kind != VV_NOISE_BINARY && noise_kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
@@ -267,14 +267,13 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
         if (s) return bail(s);
         d0 = (const float*)c->stage.p;
     }
-    const int fp16 = storage == VV_STORE_FP16;
-    hipError_t e = st->field.ensure((fp16 ? 8 : 16) * (size_t)clamp_elems(dims));
+    hipError_t e = st->field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims));
     if (!e) e = st->noise.ensure((size_t)repeat_elems(dims));
     if (!e) e = st->lic.ensure(sizeof(float) * (size_t)repeat_elems(dims));
     if (!e) e = c->counter.ensure(64);
     if (!e) e = launch_fill_u32((uint32_t*)c->counter.p, 0u, 1, c->stream);
     if (!e) e = launch_field_maxlen(d0, d0, 0.0f, n, (uint32_t*)c->counter.p, c->stream);
-    if (!e) e = launch_field_encode(d0, d0, 0.0f, dims, (const uint32_t*)c->counter.p, st->field.p, fp16, c->stream);
+    if (!e) e = launch_field_encode(d0, d0, 0.0f, dims, (const uint32_t*)c->counter.p, st->field.p, storage, c->stream);
     if (!e) e = hipStreamSynchronize(c->stream);  // the staging buffer is reused below
     if (e) return bail(fail(c, e == hipErrorOutOfMemory ? VV_ENOMEM : VV_EDEVICE, "timestep upload: %s",
                             hipGetErrorString(e)));
@@ -336,7 +335,7 @@ vv_status vv_batch_set_field(vv_batch* b, int timestep, const float* v0, const f
     if (!e) e = launch_fill_u32((uint32_t*)c->counter.p, 0u, 1, c->stream);
     if (!e) e = launch_field_maxlen(d0, d1, t, n, (uint32_t*)c->counter.p, c->stream);
     if (!e) e = launch_field_encode(d0, d1, t, st->dims, (const uint32_t*)c->counter.p, st->field.p,
-                                    st->storage == VV_STORE_FP16, c->stream);
+                                    st->storage, c->stream);
     if (!e) e = hipStreamSynchronize(c->stream);
     if (e) return fail(c, VV_EDEVICE, "timestep encode: %s", hipGetErrorString(e));
     return VV_OK;
@@ -374,7 +373,7 @@ vv_status vv_batch_run(vv_batch* b)
     vv_status s = ensure_tf(c);
     if (!s) s = ensure_weights(c);
     if (s) return s;
-    const int fp16 = b->steps[0]->storage == VV_STORE_FP16;
+    const int ftype = b->steps[0]->storage;
     const int same = c->params.freq_scale == 1.0f;
     const size_t nsteps = b->steps.size(), nv = b->views.size();
     // LIC jobs, one per time step
@@ -435,9 +434,9 @@ vv_status vv_batch_run(vv_batch* b)
     HIPCHK(c, b->vfirst.ensure(sizeof(int) * b->host_vfirst.size()));
     int cus = 0;
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const int lic_grid = ((cus * lic_batch_blocks_per_cu(fp16, same) + 7) / 8) * 8;
-    const int render_grid = cus * render_batch_blocks_per_cu(fp16, 1, bx * by);
-    std::vector<long long> key = {(long long)nsteps, (long long)nv, lic_grid, render_grid, bx, by, fp16, same,
+    const int lic_grid = ((cus * lic_batch_blocks_per_cu(ftype, same) + 7) / 8) * 8;
+    const int render_grid = cus * render_batch_blocks_per_cu(ftype, 1, bx * by);
+    std::vector<long long> key = {(long long)nsteps, (long long)nv, lic_grid, render_grid, bx, by, ftype, same,
                                   (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
                                   (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
                                   (long long)(intptr_t)b->ctr.p};
@@ -469,7 +468,7 @@ vv_status vv_batch_run(vv_batch* b)
             LB.njobs = 1;
             LB.total = b->host_first[2 * t + 1];
             LB.grid = lic_grid;
-            LB.fp16 = fp16;
+            LB.field_type = ftype;
             LB.same = same;
             e = launch_lic_batch(LB, c->stream);
             const int nvg = b->gview[g + 1] - b->gview[g];
@@ -484,7 +483,7 @@ vv_status vv_batch_run(vv_batch* b)
             RB.block[0] = bx;
             RB.block[1] = by;
             RB.grid = render_grid;
-            RB.fp16 = fp16;
+            RB.field_type = ftype;
             RB.lc = 1;
             e = launch_render_batch(RB, c->stream);
         }

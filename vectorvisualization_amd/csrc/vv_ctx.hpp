@@ -138,6 +138,11 @@ inline vv_status fail(vv_ctx* c, vv_status s, const char* fmt, ...)
                         #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
     } while (0)
 
+inline bool storage_ok(int storage)
+{
+    return storage == VV_STORE_FP32 || storage == VV_STORE_FP16 || storage == VV_STORE_UNORM8;
+}
+
 inline bool dims_ok(const int d[3])
 {
     if (!d) return false;
@@ -376,7 +381,7 @@ inline vv_status fill_render_launch(vv_ctx* c, const vv_camera* cam, RenderLaunc
     const vv_lic_params& p = c->params;
     std::memset(&R, 0, sizeof(R));
     R.field = make_clamp_desc(c->field.p, c->fdims);
-    R.field_fp16 = c->storage == VV_STORE_FP16;
+    R.field_type = c->storage;
     R.tf_ch = p.tf_index == VV_TF_INDEX_MAG ? 3 : 2;
     if (need_lic) {
         R.lic = make_repeat_desc(c->lic.p, c->ldims);

@@ -23,6 +23,8 @@
 
 namespace vv {
 
+constexpr float kInv255 = (float)(1.0 / 255.0);
+
 // ---- scalar helpers ------------------------------------------------------
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 __device__ __forceinline__ float clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }
@@ -101,8 +103,26 @@ __device__ __forceinline__ float tri(float v000, float v100, float v010, float v
 }
 
 // ---- element decoders ----------------------------------------------------
+// Field texel types: float4 (RGBA32F), half4_t (RGBA16F) and u8x4_t (RGBA8,
+// the reference's UNORM8 field of fillTexDataChar[Interp]).  ld4/ld1 return
+// raw channel values (byte values for u8x4_t); FieldScale turns a blend of
+// them into the texture value (x 1/255 for UNORM8, as for every UNORM8
+// texture here).
 struct half4_t {
     __half x, y, z, w;
+};
+
+struct u8x4_t {
+    uint8_t x, y, z, w;
+};
+
+template <typename FT>
+struct FieldScale {
+    __device__ static __forceinline__ float apply(float v) { return v; }
+};
+template <>
+struct FieldScale<u8x4_t> {
+    __device__ static __forceinline__ float apply(float v) { return v * kInv255; }
 };
 
 __device__ __forceinline__ float4 ld4(const float4* p, int i) { return p[i]; }
@@ -118,6 +138,13 @@ __device__ __forceinline__ float4 ld4(const half4_t* p, int i)
     return r;
 }
 
+__device__ __forceinline__ float4 ld4(const u8x4_t* p, int i)
+{
+    const uint32_t u = reinterpret_cast<const uint32_t*>(p)[i];
+    return make_float4((float)(u & 0xffu), (float)((u >> 8) & 0xffu), (float)((u >> 16) & 0xffu),
+                       (float)(u >> 24));
+}
+
 __device__ __forceinline__ float ld1(const float4* p, int i, int ch)
 {
     return reinterpret_cast<const float*>(p)[4 * i + ch];
@@ -126,6 +153,10 @@ __device__ __forceinline__ float ld1(const half4_t* p, int i, int ch)
 {
     unsigned short h = reinterpret_cast<const unsigned short*>(p)[4 * i + ch];
     return __half2float(__ushort_as_half(h));
+}
+__device__ __forceinline__ float ld1(const u8x4_t* p, int i, int ch)
+{
+    return (float)reinterpret_cast<const uint8_t*>(p)[4 * i + ch];
 }
 
 // rgb of a 4-channel field at a CLAMP cell
@@ -142,20 +173,39 @@ __device__ __forceinline__ float3 tri_rgb(const Vol<FT>& v, const Cell& c)
     float4 a011 = ld4(v.p, b + sz + sy);
     float4 a111 = ld4(v.p, b + sz + sy + 1);
     float3 r;
-    r.x = tri(a000.x, a100.x, a010.x, a110.x, a001.x, a101.x, a011.x, a111.x, c);
-    r.y = tri(a000.y, a100.y, a010.y, a110.y, a001.y, a101.y, a011.y, a111.y, c);
-    r.z = tri(a000.z, a100.z, a010.z, a110.z, a001.z, a101.z, a011.z, a111.z, c);
+    r.x = FieldScale<FT>::apply(tri(a000.x, a100.x, a010.x, a110.x, a001.x, a101.x, a011.x, a111.x, c));
+    r.y = FieldScale<FT>::apply(tri(a000.y, a100.y, a010.y, a110.y, a001.y, a101.y, a011.y, a111.y, c));
+    r.z = FieldScale<FT>::apply(tri(a000.z, a100.z, a010.z, a110.z, a001.z, a101.z, a011.z, a111.z, c));
     return r;
 }
 
-// one channel of a 4-channel volume
+// one channel of a 4-channel field
 template <typename FT>
 __device__ __forceinline__ float tri_ch(const Vol<FT>& v, const Cell& c, int ch)
 {
     const int b = c.base, sy = v.sy, sz = v.sz;
-    return tri(ld1(v.p, b, ch), ld1(v.p, b + 1, ch), ld1(v.p, b + sy, ch),
-               ld1(v.p, b + sy + 1, ch), ld1(v.p, b + sz, ch), ld1(v.p, b + sz + 1, ch),
-               ld1(v.p, b + sz + sy, ch), ld1(v.p, b + sz + sy + 1, ch), c);
+    return FieldScale<FT>::apply(tri(ld1(v.p, b, ch), ld1(v.p, b + 1, ch), ld1(v.p, b + sy, ch),
+                                     ld1(v.p, b + sy + 1, ch), ld1(v.p, b + sz, ch), ld1(v.p, b + sz + 1, ch),
+                                     ld1(v.p, b + sz + sy, ch), ld1(v.p, b + sz + sy + 1, ch), c));
+}
+
+// Host-side dispatch over the field texel type: fn(TypeTag<FT>{}) with FT
+// float4 (type 0), half4_t (1) or u8x4_t (2)
+template <typename T>
+struct TypeTag {
+    using type = T;
+};
+
+template <typename Fn>
+inline hipError_t with_field_type(int type, Fn&& fn)
+{
+    switch (type) {
+    case 0: fn(TypeTag<float4>{}); break;
+    case 1: fn(TypeTag<half4_t>{}); break;
+    case 2: fn(TypeTag<u8x4_t>{}); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 // scalar float volume
@@ -238,7 +288,5 @@ __device__ __forceinline__ float lut2d(const float* t, int nc, int ch, float u, 
     const float b = lerpf(t[(j1 * 256 + i0) * nc + ch], t[(j1 * 256 + i1) * nc + ch], fx);
     return lerpf(a, b, fy);
 }
-
-constexpr float kInv255 = (float)(1.0 / 255.0);
 
 }  // namespace vv

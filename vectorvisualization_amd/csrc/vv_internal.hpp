@@ -37,9 +37,12 @@ inline VolDesc make_repeat_desc(void* p, const int n[3])
 inline long long clamp_elems(const int n[3]) { return (long long)(n[0] + 2) * (n[1] + 2) * (n[2] + 2); }
 inline long long repeat_elems(const int n[3]) { return (long long)(n[0] + 1) * (n[1] + 1) * (n[2] + 1); }
 
+// bytes per field texel of a storage type (VV_STORE_FP32, _FP16, _UNORM8)
+inline unsigned long long field_texel_bytes(int field_type) { return field_type == 1 ? 8ull : field_type == 2 ? 4ull : 16ull; }
+
 struct LicLaunch {
-    VolDesc field;     // CLAMP, float4 or half4
-    int field_fp16;
+    VolDesc field;     // CLAMP, float4, half4 or UNORM8 x4
+    int field_type;    // 0 float4, 1 half4, 2 UNORM8 x4 (the VV_STORE_* value)
     VolDesc noise;     // REPEAT, uint8 (nc 1) or uchar4 (nc 4)
     int nc;
     VolDesc scalar;    // CLAMP uint8, used when gate
@@ -67,7 +70,7 @@ struct ViewBlock {
 
 struct RenderLaunch {
     VolDesc field;     // CLAMP
-    int field_fp16;
+    int field_type;    // as LicLaunch::field_type
     int tf_ch;         // 2 (.b) or 3 (.a)
     VolDesc lic;       // REPEAT, float or float4
     int lc;            // 1 or 4
@@ -99,7 +102,7 @@ struct RenderBatchLaunch {
     unsigned long long* done;     // device: completion stamp per view
     int block[2];
     int grid;                     // persistent blocks
-    int fp16, lc;                 // shared by all views
+    int field_type, lc;           // shared by all views
 };
 
 struct LicBatchLaunch {
@@ -107,7 +110,7 @@ struct LicBatchLaunch {
     const int* first;             // device: first tile of each volume
     int njobs, total;
     int grid;
-    int fp16, same;               // shared by all volumes
+    int field_type, same;         // shared by all volumes
 };
 
 struct AdjointLaunch {
@@ -126,8 +129,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long long n,
                                uint32_t* maxbits, hipStream_t s);
 hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int dims[3],
-                               const uint32_t* maxbits, void* out, int fp16, hipStream_t s);
-hipError_t launch_field_decode(const void* field, int fp16, const int dims[3], float* out4,
+                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s);
+hipError_t launch_field_decode(const void* field, int field_type, const int dims[3], float* out4,
                                hipStream_t s);
 hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int dims[3],
                             uint8_t* out, hipStream_t s);
@@ -155,8 +158,8 @@ hipError_t launch_stamp(unsigned long long* t0, hipStream_t s);
 size_t lic_args_size();
 hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
-int lic_batch_blocks_per_cu(int fp16, int same);
-int render_batch_blocks_per_cu(int fp16, int lc, int threads);
+int lic_batch_blocks_per_cu(int field_type, int same);
+int render_batch_blocks_per_cu(int field_type, int lc, int threads);
 // view-aligned slicing (ViewSlicing): planes v.(p - center) = d_i with
 // d_i = -D/2 + (i + 1/2) D/n, i = 0..n-1, front to back
 struct SliceSetup {

@@ -14,6 +14,7 @@
 #include "vv_internal.hpp"
 
 #include <cstring>
+#include <type_traits>
 
 namespace vv {
 
@@ -91,6 +92,22 @@ __device__ __forceinline__ void store_enc<half4_t>(half4_t* out, long long i, fl
     reinterpret_cast<uint2*>(out)[i] = u;
 }
 
+// UNORM8 direction byte of fillTexDataChar[Interp] (VV/dataset.cpp:760-762):
+// (unsigned char)(q*127.0 + 128.0) in double.  127q is exact in double; the
+// sum is exact too except for -2^-47 <= 127q < 0, where it rounds to 128.0
+// (the double just below 128 is 128 - 2^-46; a tie goes to the even 128).
+// So the byte is floor(127q + 128), but 128 for those q.  Without double
+// arithmetic: k = floor of 127q + 128 rounded once to float; when the exact
+// value lies just below k, fma(q, 127, 128 - k) (rounded once) is negative,
+// and fma(q, 127, 2^-47) >= 0 tells the q that round to 128.
+__device__ __forceinline__ uint32_t unorm8_dir_byte(float q)
+{
+    const float k = floorf(fmaf(q, 127.0f, 128.0f));
+    const float r = fmaf(q, 127.0f, 128.0f - k);
+    const bool to128 = k == 128.0f && fmaf(q, 127.0f, 0x1p-47f) >= 0.0f;
+    return (uint32_t)(r < 0.0f && !to128 ? k - 1.0f : k);
+}
+
 // writes the CLAMP-padded encoded field; padded faces replicate the edge
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_field_encode(const float* v0, const float* v1, float t,
@@ -108,18 +125,30 @@ __global__ void __launch_bounds__(kThreads) k_field_encode(const float* v0, cons
     long long i = ((long long)z * ny + y) * nx + x;
     float tv[3];
     float len = field_len(v0, v1, t, i, tv);
-    float r, g, b;
-    if (len == 0.0f) {
-        r = g = b = 0.5f;
+    if constexpr (std::is_same<T, u8x4_t>::value) {
+        // fillTexDataCharInterp (VV/dataset.cpp:704-823), float source branch
+        uint32_t rgb = 128u | (128u << 8) | (128u << 16);
+        if (len != 0.0f)
+            rgb = unorm8_dir_byte(tv[0] / len) | (unorm8_dir_byte(tv[1] / len) << 8) |
+                  (unorm8_dir_byte(tv[2] / len) << 16);
+        const float maxlen = __uint_as_float(*maxbits);
+        float m = maxlen > 0.0f ? len / maxlen * 255.0f : 0.0f;
+        m = m > 255.0f ? 255.0f : (m < 0.0f ? 0.0f : m);
+        reinterpret_cast<uint32_t*>(out)[q] = rgb | ((uint32_t)m << 24);
     } else {
-        r = 0.5f * tv[0] / len + 0.5f;
-        g = 0.5f * tv[1] / len + 0.5f;
-        b = 0.5f * tv[2] / len + 0.5f;
+        float r, g, b;
+        if (len == 0.0f) {
+            r = g = b = 0.5f;
+        } else {
+            r = 0.5f * tv[0] / len + 0.5f;
+            g = 0.5f * tv[1] / len + 0.5f;
+            b = 0.5f * tv[2] / len + 0.5f;
+        }
+        float maxlen = __uint_as_float(*maxbits);
+        float a = maxlen > 0.0f ? len / maxlen : 0.0f;
+        a = a > 1.0f ? 1.0f : (a < 0.0f ? 0.0f : a);
+        store_enc<T>(out, q, r, g, b, a);
     }
-    float maxlen = __uint_as_float(*maxbits);
-    float a = maxlen > 0.0f ? len / maxlen : 0.0f;
-    a = a > 1.0f ? 1.0f : (a < 0.0f ? 0.0f : a);
-    store_enc<T>(out, q, r, g, b, a);
 }
 
 template <typename T>
@@ -131,7 +160,9 @@ __global__ void __launch_bounds__(kThreads) k_field_decode(const T* field, int n
     if (i >= n) return;
     int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((long long)nx * ny));
     long long q = ((long long)(z + 1) * (ny + 2) + (y + 1)) * (nx + 2) + (x + 1);
-    out[i] = ld4(field, (int)q);
+    const float4 v = ld4(field, (int)q);
+    out[i] = make_float4(FieldScale<T>::apply(v.x), FieldScale<T>::apply(v.y), FieldScale<T>::apply(v.z),
+                         FieldScale<T>::apply(v.w));
 }
 
 // ---- noise -----------------------------------------------------------------
@@ -444,28 +475,24 @@ hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long l
 }
 
 hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int d[3],
-                               const uint32_t* maxbits, void* out, int fp16, hipStream_t s)
+                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s)
 {
     long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
-    if (fp16)
-        hipLaunchKernelGGL(k_field_encode<half4_t>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1,
-                           t, d[0], d[1], d[2], maxbits, (half4_t*)out);
-    else
-        hipLaunchKernelGGL(k_field_encode<float4>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1,
-                           t, d[0], d[1], d[2], maxbits, (float4*)out);
-    return hipGetLastError();
+    return with_field_type(field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        hipLaunchKernelGGL(k_field_encode<FT>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1, t, d[0], d[1],
+                           d[2], maxbits, (FT*)out);
+    });
 }
 
-hipError_t launch_field_decode(const void* field, int fp16, const int d[3], float* out4, hipStream_t s)
+hipError_t launch_field_decode(const void* field, int field_type, const int d[3], float* out4, hipStream_t s)
 {
     long long n = (long long)d[0] * d[1] * d[2];
-    if (fp16)
-        hipLaunchKernelGGL(k_field_decode<half4_t>, dim3(blocks_for(n)), dim3(kThreads), 0, s,
-                           (const half4_t*)field, d[0], d[1], d[2], (float4*)out4);
-    else
-        hipLaunchKernelGGL(k_field_decode<float4>, dim3(blocks_for(n)), dim3(kThreads), 0, s,
-                           (const float4*)field, d[0], d[1], d[2], (float4*)out4);
-    return hipGetLastError();
+    return with_field_type(field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        hipLaunchKernelGGL(k_field_decode<FT>, dim3(blocks_for(n)), dim3(kThreads), 0, s, (const FT*)field, d[0],
+                           d[1], d[2], (float4*)out4);
+    });
 }
 
 hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int d[3], uint8_t* out,
@@ -609,38 +636,22 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
     dim3 grid((unsigned)(a.per_xcd * 8));
     dim3 block(kThreads);
     const bool g = L.gate != 0;
-    if (L.cache && L.nc == 1 && !g) {
-        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                          L.noise.n[2] == L.field.n[2];
-        if (L.field_fp16) {
-            if (same)
-                hipLaunchKernelGGL((k_lic_cached<half4_t, true>), grid, block, 0, s, a);
-            else
-                hipLaunchKernelGGL((k_lic_cached<half4_t, false>), grid, block, 0, s, a);
-        } else {
-            if (same)
-                hipLaunchKernelGGL((k_lic_cached<float4, true>), grid, block, 0, s, a);
-            else
-                hipLaunchKernelGGL((k_lic_cached<float4, false>), grid, block, 0, s, a);
-        }
-        return hipGetLastError();
-    }
-    if (L.field_fp16) {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_lic<half4_t, 4, false>), grid, block, 0, s, a);
+    const bool cached = L.cache && L.nc == 1 && !g;
+    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                      L.noise.n[2] == L.field.n[2];
+    return with_field_type(L.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (cached && same)
+            hipLaunchKernelGGL((k_lic_cached<FT, true>), grid, block, 0, s, a);
+        else if (cached)
+            hipLaunchKernelGGL((k_lic_cached<FT, false>), grid, block, 0, s, a);
+        else if (L.nc == 4)
+            hipLaunchKernelGGL((k_lic<FT, 4, false>), grid, block, 0, s, a);
         else if (g)
-            hipLaunchKernelGGL((k_lic<half4_t, 1, true>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((k_lic<FT, 1, true>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((k_lic<half4_t, 1, false>), grid, block, 0, s, a);
-    } else {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_lic<float4, 4, false>), grid, block, 0, s, a);
-        else if (g)
-            hipLaunchKernelGGL((k_lic<float4, 1, true>), grid, block, 0, s, a);
-        else
-            hipLaunchKernelGGL((k_lic<float4, 1, false>), grid, block, 0, s, a);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_lic<FT, 1, false>), grid, block, 0, s, a);
+    });
 }
 
 }  // namespace vv
@@ -664,35 +675,27 @@ hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s)
 {
     if (B.njobs < 1 || B.grid < 1) return hipErrorInvalidValue;
     const LicArgs* jobs = (const LicArgs*)B.jobs;
-    if (B.fp16) {
+    return with_field_type(B.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
         if (B.same)
-            hipLaunchKernelGGL((k_lic_batch<half4_t, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
-                               B.njobs, B.total);
+            hipLaunchKernelGGL((k_lic_batch<FT, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first, B.njobs,
+                               B.total);
         else
-            hipLaunchKernelGGL((k_lic_batch<half4_t, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
+            hipLaunchKernelGGL((k_lic_batch<FT, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
                                B.njobs, B.total);
-    } else {
-        if (B.same)
-            hipLaunchKernelGGL((k_lic_batch<float4, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
-                               B.njobs, B.total);
-        else
-            hipLaunchKernelGGL((k_lic_batch<float4, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
-                               B.njobs, B.total);
-    }
-    return hipGetLastError();
+    });
 }
 
-int lic_batch_blocks_per_cu(int fp16, int same)
+int lic_batch_blocks_per_cu(int field_type, int same)
 {
     int n = 0;
-    hipError_t e;
-    if (fp16)
-        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<half4_t, true>, kThreads, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<half4_t, false>, kThreads, 0);
-    else
-        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<float4, true>, kThreads, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<float4, false>, kThreads, 0);
-    return (e == hipSuccess && n > 0) ? n : 1;
+    hipError_t e = hipSuccess;
+    const hipError_t d = with_field_type(field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<FT, true>, kThreads, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<FT, false>, kThreads, 0);
+    });
+    return (d == hipSuccess && e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace vv

@@ -221,6 +221,23 @@ struct FieldLoad<half4_t> {
     }
 };
 
+// UNORM8 texels: one 8-byte load holds x0 and x1 (raw byte values)
+template <>
+struct FieldLoad<u8x4_t> {
+    static constexpr unsigned kShift = 2;
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
+    {
+        const gu32* q = (const gu32*)((gchar)base + off);
+        const unsigned a = q[0], b = q[1];
+        t0[0] = (float)(a & 0xffu);
+        t0[1] = (float)((a >> 8) & 0xffu);
+        t0[2] = (float)((a >> 16) & 0xffu);
+        t1[0] = (float)(b & 0xffu);
+        t1[1] = (float)((b >> 8) & 0xffu);
+        t1[2] = (float)((b >> 16) & 0xffu);
+    }
+};
+
 // two adjacent noise bytes (x0, x1) in one 16-bit load; the address may be odd
 typedef unsigned short __attribute__((aligned(1))) u16_unaligned;
 __device__ __forceinline__ unsigned noise_pair(const unsigned char* base, unsigned off)
@@ -258,9 +275,9 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
         }
     }
     float3 r;
-    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
-    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
-    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
+    r.x = FieldScale<FT>::apply(tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f));
+    r.y = FieldScale<FT>::apply(tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f));
+    r.z = FieldScale<FT>::apply(tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f));
     return r;
 }
 
@@ -317,9 +334,9 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
     }
     noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f) * kInv255;
     float3 r;
-    r.x = tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f);
-    r.y = tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f);
-    r.z = tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f);
+    r.x = FieldScale<FT>::apply(tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f));
+    r.y = FieldScale<FT>::apply(tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f));
+    r.z = FieldScale<FT>::apply(tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f));
     return r;
 }
 

@@ -973,19 +973,38 @@ hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
     if (bx * by > kRThreads || (bx * by) % 64) return hipErrorInvalidValue;
     dim3 block(bx, by);
     dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
-    if (R.field_fp16) {
+    return with_field_type(R.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
         if (R.lc == 4)
-            hipLaunchKernelGGL((k_render<half4_t, 4>), grid, block, 0, s, R);
+            hipLaunchKernelGGL((k_render<FT, 4>), grid, block, 0, s, R);
         else
-            hipLaunchKernelGGL((k_render<half4_t, 1>), grid, block, 0, s, R);
-    } else {
-        if (R.lc == 4)
-            hipLaunchKernelGGL((k_render<float4, 4>), grid, block, 0, s, R);
-        else
-            hipLaunchKernelGGL((k_render<float4, 1>), grid, block, 0, s, R);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_render<FT, 1>), grid, block, 0, s, R);
+    });
 }
+
+namespace {
+
+// LIC variant of the on-the-fly casters: 2 cached with the noise cell on
+// the field grid, 1 cached, 0 uncached (RGBA noise or the scalar gate)
+struct LicVariant {
+    bool gate;
+    int nc;
+    int mode;
+};
+
+LicVariant lic_variant(const LicLaunch& L)
+{
+    LicVariant v;
+    v.gate = L.gate != 0;
+    v.nc = L.nc;
+    const bool cached = L.cache && L.nc == 1 && !v.gate;
+    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
+                      L.noise.n[2] == L.field.n[2];
+    v.mode = cached ? (same ? 2 : 1) : 0;
+    return v;
+}
+
+}  // namespace
 
 hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s)
 {
@@ -997,39 +1016,20 @@ hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStr
     if (e != hipSuccess) return e;
     dim3 block(bx, by);
     dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
-    const bool g = L.gate != 0;
-    if (L.cache && L.nc == 1 && !g) {
-        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                          L.noise.n[2] == L.field.n[2];
-        if (R.field_fp16) {
-            if (same)
-                hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 2>), grid, block, 0, s, R, A);
-            else
-                hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 1>), grid, block, 0, s, R, A);
-        } else {
-            if (same)
-                hipLaunchKernelGGL((k_render_fused<float4, 1, false, 2>), grid, block, 0, s, R, A);
-            else
-                hipLaunchKernelGGL((k_render_fused<float4, 1, false, 1>), grid, block, 0, s, R, A);
-        }
-        return hipGetLastError();
-    }
-    if (R.field_fp16) {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_render_fused<half4_t, 4, false, 0>), grid, block, 0, s, R, A);
-        else if (g)
-            hipLaunchKernelGGL((k_render_fused<half4_t, 1, true, 0>), grid, block, 0, s, R, A);
+    const LicVariant v = lic_variant(L);
+    return with_field_type(R.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (v.mode == 2)
+            hipLaunchKernelGGL((k_render_fused<FT, 1, false, 2>), grid, block, 0, s, R, A);
+        else if (v.mode == 1)
+            hipLaunchKernelGGL((k_render_fused<FT, 1, false, 1>), grid, block, 0, s, R, A);
+        else if (v.nc == 4)
+            hipLaunchKernelGGL((k_render_fused<FT, 4, false, 0>), grid, block, 0, s, R, A);
+        else if (v.gate)
+            hipLaunchKernelGGL((k_render_fused<FT, 1, true, 0>), grid, block, 0, s, R, A);
         else
-            hipLaunchKernelGGL((k_render_fused<half4_t, 1, false, 0>), grid, block, 0, s, R, A);
-    } else {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_render_fused<float4, 4, false, 0>), grid, block, 0, s, R, A);
-        else if (g)
-            hipLaunchKernelGGL((k_render_fused<float4, 1, true, 0>), grid, block, 0, s, R, A);
-        else
-            hipLaunchKernelGGL((k_render_fused<float4, 1, false, 0>), grid, block, 0, s, R, A);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_render_fused<FT, 1, false, 0>), grid, block, 0, s, R, A);
+    });
 }
 
 hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, hipStream_t s)
@@ -1042,39 +1042,20 @@ hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const
     if (e != hipSuccess) return e;
     dim3 block(bx, by);
     dim3 grid((R.view.width + bx - 1) / bx, (R.view.height + by - 1) / by);
-    const bool g = L.gate != 0;
-    if (L.cache && L.nc == 1 && !g) {
-        const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                          L.noise.n[2] == L.field.n[2];
-        if (R.field_fp16) {
-            if (same)
-                hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 2>), grid, block, 0, s, R, A, S);
-            else
-                hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 1>), grid, block, 0, s, R, A, S);
-        } else {
-            if (same)
-                hipLaunchKernelGGL((k_render_slices<float4, 1, false, 2>), grid, block, 0, s, R, A, S);
-            else
-                hipLaunchKernelGGL((k_render_slices<float4, 1, false, 1>), grid, block, 0, s, R, A, S);
-        }
-        return hipGetLastError();
-    }
-    if (R.field_fp16) {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_render_slices<half4_t, 4, false, 0>), grid, block, 0, s, R, A, S);
-        else if (g)
-            hipLaunchKernelGGL((k_render_slices<half4_t, 1, true, 0>), grid, block, 0, s, R, A, S);
+    const LicVariant v = lic_variant(L);
+    return with_field_type(R.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (v.mode == 2)
+            hipLaunchKernelGGL((k_render_slices<FT, 1, false, 2>), grid, block, 0, s, R, A, S);
+        else if (v.mode == 1)
+            hipLaunchKernelGGL((k_render_slices<FT, 1, false, 1>), grid, block, 0, s, R, A, S);
+        else if (v.nc == 4)
+            hipLaunchKernelGGL((k_render_slices<FT, 4, false, 0>), grid, block, 0, s, R, A, S);
+        else if (v.gate)
+            hipLaunchKernelGGL((k_render_slices<FT, 1, true, 0>), grid, block, 0, s, R, A, S);
         else
-            hipLaunchKernelGGL((k_render_slices<half4_t, 1, false, 0>), grid, block, 0, s, R, A, S);
-    } else {
-        if (L.nc == 4)
-            hipLaunchKernelGGL((k_render_slices<float4, 4, false, 0>), grid, block, 0, s, R, A, S);
-        else if (g)
-            hipLaunchKernelGGL((k_render_slices<float4, 1, true, 0>), grid, block, 0, s, R, A, S);
-        else
-            hipLaunchKernelGGL((k_render_slices<float4, 1, false, 0>), grid, block, 0, s, R, A, S);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_render_slices<FT, 1, false, 0>), grid, block, 0, s, R, A, S);
+    });
 }
 
 hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, int slices, int G,
@@ -1092,10 +1073,7 @@ hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const 
     const long long nblocks = (long long)((R.view.width + TW - 1) / TW) * ((R.view.height + TH - 1) / TH);
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     dim3 grid((unsigned)nblocks), block(kRThreads);
-    const bool g = L.gate != 0;
-    const bool cached = L.cache && L.nc == 1 && !g;
-    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                      L.noise.n[2] == L.field.n[2];
+    const LicVariant v = lic_variant(L);
 #define VV_GROUP_LAUNCH(FT, NC, GATE, MODE)                                                                 \
     do {                                                                                                   \
         if (slices)                                                                                        \
@@ -1103,49 +1081,33 @@ hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const 
         else                                                                                               \
             hipLaunchKernelGGL((k_render_group<FT, NC, GATE, MODE, false>), grid, block, 0, s, R, A, S, G); \
     } while (0)
-    if (R.field_fp16) {
-        if (cached && same)
-            VV_GROUP_LAUNCH(half4_t, 1, false, 2);
-        else if (cached)
-            VV_GROUP_LAUNCH(half4_t, 1, false, 1);
-        else if (L.nc == 4)
-            VV_GROUP_LAUNCH(half4_t, 4, false, 0);
-        else if (g)
-            VV_GROUP_LAUNCH(half4_t, 1, true, 0);
+    return with_field_type(R.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (v.mode == 2)
+            VV_GROUP_LAUNCH(FT, 1, false, 2);
+        else if (v.mode == 1)
+            VV_GROUP_LAUNCH(FT, 1, false, 1);
+        else if (v.nc == 4)
+            VV_GROUP_LAUNCH(FT, 4, false, 0);
+        else if (v.gate)
+            VV_GROUP_LAUNCH(FT, 1, true, 0);
         else
-            VV_GROUP_LAUNCH(half4_t, 1, false, 0);
-    } else {
-        if (cached && same)
-            VV_GROUP_LAUNCH(float4, 1, false, 2);
-        else if (cached)
-            VV_GROUP_LAUNCH(float4, 1, false, 1);
-        else if (L.nc == 4)
-            VV_GROUP_LAUNCH(float4, 4, false, 0);
-        else if (g)
-            VV_GROUP_LAUNCH(float4, 1, true, 0);
-        else
-            VV_GROUP_LAUNCH(float4, 1, false, 0);
-    }
+            VV_GROUP_LAUNCH(FT, 1, false, 0);
+    });
 #undef VV_GROUP_LAUNCH
-    return hipGetLastError();
 }
 
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
 {
     dim3 block(kBX, kBY);
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
-    if (A.r.field_fp16) {
+    return with_field_type(A.r.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
         if (A.r.lc == 4)
-            hipLaunchKernelGGL((k_render_adjoint<half4_t, 4>), grid, block, 0, s, A);
+            hipLaunchKernelGGL((k_render_adjoint<FT, 4>), grid, block, 0, s, A);
         else
-            hipLaunchKernelGGL((k_render_adjoint<half4_t, 1>), grid, block, 0, s, A);
-    } else {
-        if (A.r.lc == 4)
-            hipLaunchKernelGGL((k_render_adjoint<float4, 4>), grid, block, 0, s, A);
-        else
-            hipLaunchKernelGGL((k_render_adjoint<float4, 1>), grid, block, 0, s, A);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_render_adjoint<FT, 1>), grid, block, 0, s, A);
+    });
 }
 
 hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s)
@@ -1154,22 +1116,15 @@ hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s)
     if (bx * by > kRThreads || (bx * by) % 64 || B.nviews < 1) return hipErrorInvalidValue;
     dim3 block(bx, by);
     dim3 grid((unsigned)B.grid);
-    if (B.fp16) {
+    return with_field_type(B.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
         if (B.lc == 4)
-            hipLaunchKernelGGL((k_render_batch<half4_t, 4>), grid, block, 0, s, B.views, B.first, B.nviews,
-                               B.total, B.counter, B.done);
+            hipLaunchKernelGGL((k_render_batch<FT, 4>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
+                               B.counter, B.done);
         else
-            hipLaunchKernelGGL((k_render_batch<half4_t, 1>), grid, block, 0, s, B.views, B.first, B.nviews,
-                               B.total, B.counter, B.done);
-    } else {
-        if (B.lc == 4)
-            hipLaunchKernelGGL((k_render_batch<float4, 4>), grid, block, 0, s, B.views, B.first, B.nviews,
-                               B.total, B.counter, B.done);
-        else
-            hipLaunchKernelGGL((k_render_batch<float4, 1>), grid, block, 0, s, B.views, B.first, B.nviews,
-                               B.total, B.counter, B.done);
-    }
-    return hipGetLastError();
+            hipLaunchKernelGGL((k_render_batch<FT, 1>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
+                               B.counter, B.done);
+    });
 }
 
 hipError_t launch_stamp(unsigned long long* t0, hipStream_t s)
@@ -1197,17 +1152,16 @@ hipError_t launch_unpad_clamp_fixed(const long long* src, const int d[3], float*
 
 namespace vv {
 
-int render_batch_blocks_per_cu(int fp16, int lc, int threads)
+int render_batch_blocks_per_cu(int field_type, int lc, int threads)
 {
     int n = 0;
-    hipError_t e;
-    if (fp16)
-        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<half4_t, 4>, threads, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<half4_t, 1>, threads, 0);
-    else
-        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<float4, 4>, threads, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<float4, 1>, threads, 0);
-    return (e == hipSuccess && n > 0) ? n : 1;
+    hipError_t e = hipSuccess;
+    const hipError_t d = with_field_type(field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<FT, 4>, threads, 0)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<FT, 1>, threads, 0);
+    });
+    return (d == hipSuccess && e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace vv
