@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--image", type=int, default=1024)
     ap.add_argument("--L", type=int, default=40)
-    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--storage", choices=["fp32", "fp16", "unorm8"], default="fp32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip FP16 and adjoint legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -91,7 +91,7 @@ def main():
     n, L, W = args.n, args.L, args.image
     dims = (n, n, n)
     dev = torch.device("cuda:0")
-    storage = vv.STORE_FP16 if args.storage == "fp16" else vv.STORE_FP32
+    storage = {"fp32": vv.STORE_FP32, "fp16": vv.STORE_FP16, "unorm8": vv.STORE_UNORM8}[args.storage]
 
     print(f"[bench] generating {n}^3 Rankine field on {dev}", file=sys.stderr, flush=True)
     field = fields.rankine_vortices_torch(n, device=dev).contiguous()
@@ -159,7 +159,8 @@ def main():
     vox = n ** 3
     samples = vox * (2 * L + 1)
     value = samples / (step_ms / 1000.0)
-    bpv = lic_bytes_per_voxel(L, L, 6 if storage == vv.STORE_FP16 else 12)
+    # bytes of the three direction channels of one texel
+    bpv = lic_bytes_per_voxel(L, L, {vv.STORE_FP32: 12, vv.STORE_FP16: 6, vv.STORE_UNORM8: 3}[storage])
     lic_bytes = bpv * vox
     achieved = lic_bytes / (lic_ms / 1000.0) / 1e9
     tag = f"lic_rankine{n}_gauss{L}_{args.storage}"
@@ -176,7 +177,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if storage == vv.STORE_FP32 else "f16-storage/f32",
+        "dtype": {vv.STORE_FP32: "f32", vv.STORE_FP16: "f16-storage/f32",
+                  vv.STORE_UNORM8: "unorm8-storage/f32"}[storage],
         "data": "synthetic",
         "config": {
             "workload": f"LIC {n}^3 Rankine x12 + (0.1,0,0), dense uniform noise, Gaussian L={L} "
@@ -198,7 +200,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_lic_cached<half4_t>" if storage == vv.STORE_FP16 else "k_lic_cached<float4>",
+            "kernel": {vv.STORE_FP32: "k_lic_cached<float4>", vv.STORE_FP16: "k_lic_cached<half4_t>",
+                       vv.STORE_UNORM8: "k_lic_cached<u8x4_t>"}[storage],
             "algorithmic_bytes_per_voxel": bpv,
         },
     }
@@ -316,7 +319,7 @@ def batch_leg(args, dev, seed=7, nsteps=8, nviews=64, L=30):
         views.append((int(rng.integers(nsteps)), w, float(rng.uniform(0, 2 * np.pi)),
                       float(rng.uniform(-0.6, 0.6))))
     print(f"[bench] batch: timesteps {sizes}, {nviews} views", file=sys.stderr, flush=True)
-    storage = vv.STORE_FP16 if args.storage == "fp16" else vv.STORE_FP32
+    storage = {"fp32": vv.STORE_FP32, "fp16": vv.STORE_FP16, "unorm8": vv.STORE_UNORM8}[args.storage]
     ctx = vv.Context(0)
     lut, inv = vv.make_filter(vv.FILTER_TRIANGLE, L)
     ctx.set_filter(lut, inv)

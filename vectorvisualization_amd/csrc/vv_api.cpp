@@ -159,6 +159,7 @@ vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float ler
     }
     c->storage = storage;
     c->has_field = true;
+    c->lic_field_ok = false;
     return VV_OK;
 }
 
@@ -233,6 +234,7 @@ vv_status build_noise_tex(vv_ctx* c, const uint8_t* d_noise, const int dims[3], 
     for (int a = 0; a < 3; ++a) c->ndims[a] = dims[a];
     c->nch = nc;
     c->has_noise = true;
+    c->lic_field_ok = false;
     return VV_OK;
 }
 
@@ -464,6 +466,26 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     for (int a = 0; a < 3; ++a) L.tile[a] = c->lic_tile[a];
     for (int a = 0; a < 3; ++a) L.super_tile[a] = c->lic_super[a];
     L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * field_texel_bytes(L.field_type);
+    // scalar noise on the field's grid: the LIC reads the field copy that
+    // carries the noise bytes (built once per field/noise pair).  Without
+    // memory for the copy it reads the noise volume itself.
+    if (nc == 1 && c->ndims[0] == c->fdims[0] && c->ndims[1] == c->fdims[1] && c->ndims[2] == c->fdims[2]) {
+        if (!c->lic_field_ok) {
+            const hipError_t e = c->lic_field.ensure((size_t)L.field_bytes);
+            if (e == hipErrorOutOfMemory) {
+                (void)hipGetLastError();
+            } else {
+                HIPCHK(c, e);
+                HIPCHK(c, launch_pack_lic_field(c->field.p, c->storage, (const uint8_t*)c->noise.p, c->fdims,
+                                                c->lic_field.p, c->stream));
+                c->lic_field_ok = true;
+            }
+        }
+        if (c->lic_field_ok) {
+            L.field.p = c->lic_field.p;
+            L.field_has_noise = 1;
+        }
+    }
     L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
     // the cached kernels address through 32-bit byte offsets
     // and 24-bit index multiplies (plane strides below 2^24 elements)

@@ -32,6 +32,7 @@ struct BatchStep {
     float extent[3], scale[3], center[3];
     int storage;
     DevBuf field, noise, lic;
+    DevBuf lic_field;  // field copy with the noise byte in channel 3 (launch_pack_lic_field)
     int tile[3] = {16, 4, 4};
     int super[3] = {2, 8, 8};
 };
@@ -90,7 +91,8 @@ vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
 {
     const vv_lic_params& p = c->params;
     std::memset(&L, 0, sizeof(L));
-    L.field = make_clamp_desc(st->field.p, st->dims);
+    L.field = make_clamp_desc(st->lic_field.p, st->dims);  // noise on the field's grid
+    L.field_has_noise = 1;
     L.field_type = st->storage;
     L.noise = make_repeat_desc(st->noise.p, st->dims);
     L.nc = 1;
@@ -211,6 +213,7 @@ vv_status vv_batch_destroy(vv_batch* b)
         st->field.release();
         st->noise.release();
         st->lic.release();
+        st->lic_field.release();
         delete st;
     }
     DevBuf* bufs[] = {&b->jobs, &b->first, &b->rviews, &b->vfirst, &b->ctr};
@@ -258,6 +261,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
         st->field.release();
         st->noise.release();
         st->lic.release();
+        st->lic_field.release();
         delete st;
         return s;
     };
@@ -270,6 +274,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     hipError_t e = st->field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims));
     if (!e) e = st->noise.ensure((size_t)repeat_elems(dims));
     if (!e) e = st->lic.ensure(sizeof(float) * (size_t)repeat_elems(dims));
+    if (!e) e = st->lic_field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims));
     if (!e) e = c->counter.ensure(64);
     if (!e) e = launch_fill_u32((uint32_t*)c->counter.p, 0u, 1, c->stream);
     if (!e) e = launch_field_maxlen(d0, d0, 0.0f, n, (uint32_t*)c->counter.p, c->stream);
@@ -282,6 +287,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     e = c->stage.ensure((size_t)n);
     if (!e) e = launch_noise_gen(noise_seed, noise_kind, (uint64_t)t, dims, (uint8_t*)c->stage.p, c->stream);
     if (!e) e = launch_noise_tex((const uint8_t*)c->stage.p, nullptr, dims, st->noise.p, 1, c->stream);
+    if (!e) e = launch_pack_lic_field(st->field.p, storage, (const uint8_t*)st->noise.p, dims, st->lic_field.p, c->stream);
     if (!e) e = hipStreamSynchronize(c->stream);
     if (e) return bail(fail(c, VV_EDEVICE, "timestep noise: %s", hipGetErrorString(e)));
     // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
@@ -336,6 +342,8 @@ vv_status vv_batch_set_field(vv_batch* b, int timestep, const float* v0, const f
     if (!e) e = launch_field_maxlen(d0, d1, t, n, (uint32_t*)c->counter.p, c->stream);
     if (!e) e = launch_field_encode(d0, d1, t, st->dims, (const uint32_t*)c->counter.p, st->field.p,
                                     st->storage, c->stream);
+    if (!e) e = launch_pack_lic_field(st->field.p, st->storage, (const uint8_t*)st->noise.p, st->dims,
+                                      st->lic_field.p, c->stream);
     if (!e) e = hipStreamSynchronize(c->stream);
     if (e) return fail(c, VV_EDEVICE, "timestep encode: %s", hipGetErrorString(e));
     return VV_OK;

@@ -69,6 +69,12 @@ struct vv_ctx {
     int nch = 1;
     DevBuf noise;
 
+    // LIC copy of the field with the noise byte in channel 3, for a scalar
+    // noise volume on the field's grid (launch_pack_lic_field); rebuilt after
+    // either input changes
+    DevBuf lic_field;
+    bool lic_field_ok = false;
+
     // scalar gate volume
     bool has_scalar = false;
     int sdims[3] = {0, 0, 0};

@@ -41,7 +41,8 @@ inline long long repeat_elems(const int n[3]) { return (long long)(n[0] + 1) * (
 inline unsigned long long field_texel_bytes(int field_type) { return field_type == 1 ? 8ull : field_type == 2 ? 4ull : 16ull; }
 
 struct LicLaunch {
-    VolDesc field;     // CLAMP, float4, half4 or UNORM8 x4
+    VolDesc field;     // CLAMP, float4, half4 or UNORM8 x4; with a noise volume on
+                       // the same grid (nc 1) the LIC copy (launch_pack_lic_field)
     int field_type;    // 0 float4, 1 half4, 2 UNORM8 x4 (the VV_STORE_* value)
     VolDesc noise;     // REPEAT, uint8 (nc 1) or uchar4 (nc 4)
     int nc;
@@ -59,6 +60,7 @@ struct LicLaunch {
     int super_tile[3]; // tiles per super-tile along x,y,z (tile order)
     int cache;         // 1: per-lane cell caches (scalar noise, no gate)
     unsigned long long field_bytes, noise_bytes;  // padded buffer sizes
+    int field_has_noise; // field is the LIC copy: channel 3 = noise byte (same grid)
 };
 
 struct ViewBlock {
@@ -132,6 +134,11 @@ hipError_t launch_field_encode(const float* v0, const float* v1, float t, const 
                                const uint32_t* maxbits, void* out, int field_type, hipStream_t s);
 hipError_t launch_field_decode(const void* field, int field_type, const int dims[3], float* out4,
                                hipStream_t s);
+// the LIC copy of a field for a noise volume on the same grid (REPEAT-padded
+// scalar noise): rgb of the field, channel 3 the noise byte of the REPEAT
+// texel (p - 1) mod n of CLAMP-padded texel p
+hipError_t launch_pack_lic_field(const void* field, int field_type, const uint8_t* noise, const int dims[3],
+                                 void* out, hipStream_t s);
 hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int dims[3],
                             uint8_t* out, hipStream_t s);
 hipError_t launch_noise_grad_raw(const uint8_t* noise, const int dims[3], const float sd[3],

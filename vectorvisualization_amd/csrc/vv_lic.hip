@@ -165,6 +165,34 @@ __global__ void __launch_bounds__(kThreads) k_field_decode(const T* field, int n
                          FieldScale<T>::apply(v.w));
 }
 
+// LIC copy of the field for equal field and noise grids: every CLAMP-padded
+// texel p keeps its rgb and carries in channel 3 the noise byte of REPEAT
+// texel (p - 1) mod n per axis, a value each texel type holds exactly
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_pack_lic_field(const T* field, const uint8_t* noise, int nx, int ny,
+                                                             int nz, T* out)
+{
+    const int px = nx + 2, py = ny + 2, pz = nz + 2;
+    const long long np = (long long)px * py * pz;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    const int ix = (x + nx - 1) % nx, iy = (y + ny - 1) % ny, iz = (z + nz - 1) % nz;
+    const uint32_t b = noise[((long long)iz * (ny + 1) + iy) * (nx + 1) + ix];
+    if constexpr (std::is_same<T, float4>::value) {
+        float4 v = field[q];
+        v.w = (float)b;
+        out[q] = v;
+    } else if constexpr (std::is_same<T, half4_t>::value) {
+        uint2 u = reinterpret_cast<const uint2*>(field)[q];
+        u.y = (u.y & 0xffffu) | ((uint32_t)__half_as_ushort(__float2half((float)b)) << 16);
+        reinterpret_cast<uint2*>(out)[q] = u;
+    } else {
+        const uint32_t u = reinterpret_cast<const uint32_t*>(field)[q];
+        reinterpret_cast<uint32_t*>(out)[q] = (u & 0x00ffffffu) | (b << 24);
+    }
+}
+
 // ---- noise -----------------------------------------------------------------
 // noise[i] = (i+1)-th output of SplitMix64 seeded with `seed`
 __device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i)
@@ -380,7 +408,7 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
 
     const CachedArgs c = cached_args(a);
-    FieldCache fc;
+    LicFieldCache<SAME> fc;
     fc.base = -1;
     NoiseCache ncache;
     ncache.base = -1;
@@ -492,6 +520,17 @@ hipError_t launch_field_decode(const void* field, int field_type, const int d[3]
         using FT = typename decltype(tag)::type;
         hipLaunchKernelGGL(k_field_decode<FT>, dim3(blocks_for(n)), dim3(kThreads), 0, s, (const FT*)field, d[0],
                            d[1], d[2], (float4*)out4);
+    });
+}
+
+hipError_t launch_pack_lic_field(const void* field, int field_type, const uint8_t* noise, const int d[3], void* out,
+                                 hipStream_t s)
+{
+    const long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
+    return with_field_type(field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        hipLaunchKernelGGL(k_pack_lic_field<FT>, dim3(blocks_for(np)), dim3(kThreads), 0, s, (const FT*)field, noise,
+                           d[0], d[1], d[2], (FT*)out);
     });
 }
 
@@ -637,8 +676,9 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
     dim3 block(kThreads);
     const bool g = L.gate != 0;
     const bool cached = L.cache && L.nc == 1 && !g;
-    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                      L.noise.n[2] == L.field.n[2];
+    // the SAME kernels read the noise from channel 3 of the LIC field copy
+    const bool same = L.field_has_noise && L.freq == 1.0f && L.noise.n[0] == L.field.n[0] &&
+                      L.noise.n[1] == L.field.n[1] && L.noise.n[2] == L.field.n[2];
     return with_field_type(L.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
         if (cached && same)

@@ -116,10 +116,13 @@ __device__ __forceinline__ void store_repeat(T* out, int osy, int osz, int x, in
 // texels and the four differences (x1 - x0) of the cell, so the four x-lerps
 // of a hit are single fmas; the values and the operation order are exactly
 // those of tri(), so results do not change.
+// NCH = 3: the field direction; NCH = 4: also the noise byte that the LIC
+// copy of the field carries in channel 3 (equal field and noise grids)
+template <int NCH>
 struct FieldCache {
     int base;
-    float v0[3][4];   // [channel][(y,z) corner]: texel at x0
-    float dx[3][4];   // texel(x1) - texel(x0)
+    float v0[NCH][4];   // [channel][(y,z) corner]: texel at x0
+    float dx[NCH][4];   // texel(x1) - texel(x0)
 };
 
 struct NoiseCache {
@@ -184,19 +187,21 @@ typedef __attribute__((address_space(1))) unsigned gu32;
 template <typename FT>
 struct FieldLoad;
 
-// float4 texels: two 12-byte loads (rgb of x0 and of x1)
+// The loaders fill NCH channels (3: rgb, 4: rgb + channel 3) of the texels
+// x0 and x1 of a cell row.
+// float4 texels: two 12- or 16-byte loads (x0 and x1)
 template <>
 struct FieldLoad<float4> {
     static constexpr unsigned kShift = 4;
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
+    template <int NCH>
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float* t0, float* t1)
     {
         const gfloat* q = (const gfloat*)((gchar)base + off);
-        t0[0] = q[0];
-        t0[1] = q[1];
-        t0[2] = q[2];
-        t1[0] = q[4];
-        t1[1] = q[5];
-        t1[2] = q[6];
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            t0[ch] = q[ch];
+            t1[ch] = q[4 + ch];
+        }
     }
 };
 
@@ -204,7 +209,8 @@ struct FieldLoad<float4> {
 template <>
 struct FieldLoad<half4_t> {
     static constexpr unsigned kShift = 3;
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
+    template <int NCH>
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float* t0, float* t1)
     {
         const gu32* q = (const gu32*)((gchar)base + off);
         uint4 a;
@@ -218,6 +224,10 @@ struct FieldLoad<half4_t> {
         t1[0] = __half2float(__ushort_as_half((unsigned short)(a.z & 0xffffu)));
         t1[1] = __half2float(__ushort_as_half((unsigned short)(a.z >> 16)));
         t1[2] = __half2float(__ushort_as_half((unsigned short)(a.w & 0xffffu)));
+        if constexpr (NCH == 4) {
+            t0[3] = __half2float(__ushort_as_half((unsigned short)(a.y >> 16)));
+            t1[3] = __half2float(__ushort_as_half((unsigned short)(a.w >> 16)));
+        }
     }
 };
 
@@ -225,7 +235,8 @@ struct FieldLoad<half4_t> {
 template <>
 struct FieldLoad<u8x4_t> {
     static constexpr unsigned kShift = 2;
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float t0[3], float t1[3])
+    template <int NCH>
+    __device__ static __forceinline__ void pair(const char* base, unsigned off, float* t0, float* t1)
     {
         const gu32* q = (const gu32*)((gchar)base + off);
         const unsigned a = q[0], b = q[1];
@@ -235,6 +246,10 @@ struct FieldLoad<u8x4_t> {
         t1[0] = (float)(b & 0xffu);
         t1[1] = (float)((b >> 8) & 0xffu);
         t1[2] = (float)((b >> 16) & 0xffu);
+        if constexpr (NCH == 4) {
+            t0[3] = (float)(a >> 24);
+            t1[3] = (float)(b >> 24);
+        }
     }
 };
 
@@ -254,8 +269,8 @@ struct CachedArgs {
     unsigned nsy, nsz;  // noise element strides
 };
 
-template <typename FT>
-__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache& fc, const Ax& ax,
+template <typename FT, int NCH>
+__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<NCH>& fc, const Ax& ax,
                                                   const Ax& ay, const Ax& az)
 {
     const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
@@ -265,10 +280,10 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
         const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float t0[3], t1[3];
-            FieldLoad<FT>::pair(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+            float t0[NCH], t1[NCH];
+            FieldLoad<FT>::template pair<NCH>(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
+            for (int ch = 0; ch < NCH; ++ch) {
                 fc.v0[ch][k] = t0[ch];
                 fc.dx[ch][k] = t1[ch] - t0[ch];
             }
@@ -286,7 +301,7 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
 // overlap (one memory round trip instead of two); values and arithmetic
 // are those of field_from_axes and noise_from_axes.
 template <typename FT>
-__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache& fc,
+__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache<3>& fc,
                                                   NoiseCache& nc, const Ax& ax, const Ax& ay,
                                                   const Ax& az, const Ax& nx_, const Ax& ny_,
                                                   const Ax& nz_, float& noise)
@@ -303,7 +318,7 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             float t0[3], t1[3];
-            FieldLoad<FT>::pair(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+            FieldLoad<FT>::template pair<3>(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) {
                 fc.v0[ch][k] = t0[ch];
@@ -360,20 +375,78 @@ __device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedA
     return tri_cached(nc.v0, nc.dx, ax.f, ay.f, az.f) * kInv255;
 }
 
+// REPEAT corner index (floor mod n) with float arithmetic instead of an
+// integer remainder, which keeps the rarely taken noise fallback of the SAME
+// kernels light on registers.  The floor is limited to +-2^20 first, so the
+// quotient estimate fl * rcp(n) is off by at most one and one correction
+// step lands in [0, n-1]; the final limit only guards that range.  Equal to
+// repeat_idx for |floor| <= 2^20 (a million texels outside the volume).
+__device__ __forceinline__ unsigned repeat_idx_f(float fl, float nm1)
+{
+    const float n = nm1 + 1.0f;
+    fl = __builtin_amdgcn_fmed3f(fl, -1048576.0f, 1048576.0f);
+    float w = fl - n * floorf(fl * __builtin_amdgcn_rcpf(n));
+    w = w >= n ? w - n : (w < 0.0f ? w + n : w);
+    return (unsigned)(int)__builtin_amdgcn_fmed3f(w, 0.0f, nm1);
+}
+
+// noise sample of a REPEAT cell on the field's grid, without the cache (a
+// lane whose streamline left the volume)
+__device__ __forceinline__ float noise_direct(const LicArgs& a, const CachedArgs& c, const Ax& ax, const Ax& ay,
+                                              const Ax& az)
+{
+    const unsigned base = __umul24(repeat_idx_f(az.fl, c.fnm1[2]), c.nsz) +
+                          __umul24(repeat_idx_f(ay.fl, c.fnm1[1]), c.nsy) + repeat_idx_f(ax.fl, c.fnm1[0]);
+    const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
+    float v0[4], dx[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const unsigned u = noise_pair(c.nr, base + off[k]);
+        v0[k] = (float)(u & 0xffu);
+        dx[k] = (float)(u >> 8) - v0[k];
+    }
+    return tri_cached(v0, dx, ax.f, ay.f, az.f) * kInv255;
+}
+
+// Equal field and noise grids with freq 1 (SAME): the LIC copy of the field
+// holds, in channel 3 of the CLAMP-padded texel p, the noise byte of REPEAT
+// texel (p - 1) mod n per axis.  While every floor lies in [-1, n-1] the
+// CLAMP cell (clamp(floor) + 1, + 2) and the REPEAT cell (floor mod n, + 1)
+// then hold the same noise bytes, so the field fill brings the noise cell
+// with it: no noise loads.  Outside that range (a streamline that left the
+// volume) the noise comes from the REPEAT array.  The noise value and its
+// arithmetic are those of noise_from_axes.
+__device__ __forceinline__ bool same_cells(const CachedArgs& c, const Ax& ax, const Ax& ay, const Ax& az)
+{
+    return ax.fl >= -1.0f && ax.fl <= c.fnm1[0] && ay.fl >= -1.0f && ay.fl <= c.fnm1[1] && az.fl >= -1.0f &&
+           az.fl <= c.fnm1[2];
+}
+
+template <typename FT>
+__device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& c, const FieldCache<4>& fc,
+                                            const Ax& ax, const Ax& ay, const Ax& az)
+{
+    if (same_cells(c, ax, ay, az)) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f) * kInv255;
+    return noise_direct(a, c, ax, ay, az);
+}
+
 // LIC of one seed with the per-lane cell caches (scalar noise without gate):
 // the weighted noise sum, before the final scaling.  SAME: the noise grid
 // equals the field grid and freq == 1, so the noise cell of the corrector
 // position reuses the corrector's floors and fractions.  The caches may hold
 // cells of an earlier seed (the ray caster keeps them along a ray): they only
 // decide whether a cell is reloaded, never its values.
+template <bool SAME>
+using LicFieldCache = FieldCache<SAME ? 4 : 3>;
+
 template <typename FT, bool SAME>
-__device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, FieldCache& fc,
+__device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, LicFieldCache<SAME>& fc,
                                                   NoiseCache& ncache, float sx, float sy, float sz)
 {
     const int nx = a.field.n[0], ny = a.field.n[1], nz = a.field.n[2];
     auto noise = [&](float px, float py, float pz, const Ax& ax, const Ax& ay, const Ax& az) -> float {
         if constexpr (SAME) {
-            return noise_from_axes(a, c, ncache, ax, ay, az);
+            return noise_same<FT>(a, c, fc, ax, ay, az);
         } else {
             const float q0 = px * a.freq, q1 = py * a.freq, q2 = pz * a.freq;
             return noise_from_axes(a, c, ncache, axis(q0, a.noise.n[0]), axis(q1, a.noise.n[1]),
@@ -413,7 +486,8 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
             az = axis(pz, nz);
             float nval;
             if constexpr (SAME) {
-                s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, ax, ay, az, nval);
+                s = field_from_axes<FT>(c, fc, ax, ay, az);
+                nval = noise_same<FT>(a, c, fc, ax, ay, az);
             } else {
                 s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, axis(px * a.freq, a.noise.n[0]),
                                         axis(py * a.freq, a.noise.n[1]), axis(pz * a.freq, a.noise.n[2]),

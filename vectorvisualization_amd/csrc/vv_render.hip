@@ -657,7 +657,7 @@ k_render_fused(RenderLaunch a, LicArgs L)
             const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                              a.field.sz};
             const CachedArgs c = cached_args(L);
-            FieldCache fc;
+            LicFieldCache<MODE == 2> fc;
             fc.base = -1;
             NoiseCache nc;
             nc.base = -1;
@@ -726,7 +726,7 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
         const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                          a.field.sz};
         const CachedArgs c = cached_args(L);
-        FieldCache fc;
+        LicFieldCache<MODE == 2> fc;
         fc.base = -1;
         NoiseCache nc;
         nc.base = -1;
@@ -822,7 +822,7 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
     const ViewBlock& vb = a.view;
     const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy, a.field.sz};
     const CachedArgs c = cached_args(L);
-    FieldCache fc;
+    LicFieldCache<MODE == 2> fc;
     fc.base = -1;
     NoiseCache nc;
     nc.base = -1;
@@ -998,8 +998,9 @@ LicVariant lic_variant(const LicLaunch& L)
     v.gate = L.gate != 0;
     v.nc = L.nc;
     const bool cached = L.cache && L.nc == 1 && !v.gate;
-    const bool same = L.freq == 1.0f && L.noise.n[0] == L.field.n[0] && L.noise.n[1] == L.field.n[1] &&
-                      L.noise.n[2] == L.field.n[2];
+    // the SAME kernels read the noise from channel 3 of the LIC field copy
+    const bool same = L.field_has_noise && L.freq == 1.0f && L.noise.n[0] == L.field.n[0] &&
+                      L.noise.n[1] == L.field.n[1] && L.noise.n[2] == L.field.n[2];
     v.mode = cached ? (same ? 2 : 1) : 0;
     return v;
 }
