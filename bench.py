@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--no-extras", action="store_true", help="skip FP16 and adjoint legs")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched timesteps x views leg")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the FP64 check of the 256^3 LIC + 1024^2 image + adjoint")
     ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 FP16-storage leg")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--pmc-probe", action="store_true",
@@ -292,6 +294,9 @@ def main():
     if not args.no_precision:
         out["precision"] = precision_leg(dev)
 
+    if not args.no_target and n == 256 and W == 1024:
+        out["fp64_check"] = fp64_check_leg(dev, field, L)
+
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(field.cpu().numpy(), dims, L, lut, inv, p,
                                            args.cpu_seconds, samples / vox)
@@ -491,6 +496,95 @@ def precision_leg(dev, n=512, L=60, box=64, W=1024):
         out[f"{name}_vs_fp64_{box}^3"] = {"lic_max_rel": float(d.max()) / rscale,
                                          "lic_rms_rel": float(np.sqrt(np.mean(d ** 2))) / rscale}
     out["oracle_box_s"] = oracle_s
+    return out
+
+
+def fp64_check_leg(dev, field, L, W=1024, box_threads=16):
+    """The benchmark workload itself (256^3 LIC, 1024^2 frame, TF/opacity
+    adjoint) against the FP64 oracle, for FP32, FP16 and UNORM8 field storage:
+    LIC, image and gradient errors.  Image errors are also given over the
+    pixels whose early-termination index (dest.a > 0.95) is the FP64 one; the
+    index of the GPU run is that of the oracle's FP32 ray cast of the GPU's own
+    LIC volume on the same stored field (the GPU image equals it to 1e-5)."""
+    import numpy as np
+    import torch
+
+    import vectorvisualization_amd as vv
+    from vectorvisualization_amd import fields
+    from oracle import oracle as orc
+
+    n = field.shape[0]
+    dims = (n, n, n)
+    try:
+        cores = max(1, min(len(os.sched_getaffinity(0)), box_threads))
+    except AttributeError:
+        cores = 1
+    print(f"[bench] fp64 check: {n}^3 LIC + {W}^2 frame + adjoint vs the FP64 oracle ({cores} threads)",
+          file=sys.stderr, flush=True)
+    host = field.cpu().numpy()
+    lut, inv = vv.make_filter(vv.FILTER_GAUSSIAN, L)
+    rgba, op = fields.tf_magnitude()
+    dimg = (np.random.default_rng(17).normal(size=W * W * 4) * 1e-3).astype(np.float32)
+    ctx = vv.Context(0)
+    ctx.generate_noise(1, vv.NOISE_UNIFORM, 0.0, dims)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    p.step_size_lic = 0.005
+    p.step_size_vol = 1.0 / 256.0
+    p.gradient_scale = 3.0
+    p.tf_index = vv.TF_INDEX_MAG
+    ctx.set_lic_params(p)
+    ctx.set_tf(rgba, op)
+    gpu = {}
+    for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16), ("unorm8", vv.STORE_UNORM8)):
+        ctx.upload_field(field, dims, storage=st)
+        ctx.compute_lic()
+        cen = ctx.field_geometry()[2]
+        cam = vv.default_camera(W, W, cen)
+        img = ctx.render(cam)
+        dtf, dsig = ctx.render_adjoint(cam, dimg)
+        gpu[name] = dict(lic=ctx.download_lic(), img=img, dtf=dtf, dsig=dsig)
+    ctx.close()
+    noise = orc.gen_noise(1, vv.NOISE_UNIFORM, 0.0, dims)
+    op64 = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.005, inv_filter_area=inv,
+                      gradient_scale=3.0, step_size_vol=1.0 / 256.0, tf_index=1)
+    view = orc.view_setup(W, W)
+    t = time.perf_counter()
+    f4 = orc.encode_field(host, dims, f64=True)
+    lic64 = orc.lic(f4, dims, noise, 1, dims, lut, op64, nthreads=cores)
+    img64, ns64, _ = orc.raycast(f4, dims, lic64, 1, dims, rgba, op, op64, view, nthreads=cores, f64=True)
+    dtf64, dsig64 = orc.raycast_adjoint(f4, dims, lic64, 1, dims, rgba, op, op64, view, dimg)
+    oracle_s = time.perf_counter() - t
+    del f4
+    lscale = float(np.max(np.abs(lic64)))
+    out = {"workload": f"{n}^3 Rankine x12, dense uniform noise, Gaussian L={L} h=0.005; {W}^2 frame step "
+                       f"1/256, vector-length TF; adjoint for a seeded dL/dimage (N(0, 1e-3))",
+           "oracle_s": oracle_s, "oracle_threads": cores}
+    for name, g in gpu.items():
+        st = {"fp32": vv.STORE_FP32, "fp16": vv.STORE_FP16, "unorm8": vv.STORE_UNORM8}[name]
+        if st == vv.STORE_UNORM8:
+            f4s = orc.field_bytes_as_float(orc.encode_field_u8(host, dims))
+        else:
+            f4s = orc.encode_field(host, dims)
+            if st == vv.STORE_FP16:
+                f4s = orc.f16_to_f32(orc.f32_to_f16(f4s))
+        ps = orc.Params.from_buffer_copy(op64)
+        ps.field_unorm8 = 1 if st == vv.STORE_UNORM8 else 0
+        _, ns_s, _ = orc.raycast(f4s, dims, g["lic"], 1, dims, rgba, op, ps, view, nthreads=cores)
+        same = ns_s == ns64
+        di = np.abs(g["img"].astype(np.float64) - img64).reshape(-1, 4)
+        dl = np.abs(g["lic"].astype(np.float64) - lic64)
+        out[name] = {
+            "lic_max_rel": float(dl.max()) / lscale,
+            "image_max_abs_same_termination": float(di[same].max()),
+            "image_max_abs_all_pixels": float(di.max()),
+            "image_rms": float(np.sqrt(np.mean(di ** 2))),
+            "termination_index_differs_frac": float(1.0 - same.mean()),
+            "dtf_max_rel": float(np.max(np.abs(g["dtf"] - dtf64)) / np.max(np.abs(dtf64))),
+            "dsigma_max_rel": float(np.max(np.abs(g["dsig"] - dsig64)) / np.max(np.abs(dsig64))),
+        }
+    out["fp16_meets_1e-3_image_bound"] = out["fp16"]["image_max_abs_same_termination"] <= 1e-3
     return out
 
 

@@ -143,6 +143,41 @@ def test_lic_parity(ctx, case):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8])
+def test_lic_streamlines_leaving_the_volume(ctx, storage):
+    # h_e * n = 2.4 voxels per step over 10 steps: most streamlines leave the
+    # 16^3 box and sample the REPEAT noise one or two periods away, where the
+    # noise no longer comes from the field texel's channel 3
+    n, L, h = 16, 10, 0.5
+    dims = (n, n, n)
+    rng = np.random.default_rng(21)
+    v = rng.normal(size=(n, n, n, 3)).astype(np.float32)
+    v[..., 0] += 2.0  # mostly along +x, so the lines run far out of the box
+    ctx.upload_field(v, dims, storage=storage)
+    ctx.generate_noise(4, vv.NOISE_UNIFORM, 0.0, dims)
+    lut, inv = vv.make_filter(vv.FILTER_TRIANGLE, L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = L
+    p.step_size_lic = h
+    p.gradient_scale = 3.0
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    got = ctx.download_lic()
+    from workloads import oracle_field
+    op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=h, inv_filter_area=inv, gradient_scale=3.0,
+                    field_unorm8=1 if storage == vv.STORE_UNORM8 else 0)
+    ref = orc.lic(oracle_field(v, dims, storage), dims, orc.gen_noise(4, vv.NOISE_UNIFORM, 0.0, dims), 1, dims,
+                  lut, op)
+    _report("lic-outside", got, ref)
+    assert rel_err(got, ref) <= TOL
+    # the uncached kernel (separate noise volume throughout) gives the same values
+    ctx.set_tuning("lic.cache", 0)
+    ctx.compute_lic()
+    ctx.set_tuning("lic.cache", 1)
+    assert rel_err(ctx.download_lic(), ref) <= TOL
+
+
 def test_lic_scalar_gate(ctx):
     n = 16
     rng = np.random.default_rng(0)

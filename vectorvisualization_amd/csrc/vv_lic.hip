@@ -409,9 +409,9 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
 
     const CachedArgs c = cached_args(a);
     LicFieldCache<SAME> fc;
-    fc.base = -1;
+    fc.reset();
     NoiseCache ncache;
-    ncache.base = -1;
+    ncache.reset();
     const float acc = lic_point_cached<FT, SAME>(a, c, fc, ncache, sx, sy, sz);
     store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
 }

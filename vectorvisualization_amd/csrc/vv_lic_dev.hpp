@@ -118,16 +118,23 @@ __device__ __forceinline__ void store_repeat(T* out, int osy, int osz, int x, in
 // those of tri(), so results do not change.
 // NCH = 3: the field direction; NCH = 4: also the noise byte that the LIC
 // copy of the field carries in channel 3 (equal field and noise grids)
+// A cache is keyed by the three floors of its cell (x = u n - 0.5), so a hit
+// costs three compares and no index arithmetic; NaN keys never hit.  (Cells
+// outside the volume with different floors but the same clamped texels are
+// reloaded, with the same values.)
 template <int NCH>
 struct FieldCache {
-    int base;
+    float kx, ky, kz;   // floors of the cached cell
+    bool same;          // NCH 4: the cell's noise bytes are those of the REPEAT cell
     float v0[NCH][4];   // [channel][(y,z) corner]: texel at x0
     float dx[NCH][4];   // texel(x1) - texel(x0)
+    __device__ __forceinline__ void reset() { kx = ky = kz = __builtin_nanf(""); same = false; }
 };
 
 struct NoiseCache {
-    int base;
+    float kx, ky, kz;
     float v0[4], dx[4];
+    __device__ __forceinline__ void reset() { kx = ky = kz = __builtin_nanf(""); }
 };
 
 __device__ __forceinline__ float tri_cached(const float* v0, const float* dx, float fx, float fy, float fz)
@@ -269,14 +276,36 @@ struct CachedArgs {
     unsigned nsy, nsz;  // noise element strides
 };
 
+__device__ __forceinline__ bool key_hit(float kx, float ky, float kz, const Ax& ax, const Ax& ay, const Ax& az)
+{
+    return ax.fl == kx && ay.fl == ky && az.fl == kz;
+}
+
+// Equal field and noise grids with freq 1 (SAME): the LIC copy of the field
+// holds, in channel 3 of the CLAMP-padded texel p, the noise byte of REPEAT
+// texel (p - 1) mod n per axis.  While every floor lies in [-1, n-1] the
+// CLAMP cell (clamp(floor) + 1, + 2) and the REPEAT cell (floor mod n, + 1)
+// then hold the same noise bytes, so the field fill brings the noise cell
+// with it: no noise loads.  Outside that range (a streamline that left the
+// volume) the noise comes from the REPEAT array.  The noise value and its
+// arithmetic are those of noise_from_axes.
+__device__ __forceinline__ bool same_cells(const CachedArgs& c, const Ax& ax, const Ax& ay, const Ax& az)
+{
+    return ax.fl >= -1.0f && ax.fl <= c.fnm1[0] && ay.fl >= -1.0f && ay.fl <= c.fnm1[1] && az.fl >= -1.0f &&
+           az.fl <= c.fnm1[2];
+}
+
 template <typename FT, int NCH>
 __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<NCH>& fc, const Ax& ax,
                                                   const Ax& ay, const Ax& az)
 {
-    const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
-                          __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
-    if ((int)base != fc.base) {
-        fc.base = (int)base;
+    if (!key_hit(fc.kx, fc.ky, fc.kz, ax, ay, az)) {
+        fc.kx = ax.fl;
+        fc.ky = ay.fl;
+        fc.kz = az.fl;
+        if constexpr (NCH == 4) fc.same = same_cells(c, ax, ay, az);
+        const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
+                              __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
         const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -306,14 +335,12 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
                                                   const Ax& az, const Ax& nx_, const Ax& ny_,
                                                   const Ax& nz_, float& noise)
 {
-    const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
-                        __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
-    const unsigned nb = __umul24(repeat_idx(nz_.fl, a.noise.n[2]), c.nsz) +
-                        __umul24(repeat_idx(ny_.fl, a.noise.n[1]), c.nsy) + repeat_idx(nx_.fl, a.noise.n[0]);
-    const bool fmiss = (int)fb != fc.base;
-    const bool nmiss = (int)nb != nc.base;
+    const bool fmiss = !key_hit(fc.kx, fc.ky, fc.kz, ax, ay, az);
+    const bool nmiss = !key_hit(nc.kx, nc.ky, nc.kz, nx_, ny_, nz_);
     unsigned short nraw[4];
     if (fmiss) {
+        const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
+                            __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
         const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -327,19 +354,25 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
         }
     }
     if (nmiss) {
+        const unsigned nb = __umul24(repeat_idx(nz_.fl, a.noise.n[2]), c.nsz) +
+                            __umul24(repeat_idx(ny_.fl, a.noise.n[1]), c.nsy) + repeat_idx(nx_.fl, a.noise.n[0]);
         const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) nraw[k] = (unsigned short)noise_pair(c.nr, nb + off[k]);
     }
     if (fmiss) {
-        fc.base = (int)fb;
+        fc.kx = ax.fl;
+        fc.ky = ay.fl;
+        fc.kz = az.fl;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
 #pragma unroll
             for (int ch = 0; ch < 3; ++ch) fc.dx[ch][k] = fc.dx[ch][k] - fc.v0[ch][k];
     }
     if (nmiss) {
-        nc.base = (int)nb;
+        nc.kx = nx_.fl;
+        nc.ky = ny_.fl;
+        nc.kz = nz_.fl;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const float t0 = (float)(nraw[k] & 0xffu), t1 = (float)(nraw[k] >> 8);
@@ -358,10 +391,12 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
 __device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedArgs& c, NoiseCache& nc,
                                                  const Ax& ax, const Ax& ay, const Ax& az)
 {
-    const unsigned base = __umul24(repeat_idx(az.fl, a.noise.n[2]), c.nsz) +
-                          __umul24(repeat_idx(ay.fl, a.noise.n[1]), c.nsy) + repeat_idx(ax.fl, a.noise.n[0]);
-    if ((int)base != nc.base) {
-        nc.base = (int)base;
+    if (!key_hit(nc.kx, nc.ky, nc.kz, ax, ay, az)) {
+        nc.kx = ax.fl;
+        nc.ky = ay.fl;
+        nc.kz = az.fl;
+        const unsigned base = __umul24(repeat_idx(az.fl, a.noise.n[2]), c.nsz) +
+                              __umul24(repeat_idx(ay.fl, a.noise.n[1]), c.nsy) + repeat_idx(ax.fl, a.noise.n[0]);
         const unsigned off[4] = {0u, c.nsy, c.nsz, c.nsz + c.nsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -408,25 +443,13 @@ __device__ __forceinline__ float noise_direct(const LicArgs& a, const CachedArgs
     return tri_cached(v0, dx, ax.f, ay.f, az.f) * kInv255;
 }
 
-// Equal field and noise grids with freq 1 (SAME): the LIC copy of the field
-// holds, in channel 3 of the CLAMP-padded texel p, the noise byte of REPEAT
-// texel (p - 1) mod n per axis.  While every floor lies in [-1, n-1] the
-// CLAMP cell (clamp(floor) + 1, + 2) and the REPEAT cell (floor mod n, + 1)
-// then hold the same noise bytes, so the field fill brings the noise cell
-// with it: no noise loads.  Outside that range (a streamline that left the
-// volume) the noise comes from the REPEAT array.  The noise value and its
-// arithmetic are those of noise_from_axes.
-__device__ __forceinline__ bool same_cells(const CachedArgs& c, const Ax& ax, const Ax& ay, const Ax& az)
-{
-    return ax.fl >= -1.0f && ax.fl <= c.fnm1[0] && ay.fl >= -1.0f && ay.fl <= c.fnm1[1] && az.fl >= -1.0f &&
-           az.fl <= c.fnm1[2];
-}
-
+// noise of the cell just fetched into fc (SAME): channel 3 of the cache, or
+// the REPEAT array for a cell outside the volume
 template <typename FT>
 __device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& c, const FieldCache<4>& fc,
                                             const Ax& ax, const Ax& ay, const Ax& az)
 {
-    if (same_cells(c, ax, ay, az)) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f) * kInv255;
+    if (fc.same) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f) * kInv255;
     return noise_direct(a, c, ax, ay, az);
 }
 

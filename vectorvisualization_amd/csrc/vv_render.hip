@@ -658,9 +658,9 @@ k_render_fused(RenderLaunch a, LicArgs L)
                              a.field.sz};
             const CachedArgs c = cached_args(L);
             LicFieldCache<MODE == 2> fc;
-            fc.base = -1;
+            fc.reset();
             NoiseCache nc;
-            nc.base = -1;
+            nc.reset();
             float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
             for (long long k = 0; k < a.max_samples; ++k) {
                 float l4[4];
@@ -727,9 +727,9 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
                          a.field.sz};
         const CachedArgs c = cached_args(L);
         LicFieldCache<MODE == 2> fc;
-        fc.base = -1;
+        fc.reset();
         NoiseCache nc;
-        nc.base = -1;
+        nc.reset();
         float dest[4] = {0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < S.n && vdir != 0.0f; ++k) {
             if (!(dest[3] < 0.95f)) break;
@@ -823,9 +823,9 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
     const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy, a.field.sz};
     const CachedArgs c = cached_args(L);
     LicFieldCache<MODE == 2> fc;
-    fc.base = -1;
+    fc.reset();
     NoiseCache nc;
-    nc.base = -1;
+    nc.reset();
     float dest[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned long long ns = 0;
     Ray r;
