@@ -419,9 +419,12 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
 }
 
 // dsigma accumulation window in LDS: entries (int64 fixed point), one
-// window per kSub samples of a segment
-constexpr int kWin = 6144;
-constexpr int kSub = 32;
+// window per kSub samples of a segment.  Sized for occupancy: 32 KB of
+// window (45 KB of LDS per block) lets three blocks share a CU, and the
+// voxel box of 16 samples of a 16x16-ray tile fits it at 256^3 / 1024^2
+// (a box that does not fit goes to global atomics, with the same sums).
+constexpr int kWin = 4096;
+constexpr int kSub = 16;
 constexpr int kNSub = kSeg / kSub;
 
 template <typename FT, int LC>
@@ -446,8 +449,11 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
     const long long nrays = W * H;
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
-    float4* ck = reinterpret_cast<float4*>(A.ckpt);                                   // [c][2][ray]
-    float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;  // [k][2][ray]
+    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
+    // scratch column of one segment: [k][ray] D before sample k, with the
+    // sample's clamp mask in the sign bits (D lies in [0,1]); then
+    // [kSeg + m][ray] the position at the start of part m
+    float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;
 
     // phase 1: forward with a checkpoint (pos, dest) every kSeg samples
     Ray r;
@@ -502,13 +508,16 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
             float d[4] = {cd.x, cd.y, cd.z, cd.w};
             int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
             for (long long k = k0; k < k1; ++k) {
+                if ((k - k0) % kSub == 0)
+                    seg[(kSeg + (k - k0) / kSub) * nrays + pix] = make_float4(p[0], p[1], p[2], 0.f);
                 Sample s;
                 eval_sample<FT, LC>(a, tf, op, r, p, s);
-                const long long slot = (k - k0) * 2;
-                const float4 dprev = make_float4(d[0], d[1], d[2], d[3]);
+                unsigned dbits[4];
+                for (int q = 0; q < 4; ++q) dbits[q] = __float_as_uint(d[q]);
                 const int mask = composite(d, s);
-                seg[slot * nrays + pix] = make_float4(p[0], p[1], p[2], __int_as_float(mask));
-                seg[(slot + 1) * nrays + pix] = dprev;
+                for (int q = 0; q < 4; ++q) dbits[q] |= (unsigned)((mask >> q) & 1) << 31;
+                seg[(k - k0) * nrays + pix] = make_float4(__uint_as_float(dbits[0]), __uint_as_float(dbits[1]),
+                                                          __uint_as_float(dbits[2]), __uint_as_float(dbits[3]));
                 if (A.dsigma_fx) {
                     for (int q = 0; q < 3; ++q) {
                         float f;
@@ -542,12 +551,19 @@ __global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
             const long long ka = k0 + (long long)m * kSub;
             const long long kb = min(k1, ka + kSub);
             if (mine) {
+                const float4 ps = seg[(kSeg + m) * nrays + pix];
                 for (long long k = kb - 1; k >= ka; --k) {
-                    const long long slot = (k - k0) * 2;
-                    const float4 sp = seg[slot * nrays + pix];
-                    const float4 dk = seg[(slot + 1) * nrays + pix];
-                    const float pk[3] = {sp.x, sp.y, sp.z};
-                    const int mask = __float_as_int(sp.w);
+                    const float4 dm = seg[(k - k0) * nrays + pix];
+                    const unsigned db[4] = {__float_as_uint(dm.x), __float_as_uint(dm.y), __float_as_uint(dm.z),
+                                            __float_as_uint(dm.w)};
+                    int mask = 0;
+                    for (int q = 0; q < 4; ++q) mask |= (int)(db[q] >> 31) << q;
+                    const float4 dk = make_float4(__uint_as_float(db[0] & 0x7fffffffu), __uint_as_float(db[1] & 0x7fffffffu),
+                                                  __uint_as_float(db[2] & 0x7fffffffu), __uint_as_float(db[3] & 0x7fffffffu));
+                    // the sample position: the forward fma chain from the part's start
+                    float pk[3] = {ps.x, ps.y, ps.z};
+                    for (long long t = ka; t < k; ++t)
+                        for (int q = 0; q < 3; ++q) pk[q] = fmaf(r.dir[q], a.step, pk[q]);
                     Sample s;
                     eval_sample<FT, LC>(a, tf, op, r, pk, s);
                     for (int q = 0; q < 4; ++q)
