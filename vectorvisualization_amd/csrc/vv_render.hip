@@ -419,16 +419,17 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
 }
 
 // dsigma accumulation window in LDS: entries (int64 fixed point), one
-// window per kSub samples of a segment.  Sized for occupancy: 32 KB of
-// window (45 KB of LDS per block) lets three blocks share a CU, and the
-// voxel box of 16 samples of a 16x16-ray tile fits it at 256^3 / 1024^2
-// (a box that does not fit goes to global atomics, with the same sums).
-constexpr int kWin = 4096;
+// window per kSub samples of a segment.  Sized for occupancy: 24 KB of
+// window (38 KB of LDS per block) lets four blocks share a CU, matching the
+// kernel's 4 waves per SIMD (at most 128 VGPRs), and the voxel box of 16
+// samples of a 16x16-ray tile fits it at 256^3 / 1024^2 (a box that does
+// not fit goes to global atomics, with the same sums).
+constexpr int kWin = 3072;
 constexpr int kSub = 16;
 constexpr int kNSub = kSeg / kSub;
 
 template <typename FT, int LC>
-__global__ void __launch_bounds__(kRThreads) k_render_adjoint(AdjointLaunch A)
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4))) k_render_adjoint(AdjointLaunch A)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
