@@ -210,6 +210,30 @@ def test_lic_non_cubic(ctx):
     assert rel_err(got, ref) <= TOL
 
 
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_UNORM8])
+def test_lic_non_cubic_same_grid(ctx, storage):
+    # noise on the field's non-cubic grid: the LIC field copy carries the
+    # noise bytes, wrapped per axis with that axis's size
+    dims = (24, 16, 12)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(dims[2], dims[1], dims[0], 3)).astype(np.float32)
+    ctx.upload_field(v, dims, spacing=(1.0, 1.0, 2.0), storage=storage)
+    ctx.generate_noise(5, vv.NOISE_UNIFORM, 0.0, dims)
+    p = vv.default_lic_params()
+    p.steps_forward, p.steps_backward = 7, 11
+    p.step_size_lic = 0.05
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    got = ctx.download_lic()
+    ext, sc, cen = ctx.field_geometry()
+    from workloads import oracle_field
+    op = orc.params(steps_forward=7, steps_backward=11, step_size_lic=0.05,
+                    field_unorm8=1 if storage == vv.STORE_UNORM8 else 0)
+    ref = orc.lic(oracle_field(v, dims, storage), dims, orc.gen_noise(5, vv.NOISE_UNIFORM, 0.0, dims), 1, dims,
+                  np.full(256, 255, np.uint8), op, scale=sc)
+    assert rel_err(got, ref) <= TOL
+
+
 def test_lic_deterministic(ctx):
     lic_setup(ctx, n=24, kind="rankine", L=16)
     ctx.compute_lic()
