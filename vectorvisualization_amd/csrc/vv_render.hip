@@ -657,7 +657,7 @@ __global__ void k_fixed_to_float(const long long* src, float* dst, long long n, 
 // no gate), which persist along the ray; 2: the same with the noise grid
 // equal to the field grid at freq 1.
 template <typename FT, int NC, bool GATE, int MODE>
-__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_render_fused(RenderLaunch a, LicArgs L)
 {
     __shared__ float tf[1024];
@@ -719,7 +719,7 @@ k_render_fused(RenderLaunch a, LicArgs L)
 __device__ __forceinline__ float half_rn(float x) { return __half2float(__float2half_rn(x)); }
 
 template <typename FT, int NC, bool GATE, int MODE>
-__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
 {
     __shared__ float tf[1024];
@@ -815,7 +815,7 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
 //                    at most 0.05; stop once dest.a >= 0.95; dest rounded to
 //                    FP16 after every composited sample
 template <typename FT, int NC, bool GATE, int MODE, bool SLICES>
-__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
 {
     __shared__ float tf[1024];
