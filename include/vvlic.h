@@ -395,7 +395,8 @@ vv_status vv_anim_destroy(vv_anim* a);
  * "render.block_x", "render.block_y" (pixel tile of the ray caster, 64 to
  * 256 threads in multiples of 64), "render.ray_group" (lanes per ray of
  * vv_render_fused / vv_render_slices, a power of two up to 64; 0 = enough
- * lanes for about 2^21 in total; the image does not depend on it).
+ * lanes for about 2^21 in total with an FP32 field and 2^22 with FP16 or
+ * UNORM8, at most 32 per ray; the image does not depend on it).
  * Results do not depend on these settings. */
 vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
 vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);

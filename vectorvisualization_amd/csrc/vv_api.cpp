@@ -667,12 +667,18 @@ vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device
 // lanes per ray of the on-the-fly casters: the tuning value, or enough to
 // put about 2^21 lanes to work (measured on MI355X at 256^3, L = 40: 2 lanes
 // per ray at 1024^2, 8 at 512^2, 32 at 256^2, 64 below)
+// Lanes per ray of the on-the-fly casters: enough lanes in flight to cover
+// the serial chain of samples.  Measured at 256^3 L = 40 on 256^2..1024^2
+// views: about 2^21 lanes for an FP32 field (its LIC is bound by L1 gathers,
+// more lanes do not help) and 2^22 for FP16 / UNORM8 fields, at most 32
+// lanes per ray.
 static int ray_group_for(const vv_ctx* c, const vv_camera* cam)
 {
     if (c->ray_group > 0) return c->ray_group;
     const long long rays = (long long)cam->width * cam->height;
+    const long long lanes = c->storage == VV_STORE_FP32 ? (1ll << 21) : (1ll << 22);
     int G = 1;
-    while (G < 64 && rays * G < (1ll << 21)) G *= 2;
+    while (G < 32 && rays * G < lanes) G *= 2;
     return G;
 }
 
