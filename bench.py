@@ -215,6 +215,14 @@ def main():
         rate = l1 / (lic_ms / 1000.0) / (MI355X_CUS * MI355X_CLOCK_HZ)
         out["roofline"]["l1_gather"] = {"accesses_per_launch": l1, "per_clock_per_cu": rate, "peak": 1.0,
                                         "frac": rate, "hbm_traffic_gbs": (traffic or 0) / (lic_ms / 1000.0) / 1e9}
+    # L2 (TCC) requests of the same kernel at 128 B per line, against 16
+    # channels x 128 B per clock per XCD (an assumed CDNA L2 figure)
+    l2 = _profile_value(f"{tag}_pmc.txt", "TCC requests per launch")
+    if l2 is not None:
+        l2_gbs = l2 * 128.0 / (lic_ms / 1000.0) / 1e9
+        l2_peak = 8 * 16 * 128.0 * MI355X_CLOCK_HZ / 1e9
+        out["roofline"]["l2"] = {"requests_per_launch": l2, "achieved": l2_gbs, "peak": l2_peak, "unit": "GB/s",
+                                 "frac": l2_gbs / l2_peak, "peak_basis": "16 channels x 128 B/clk per XCD, 8 XCDs"}
 
     if not args.no_extras:
         def timed(fn, reps=3):
