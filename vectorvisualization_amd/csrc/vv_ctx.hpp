@@ -101,7 +101,7 @@ struct vv_ctx {
 
     // launch tuning (vv_set_tuning)
     int lic_tile[3] = {16, 4, 4};
-    int lic_super[3] = {2, 8, 8};
+    int lic_super[3] = {2, 16, 16};
     int lic_cache = 1;
     int adj_window = 1;
     int render_block[2] = {16, 16};
