@@ -155,6 +155,16 @@ vv_status vv_default_camera(vv_camera* cam, int width, int height, const float c
 vv_status vv_upload_field(vv_ctx* ctx, const float* v0, const float* v1, float lerp_t,
                           const int dims[3], const float spacing[3], int storage,
                           int on_device);
+/* Field encodings of vv_upload_field_ex. */
+typedef enum vv_field_encoding {
+    VV_ENCODE_INTERP = 0,        /* a = |v|/max|v| (fillTexDataFloatInterp / CharInterp), as vv_upload_field */
+    VV_ENCODE_RAW_MAGNITUDE = 1  /* a = |v| unnormalised (fillTexDataFloat, VV/dataset.cpp:448-529, the
+                                  * encoding of VectorDataSet::createTexture :213-288); FP32 / FP16 only */
+} vv_field_encoding;
+/* vv_upload_field with a choice of field encoding. */
+vv_status vv_upload_field_ex(vv_ctx* ctx, const float* v0, const float* v1, float lerp_t,
+                             const int dims[3], const float spacing[3], int storage, int encoding,
+                             int on_device);
 /* extent[3], scale[3], center[3] of the loaded field (VolumeData). */
 vv_status vv_field_geometry(const vv_ctx* ctx, float extent[3], float scale[3],
                             float center[3]);

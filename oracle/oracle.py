@@ -136,6 +136,8 @@ def _load() -> C.CDLL:
         "orc_encode_field_f32": (None, [P, P, F, I, I, I, P]),
         "orc_encode_field_u8": (None, [P, P, F, I, I, I, P]),
         "orc_encode_field_f64": (None, [P, P, D, I, I, I, P]),
+        "orc_encode_field_raw_f32": (None, [P, P, F, I, I, I, P]),
+        "orc_encode_field_raw_f64": (None, [P, P, D, I, I, I, P]),
         "orc_lic_weights_f32": (None, [P, I, I, I, P]),
         "orc_lic_weights_f64": (None, [P, I, I, I, P]),
         "orc_lic_f32": (None, [P, I, I, I, P, P, I, P, P, P, P, I, C.POINTER(Params), I, I, I, P]),
@@ -287,12 +289,16 @@ def light_position(center=(0.5, 0.5, 0.5), cam_pos=(0, 0, 0), cam_quat=(0, 0, 0,
     return out
 
 
-def encode_field(v0: np.ndarray, dims, v1=None, t=0.0, f64=False) -> np.ndarray:
+def encode_field(v0: np.ndarray, dims, v1=None, t=0.0, f64=False, raw_magnitude=False) -> np.ndarray:
+    """fillTexDataFloatInterp; raw_magnitude: fillTexDataFloat (a = |v|)."""
     nx, ny, nz = dims
     v0 = np.ascontiguousarray(v0, np.float32).ravel()
     v1 = None if v1 is None else np.ascontiguousarray(v1, np.float32).ravel()
     out = np.empty(4 * nx * ny * nz, np.float64 if f64 else np.float32)
-    fn = LIB.orc_encode_field_f64 if f64 else LIB.orc_encode_field_f32
+    if raw_magnitude:
+        fn = LIB.orc_encode_field_raw_f64 if f64 else LIB.orc_encode_field_raw_f32
+    else:
+        fn = LIB.orc_encode_field_f64 if f64 else LIB.orc_encode_field_f32
     fn(_p(v0), _p(v1), t, nx, ny, nz, _p(out))
     return out
 

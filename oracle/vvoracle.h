@@ -124,6 +124,12 @@ void orc_light_position(const double center[3], const double cam_pos[3],
 /* ---- floating point work, two precisions ---- */
 void orc_encode_field_f32(const float* v0, const float* v1, float t,
                           int nx, int ny, int nz, float* out4);
+/* fillTexDataFloat (VV/dataset.cpp:448-529): as orc_encode_field_* but
+ * a = |v| unnormalised */
+void orc_encode_field_raw_f32(const float* v0, const float* v1, float t,
+                              int nx, int ny, int nz, float* out);
+void orc_encode_field_raw_f64(const float* v0, const float* v1, double t,
+                              int nx, int ny, int nz, double* out);
 /* UNORM8 field (fillTexDataCharInterp, VV/dataset.cpp:704-823): 4 bytes per
  * voxel, direction bytes and |v|/max as a byte; see vvoracle.c */
 void orc_encode_field_u8(const float* v0, const float* v1, float t,

@@ -142,13 +142,10 @@ static inline R FN(lut1d_u8)(const uint8_t* t, int n, R u)
 /* a = clamp(|v|/max|v|, 0, 1).  An all-zero field (max 0) gets a = 0 */
 /* (the reference would store 0/0).                                   */
 /* ------------------------------------------------------------------ */
-#if ORC_DOUBLE
-void orc_encode_field_f64(const float* v0, const float* v1, double t,
-                          int nx, int ny, int nz, double* out)
-#else
-void orc_encode_field_f32(const float* v0, const float* v1, float t,
-                          int nx, int ny, int nz, float* out)
-#endif
+/* raw = 0: fillTexDataFloatInterp (VV/dataset.cpp:533-635), a = |v|/max
+ * clamped to [0,1]; raw = 1: fillTexDataFloat (:448-529), a = |v| as is
+ * (its max normalisation is commented out, :517-525) */
+static void FN(encode_field)(const float* v0, const float* v1, R t, int nx, int ny, int nz, int raw, R* out)
 {
     long n = (long)nx * ny * nz;
     R maxlen = K(-1);
@@ -157,7 +154,7 @@ void orc_encode_field_f32(const float* v0, const float* v1, float t,
         R tv[3];
         for (int c = 0; c < 3; ++c) {
             R a = (R)v0[3 * i + c], b = (R)v1[3 * i + c];
-            tv[c] = a + (R)t * (b - a);
+            tv[c] = a + t * (b - a);
         }
         R len = SQRT(tv[0] * tv[0] + tv[1] * tv[1] + tv[2] * tv[2]);
         if (len < K(1e-5f)) {
@@ -169,11 +166,32 @@ void orc_encode_field_f32(const float* v0, const float* v1, float t,
         out[4 * i + 3] = len;
         if (len > maxlen) maxlen = len;
     }
+    if (raw) return;
     for (long i = 0; i < n; ++i) {
         R a = maxlen > K(0) ? out[4 * i + 3] / maxlen : K(0);
         out[4 * i + 3] = a > K(1) ? K(1) : (a < K(0) ? K(0) : a);
     }
 }
+
+#if ORC_DOUBLE
+void orc_encode_field_f64(const float* v0, const float* v1, double t, int nx, int ny, int nz, double* out)
+{
+    encode_field_f64(v0, v1, t, nx, ny, nz, 0, out);
+}
+void orc_encode_field_raw_f64(const float* v0, const float* v1, double t, int nx, int ny, int nz, double* out)
+{
+    encode_field_f64(v0, v1, t, nx, ny, nz, 1, out);
+}
+#else
+void orc_encode_field_f32(const float* v0, const float* v1, float t, int nx, int ny, int nz, float* out)
+{
+    encode_field_f32(v0, v1, t, nx, ny, nz, 0, out);
+}
+void orc_encode_field_raw_f32(const float* v0, const float* v1, float t, int nx, int ny, int nz, float* out)
+{
+    encode_field_f32(v0, v1, t, nx, ny, nz, 1, out);
+}
+#endif
 
 /* ------------------------------------------------------------------ */
 /* LIC kernel weights, one per convolution sample, in sample order:    */

@@ -22,6 +22,8 @@ from ._lib import (  # noqa: F401
     STORE_FP16,
     STORE_FP32,
     STORE_UNORM8,
+    ENCODE_INTERP,
+    ENCODE_RAW_MAGNITUDE,
     FILTER_BOX,
     FILTER_COS2,
     FILTER_GAUSSIAN,

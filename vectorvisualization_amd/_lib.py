@@ -18,6 +18,7 @@ VV_OK, VV_EINVAL, VV_ENOMEM, VV_EDEVICE, VV_EUNSUPPORTED, VV_ESTATE = range(6)
 STATUS_NAMES = {0: "VV_OK", 1: "VV_EINVAL", 2: "VV_ENOMEM", 3: "VV_EDEVICE",
                 4: "VV_EUNSUPPORTED", 5: "VV_ESTATE", 6: "VV_EIO"}
 STORE_FP32, STORE_FP16, STORE_UNORM8 = 0, 1, 2
+ENCODE_INTERP, ENCODE_RAW_MAGNITUDE = 0, 1
 FILTER_BOX, FILTER_TRIANGLE, FILTER_COS2, FILTER_GAUSSIAN = range(4)
 NOISE_BINARY, NOISE_UNIFORM = 0, 1
 ILLUM_LIC, ILLUM_GRADIENT = 0, 1
@@ -114,7 +115,7 @@ def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
             f"libvvlic.so not found at {LIB_PATH}; build it with "
-            "`python -m vectorvisualization_amd.build` (hipcc, gfx950)")
+            "`python vectorvisualization_amd/build.py` (hipcc, gfx950)")
     lib = C.CDLL(str(LIB_PATH))
     P, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
     IP = C.POINTER(C.c_int)
@@ -130,6 +131,7 @@ def _load() -> C.CDLL:
         "vv_default_lic_params": (I, [C.POINTER(LicParams)]),
         "vv_default_camera": (I, [C.POINTER(Camera), I, I, FP]),
         "vv_upload_field": (I, [P, P, P, F, IP, FP, I, I]),
+        "vv_upload_field_ex": (I, [P, P, P, F, IP, FP, I, I, I]),
         "vv_field_geometry": (I, [P, FP, FP, FP]),
         "vv_download_field": (I, [P, P, I]),
         "vv_upload_noise": (I, [P, u8p, IP, I, u8p, I]),
@@ -205,7 +207,7 @@ LIB = _load()
 # every function declared in include/vvlic.h
 EXPORTED = [
     "vv_create", "vv_destroy", "vv_last_error", "vv_abi_version", "vv_stream", "vv_synchronize",
-    "vv_default_lic_params", "vv_default_camera", "vv_upload_field", "vv_field_geometry",
+    "vv_default_lic_params", "vv_default_camera", "vv_upload_field", "vv_upload_field_ex", "vv_field_geometry",
     "vv_download_field", "vv_upload_noise", "vv_generate_noise", "vv_download_noise",
     "vv_noise_info", "vv_make_filter", "vv_set_filter", "vv_set_tf", "vv_set_tf_u8",
     "vv_default_tf_u8", "vv_upload_scalar", "vv_set_lic_params", "vv_get_lic_params",
@@ -476,12 +478,16 @@ class Context:
 
     # -- inputs --
     def upload_field(self, v0, dims, v1=None, lerp_t: float = 0.0, spacing=(1.0, 1.0, 1.0),
-                     storage: int = STORE_FP32) -> None:
+                     storage: int = STORE_FP32, encoding: int = ENCODE_INTERP) -> None:
         p0, dev = _ptr(v0)
         p1 = _ptr(v1)[0] if v1 is not None else None
         with self.ordered(v0, v1):
-            self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
-                                          int(storage), dev))
+            if encoding == ENCODE_INTERP:
+                self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
+                                              int(storage), dev))
+            else:
+                self._chk(LIB.vv_upload_field_ex(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
+                                                 int(storage), int(encoding), dev))
 
     def set_line_material(self, m: LineMaterial) -> None:
         self._chk(LIB.vv_set_line_material(self._h, C.byref(m)))

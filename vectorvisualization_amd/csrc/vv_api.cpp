@@ -104,7 +104,17 @@ vv_status vv_default_camera(vv_camera* cam, int width, int height, const float c
 vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float lerp_t, const int dims[3],
                           const float spacing[3], int storage, int on_device)
 {
+    return vv_upload_field_ex(c, v0, v1, lerp_t, dims, spacing, storage, VV_ENCODE_INTERP, on_device);
+}
+
+vv_status vv_upload_field_ex(vv_ctx* c, const float* v0, const float* v1, float lerp_t, const int dims[3],
+                             const float spacing[3], int storage, int encoding, int on_device)
+{
     if (!c) return VV_EINVAL;
+    if (encoding != VV_ENCODE_INTERP && encoding != VV_ENCODE_RAW_MAGNITUDE)
+        return fail(c, VV_EINVAL, "unknown field encoding %d", encoding);
+    if (encoding == VV_ENCODE_RAW_MAGNITUDE && storage == VV_STORE_UNORM8)
+        return fail(c, VV_EINVAL, "the raw-magnitude encoding has float texels only (FP32, FP16)");
     if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
     if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
     if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage %d", storage);
@@ -140,7 +150,8 @@ vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float ler
     uint32_t* maxbits = (uint32_t*)c->counter.p;
     HIPCHK(c, launch_fill_u32(maxbits, 0u, 1, c->stream));
     HIPCHK(c, launch_field_maxlen(d0, d1, lerp_t, n, maxbits, c->stream));
-    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, storage, c->stream));
+    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, storage, c->stream,
+                                  encoding == VV_ENCODE_RAW_MAGNITUDE));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
     float vol[3], maxvol = 0.0f;

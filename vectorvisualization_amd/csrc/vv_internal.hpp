@@ -131,7 +131,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long long n,
                                uint32_t* maxbits, hipStream_t s);
 hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int dims[3],
-                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s);
+                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s,
+                               int raw = 0);  // raw: a = |v| (fillTexDataFloat), float types only
 hipError_t launch_field_decode(const void* field, int field_type, const int dims[3], float* out4,
                                hipStream_t s);
 // the LIC copy of a field for a noise volume on the same grid (REPEAT-padded

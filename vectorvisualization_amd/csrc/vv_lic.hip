@@ -112,7 +112,7 @@ __device__ __forceinline__ uint32_t unorm8_dir_byte(float q)
 template <typename T>
 __global__ void __launch_bounds__(kThreads) k_field_encode(const float* v0, const float* v1, float t,
                                                            int nx, int ny, int nz,
-                                                           const uint32_t* maxbits, T* out)
+                                                           const uint32_t* maxbits, int raw, T* out)
 {
     const int px = nx + 2, py = ny + 2, pz = nz + 2;
     const long long np = (long long)px * py * pz;
@@ -144,9 +144,12 @@ __global__ void __launch_bounds__(kThreads) k_field_encode(const float* v0, cons
             g = 0.5f * tv[1] / len + 0.5f;
             b = 0.5f * tv[2] / len + 0.5f;
         }
-        float maxlen = __uint_as_float(*maxbits);
-        float a = maxlen > 0.0f ? len / maxlen : 0.0f;
-        a = a > 1.0f ? 1.0f : (a < 0.0f ? 0.0f : a);
+        float a = len;  // raw: fillTexDataFloat keeps |v| (VV/dataset.cpp:512-513)
+        if (!raw) {
+            float maxlen = __uint_as_float(*maxbits);
+            a = maxlen > 0.0f ? len / maxlen : 0.0f;
+            a = a > 1.0f ? 1.0f : (a < 0.0f ? 0.0f : a);
+        }
         store_enc<T>(out, q, r, g, b, a);
     }
 }
@@ -503,13 +506,13 @@ hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long l
 }
 
 hipError_t launch_field_encode(const float* v0, const float* v1, float t, const int d[3],
-                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s)
+                               const uint32_t* maxbits, void* out, int field_type, hipStream_t s, int raw)
 {
     long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
     return with_field_type(field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
         hipLaunchKernelGGL(k_field_encode<FT>, dim3(blocks_for(np)), dim3(kThreads), 0, s, v0, v1, t, d[0], d[1],
-                           d[2], maxbits, (FT*)out);
+                           d[2], maxbits, raw, (FT*)out);
     });
 }
 
