@@ -64,6 +64,36 @@ def test_field_encode_unorm8_bitexact(ctx):
     assert np.all(z == [128, 128, 128, 0])
 
 
+@pytest.mark.parametrize("storage", [vv.STORE_FP32, vv.STORE_FP16])
+def test_field_encode_raw_magnitude(ctx, storage):
+    # vv_upload_field_ex with the encoding of createTexture / fillTexDataFloat
+    n = 16
+    v = field("rankine", n) * 3.0
+    ctx.upload_field(v, (n, n, n), storage=storage, encoding=vv.ENCODE_RAW_MAGNITUDE)
+    got = ctx.download_field((n, n, n))
+    ref = orc.encode_field(v, (n, n, n), raw_magnitude=True)
+    if storage == vv.STORE_FP16:
+        ref = orc.f16_to_f32(orc.f32_to_f16(ref))
+    np.testing.assert_array_equal(got, ref)
+    assert got.reshape(-1, 4)[:, 3].max() > 1.0
+    # the vector-length TF index reads the raw length (clamped by the TF lookup)
+    c = lic_setup(ctx, n=n, kind="rankine", L=6)
+    ctx.upload_field(v, (n, n, n), storage=storage, encoding=vv.ENCODE_RAW_MAGNITUDE)
+    ctx.compute_lic()
+    c["f4"] = ref
+    img, rimg, nsamp, total = _render_case(ctx, c, 40, 40, tf_index=vv.TF_INDEX_MAG, tf=vv.fields.tf_random(2))
+    assert nsamp == total
+    assert rel_err(img, rimg) <= TOL
+
+
+def test_field_encoding_errors(ctx):
+    v = field("abc", 8)
+    with pytest.raises(vv.VVError):
+        ctx.upload_field(v, (8, 8, 8), storage=vv.STORE_UNORM8, encoding=vv.ENCODE_RAW_MAGNITUDE)
+    with pytest.raises(vv.VVError):
+        ctx.upload_field(v, (8, 8, 8), encoding=7)
+
+
 def test_field_zero_vector(ctx):
     n = 4
     v = np.zeros((n, n, n, 3), np.float32)

@@ -204,6 +204,17 @@ def test_encode_closed_form_and_zero_vector():
     assert np.all(got32[0, 0, 0] == [0.5, 0.5, 0.5, 0.0])
 
 
+def test_encode_raw_magnitude_keeps_the_length():
+    # fillTexDataFloat: rgb as the interpolated encoding, a = |v| unnormalised
+    v = np.array([[3, 4, 0], [0, 0, 0], [0, 0.5, 0], [1, 2, 2]], np.float32)
+    for f64 in (False, True):
+        raw = orc.encode_field(v, (4, 1, 1), f64=f64, raw_magnitude=True).reshape(4, 4)
+        ref = orc.encode_field(v, (4, 1, 1), f64=f64).reshape(4, 4)
+        np.testing.assert_array_equal(raw[:, :3], ref[:, :3])
+        np.testing.assert_array_equal(raw[:, 3], [5.0, 0.0, 0.5, 3.0])
+        np.testing.assert_array_equal(ref[:, 3], np.array([1.0, 0.0, 0.1, 0.6], raw.dtype))
+
+
 def test_encode_u8_known_answers():
     # fillTexDataCharInterp: byte = (unsigned char)(v/|v|*127.0 + 128.0), truncated
     v = np.array([[1, 0, 0], [-1, 0, 0], [0, 0, 0], [3, 4, 0], [-3, -4, 0],
