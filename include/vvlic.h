@@ -356,6 +356,13 @@ vv_status vv_noise_file_write(const char* path, const uint8_t* noise, const int 
  * (loadGradients / saveGradients, VV/gradient.cpp:112-187) */
 vv_status vv_load_noise_file(vv_ctx* ctx, const char* path, int with_gradients,
                              int use_gradient_cache);
+/* NoiseDataSet::loadData with its fallback (VV/dataset.cpp:1124-1172): when the
+ * noise file cannot be read (path NULL, missing, short or malformed), 256^3
+ * binary white noise is generated instead, 255 with probability 1/6 as the
+ * reference's floor(0.6 rand()/(RAND_MAX+1) + 0.5) * 255 (:1158-1162), from
+ * `seed` (vv_generate_noise).  *generated (may be NULL) is 1 when it was. */
+vv_status vv_load_noise_or_generate(vv_ctx* ctx, const char* path, int with_gradients,
+                                    int use_gradient_cache, uint64_t seed, int* generated);
 /* pngRead (VV/imageUtils.cpp:19-96): 8-bit gray, gray+alpha, RGB or RGBA, rows in file
  * order.  out == NULL reads the header only. */
 vv_status vv_png_read(const char* path, uint8_t* out, size_t out_bytes, int* width, int* height,

@@ -120,3 +120,39 @@ def test_load_scalar_dat_gates_lic(ctx, tmp_path):
                   np.full(256, 255, np.uint8), op, scalar=su8, sdims=dims)
     assert np.any(ref == 0) and np.any(ref != 0)
     assert rel_err(ctx.download_lic(), ref) <= 1e-5
+
+
+def test_noise_fallback_generates_white_noise(ctx, tmp_path):
+    # NoiseDataSet::loadData: an unreadable noise file gives 256^3 binary
+    # noise at density 1/6 (the reference's floor(0.6 u + 0.5) * 255)
+    ref = orc.gen_noise(9, vv.NOISE_BINARY, 1.0 / 6.0, (256, 256, 256))
+    assert abs(np.mean(ref == 255) - 1.0 / 6.0) < 1e-3
+    for path in (None, tmp_path / "missing.raw"):
+        assert ctx.load_noise_or_generate(path, seed=9) is True
+        np.testing.assert_array_equal(ctx.download_noise(), ref)
+    short = tmp_path / "short.raw"
+    short.write_bytes(np.array([10, 10, 10], np.int32).tobytes() + bytes(5))
+    assert ctx.load_noise_or_generate(short, seed=9) is True
+    assert ctx.noise_info()[0] == [256, 256, 256]
+    # a readable file is used as is
+    dims = (12, 10, 8)
+    noise = orc.gen_noise(2, vv.NOISE_UNIFORM, 0.0, dims)
+    good = tmp_path / "good.raw"
+    L.noise_file_write(good, noise, dims)
+    assert ctx.load_noise_or_generate(good, seed=9) is False
+    np.testing.assert_array_equal(ctx.download_noise(), noise)
+
+
+def test_failed_filter_and_tf_loads_keep_the_defaults(ctx, tmp_path):
+    # a missing filter PNG leaves the box filter (VV/3DLIC.cpp:725-729), a
+    # missing TF file set leaves the caller's table untouched
+    p = vv.default_lic_params()
+    p.steps_forward = p.steps_backward = 6
+    ctx.set_lic_params(p)
+    with pytest.raises(vv.VVError):
+        ctx.load_filter_png(tmp_path / "nofilter.png")
+    np.testing.assert_array_equal(ctx.lic_weights(), np.ones(13, np.float32))
+    tf5 = np.arange(1280, dtype=np.uint8) % 251
+    with pytest.raises(vv.VVError):
+        L.tf_load_png(tmp_path / "notf", tf5)
+    np.testing.assert_array_equal(tf5, np.arange(1280, dtype=np.uint8) % 251)

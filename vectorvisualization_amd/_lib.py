@@ -176,6 +176,7 @@ def _load() -> C.CDLL:
         "vv_noise_file_read": (I, [C.c_char_p, IP, u8p, C.c_size_t]),
         "vv_noise_file_write": (I, [C.c_char_p, u8p, IP]),
         "vv_load_noise_file": (I, [P, C.c_char_p, I, I]),
+        "vv_load_noise_or_generate": (I, [P, C.c_char_p, I, I, C.c_uint64, IP]),
         "vv_png_read": (I, [C.c_char_p, u8p, C.c_size_t, IP, IP, IP]),
         "vv_png_write": (I, [C.c_char_p, u8p, I, I, I, I]),
         "vv_image_write_png": (I, [C.c_char_p, FP, I, I]),
@@ -218,7 +219,7 @@ EXPORTED = [
     "vv_batch_get_tuning", "vv_batch_run", "vv_batch_view_latency", "vv_batch_samples",
     "vv_batch_download_lic", "vv_dat_parse", "vv_dat_read", "vv_dat_read_field",
     "vv_load_field_dat", "vv_load_scalar_dat", "vv_noise_file_read", "vv_noise_file_write",
-    "vv_load_noise_file", "vv_png_read", "vv_png_write", "vv_image_write_png",
+    "vv_load_noise_file", "vv_load_noise_or_generate", "vv_png_read", "vv_png_write", "vv_image_write_png",
     "vv_load_filter_png", "vv_tf_load_png", "vv_tf_save_png", "vv_batch_set_field",
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
@@ -510,6 +511,16 @@ class Context:
     def load_noise_file(self, path, with_gradients: bool = False, use_gradient_cache: bool = True) -> None:
         self._chk(LIB.vv_load_noise_file(self._h, _path(path), int(bool(with_gradients)),
                                          int(bool(use_gradient_cache))))
+
+    def load_noise_or_generate(self, path, with_gradients: bool = False, use_gradient_cache: bool = True,
+                               seed: int = 1) -> bool:
+        """NoiseDataSet::loadData: the noise file, or generated 256^3 white noise
+        when it cannot be read; returns True when the noise was generated."""
+        g = C.c_int(0)
+        p = None if path is None else _path(path)
+        self._chk(LIB.vv_load_noise_or_generate(self._h, p, int(bool(with_gradients)),
+                                                int(bool(use_gradient_cache)), C.c_uint64(seed), C.byref(g)))
+        return bool(g.value)
 
     def load_filter_png(self, path) -> None:
         self._chk(LIB.vv_load_filter_png(self._h, _path(path)))

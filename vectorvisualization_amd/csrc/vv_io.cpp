@@ -505,6 +505,28 @@ vv_status vv_load_noise_file(vv_ctx* ctx, const char* path, int with_gradients, 
     return VV_OK;
 }
 
+vv_status vv_load_noise_or_generate(vv_ctx* ctx, const char* path, int with_gradients, int use_gradient_cache,
+                                    uint64_t seed, int* generated)
+{
+    if (!ctx) return VV_EINVAL;
+    if (generated) *generated = 0;
+    int dims[3];
+    const vv_status r = path ? vv_noise_file_read(path, dims, nullptr, 0) : VV_EIO;
+    if (r == VV_OK) {
+        // the size check reads the whole file; a short file falls back too
+        const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+        std::vector<uint8_t> probe(n);
+        if (vv_noise_file_read(path, dims, probe.data(), n) == VV_OK)
+            return vv_load_noise_file(ctx, path, with_gradients, use_gradient_cache);
+    }
+    // "NoiseData:  Using random noise 256x256x256": 255 where
+    // floor(0.6 u + 0.5) = 1, i.e. u >= 5/6
+    const int d256[3] = {256, 256, 256};
+    const vv_status s = vv_generate_noise(ctx, seed, VV_NOISE_BINARY, 1.0 / 6.0, d256, with_gradients);
+    if (s == VV_OK && generated) *generated = 1;
+    return s;
+}
+
 vv_status vv_png_read(const char* path, uint8_t* out, size_t out_bytes, int* width, int* height, int* channels)
 {
     if (!path) return fail(nullptr, VV_EINVAL, "NULL path");
