@@ -215,11 +215,11 @@ vv_status vv_get_lic_params(const vv_ctx* ctx, vv_lic_params* p);
 /* ---- a5/a6: LIC volume (Renderer::updateLICVolume) -------------------- */
 /* Computes the LIC volume at the field's resolution into the context's
  * volume buffer: 1 float per voxel (scalar noise) or 4 (RGBA noise).
- * With scalar noise on the field's grid, the first call after a field or
- * noise upload also builds a LIC copy of the field that carries the noise
- * bytes (one more field-sized buffer, kept until the next upload); the LIC
- * then reads no separate noise volume.  Without memory for it, the LIC reads
- * the noise volume itself (same results). */
+ * With scalar noise on the field's grid, freq_scale 1 and no scalar gate,
+ * the first call after a field or noise upload also builds a LIC copy of the
+ * field that carries the noise bytes (one more field-sized buffer, kept until
+ * the next upload); the LIC then reads no separate noise volume.  Without
+ * memory for it, the LIC reads the noise volume itself (same results). */
 vv_status vv_compute_lic(vv_ctx* ctx);
 vv_status vv_lic_info(const vv_ctx* ctx, int dims[3], int* channels);
 vv_status vv_download_lic(vv_ctx* ctx, float* out, int on_device);

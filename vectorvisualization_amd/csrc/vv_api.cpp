@@ -477,10 +477,16 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     for (int a = 0; a < 3; ++a) L.tile[a] = c->lic_tile[a];
     for (int a = 0; a < 3; ++a) L.super_tile[a] = c->lic_super[a];
     L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * field_texel_bytes(L.field_type);
-    // scalar noise on the field's grid: the LIC reads the field copy that
-    // carries the noise bytes (built once per field/noise pair).  Without
-    // memory for the copy it reads the noise volume itself.
-    if (nc == 1 && c->ndims[0] == c->fdims[0] && c->ndims[1] == c->fdims[1] && c->ndims[2] == c->fdims[2]) {
+    L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
+    // the cached kernels address through 32-bit byte offsets
+    // and 24-bit index multiplies (plane strides below 2^24 elements)
+    L.cache = c->lic_cache && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
+              (long long)L.field.sz < (1ll << 24) && (long long)L.noise.sz < (1ll << 24);
+    // scalar noise on the field's grid at freq 1 (the cached SAME kernels):
+    // the LIC reads the field copy that carries the noise bytes, built once per
+    // field/noise pair.  Without memory for the copy it reads the noise volume.
+    if (L.cache && nc == 1 && !L.gate && L.freq == 1.0f && c->ndims[0] == c->fdims[0] &&
+        c->ndims[1] == c->fdims[1] && c->ndims[2] == c->fdims[2]) {
         if (!c->lic_field_ok) {
             const hipError_t e = c->lic_field.ensure((size_t)L.field_bytes);
             if (e == hipErrorOutOfMemory) {
@@ -497,11 +503,6 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
             L.field_has_noise = 1;
         }
     }
-    L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
-    // the cached kernels address through 32-bit byte offsets
-    // and 24-bit index multiplies (plane strides below 2^24 elements)
-    L.cache = c->lic_cache && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
-              (long long)L.field.sz < (1ll << 24) && (long long)L.noise.sz < (1ll << 24);
     return VV_OK;
 }
 
