@@ -269,8 +269,9 @@ vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dima
  * transfer function at run time.  Timesteps are resident volumes (field +
  * seeded scalar noise + LIC volume) of any size; views render one timestep
  * each into a caller-owned device image (width*height*4 float32).
- * vv_batch_run computes every LIC volume and every view with two persistent
- * kernels captured once into a hipGraph and replayed on later runs; it
+ * vv_batch_run computes every LIC volume (one block per tile) and every view
+ * (a persistent kernel over pixel tiles), launches captured once into a
+ * hipGraph and replayed on later runs; it
  * returns without waiting.  Replaces the reference's per-frame re-encode +
  * LIC recompute of unsteady data (VV/3DLIC.cpp:129-172). */
 typedef struct vv_batch vv_batch;

@@ -442,9 +442,8 @@ vv_status vv_batch_run(vv_batch* b)
     HIPCHK(c, b->vfirst.ensure(sizeof(int) * b->host_vfirst.size()));
     int cus = 0;
     HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-    const int lic_grid = ((cus * lic_batch_blocks_per_cu(ftype, same) + 7) / 8) * 8;
     const int render_grid = cus * render_batch_blocks_per_cu(ftype, 1, bx * by);
-    std::vector<long long> key = {(long long)nsteps, (long long)nv, lic_grid, render_grid, bx, by, ftype, same,
+    std::vector<long long> key = {(long long)nsteps, (long long)nv, render_grid, bx, by, ftype, same,
                                   (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
                                   (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
                                   (long long)(intptr_t)b->ctr.p};
@@ -475,7 +474,6 @@ vv_status vv_batch_run(vv_batch* b)
             LB.first = (const int*)b->first.p + 2 * t;
             LB.njobs = 1;
             LB.total = b->host_first[2 * t + 1];
-            LB.grid = lic_grid;
             LB.field_type = ftype;
             LB.same = same;
             e = launch_lic_batch(LB, c->stream);

@@ -110,8 +110,7 @@ struct RenderBatchLaunch {
 struct LicBatchLaunch {
     const void* jobs;             // device array of per-volume kernel arguments
     const int* first;             // device: first tile of each volume
-    int njobs, total;
-    int grid;
+    int njobs, total;             // volumes, tiles (one block each)
     int field_type, same;         // shared by all volumes
 };
 
@@ -166,7 +165,6 @@ hipError_t launch_stamp(unsigned long long* t0, hipStream_t s);
 size_t lic_args_size();
 hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
-int lic_batch_blocks_per_cu(int field_type, int same);
 int render_batch_blocks_per_cu(int field_type, int lc, int threads);
 // view-aligned slicing (ViewSlicing): planes v.(p - center) = d_i with
 // d_i = -D/2 + (i + 1/2) D/n, i = 0..n-1, front to back

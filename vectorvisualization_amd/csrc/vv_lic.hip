@@ -428,23 +428,23 @@ __global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
     lic_cached_tile<FT, SAME>(a, tile);
 }
 
-// Persistent LIC over the tiles of several volumes (one LicArgs per volume,
-// same storage and noise grid relation).  Tile i of the concatenated list
-// goes to block i mod gridDim.x; the XCD order of each volume's tiles is the
-// one of k_lic_cached, so every XCD works on a contiguous part.
+// LIC over the tiles of several volumes (one LicArgs per volume, same
+// storage and noise grid relation), one block per tile of the concatenated
+// list, so the hardware hands tiles to CUs as they free up.  Each volume holds
+// a multiple of 8 tiles, so the XCD order of its tiles is the one of
+// k_lic_cached and every XCD works on a contiguous part.
 template <typename FT, bool SAME>
 __global__ void __launch_bounds__(kThreads) k_lic_batch(const LicArgs* __restrict__ jobs,
-                                                        const int* __restrict__ first, int njobs, int total)
+                                                        const int* __restrict__ first, int njobs)
 {
-    for (int item = blockIdx.x; item < total; item += gridDim.x) {
-        int j = 0;
-        while (j + 1 < njobs && item >= first[j + 1]) ++j;
-        j = __builtin_amdgcn_readfirstlane(j);  // block-uniform: descriptor loads go to SGPRs
-        const int i = item - first[j];
-        const LicArgs& a = jobs[j];
-        const int tile = (i & 7) * a.per_xcd + (i >> 3);
-        if (tile < a.nblocks) lic_cached_tile<FT, SAME>(a, tile);
-    }
+    const int item = blockIdx.x;
+    int j = 0;
+    while (j + 1 < njobs && item >= first[j + 1]) ++j;
+    j = __builtin_amdgcn_readfirstlane(j);  // block-uniform: descriptor loads go to SGPRs
+    const int i = item - first[j];
+    const LicArgs a = jobs[j];  // by value: block-uniform, kept in SGPRs
+    const int tile = (i & 7) * a.per_xcd + (i >> 3);
+    if (tile < a.nblocks) lic_cached_tile<FT, SAME>(a, tile);
 }
 
 template <typename FT, int NC, bool GATE>
@@ -716,29 +716,15 @@ hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles)
 
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s)
 {
-    if (B.njobs < 1 || B.grid < 1) return hipErrorInvalidValue;
+    if (B.njobs < 1 || B.total < 1) return hipErrorInvalidValue;
     const LicArgs* jobs = (const LicArgs*)B.jobs;
     return with_field_type(B.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
         if (B.same)
-            hipLaunchKernelGGL((k_lic_batch<FT, true>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first, B.njobs,
-                               B.total);
+            hipLaunchKernelGGL((k_lic_batch<FT, true>), dim3(B.total), dim3(kThreads), 0, s, jobs, B.first, B.njobs);
         else
-            hipLaunchKernelGGL((k_lic_batch<FT, false>), dim3(B.grid), dim3(kThreads), 0, s, jobs, B.first,
-                               B.njobs, B.total);
+            hipLaunchKernelGGL((k_lic_batch<FT, false>), dim3(B.total), dim3(kThreads), 0, s, jobs, B.first, B.njobs);
     });
-}
-
-int lic_batch_blocks_per_cu(int field_type, int same)
-{
-    int n = 0;
-    hipError_t e = hipSuccess;
-    const hipError_t d = with_field_type(field_type, [&](auto tag) {
-        using FT = typename decltype(tag)::type;
-        e = same ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<FT, true>, kThreads, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_lic_batch<FT, false>, kThreads, 0);
-    });
-    return (d == hipSuccess && e == hipSuccess && n > 0) ? n : 1;
 }
 
 }  // namespace vv

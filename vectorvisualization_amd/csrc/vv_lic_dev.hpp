@@ -478,13 +478,16 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
     };
     Ax ax = axis(sx, nx), ay = axis(sy, ny), az = axis(sz, nz);
     const float3 s0 = field_from_axes<FT>(c, fc, ax, ay, az);
-    float acc = noise(sx, sy, sz, ax, ay, az) * a.w[0];
+    // the weights are global memory; the cast keeps the batch kernel, which
+    // reads this pointer from a descriptor, on scalar loads
+    const gfloat* w = (const gfloat*)a.w;
+    float acc = noise(sx, sy, sz, ax, ay, az) * w[0];
 
 #pragma unroll 1
     for (int dirn = 0; dirn < 2; ++dirn) {
         const int steps = dirn == 0 ? a.B : a.F;
         const float h = dirn == 0 ? -a.he : a.he;
-        const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
+        const gfloat* wk = dirn == 0 ? w + 1 : w + 1 + a.B;
         float px = sx, py = sy, pz = sz;
         float3 s = s0;
 #pragma unroll 1
