@@ -288,12 +288,8 @@ __device__ __forceinline__ bool key_hit(float kx, float ky, float kz, const Ax& 
 // then hold the same noise bytes, so the field fill brings the noise cell
 // with it: no noise loads.  Outside that range (a streamline that left the
 // volume) the noise comes from the REPEAT array.  The noise value and its
-// arithmetic are those of noise_from_axes.
-__device__ __forceinline__ bool same_cells(const CachedArgs& c, const Ax& ax, const Ax& ay, const Ax& az)
-{
-    return ax.fl >= -1.0f && ax.fl <= c.fnm1[0] && ay.fl >= -1.0f && ay.fl <= c.fnm1[1] && az.fl >= -1.0f &&
-           az.fl <= c.fnm1[2];
-}
+// arithmetic are those of noise_from_axes.  field_from_axes sets fc.same
+// for a cell whose three floors lie in that range.
 
 template <typename FT, int NCH>
 __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<NCH>& fc, const Ax& ax,
@@ -303,9 +299,13 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
         fc.kx = ax.fl;
         fc.ky = ay.fl;
         fc.kz = az.fl;
-        if constexpr (NCH == 4) fc.same = same_cells(c, ax, ay, az);
-        const unsigned base = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
-                              __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
+        // clamped floors; a floor the clamp leaves unchanged lies in [-1, n-1]
+        const float mx = __builtin_amdgcn_fmed3f(ax.fl, -1.0f, c.fnm1[0]);
+        const float my = __builtin_amdgcn_fmed3f(ay.fl, -1.0f, c.fnm1[1]);
+        const float mz = __builtin_amdgcn_fmed3f(az.fl, -1.0f, c.fnm1[2]);
+        if constexpr (NCH == 4) fc.same = mx == ax.fl && my == ay.fl && mz == az.fl;
+        const unsigned base = __umul24((unsigned)((int)mz + 1), c.fsz) + __umul24((unsigned)((int)my + 1), c.fsy) +
+                              (unsigned)((int)mx + 1);
         const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
