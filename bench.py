@@ -224,6 +224,17 @@ def main():
         out["roofline"]["l2"] = {"requests_per_launch": l2, "achieved": l2_gbs, "peak": l2_peak, "unit": "GB/s",
                                  "frac": l2_gbs / l2_peak, "peak_basis": "16 channels x 128 B/clk per XCD, 8 XCDs"}
 
+    # the bound that holds the LIC kernel: FP32 fma/add/sub/mul wave64
+    # instructions (SQ_INSTS_VALU_{FMA,ADD,MUL}_F32 of the same kernel), which
+    # issue once per clock per CU (157.3 TFLOP/s FP32 counts packed FMAs)
+    valu = _profile_value(f"{tag}_pmc.txt", "valu_f32_insts_per_launch")
+    if valu is not None:
+        rate = valu / (lic_ms / 1000.0) / (MI355X_CUS * MI355X_CLOCK_HZ)
+        out["roofline"]["valu_f32"] = {"insts_per_launch": valu, "per_clock_per_cu": rate, "peak": 1.0,
+                                       "frac": rate,
+                                       "peak_basis": "one unpacked wave64 FP32 fma/add/mul per clock per CU "
+                                                     "(157.3 TFLOP/s = 256 CUs x 2.4 GHz x 64 lanes x 2 packed x 2)"}
+
     if not args.no_extras:
         def timed(fn, reps=3):
             fn()
