@@ -151,7 +151,9 @@ vv_status vv_default_camera(vv_camera* cam, int width, int height, const float c
  * (v/|v|*127 + 128), 128 for |v| < 1e-5, a = (unsigned char)(|v|/max*255),
  * sampled as byte/255.
  * spacing = DAT slice distances; extent/scale/center follow
- * VV/dataset.cpp:144-176. */
+ * VV/dataset.cpp:144-176.
+ * Volumes (field, noise, scalar, LIC) take 2..4096 samples per axis and fewer
+ * than 2^29 in total, so padded element indices stay 32-bit. */
 vv_status vv_upload_field(vv_ctx* ctx, const float* v0, const float* v1, float lerp_t,
                           const int dims[3], const float spacing[3], int storage,
                           int on_device);

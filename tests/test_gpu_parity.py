@@ -412,6 +412,14 @@ def test_errors_leave_context_usable(ctx):
     assert e.value.status == 5  # VV_ESTATE
     with pytest.raises(vv.VVError):
         ctx.upload_field(np.zeros(3, np.float32), (1, 1, 1))
+    # 2^30 voxels: over the total limit that keeps padded indices 32-bit; the
+    # dims are rejected before the (small) array is read
+    with pytest.raises(vv.VVError) as e:
+        ctx.upload_field(np.zeros(3, np.float32), (4096, 4096, 64))
+    assert e.value.status == 1 and "2^29" in str(e.value)
+    with pytest.raises(vv.VVError) as e:
+        ctx.generate_noise(1, vv.NOISE_UNIFORM, 0.0, (4096, 4096, 64))
+    assert e.value.status == 1
     bad = np.ones((4, 4, 4, 3), np.float32)
     bad[0, 0, 0, 0] = np.nan
     with pytest.raises(vv.VVError) as e:

@@ -33,7 +33,7 @@ vv_status anim_alloc(vv_ctx* c, int nsteps, const int dims[3], const float spaci
     if (!out) return fail(c, VV_EINVAL, "out is NULL");
     *out = nullptr;
     if (nsteps < 1) return fail(c, VV_EINVAL, "a series needs at least one time step");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims %s", kDimsRule);
     if (interp < 1) return fail(c, VV_EINVAL, "interp_size must be >= 1");
     if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage");
     vv_anim* a = new (std::nothrow) vv_anim();
@@ -103,7 +103,7 @@ vv_status vv_anim_load_dat(vv_ctx* ctx, const char* dat_path, int interp_size, i
     vv_dat_info info;
     vv_status s = vv_dat_parse(dat_path, &info);
     if (s) return fail(ctx, s, "%s", g_create_error.c_str());
-    if (!dims_ok(info.dims)) return fail(ctx, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (!dims_ok(info.dims)) return fail(ctx, VV_EINVAL, "field dims %s", kDimsRule);
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int nsteps = info.t_end - info.t_begin + 1;
     vv_anim* a = nullptr;

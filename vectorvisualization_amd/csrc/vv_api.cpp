@@ -116,7 +116,7 @@ vv_status vv_upload_field_ex(vv_ctx* c, const float* v0, const float* v1, float 
     if (encoding == VV_ENCODE_RAW_MAGNITUDE && storage == VV_STORE_UNORM8)
         return fail(c, VV_EINVAL, "the raw-magnitude encoding has float texels only (FP32, FP16)");
     if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims %s", kDimsRule);
     if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage %d", storage);
     if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
     float sd[3] = {1, 1, 1};
@@ -256,7 +256,7 @@ vv_status vv_upload_noise(vv_ctx* c, const uint8_t* noise, const int dims[3], in
 {
     if (!c) return VV_EINVAL;
     if (!noise) return fail(c, VV_EINVAL, "noise is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
     HIPCHK(c, hipSetDevice(c->device));
     vv_status st = upload(c, c->stage, noise, (size_t)nvox(dims), on_device);
     if (st) return st;
@@ -267,7 +267,7 @@ vv_status vv_generate_noise(vv_ctx* c, uint64_t seed, int kind, double density, 
                             int with_gradients)
 {
     if (!c) return VV_EINVAL;
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
     if (kind != VV_NOISE_BINARY && kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
     if (kind == VV_NOISE_BINARY && !(density >= 0.0 && density <= 1.0))
         return fail(c, VV_EINVAL, "density must be in [0,1]");
@@ -396,7 +396,7 @@ vv_status vv_upload_scalar(vv_ctx* c, const uint8_t* scalar, const int dims[3], 
 {
     if (!c) return VV_EINVAL;
     if (!scalar) return fail(c, VV_EINVAL, "scalar is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "scalar dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "scalar dims %s", kDimsRule);
     HIPCHK(c, hipSetDevice(c->device));
     vv_status st = upload(c, c->stage, scalar, (size_t)nvox(dims), on_device);
     if (st) return st;
@@ -559,7 +559,7 @@ vv_status vv_upload_lic(vv_ctx* c, const float* lic, int channels, const int dim
     if (!c) return VV_EINVAL;
     if (!lic) return fail(c, VV_EINVAL, "lic is NULL");
     if (channels != 1 && channels != 4) return fail(c, VV_EINVAL, "channels must be 1 or 4");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "lic dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "lic dims %s", kDimsRule);
     HIPCHK(c, hipSetDevice(c->device));
     const size_t bytes = sizeof(float) * channels * (size_t)nvox(dims);
     vv_status st = upload(c, c->stage, lic, bytes, on_device);

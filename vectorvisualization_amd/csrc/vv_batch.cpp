@@ -237,7 +237,7 @@ vv_status vv_batch_add_timestep(vv_batch* b, const float* v0, const int dims[3],
     if (!b) return VV_EINVAL;
     vv_ctx* c = b->ctx;
     if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims must be in [2,4096] per axis");
+    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims %s", kDimsRule);
     if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage");
     if (!b->steps.empty() && b->steps[0]->storage != storage)
         return fail(c, VV_EINVAL, "all timesteps of a batch share one storage type");

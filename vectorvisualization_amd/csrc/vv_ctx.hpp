@@ -149,6 +149,9 @@ inline bool storage_ok(int storage)
     return storage == VV_STORE_FP32 || storage == VV_STORE_FP16 || storage == VV_STORE_UNORM8;
 }
 
+// volume sizes dims_ok accepts (error messages)
+constexpr const char* kDimsRule = "must be in [2,4096] per axis, under 2^29 voxels in total";
+
 inline bool dims_ok(const int d[3])
 {
     if (!d) return false;

@@ -466,7 +466,7 @@ vv_status vv_noise_file_read(const char* path, int dims[3], uint8_t* out, size_t
 vv_status vv_noise_file_write(const char* path, const uint8_t* noise, const int dims[3])
 {
     if (!path || !noise || !dims) return fail(nullptr, VV_EINVAL, "NULL argument");
-    if (!dims_ok(dims)) return fail(nullptr, VV_EINVAL, "noise dims must be in [2,4096]");
+    if (!dims_ok(dims)) return fail(nullptr, VV_EINVAL, "noise dims %s", kDimsRule);
     File fp(path, "wb");
     const int32_t hdr[3] = {dims[0], dims[1], dims[2]};
     const size_t n = (size_t)dims[0] * dims[1] * dims[2];
