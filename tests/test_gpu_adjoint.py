@@ -135,6 +135,34 @@ def test_autograd_matches_central_differences(ctx):
     assert worst < 2e-2
 
 
+def test_autograd_backward_uses_its_own_forward_state(ctx):
+    """Two forward passes on one engine (different TFs, one with an opacity
+    scale), then one backward: each gradient equals that of a forward and
+    backward pair on its own (the adjoint sums are deterministic, so equal)."""
+    render_tf, cam, op, target = _autograd_scene(ctx)
+    d, _ = ctx.lic_info()
+    sig0 = torch.tensor(np.random.default_rng(4).uniform(0.5, 1.2, int(np.prod(d))).astype(np.float32),
+                        device="cuda")
+
+    def fresh():
+        tf_a = torch.tensor(fields.tf_random(5)[0].reshape(256, 4), device="cuda", requires_grad=True)
+        tf_b = torch.tensor(fields.tf_random(7)[0].reshape(256, 4), device="cuda", requires_grad=True)
+        return tf_a, tf_b, sig0.clone().requires_grad_(True)
+
+    def loss(tf, sig=None):
+        return ((render_tf(ctx, cam, tf, op, sig) - target) ** 2).sum()
+
+    tf_a, tf_b, sig = fresh()
+    loss(tf_a, sig).backward()
+    loss(tf_b).backward()
+    ref = (tf_a.grad.clone(), sig.grad.clone(), tf_b.grad.clone())
+    tf_a, tf_b, sig = fresh()
+    (loss(tf_a, sig) + loss(tf_b)).backward()
+    assert torch.equal(tf_a.grad, ref[0])
+    assert torch.equal(sig.grad, ref[1])
+    assert torch.equal(tf_b.grad, ref[2])
+
+
 def test_adam_fits_transfer_function(ctx):
     render_tf, cam, op, target = _autograd_scene(ctx)
     tf = torch.tensor(fields.tf_random(5)[0].reshape(256, 4), device="cuda", requires_grad=True)

@@ -428,3 +428,16 @@ def test_errors_leave_context_usable(ctx):
     lic_setup(ctx, n=8, L=2)
     ctx.compute_lic()
     assert ctx.download_lic().size == 8 ** 3
+    # cameras the ray setup cannot use are rejected before any launch
+    cen = ctx.field_geometry()[2]
+    for q, v in ((0, float("nan")), (3, 0.0)):
+        cam = vv.default_camera(8, 8, cen)
+        cam.quat[q] = v
+        with pytest.raises(vv.VVError) as e:
+            ctx.render(cam)
+        assert e.value.status == 1
+    cam = vv.default_camera(8, 8, cen)
+    cam.light_specular = float("inf")
+    with pytest.raises(vv.VVError):
+        ctx.render(cam)
+    assert ctx.render(vv.default_camera(8, 8, cen)).size == 8 * 8 * 4

@@ -1,10 +1,15 @@
 """torch.autograd.Function over the ray caster and its hand-written adjoint.
 
-``render_tf(engine, cam, tf_rgba, sigma=None)`` returns the RGBA image
-(height, width, 4) rendered by the engine's fused ray caster from its current
+``render_tf(engine, cam, tf_rgba, opacity, sigma=None)`` returns the RGBA
+image (height, width, 4) rendered by the engine's ray caster from its current
 LIC volume and field.  Gradients flow to the 256x4 RGBA transfer function and
 to the per-voxel opacity scale; both are computed by ``vv_render_adjoint``
 (checkpointed, deterministic) on the GPU.
+
+The backward pass sets the transfer function, opacity table and opacity scale
+of its own forward pass again before the adjoint, so several renders may share
+one engine before a single backward.  The field, the LIC volume and the LIC
+parameters must stay as they were at the forward pass.
 """
 from __future__ import annotations
 
@@ -19,20 +24,24 @@ class _RenderTF(torch.autograd.Function):
     def forward(fctx, tf_rgba, sigma, engine: Context, cam: Camera, opacity: np.ndarray):
         if not tf_rgba.is_cuda:
             raise ValueError("tf_rgba must be a CUDA tensor")
-        rgba = tf_rgba.detach().to(torch.float32).contiguous().cpu().numpy()
-        engine.set_tf(rgba.ravel(), opacity)
-        if sigma is not None:
-            engine.set_opacity_scale(sigma.detach().to(torch.float32).contiguous())
-        else:
-            engine.set_opacity_scale(None)
+        rgba = tf_rgba.detach().to(torch.float32).contiguous().cpu().numpy().ravel()
+        opacity = np.ascontiguousarray(opacity, np.float32).copy()
+        sig = None if sigma is None else sigma.detach().to(torch.float32).contiguous().clone()
+        _RenderTF._apply_state(engine, rgba, opacity, sig)
         img = torch.empty(cam.height * cam.width * 4, dtype=torch.float32, device=tf_rgba.device)
         engine.render(cam, out=img)
         engine.synchronize()
         fctx.engine = engine
         fctx.cam = cam
+        fctx.state = (rgba, opacity, sig)
         fctx.has_sigma = sigma is not None
         fctx.sigma_shape = None if sigma is None else sigma.shape
         return img.view(cam.height, cam.width, 4)
+
+    @staticmethod
+    def _apply_state(engine: Context, rgba: np.ndarray, opacity: np.ndarray, sigma) -> None:
+        engine.set_tf(rgba, opacity)
+        engine.set_opacity_scale(sigma)
 
     @staticmethod
     def backward(fctx, grad_img):
@@ -43,6 +52,8 @@ class _RenderTF(torch.autograd.Function):
         if fctx.has_sigma and fctx.needs_input_grad[1]:
             dsig = torch.empty(int(np.prod(fctx.sigma_shape)), dtype=torch.float32, device=g.device)
         torch.cuda.current_stream().synchronize()
+        # the state of this forward pass (another forward may have run since)
+        _RenderTF._apply_state(engine, *fctx.state)
         engine.render_adjoint(cam, g, dtf, dsig)
         engine.synchronize()
         dsig_out = None if dsig is None else dsig.view(fctx.sigma_shape)

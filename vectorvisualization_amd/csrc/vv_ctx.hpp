@@ -316,6 +316,15 @@ inline vv_status check_camera(vv_ctx* c, const vv_camera* cam)
     for (int a = 0; a < 3; ++a)
         if (!std::isfinite(cam->pos[a]) || !std::isfinite(cam->light_pos[a]))
             return fail(c, VV_EINVAL, "camera/light position not finite");
+    double qn = 0.0;
+    for (int a = 0; a < 4; ++a) {
+        if (!std::isfinite(cam->quat[a])) return fail(c, VV_EINVAL, "camera orientation not finite");
+        qn += (double)cam->quat[a] * cam->quat[a];
+    }
+    if (!(qn > 0.0)) return fail(c, VV_EINVAL, "camera orientation is the zero quaternion");
+    if (!std::isfinite(cam->light_diffuse) || !std::isfinite(cam->light_specular) ||
+        !std::isfinite(cam->light_ambient))
+        return fail(c, VV_EINVAL, "light intensities not finite");
     if (cam->spec_exponent < 0 || cam->spec_exponent > 1024)
         return fail(c, VV_EINVAL, "spec exponent out of range");
     return VV_OK;
