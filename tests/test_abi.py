@@ -89,3 +89,25 @@ def test_null_arguments_rejected():
     assert vv.LIB.vv_default_camera(None, 1, 1, None) == 1
     assert vv.LIB.vv_synchronize(None) == 1
     assert vv.LIB.vv_destroy(None) == 0
+
+
+def test_binding_refuses_short_or_mistyped_buffers():
+    # the library reads or writes as many elements as the dims say; the
+    # binding checks every buffer before its address reaches the C ABI
+    a = np.zeros(12, np.float32)
+    assert _lib._buf(a, "float32", 12, "field")[0] == a.ctypes.data
+    with pytest.raises(vv.VVError) as e:
+        _lib._buf(a, "float32", 13, "field")
+    assert e.value.status == 1 and "13 elements needed" in str(e.value)
+    with pytest.raises(vv.VVError) as e:
+        _lib._buf(a.astype(np.float64), "float32", 12, "field")
+    assert "float32 buffer expected, got float64" in str(e.value)
+    with pytest.raises(vv.VVError):
+        _lib._buf(np.zeros(8, np.uint8), "float32", 2, "image")
+    torch = pytest.importorskip("torch")
+    t = torch.zeros(12, dtype=torch.float32)
+    assert _lib._buf(t, "float32", 12, "field") == (t.data_ptr(), 0)
+    with pytest.raises(vv.VVError):
+        _lib._buf(t, "uint8", 12, "noise")
+    with pytest.raises(ValueError):
+        _lib._buf(np.zeros((4, 4), np.float32)[:, ::2], "float32", 8, "field")  # not contiguous

@@ -248,6 +248,28 @@ def _ptr(a) -> tuple[int, int]:
     raise TypeError(f"unsupported buffer type {type(a)}")
 
 
+def _buf(a, dtype: str, count: int, what: str) -> tuple[int, int]:
+    """(address, on_device) of a numpy array or torch tensor holding at least
+    `count` elements of `dtype` ("float32" or "uint8"); the library reads or
+    writes that many, so a shorter or differently typed buffer is refused
+    here (VV_EINVAL) instead of being overrun."""
+    if isinstance(a, np.ndarray):
+        got_type, n = str(a.dtype), a.size
+    elif hasattr(a, "data_ptr"):
+        got_type, n = str(a.dtype).replace("torch.", ""), a.numel()
+    else:
+        raise TypeError(f"{what}: unsupported buffer type {type(a)}")
+    if got_type != dtype:
+        raise VVError(1, f"{what}: {dtype} buffer expected, got {got_type}")
+    if n < count:
+        raise VVError(1, f"{what}: {count} elements needed, the buffer holds {n}")
+    return _ptr(a)
+
+
+def _nvox(dims) -> int:
+    return int(dims[0]) * int(dims[1]) * int(dims[2])
+
+
 def _is_cuda(a) -> bool:
     return a is not None and getattr(a, "is_cuda", False)
 
@@ -480,8 +502,8 @@ class Context:
     # -- inputs --
     def upload_field(self, v0, dims, v1=None, lerp_t: float = 0.0, spacing=(1.0, 1.0, 1.0),
                      storage: int = STORE_FP32, encoding: int = ENCODE_INTERP) -> None:
-        p0, dev = _ptr(v0)
-        p1 = _ptr(v1)[0] if v1 is not None else None
+        p0, dev = _buf(v0, "float32", 3 * _nvox(dims), "field")
+        p1 = _buf(v1, "float32", 3 * _nvox(dims), "second field")[0] if v1 is not None else None
         with self.ordered(v0, v1):
             if encoding == ENCODE_INTERP:
                 self._chk(LIB.vv_upload_field(self._h, p0, p1, float(lerp_t), _i3(dims), _f3(spacing),
@@ -536,8 +558,8 @@ class Context:
         return out
 
     def upload_noise(self, noise, dims, with_gradients: bool = False, grad_rgb=None) -> None:
-        p, dev = _ptr(noise)
-        g = _ptr(grad_rgb)[0] if grad_rgb is not None else None
+        p, dev = _buf(noise, "uint8", _nvox(dims), "noise")
+        g = _buf(grad_rgb, "uint8", 3 * _nvox(dims), "noise gradients")[0] if grad_rgb is not None else None
         with self.ordered(noise, grad_rgb):
             self._chk(LIB.vv_upload_noise(self._h, p, _i3(dims), int(bool(with_gradients)), g, dev))
 
@@ -572,7 +594,7 @@ class Context:
         self._chk(LIB.vv_set_tf_u8(self._h, tf5.ctypes.data))
 
     def upload_scalar(self, scalar, dims) -> None:
-        p, dev = _ptr(scalar)
+        p, dev = _buf(scalar, "uint8", _nvox(dims), "scalar volume")
         with self.ordered(scalar):
             self._chk(LIB.vv_upload_scalar(self._h, p, _i3(dims), dev))
 
@@ -594,17 +616,25 @@ class Context:
         self._chk(LIB.vv_lic_info(self._h, d, C.byref(ch)))
         return list(d), ch.value
 
+    def _lic_voxels(self) -> int:
+        """Voxels of the LIC grid (the opacity scale's size); 0 without a
+        LIC volume, which the library then reports itself."""
+        try:
+            return _nvox(self.lic_info()[0])
+        except VVError:
+            return 0
+
     def download_lic(self, out=None) -> np.ndarray:
         d, ch = self.lic_info()
         if out is None:
             out = np.empty(int(np.prod(d)) * ch, np.float32)
-        p, dev = _ptr(out)
+        p, dev = _buf(out, "float32", _nvox(d) * ch, "LIC output")
         with self.ordered(out):
             self._chk(LIB.vv_download_lic(self._h, p, dev))
         return out
 
     def upload_lic(self, lic, channels: int, dims) -> None:
-        p, dev = _ptr(lic)
+        p, dev = _buf(lic, "float32", _nvox(dims) * int(channels), "LIC volume")
         with self.ordered(lic):
             self._chk(LIB.vv_upload_lic(self._h, p, int(channels), _i3(dims), dev))
 
@@ -626,7 +656,7 @@ class Context:
     def render(self, cam: Camera, out=None, count_samples: bool = False):
         if out is None:
             out = np.empty(cam.width * cam.height * 4, np.float32)
-        p, dev = _ptr(out)
+        p, dev = _buf(out, "float32", cam.width * cam.height * 4, "image")
         ns = C.c_uint64()
         with self.ordered(out):
             self._chk(LIB.vv_render(self._h, C.byref(cam), p, dev,
@@ -637,7 +667,7 @@ class Context:
         """Fused ray cast: the LIC of every sample computed on the fly."""
         if out is None:
             out = np.empty(cam.width * cam.height * 4, np.float32)
-        p, dev = _ptr(out)
+        p, dev = _buf(out, "float32", cam.width * cam.height * 4, "image")
         ns = C.c_uint64()
         with self.ordered(out):
             self._chk(LIB.vv_render_fused(self._h, C.byref(cam), p, dev,
@@ -648,7 +678,7 @@ class Context:
         """View-aligned slicing with on-the-fly LIC (FP16 render target)."""
         if out is None:
             out = np.empty(cam.width * cam.height * 4, np.float32)
-        p, dev = _ptr(out)
+        p, dev = _buf(out, "float32", cam.width * cam.height * 4, "image")
         ns = C.c_uint64()
         with self.ordered(out):
             self._chk(LIB.vv_render_slices(self._h, C.byref(cam), p, dev,
@@ -664,7 +694,7 @@ class Context:
         if sigma is None:
             self._chk(LIB.vv_set_opacity_scale(self._h, None, 0))
             return
-        p, dev = _ptr(sigma)
+        p, dev = _buf(sigma, "float32", self._lic_voxels(), "opacity scale")
         with self.ordered(sigma):
             self._chk(LIB.vv_set_opacity_scale(self._h, p, dev))
 
@@ -677,9 +707,9 @@ class Context:
             if dsigma is None and want_sigma:
                 d, _ = self.lic_info()
                 dsigma = np.zeros(int(np.prod(d)), np.float32)
-        pi, dev = _ptr(dimage)
-        pt = _ptr(dtf)[0]
-        ps = _ptr(dsigma)[0] if dsigma is not None else None
+        pi, dev = _buf(dimage, "float32", cam.width * cam.height * 4, "image gradient")
+        pt = _buf(dtf, "float32", 1024, "TF gradient")[0]
+        ps = _buf(dsigma, "float32", self._lic_voxels(), "opacity-scale gradient")[0] if dsigma is not None else None
         with self.ordered(dimage, dtf, dsigma):
             self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
         return dtf, dsigma
@@ -694,6 +724,7 @@ class Batch:
         self._ctx = ctx
         self._h = h
         self._images = []
+        self._dims = []  # per time step, for the buffer checks of set_field
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -708,15 +739,16 @@ class Batch:
 
     def add_timestep(self, v, dims, spacing=(1.0, 1.0, 1.0), storage: int = STORE_FP32,
                      noise_seed: int = 1, noise_kind: int = NOISE_UNIFORM, density: float = 0.0) -> int:
-        p, dev = _ptr(v)
+        p, dev = _buf(v, "float32", 3 * _nvox(dims), "time step field")
         with self._ctx.ordered(v):
             self._ctx._chk(LIB.vv_batch_add_timestep(self._h, p, _i3(dims), _f3(spacing),
                                                      int(storage), C.c_uint64(noise_seed),
                                                      int(noise_kind), float(density), dev))
+        self._dims.append(tuple(int(x) for x in dims))
         return self.info()[0] - 1
 
     def add_view(self, timestep: int, cam: Camera, image) -> int:
-        p, dev = _ptr(image)
+        p, dev = _buf(image, "float32", cam.width * cam.height * 4, "batch image")
         if not dev:
             raise ValueError("batch images must be device tensors")
         self._ctx._chk(LIB.vv_batch_add_view(self._h, int(timestep), C.byref(cam), p))
@@ -752,8 +784,9 @@ class Batch:
         return t.value
 
     def set_field(self, timestep: int, v0, v1=None, lerp_t: float = 0.0) -> None:
-        p0, dev = _ptr(v0)
-        p1 = _ptr(v1)[0] if v1 is not None else None
+        n = 3 * _nvox(self._dims[timestep]) if 0 <= timestep < len(self._dims) else 0
+        p0, dev = _buf(v0, "float32", n, "time step field")
+        p1 = _buf(v1, "float32", n, "second time step field")[0] if v1 is not None else None
         with self._ctx.ordered(v0, v1):
             self._ctx._chk(LIB.vv_batch_set_field(self._h, int(timestep), p0, p1, float(lerp_t), dev))
 
@@ -775,7 +808,7 @@ class Anim:
             ctx._chk(LIB.vv_anim_load_dat(ctx.handle, _path(dat_path), int(interp_size), int(storage),
                                           C.byref(h)))
         else:
-            ptrs = [_ptr(s) for s in steps]
+            ptrs = [_buf(s, "float32", 3 * _nvox(dims), f"time step {i}") for i, s in enumerate(steps)]
             devs = {d for _, d in ptrs}
             if len(devs) != 1:
                 raise ValueError("all time steps must be host arrays or all device tensors")
