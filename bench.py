@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only 3 LIC launches and 3 copies of 256 MiB (rocprof counter runs)")
+    ap.add_argument("--pmc-probe-fused", action="store_true",
+                    help="only 3 fused on-the-fly LIC ray casts of the image (rocprof counter runs)")
     return ap.parse_args()
 
 
@@ -131,6 +133,16 @@ def main():
         torch.cuda.synchronize()
         ctx.close()
         print("[bench] pmc probe done", file=sys.stderr, flush=True)
+        return
+
+    if args.pmc_probe_fused:
+        # counter runs of the fused caster: every sample integrates its own
+        # streamline (no LIC volume), same field, filter and TF as the step
+        for _ in range(3):
+            ctx.render_fused(cam, out=image)
+        ctx.synchronize()
+        ctx.close()
+        print("[bench] fused pmc probe done", file=sys.stderr, flush=True)
         return
 
     print("[bench] warmup", file=sys.stderr, flush=True)
