@@ -1,0 +1,128 @@
+"""Seeded random configurations against the oracle: the LIC volume, the ray
+cast of it and the fused on-the-fly caster, each within 1e-5 relative in
+FP32.  Every case draws its own combination of storage, non-cubic dims,
+spacing, noise grid (the field's or another), noise kind and density,
+frequency scale, gradient (RGBA) noise or scalar gate, filter, forward and
+backward step counts, LIC step, camera, image size, sampling step, TF index,
+transfer function and opacity scale, so paths the hand-written cases pin one
+at a time are also met in combination."""
+import numpy as np
+import pytest
+
+import vectorvisualization_amd as vv
+from vectorvisualization_amd import fields
+from oracle import oracle as orc
+
+from conftest import rel_err
+from workloads import oracle_field, oracle_noise
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+CASES = 24
+
+
+def _draw(seed):
+    rng = np.random.default_rng(1000 + seed)
+    dims = tuple(int(x) for x in rng.integers(6, 19, 3))
+    same_grid = rng.random() < 0.6
+    ndims = dims if same_grid else tuple(int(x) for x in rng.integers(5, 17, 3))
+    grad = bool(rng.random() < 0.2)
+    F, B = (int(x) for x in rng.integers(1, 13, 2))
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    return dict(
+        dims=dims, ndims=ndims, grad=grad,
+        storage=int(rng.choice([vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8])),
+        spacing=tuple(float(x) for x in rng.choice([0.5, 1.0, 1.5, 2.0], 3)),
+        freq=1.0 if rng.random() < 0.7 else float(rng.choice([0.5, 1.5, 2.0])),
+        gate=bool(not grad and rng.random() < 0.15),
+        noise_kind=int(rng.choice([vv.NOISE_BINARY, vv.NOISE_UNIFORM])),
+        density=float(rng.uniform(0.05, 0.6)),
+        filt=int(rng.integers(0, 4)), F=F, B=B,
+        h=float(np.float32(rng.uniform(0.002, 0.05))),
+        gradient_scale=float(np.float32(rng.uniform(1.0, 30.0))),
+        w=int(rng.integers(12, 41)), hgt=int(rng.integers(12, 41)),
+        quat=[float(np.float32(x)) for x in q],
+        step=float(rng.choice([1.0 / 64.0, 1.0 / 96.0, 1.0 / 128.0])),
+        tf_index=int(rng.choice([vv.TF_INDEX_B, vv.TF_INDEX_MAG])),
+        tf_seed=int(rng.integers(0, 100)),
+        sigma=bool(rng.random() < 0.3),
+        zero_vectors=int(rng.integers(0, 4)),
+        vseed=int(rng.integers(0, 2 ** 31)),
+    )
+
+
+@pytest.mark.parametrize("seed", range(CASES))
+def test_random_configuration(ctx, seed):
+    c = _draw(seed)
+    print(f"case {seed}: {c}", flush=True)
+    dims, ndims = c["dims"], c["ndims"]
+    rng = np.random.default_rng(c["vseed"])
+    v = rng.normal(size=(dims[2], dims[1], dims[0], 3)).astype(np.float32)
+    for _ in range(c["zero_vectors"]):
+        v[tuple(int(rng.integers(0, d)) for d in (dims[2], dims[1], dims[0]))] = 0.0
+    ctx.upload_field(v, dims, spacing=c["spacing"], storage=c["storage"])
+    ctx.generate_noise(seed + 1, c["noise_kind"], c["density"], ndims, with_gradients=c["grad"])
+    L = max(c["F"], c["B"])
+    lut, inv = vv.make_filter(c["filt"], L)
+    ctx.set_filter(lut, inv)
+    scalar = None
+    if c["gate"]:
+        scalar = rng.integers(0, 120, int(np.prod(dims)), dtype=np.uint8)
+        ctx.upload_scalar(scalar, dims)
+    illum = vv.ILLUM_GRADIENT if c["grad"] else vv.ILLUM_LIC
+    p = vv.default_lic_params()
+    p.steps_forward, p.steps_backward = c["F"], c["B"]
+    p.step_size_lic = c["h"]
+    p.gradient_scale = c["gradient_scale"]
+    p.freq_scale = c["freq"]
+    p.illum_mode = illum
+    p.gate_enable = 1 if c["gate"] else 0
+    p.tf_index = c["tf_index"]
+    p.step_size_vol = c["step"]
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    lic = ctx.download_lic()
+    ext, sc, cen = ctx.field_geometry()
+
+    f4 = oracle_field(v, dims, c["storage"])
+    noise = oracle_noise(seed + 1, c["noise_kind"], c["density"], ndims, c["grad"])
+    nch = 4 if c["grad"] else 1
+    op = orc.params(steps_forward=c["F"], steps_backward=c["B"], step_size_lic=c["h"], inv_filter_area=inv,
+                    gradient_scale=c["gradient_scale"], freq_scale=c["freq"], illum_mode=1 if c["grad"] else 0,
+                    gate_enable=1 if c["gate"] else 0, field_unorm8=1 if c["storage"] == vv.STORE_UNORM8 else 0,
+                    tf_index=c["tf_index"], step_size_vol=c["step"],
+                    alpha_correction=float(np.float32(c["step"]) * np.float32(128.0)))
+    ref = orc.lic(f4, dims, noise, nch, ndims, lut, op, scale=sc, scalar=scalar,
+                  sdims=dims if scalar is not None else None)
+    print(f"lic rel err {rel_err(lic, ref):.2e}", flush=True)
+    assert np.all(np.isfinite(lic))
+    assert rel_err(lic, ref) <= TOL
+
+    rgba, opac = fields.tf_random(c["tf_seed"])
+    opac = (opac * np.float32(0.4)).astype(np.float32)
+    ctx.set_tf(rgba, opac)
+    sigma = None
+    if c["sigma"]:
+        sigma = rng.uniform(0.3, 1.5, int(np.prod(dims))).astype(np.float32)
+        ctx.set_opacity_scale(sigma)
+    cam = vv.default_camera(c["w"], c["hgt"], cen)
+    for i in range(4):
+        cam.quat[i] = c["quat"][i]
+    img, nsamp = ctx.render(cam, count_samples=True)
+    view = orc.view_setup(c["w"], c["hgt"], quat=c["quat"], center=cen, extent=ext, scale=sc)
+    op.light[0], op.light[1], op.light[2] = list(cam.light_pos)
+    rimg, _, total = orc.raycast(f4, dims, lic, nch, dims, rgba, opac, op, view, sigma=sigma)
+    print(f"render rel err {rel_err(img, rimg):.2e}, samples {nsamp}/{total}", flush=True)
+    assert nsamp == total
+    assert rel_err(img, rimg) <= TOL
+
+    if seed % 2 == 0:
+        fimg, fn = ctx.render_fused(cam, count_samples=True)
+        fref, _, ftotal = orc.raycast_fused(f4, dims, noise, nch, ndims, lut, rgba, opac, op, view, scalar=scalar,
+                                            sdims=dims if scalar is not None else None, sigma=sigma,
+                                            sigma_dims=dims if sigma is not None else None)
+        print(f"fused rel err {rel_err(fimg, fref):.2e}, samples {fn}/{ftotal}", flush=True)
+        assert fn == ftotal
+        assert rel_err(fimg, fref) <= TOL
