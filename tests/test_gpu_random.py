@@ -5,7 +5,8 @@ spacing, noise grid (the field's or another), noise kind and density,
 frequency scale, gradient (RGBA) noise or scalar gate, filter, forward and
 backward step counts, LIC step, camera, image size, sampling step, TF index,
 transfer function and opacity scale, so paths the hand-written cases pin one
-at a time are also met in combination."""
+at a time are also met in combination.  Eight more draws check the TF and
+opacity-scale adjoint against the FP64 oracle adjoint."""
 import numpy as np
 import pytest
 
@@ -53,8 +54,9 @@ def _draw(seed):
     )
 
 
-@pytest.mark.parametrize("seed", range(CASES))
-def test_random_configuration(ctx, seed):
+def _load(ctx, seed):
+    """Uploads case `seed` into ctx, computes its LIC volume and returns the
+    draw, the random stream and what the oracle needs for the same case."""
     c = _draw(seed)
     print(f"case {seed}: {c}", flush=True)
     dims, ndims = c["dims"], c["ndims"]
@@ -71,29 +73,44 @@ def test_random_configuration(ctx, seed):
     if c["gate"]:
         scalar = rng.integers(0, 120, int(np.prod(dims)), dtype=np.uint8)
         ctx.upload_scalar(scalar, dims)
-    illum = vv.ILLUM_GRADIENT if c["grad"] else vv.ILLUM_LIC
     p = vv.default_lic_params()
     p.steps_forward, p.steps_backward = c["F"], c["B"]
     p.step_size_lic = c["h"]
     p.gradient_scale = c["gradient_scale"]
     p.freq_scale = c["freq"]
-    p.illum_mode = illum
+    p.illum_mode = vv.ILLUM_GRADIENT if c["grad"] else vv.ILLUM_LIC
     p.gate_enable = 1 if c["gate"] else 0
     p.tf_index = c["tf_index"]
     p.step_size_vol = c["step"]
     ctx.set_lic_params(p)
     ctx.compute_lic()
-    lic = ctx.download_lic()
-    ext, sc, cen = ctx.field_geometry()
-
     f4 = oracle_field(v, dims, c["storage"])
     noise = oracle_noise(seed + 1, c["noise_kind"], c["density"], ndims, c["grad"])
-    nch = 4 if c["grad"] else 1
     op = orc.params(steps_forward=c["F"], steps_backward=c["B"], step_size_lic=c["h"], inv_filter_area=inv,
                     gradient_scale=c["gradient_scale"], freq_scale=c["freq"], illum_mode=1 if c["grad"] else 0,
                     gate_enable=1 if c["gate"] else 0, field_unorm8=1 if c["storage"] == vv.STORE_UNORM8 else 0,
                     tf_index=c["tf_index"], step_size_vol=c["step"],
                     alpha_correction=float(np.float32(c["step"]) * np.float32(128.0)))
+    return c, rng, dict(f4=f4, noise=noise, nch=4 if c["grad"] else 1, lut=lut, op=op, scalar=scalar)
+
+
+def _camera(ctx, c, o):
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(c["w"], c["hgt"], cen)
+    for i in range(4):
+        cam.quat[i] = c["quat"][i]
+    view = orc.view_setup(c["w"], c["hgt"], quat=c["quat"], center=cen, extent=ext, scale=sc)
+    o["op"].light[0], o["op"].light[1], o["op"].light[2] = list(cam.light_pos)
+    return cam, view
+
+
+@pytest.mark.parametrize("seed", range(CASES))
+def test_random_configuration(ctx, seed):
+    c, rng, o = _load(ctx, seed)
+    dims, ndims = c["dims"], c["ndims"]
+    f4, noise, nch, lut, op, scalar = o["f4"], o["noise"], o["nch"], o["lut"], o["op"], o["scalar"]
+    lic = ctx.download_lic()
+    sc = ctx.field_geometry()[1]
     ref = orc.lic(f4, dims, noise, nch, ndims, lut, op, scale=sc, scalar=scalar,
                   sdims=dims if scalar is not None else None)
     print(f"lic rel err {rel_err(lic, ref):.2e}", flush=True)
@@ -107,12 +124,8 @@ def test_random_configuration(ctx, seed):
     if c["sigma"]:
         sigma = rng.uniform(0.3, 1.5, int(np.prod(dims))).astype(np.float32)
         ctx.set_opacity_scale(sigma)
-    cam = vv.default_camera(c["w"], c["hgt"], cen)
-    for i in range(4):
-        cam.quat[i] = c["quat"][i]
+    cam, view = _camera(ctx, c, o)
     img, nsamp = ctx.render(cam, count_samples=True)
-    view = orc.view_setup(c["w"], c["hgt"], quat=c["quat"], center=cen, extent=ext, scale=sc)
-    op.light[0], op.light[1], op.light[2] = list(cam.light_pos)
     rimg, _, total = orc.raycast(f4, dims, lic, nch, dims, rgba, opac, op, view, sigma=sigma)
     print(f"render rel err {rel_err(img, rimg):.2e}, samples {nsamp}/{total}", flush=True)
     assert nsamp == total
@@ -126,3 +139,29 @@ def test_random_configuration(ctx, seed):
         print(f"fused rel err {rel_err(fimg, fref):.2e}, samples {fn}/{ftotal}", flush=True)
         assert fn == ftotal
         assert rel_err(fimg, fref) <= TOL
+
+
+@pytest.mark.parametrize("seed", range(100, 108))
+def test_random_adjoint(ctx, seed):
+    # TF and opacity-scale gradients of random configurations against the
+    # FP64 oracle adjoint, bound 2e-3 of the largest entry as for the fixed
+    # cases.  The opacity table is scaled by 0.05 so that no ray reaches the
+    # 0.95 termination, where FP32 and FP64 may stop one sample apart.
+    c, rng, o = _load(ctx, seed)
+    dims = c["dims"]
+    rgba, opac = fields.tf_random(c["tf_seed"])
+    opac = (opac * np.float32(0.05)).astype(np.float32)
+    ctx.set_tf(rgba, opac)
+    sigma = rng.uniform(0.3, 1.5, int(np.prod(dims))).astype(np.float32)
+    ctx.set_opacity_scale(sigma)
+    cam, view = _camera(ctx, c, o)
+    dimg = rng.normal(size=c["w"] * c["hgt"] * 4).astype(np.float32)
+    dtf, dsig = ctx.render_adjoint(cam, dimg)
+    lic = ctx.download_lic()
+    rdtf, rdsig = orc.raycast_adjoint(o["f4"].astype(np.float64), dims, lic, o["nch"], dims, rgba, opac, o["op"],
+                                      view, dimg, sigma=sigma)
+    assert np.max(np.abs(rdtf)) > 0 and np.max(np.abs(rdsig)) > 0
+    e_tf = np.max(np.abs(dtf - rdtf)) / np.max(np.abs(rdtf))
+    e_sg = np.max(np.abs(dsig - rdsig)) / np.max(np.abs(rdsig))
+    print(f"adjoint rel err: dTF {e_tf:.2e}  dsigma {e_sg:.2e}", flush=True)
+    assert e_tf <= 2e-3 and e_sg <= 2e-3
