@@ -1,12 +1,12 @@
-"""Seeded random configurations against the oracle: the LIC volume, the ray
-cast of it and the fused on-the-fly caster, each within 1e-5 relative in
-FP32.  Every case draws its own combination of storage, non-cubic dims,
-spacing, noise grid (the field's or another), noise kind and density,
-frequency scale, gradient (RGBA) noise or scalar gate, filter, forward and
-backward step counts, LIC step, camera, image size, sampling step, TF index,
-transfer function and opacity scale, so paths the hand-written cases pin one
-at a time are also met in combination.  Eight more draws check the TF and
-opacity-scale adjoint against the FP64 oracle adjoint."""
+"""Seeded random configurations against the oracle: the LIC volume, the ray cast
+of it and the on-the-fly casters (fused, or view-aligned slicing), each within
+1e-5 relative in FP32. Every case draws its own combination of storage, non-
+cubic dims, spacing, noise grid (the field's or another), noise kind and
+density, frequency scale, gradient (RGBA) noise or scalar gate, filter,
+forward and backward step counts, LIC step, camera, image size, sampling step,
+TF index, transfer function and opacity scale, so paths the hand-written cases
+pin one at a time are also met in combination. Eight more draws check the TF
+and opacity-scale adjoint against the FP64 oracle adjoint."""
 import numpy as np
 import pytest
 
@@ -139,6 +139,14 @@ def test_random_configuration(ctx, seed):
         print(f"fused rel err {rel_err(fimg, fref):.2e}, samples {fn}/{ftotal}", flush=True)
         assert fn == ftotal
         assert rel_err(fimg, fref) <= TOL
+    else:
+        simg, sn = ctx.render_slices(cam, count_samples=True)
+        sref, _, stotal = orc.render_slices(f4, dims, noise, nch, ndims, lut, rgba, opac, op, view, scalar=scalar,
+                                            sdims=dims if scalar is not None else None, sigma=sigma,
+                                            sigma_dims=dims if sigma is not None else None)
+        print(f"slices rel err {rel_err(simg, sref):.2e}, samples {sn}/{stotal}", flush=True)
+        assert sn == stotal
+        assert rel_err(simg, sref) <= TOL
 
 
 @pytest.mark.parametrize("seed", range(100, 108))
