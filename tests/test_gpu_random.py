@@ -1,7 +1,7 @@
 """Seeded random configurations against the oracle: the LIC volume, the ray cast
 of it and the on-the-fly casters (fused, or view-aligned slicing), each within
-1e-5 relative in FP32. Every case draws its own combination of storage, non-
-cubic dims, spacing, noise grid (the field's or another), noise kind and
+1e-5 relative in FP32. Every case draws its own combination of storage,
+non-cubic dims, spacing, noise grid (the field's or another), noise kind and
 density, frequency scale, gradient (RGBA) noise or scalar gate, filter,
 forward and backward step counts, LIC step, camera, image size, sampling step,
 TF index, transfer function and opacity scale, so paths the hand-written cases
