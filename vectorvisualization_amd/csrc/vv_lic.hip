@@ -419,8 +419,12 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
 }
 
+// SAME: at most 80 VGPRs (6 waves per SIMD instead of 5, 1-2 registers
+// spilled), measured 1.5 % faster at 256^3 (FP32 5.71 vs 5.80 ms, FP16 4.79
+// vs 4.86 ms); the separate-noise variants keep their 86 registers
 template <typename FT, bool SAME>
-__global__ void __launch_bounds__(kThreads) k_lic_cached(LicArgs a)
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SAME ? 6 : 1)))
+k_lic_cached(LicArgs a)
 {
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.per_xcd + (b >> 3);
