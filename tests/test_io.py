@@ -108,6 +108,16 @@ def test_png_rejects_what_pngread_rejects(tmp_path):
         L.png_read(f)
     with pytest.raises(vv.VVError):
         L.png_read(tmp_path / "missing.png")
+    # a header claiming 65536 x 65536 RGBA (16 GiB of pixels) is refused
+    # before anything is allocated
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    huge = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 65536, 65536, 8, 6, 0, 0, 0)) +
+            chunk(b"IDAT", zlib.compress(b"\x00" * 16)) + chunk(b"IEND", b""))
+    f.write_bytes(huge)
+    with pytest.raises(vv.VVError) as e:
+        L.png_read(f)
+    assert e.value.status == 4 and "2^28" in str(e.value)
 
 
 def test_image_write_png(tmp_path):

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -143,6 +144,20 @@ inline vv_status fail(vv_ctx* c, vv_status s, const char* fmt, ...)
             return fail((c), e_ == hipErrorOutOfMemory ? VV_ENOMEM : VV_EDEVICE, "%s: %s (%s:%d)", \
                         #expr, hipGetErrorString(e_), __FILE__, __LINE__);                     \
     } while (0)
+
+// The C ABI lets no C++ exception through: a host allocation sized by an
+// input (a file's contents) that fails reports VV_ENOMEM instead.
+template <typename F>
+inline vv_status guarded(vv_ctx* c, F&& f)
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        return fail(c, VV_ENOMEM, "out of host memory");
+    } catch (...) {
+        return fail(c, VV_EDEVICE, "unexpected host error");
+    }
+}
 
 inline bool storage_ok(int storage)
 {
