@@ -20,186 +20,204 @@ const char* vv_last_error(const vv_ctx* ctx)
 
 vv_status vv_create(vv_ctx** out, int device)
 {
-    if (!out) return fail(nullptr, VV_EINVAL, "out is NULL");
-    *out = nullptr;
-    int n = 0;
-    hipError_t e = hipGetDeviceCount(&n);
-    if (e != hipSuccess || n == 0) return fail(nullptr, VV_EDEVICE, "no HIP device available");
-    if (device < 0 || device >= n) return fail(nullptr, VV_EINVAL, "device %d out of range", device);
-    e = hipSetDevice(device);
-    if (e != hipSuccess) return fail(nullptr, VV_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
-    vv_ctx* c = new (std::nothrow) vv_ctx();
-    if (!c) return fail(nullptr, VV_ENOMEM, "out of host memory");
-    c->device = device;
-    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete c;
-        return fail(nullptr, VV_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
-    }
-    default_params(&c->params);
-    vv_default_line_material(&c->line_mat);
-    uint8_t t5[256 * 5];
-    default_tf5(t5);
-    tf5_to_float(t5, c->tf);
-    // default filter: box (VV/3DLIC.cpp:725-729 fallback)
-    c->lut.assign(256, 255);
-    c->inv_area = 0.5f;
-    *out = c;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!out) return fail(nullptr, VV_EINVAL, "out is NULL");
+        *out = nullptr;
+        int n = 0;
+        hipError_t e = hipGetDeviceCount(&n);
+        if (e != hipSuccess || n == 0) return fail(nullptr, VV_EDEVICE, "no HIP device available");
+        if (device < 0 || device >= n) return fail(nullptr, VV_EINVAL, "device %d out of range", device);
+        e = hipSetDevice(device);
+        if (e != hipSuccess) return fail(nullptr, VV_EDEVICE, "hipSetDevice: %s", hipGetErrorString(e));
+        vv_ctx* c = new (std::nothrow) vv_ctx();
+        if (!c) return fail(nullptr, VV_ENOMEM, "out of host memory");
+        c->device = device;
+        e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete c;
+            return fail(nullptr, VV_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
+        }
+        default_params(&c->params);
+        vv_default_line_material(&c->line_mat);
+        uint8_t t5[256 * 5];
+        default_tf5(t5);
+        tf5_to_float(t5, c->tf);
+        // default filter: box (VV/3DLIC.cpp:725-729 fallback)
+        c->lut.assign(256, 255);
+        c->inv_area = 0.5f;
+        *out = c;
+        return VV_OK;
+    });
 }
 
 vv_status vv_destroy(vv_ctx* c)
 {
-    if (!c) return VV_OK;
-    (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
-    DevBuf* bufs[] = {&c->field, &c->noise, &c->scalar, &c->tf_dev, &c->lic, &c->weights,
-                      &c->sigma, &c->stage, &c->stage2, &c->image, &c->counter, &c->adj_ckpt,
-                      &c->adj_acc, &c->adj_dimg, &c->adj_out};
-    for (DevBuf* b : bufs) b->release();
-    (void)hipStreamDestroy(c->stream);
-    delete c;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_OK;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        DevBuf* bufs[] = {&c->field, &c->noise, &c->scalar, &c->tf_dev, &c->lic, &c->weights,
+                          &c->sigma, &c->stage, &c->stage2, &c->image, &c->counter, &c->adj_ckpt,
+                          &c->adj_acc, &c->adj_dimg, &c->adj_out};
+        for (DevBuf* b : bufs) b->release();
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return VV_OK;
+    });
 }
 
 void* vv_stream(vv_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 vv_status vv_synchronize(vv_ctx* c)
 {
-    if (!c) return VV_EINVAL;
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
 }
 
 vv_status vv_default_lic_params(vv_lic_params* p)
 {
-    if (!p) return VV_EINVAL;
-    default_params(p);
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!p) return VV_EINVAL;
+        default_params(p);
+        return VV_OK;
+    });
 }
 
 vv_status vv_default_camera(vv_camera* cam, int width, int height, const float center[3])
 {
-    if (!cam || width < 1 || height < 1) return VV_EINVAL;
-    std::memset(cam, 0, sizeof(*cam));
-    cam->width = width;
-    cam->height = height;
-    cam->fovy_deg = 35.0f;
-    cam->dist = 4.0f;
-    cam->quat[3] = 1.0f;
-    const float c0 = center ? center[0] : 0.5f, c1 = center ? center[1] : 0.5f,
-                c2 = center ? center[2] : 0.5f;
-    // updateLightPos with identity rotations: center + (0,0,light distance 1)
-    cam->light_pos[0] = c0;
-    cam->light_pos[1] = c1;
-    cam->light_pos[2] = c2 + 1.0f;
-    cam->light_diffuse = 1.0f;
-    cam->light_specular = 1.0f;
-    cam->light_ambient = 0.0f;
-    cam->spec_exponent = 40;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!cam || width < 1 || height < 1) return VV_EINVAL;
+        std::memset(cam, 0, sizeof(*cam));
+        cam->width = width;
+        cam->height = height;
+        cam->fovy_deg = 35.0f;
+        cam->dist = 4.0f;
+        cam->quat[3] = 1.0f;
+        const float c0 = center ? center[0] : 0.5f, c1 = center ? center[1] : 0.5f,
+                    c2 = center ? center[2] : 0.5f;
+        // updateLightPos with identity rotations: center + (0,0,light distance 1)
+        cam->light_pos[0] = c0;
+        cam->light_pos[1] = c1;
+        cam->light_pos[2] = c2 + 1.0f;
+        cam->light_diffuse = 1.0f;
+        cam->light_specular = 1.0f;
+        cam->light_ambient = 0.0f;
+        cam->spec_exponent = 40;
+        return VV_OK;
+    });
 }
 
 // ---- field -------------------------------------------------------------------
 vv_status vv_upload_field(vv_ctx* c, const float* v0, const float* v1, float lerp_t, const int dims[3],
                           const float spacing[3], int storage, int on_device)
 {
-    return vv_upload_field_ex(c, v0, v1, lerp_t, dims, spacing, storage, VV_ENCODE_INTERP, on_device);
+    return guarded(c, [&]() -> vv_status {
+        return vv_upload_field_ex(c, v0, v1, lerp_t, dims, spacing, storage, VV_ENCODE_INTERP, on_device);
+    });
 }
 
 vv_status vv_upload_field_ex(vv_ctx* c, const float* v0, const float* v1, float lerp_t, const int dims[3],
                              const float spacing[3], int storage, int encoding, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (encoding != VV_ENCODE_INTERP && encoding != VV_ENCODE_RAW_MAGNITUDE)
-        return fail(c, VV_EINVAL, "unknown field encoding %d", encoding);
-    if (encoding == VV_ENCODE_RAW_MAGNITUDE && storage == VV_STORE_UNORM8)
-        return fail(c, VV_EINVAL, "the raw-magnitude encoding has float texels only (FP32, FP16)");
-    if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims %s", kDimsRule);
-    if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage %d", storage);
-    if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
-    float sd[3] = {1, 1, 1};
-    if (spacing)
-        for (int a = 0; a < 3; ++a) {
-            if (!(spacing[a] > 0.0f) || !std::isfinite(spacing[a]))
-                return fail(c, VV_EINVAL, "spacing must be > 0");
-            sd[a] = spacing[a];
-        }
-    HIPCHK(c, hipSetDevice(c->device));
-    const long long n = nvox(dims);
-    const size_t raw = sizeof(float) * 3 * (size_t)n;
-    const float* d0 = v0;
-    const float* d1 = v1 ? v1 : v0;
-    if (!on_device) {
-        for (long long i = 0; i < 3 * n; ++i)
-            if (!std::isfinite(v0[i]) || (v1 && !std::isfinite(v1[i])))
-                return fail(c, VV_EINVAL, "field contains a non-finite value at %lld", i / 3);
-        vv_status st = upload(c, c->stage, v0, raw, 0);
-        if (st) return st;
-        d0 = (const float*)c->stage.p;
-        d1 = d0;
-        if (v1) {
-            st = upload(c, c->stage2, v1, raw, 0);
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (encoding != VV_ENCODE_INTERP && encoding != VV_ENCODE_RAW_MAGNITUDE)
+            return fail(c, VV_EINVAL, "unknown field encoding %d", encoding);
+        if (encoding == VV_ENCODE_RAW_MAGNITUDE && storage == VV_STORE_UNORM8)
+            return fail(c, VV_EINVAL, "the raw-magnitude encoding has float texels only (FP32, FP16)");
+        if (!v0) return fail(c, VV_EINVAL, "field data is NULL");
+        if (!dims_ok(dims)) return fail(c, VV_EINVAL, "field dims %s", kDimsRule);
+        if (!storage_ok(storage)) return fail(c, VV_EINVAL, "unknown storage %d", storage);
+        if (!std::isfinite(lerp_t)) return fail(c, VV_EINVAL, "lerp_t not finite");
+        float sd[3] = {1, 1, 1};
+        if (spacing)
+            for (int a = 0; a < 3; ++a) {
+                if (!(spacing[a] > 0.0f) || !std::isfinite(spacing[a]))
+                    return fail(c, VV_EINVAL, "spacing must be > 0");
+                sd[a] = spacing[a];
+            }
+        HIPCHK(c, hipSetDevice(c->device));
+        const long long n = nvox(dims);
+        const size_t raw = sizeof(float) * 3 * (size_t)n;
+        const float* d0 = v0;
+        const float* d1 = v1 ? v1 : v0;
+        if (!on_device) {
+            for (long long i = 0; i < 3 * n; ++i)
+                if (!std::isfinite(v0[i]) || (v1 && !std::isfinite(v1[i])))
+                    return fail(c, VV_EINVAL, "field contains a non-finite value at %lld", i / 3);
+            vv_status st = upload(c, c->stage, v0, raw, 0);
             if (st) return st;
-            d1 = (const float*)c->stage2.p;
+            d0 = (const float*)c->stage.p;
+            d1 = d0;
+            if (v1) {
+                st = upload(c, c->stage2, v1, raw, 0);
+                if (st) return st;
+                d1 = (const float*)c->stage2.p;
+            }
         }
-    }
-    HIPCHK(c, c->field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims)));
-    HIPCHK(c, c->counter.ensure(64));
-    uint32_t* maxbits = (uint32_t*)c->counter.p;
-    HIPCHK(c, launch_fill_u32(maxbits, 0u, 1, c->stream));
-    HIPCHK(c, launch_field_maxlen(d0, d1, lerp_t, n, maxbits, c->stream));
-    HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, storage, c->stream,
-                                  encoding == VV_ENCODE_RAW_MAGNITUDE));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
-    float vol[3], maxvol = 0.0f;
-    int maxtex = 0;
-    for (int a = 0; a < 3; ++a) {
-        c->fdims[a] = dims[a];
-        c->spacing[a] = sd[a];
-        vol[a] = (float)dims[a] * sd[a];
-        if (vol[a] > maxvol) maxvol = vol[a];
-        if (dims[a] > maxtex) maxtex = dims[a];
-    }
-    for (int a = 0; a < 3; ++a) {
-        c->scale[a] = (float)maxtex / ((float)dims[a] * sd[a]);
-        c->extent[a] = (float)dims[a] * sd[a] / maxvol;
-        c->center[a] = c->extent[a] / 2.0f;
-    }
-    c->storage = storage;
-    c->has_field = true;
-    c->lic_field_ok = false;
-    return VV_OK;
+        HIPCHK(c, c->field.ensure((size_t)field_texel_bytes(storage) * (size_t)clamp_elems(dims)));
+        HIPCHK(c, c->counter.ensure(64));
+        uint32_t* maxbits = (uint32_t*)c->counter.p;
+        HIPCHK(c, launch_fill_u32(maxbits, 0u, 1, c->stream));
+        HIPCHK(c, launch_field_maxlen(d0, d1, lerp_t, n, maxbits, c->stream));
+        HIPCHK(c, launch_field_encode(d0, d1, lerp_t, dims, maxbits, c->field.p, storage, c->stream,
+                                      encoding == VV_ENCODE_RAW_MAGNITUDE));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        // VectorDataSet::loadData geometry (VV/dataset.cpp:144-176)
+        float vol[3], maxvol = 0.0f;
+        int maxtex = 0;
+        for (int a = 0; a < 3; ++a) {
+            c->fdims[a] = dims[a];
+            c->spacing[a] = sd[a];
+            vol[a] = (float)dims[a] * sd[a];
+            if (vol[a] > maxvol) maxvol = vol[a];
+            if (dims[a] > maxtex) maxtex = dims[a];
+        }
+        for (int a = 0; a < 3; ++a) {
+            c->scale[a] = (float)maxtex / ((float)dims[a] * sd[a]);
+            c->extent[a] = (float)dims[a] * sd[a] / maxvol;
+            c->center[a] = c->extent[a] / 2.0f;
+        }
+        c->storage = storage;
+        c->has_field = true;
+        c->lic_field_ok = false;
+        return VV_OK;
+    });
 }
 
 vv_status vv_field_geometry(const vv_ctx* c, float extent[3], float scale[3], float center[3])
 {
-    if (!c || !c->has_field) return VV_ESTATE;
-    for (int a = 0; a < 3; ++a) {
-        if (extent) extent[a] = c->extent[a];
-        if (scale) scale[a] = c->scale[a];
-        if (center) center[a] = c->center[a];
-    }
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !c->has_field) return VV_ESTATE;
+        for (int a = 0; a < 3; ++a) {
+            if (extent) extent[a] = c->extent[a];
+            if (scale) scale[a] = c->scale[a];
+            if (center) center[a] = c->center[a];
+        }
+        return VV_OK;
+    });
 }
 
 vv_status vv_download_field(vv_ctx* c, float* out4, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!c->has_field) return fail(c, VV_ESTATE, "no field");
-    if (!out4) return fail(c, VV_EINVAL, "out is NULL");
-    const size_t bytes = sizeof(float) * 4 * (size_t)nvox(c->fdims);
-    float* dst = out4;
-    if (!on_device) {
-        HIPCHK(c, c->stage.ensure(bytes));
-        dst = (float*)c->stage.p;
-    }
-    HIPCHK(c, launch_field_decode(c->field.p, c->storage, c->fdims, dst, c->stream));
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(out4, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!c->has_field) return fail(c, VV_ESTATE, "no field");
+        if (!out4) return fail(c, VV_EINVAL, "out is NULL");
+        const size_t bytes = sizeof(float) * 4 * (size_t)nvox(c->fdims);
+        float* dst = out4;
+        if (!on_device) {
+            HIPCHK(c, c->stage.ensure(bytes));
+            dst = (float*)c->stage.p;
+        }
+        HIPCHK(c, launch_field_decode(c->field.p, c->storage, c->fdims, dst, c->stream));
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(out4, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
 }
 
 // ---- noise -------------------------------------------------------------------
@@ -254,184 +272,210 @@ vv_status build_noise_tex(vv_ctx* c, const uint8_t* d_noise, const int dims[3], 
 vv_status vv_upload_noise(vv_ctx* c, const uint8_t* noise, const int dims[3], int with_gradients,
                           const uint8_t* grad_rgb, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!noise) return fail(c, VV_EINVAL, "noise is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
-    HIPCHK(c, hipSetDevice(c->device));
-    vv_status st = upload(c, c->stage, noise, (size_t)nvox(dims), on_device);
-    if (st) return st;
-    return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, grad_rgb, on_device);
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!noise) return fail(c, VV_EINVAL, "noise is NULL");
+        if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
+        HIPCHK(c, hipSetDevice(c->device));
+        vv_status st = upload(c, c->stage, noise, (size_t)nvox(dims), on_device);
+        if (st) return st;
+        return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, grad_rgb, on_device);
+    });
 }
 
 vv_status vv_generate_noise(vv_ctx* c, uint64_t seed, int kind, double density, const int dims[3],
                             int with_gradients)
 {
-    if (!c) return VV_EINVAL;
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
-    if (kind != VV_NOISE_BINARY && kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
-    if (kind == VV_NOISE_BINARY && !(density >= 0.0 && density <= 1.0))
-        return fail(c, VV_EINVAL, "density must be in [0,1]");
-    HIPCHK(c, hipSetDevice(c->device));
-    double t = std::floor((kind == VV_NOISE_BINARY ? density : 0.0) * 4294967296.0);
-    if (t < 0) t = 0;
-    if (t > 4294967296.0) t = 4294967296.0;
-    HIPCHK(c, c->stage.ensure((size_t)nvox(dims)));
-    HIPCHK(c, launch_noise_gen(seed, kind, (uint64_t)t, dims, (uint8_t*)c->stage.p, c->stream));
-    return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, nullptr, 0);
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!dims_ok(dims)) return fail(c, VV_EINVAL, "noise dims %s", kDimsRule);
+        if (kind != VV_NOISE_BINARY && kind != VV_NOISE_UNIFORM) return fail(c, VV_EINVAL, "noise kind");
+        if (kind == VV_NOISE_BINARY && !(density >= 0.0 && density <= 1.0))
+            return fail(c, VV_EINVAL, "density must be in [0,1]");
+        HIPCHK(c, hipSetDevice(c->device));
+        double t = std::floor((kind == VV_NOISE_BINARY ? density : 0.0) * 4294967296.0);
+        if (t < 0) t = 0;
+        if (t > 4294967296.0) t = 4294967296.0;
+        HIPCHK(c, c->stage.ensure((size_t)nvox(dims)));
+        HIPCHK(c, launch_noise_gen(seed, kind, (uint64_t)t, dims, (uint8_t*)c->stage.p, c->stream));
+        return build_noise_tex(c, (const uint8_t*)c->stage.p, dims, with_gradients, nullptr, 0);
+    });
 }
 
 vv_status vv_download_noise(vv_ctx* c, uint8_t* out, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!c->has_noise) return fail(c, VV_ESTATE, "no noise");
-    if (!out) return fail(c, VV_EINVAL, "out is NULL");
-    const size_t bytes = (size_t)c->nch * (size_t)nvox(c->ndims);
-    uint8_t* dst = out;
-    if (!on_device) {
-        HIPCHK(c, c->stage.ensure(bytes));
-        dst = (uint8_t*)c->stage.p;
-    }
-    HIPCHK(c, launch_noise_untex(c->noise.p, c->nch, c->ndims, dst, c->stream));
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!c->has_noise) return fail(c, VV_ESTATE, "no noise");
+        if (!out) return fail(c, VV_EINVAL, "out is NULL");
+        const size_t bytes = (size_t)c->nch * (size_t)nvox(c->ndims);
+        uint8_t* dst = out;
+        if (!on_device) {
+            HIPCHK(c, c->stage.ensure(bytes));
+            dst = (uint8_t*)c->stage.p;
+        }
+        HIPCHK(c, launch_noise_untex(c->noise.p, c->nch, c->ndims, dst, c->stream));
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
 }
 
 vv_status vv_noise_info(const vv_ctx* c, int dims[3], int* channels)
 {
-    if (!c || !c->has_noise) return VV_ESTATE;
-    if (dims)
-        for (int a = 0; a < 3; ++a) dims[a] = c->ndims[a];
-    if (channels) *channels = c->nch;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !c->has_noise) return VV_ESTATE;
+        if (dims)
+            for (int a = 0; a < 3; ++a) dims[a] = c->ndims[a];
+        if (channels) *channels = c->nch;
+        return VV_OK;
+    });
 }
 
 // ---- filter ------------------------------------------------------------------
 vv_status vv_make_filter(int kind, int length, float sigma, uint8_t lut[256], int* width, float* inv_area)
 {
-    if (!lut || !width || !inv_area) return VV_EINVAL;
-    const int W = 256;
-    if (kind == VV_FILTER_BOX) {
-        // LICFilter::createBoxFilter (VV/dataset.cpp:1405-1413)
-        std::memset(lut, 255, W);
-        *width = W;
-        *inv_area = 0.5f;
-        return VV_OK;
-    }
-    if (kind < VV_FILTER_TRIANGLE || kind > VV_FILTER_GAUSSIAN || length <= 0) return VV_EINVAL;
-    double sg = sigma > 0.0f ? (double)sigma : length / 4.0;
-    const double pi = 3.14159265358979323846;
-    for (int i = 0; i < W; ++i) {
-        const double t = (i + 0.5) / W;
-        const double cc = 2.0 * t - 1.0;
-        double w;
-        if (kind == VV_FILTER_TRIANGLE) {
-            w = 1.0 - std::fabs(cc);
-        } else if (kind == VV_FILTER_COS2) {
-            const double q = std::cos(pi * (t - 0.5));
-            w = q * q;
-        } else {
-            const double s = cc * length;
-            w = std::exp(-s * s / (2.0 * sg * sg));
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!lut || !width || !inv_area) return VV_EINVAL;
+        const int W = 256;
+        if (kind == VV_FILTER_BOX) {
+            // LICFilter::createBoxFilter (VV/dataset.cpp:1405-1413)
+            std::memset(lut, 255, W);
+            *width = W;
+            *inv_area = 0.5f;
+            return VV_OK;
         }
-        double b = std::floor(255.0 * w + 0.5);
-        lut[i] = (uint8_t)(b < 0 ? 0 : (b > 255 ? 255 : b));
-    }
-    *width = W;
-    float area = 0.0f;
-    for (int i = 0; i < W; ++i) area += lut[i];
-    *inv_area = 0.5f * W * 255.0f / area;
-    return VV_OK;
+        if (kind < VV_FILTER_TRIANGLE || kind > VV_FILTER_GAUSSIAN || length <= 0) return VV_EINVAL;
+        double sg = sigma > 0.0f ? (double)sigma : length / 4.0;
+        const double pi = 3.14159265358979323846;
+        for (int i = 0; i < W; ++i) {
+            const double t = (i + 0.5) / W;
+            const double cc = 2.0 * t - 1.0;
+            double w;
+            if (kind == VV_FILTER_TRIANGLE) {
+                w = 1.0 - std::fabs(cc);
+            } else if (kind == VV_FILTER_COS2) {
+                const double q = std::cos(pi * (t - 0.5));
+                w = q * q;
+            } else {
+                const double s = cc * length;
+                w = std::exp(-s * s / (2.0 * sg * sg));
+            }
+            double b = std::floor(255.0 * w + 0.5);
+            lut[i] = (uint8_t)(b < 0 ? 0 : (b > 255 ? 255 : b));
+        }
+        *width = W;
+        float area = 0.0f;
+        for (int i = 0; i < W; ++i) area += lut[i];
+        *inv_area = 0.5f * W * 255.0f / area;
+        return VV_OK;
+    });
 }
 
 vv_status vv_set_filter(vv_ctx* c, const uint8_t* lut, int width, float inv_area)
 {
-    if (!c) return VV_EINVAL;
-    if (!lut) return fail(c, VV_EINVAL, "lut is NULL");
-    if (width < 1 || width > 4096 || (width & (width - 1)))
-        return fail(c, VV_EINVAL, "filter width must be a power of two <= 4096");
-    float area = 0.0f;
-    for (int i = 0; i < width; ++i) area += lut[i];
-    if (!(area > 0.0f)) return fail(c, VV_EINVAL, "filter is all zero");
-    if (!std::isfinite(inv_area)) return fail(c, VV_EINVAL, "inv_area not finite");
-    c->lut.assign(lut, lut + width);
-    // calcFilterKernelInvArea (VV/dataset.cpp:1503-1512)
-    c->inv_area = inv_area > 0.0f ? inv_area : 0.5f * width * 255.0f / area;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!lut) return fail(c, VV_EINVAL, "lut is NULL");
+        if (width < 1 || width > 4096 || (width & (width - 1)))
+            return fail(c, VV_EINVAL, "filter width must be a power of two <= 4096");
+        float area = 0.0f;
+        for (int i = 0; i < width; ++i) area += lut[i];
+        if (!(area > 0.0f)) return fail(c, VV_EINVAL, "filter is all zero");
+        if (!std::isfinite(inv_area)) return fail(c, VV_EINVAL, "inv_area not finite");
+        c->lut.assign(lut, lut + width);
+        // calcFilterKernelInvArea (VV/dataset.cpp:1503-1512)
+        c->inv_area = inv_area > 0.0f ? inv_area : 0.5f * width * 255.0f / area;
+        return VV_OK;
+    });
 }
 
 // ---- transfer function -------------------------------------------------------
 vv_status vv_set_tf(vv_ctx* c, const float* rgba, const float* opacity)
 {
-    if (!c) return VV_EINVAL;
-    if (!rgba || !opacity) return fail(c, VV_EINVAL, "tf arrays NULL");
-    for (int i = 0; i < 1024; ++i)
-        if (!std::isfinite(rgba[i])) return fail(c, VV_EINVAL, "tf rgba not finite");
-    for (int i = 0; i < 256; ++i)
-        if (!std::isfinite(opacity[i])) return fail(c, VV_EINVAL, "tf opacity not finite");
-    std::memcpy(c->tf, rgba, sizeof(float) * 1024);
-    std::memcpy(c->tf + 1024, opacity, sizeof(float) * 256);
-    c->tf_dirty = true;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!rgba || !opacity) return fail(c, VV_EINVAL, "tf arrays NULL");
+        for (int i = 0; i < 1024; ++i)
+            if (!std::isfinite(rgba[i])) return fail(c, VV_EINVAL, "tf rgba not finite");
+        for (int i = 0; i < 256; ++i)
+            if (!std::isfinite(opacity[i])) return fail(c, VV_EINVAL, "tf opacity not finite");
+        std::memcpy(c->tf, rgba, sizeof(float) * 1024);
+        std::memcpy(c->tf + 1024, opacity, sizeof(float) * 256);
+        c->tf_dirty = true;
+        return VV_OK;
+    });
 }
 
 vv_status vv_set_tf_u8(vv_ctx* c, const uint8_t* tf5)
 {
-    if (!c) return VV_EINVAL;
-    if (!tf5) return fail(c, VV_EINVAL, "tf NULL");
-    tf5_to_float(tf5, c->tf);
-    c->tf_dirty = true;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!tf5) return fail(c, VV_EINVAL, "tf NULL");
+        tf5_to_float(tf5, c->tf);
+        c->tf_dirty = true;
+        return VV_OK;
+    });
 }
 
 vv_status vv_default_tf_u8(uint8_t tf5[256 * 5])
 {
-    if (!tf5) return VV_EINVAL;
-    default_tf5(tf5);
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!tf5) return VV_EINVAL;
+        default_tf5(tf5);
+        return VV_OK;
+    });
 }
 
 // ---- scalar gate volume --------------------------------------------------------
 vv_status vv_upload_scalar(vv_ctx* c, const uint8_t* scalar, const int dims[3], int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!scalar) return fail(c, VV_EINVAL, "scalar is NULL");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "scalar dims %s", kDimsRule);
-    HIPCHK(c, hipSetDevice(c->device));
-    vv_status st = upload(c, c->stage, scalar, (size_t)nvox(dims), on_device);
-    if (st) return st;
-    HIPCHK(c, c->scalar.ensure((size_t)clamp_elems(dims)));
-    HIPCHK(c, launch_pad_clamp_u8((const uint8_t*)c->stage.p, dims, (uint8_t*)c->scalar.p, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int a = 0; a < 3; ++a) c->sdims[a] = dims[a];
-    c->has_scalar = true;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!scalar) return fail(c, VV_EINVAL, "scalar is NULL");
+        if (!dims_ok(dims)) return fail(c, VV_EINVAL, "scalar dims %s", kDimsRule);
+        HIPCHK(c, hipSetDevice(c->device));
+        vv_status st = upload(c, c->stage, scalar, (size_t)nvox(dims), on_device);
+        if (st) return st;
+        HIPCHK(c, c->scalar.ensure((size_t)clamp_elems(dims)));
+        HIPCHK(c, launch_pad_clamp_u8((const uint8_t*)c->stage.p, dims, (uint8_t*)c->scalar.p, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int a = 0; a < 3; ++a) c->sdims[a] = dims[a];
+        c->has_scalar = true;
+        return VV_OK;
+    });
 }
 
 // ---- params ------------------------------------------------------------------
 vv_status vv_set_lic_params(vv_ctx* c, const vv_lic_params* p)
 {
-    if (!c) return VV_EINVAL;
-    vv_status st = check_params(c, p);
-    if (st) return st;
-    c->params = *p;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        vv_status st = check_params(c, p);
+        if (st) return st;
+        c->params = *p;
+        return VV_OK;
+    });
 }
 
 vv_status vv_get_lic_params(const vv_ctx* c, vv_lic_params* p)
 {
-    if (!c || !p) return VV_EINVAL;
-    *p = c->params;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !p) return VV_EINVAL;
+        *p = c->params;
+        return VV_OK;
+    });
 }
 
 vv_status vv_lic_weights(const vv_ctx* c, float* out)
 {
-    if (!c || !out) return VV_EINVAL;
-    std::vector<float> w;
-    lic_weights(c, w);
-    std::memcpy(out, w.data(), sizeof(float) * w.size());
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !out) return VV_EINVAL;
+        std::vector<float> w;
+        lic_weights(c, w);
+        std::memcpy(out, w.data(), sizeof(float) * w.size());
+        return VV_OK;
+    });
 }
 
 // ---- LIC ---------------------------------------------------------------------
@@ -510,170 +554,186 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
 
 vv_status vv_compute_lic(vv_ctx* c)
 {
-    if (!c) return VV_EINVAL;
-    if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
-    LicLaunch L;
-    vv_status st = fill_lic_launch(c, L, nullptr);
-    if (st) return st;
-    const int nc = c->nch;
-    HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
-    L.out = make_repeat_desc(c->lic.p, c->fdims);
-    HIPCHK(c, launch_lic(L, c->stream));  // also writes the REPEAT halo planes
-    // the opacity scale lives on the LIC grid: drop it when that grid changes
-    if (c->ldims[0] != c->fdims[0] || c->ldims[1] != c->fdims[1] || c->ldims[2] != c->fdims[2])
-        c->has_sigma = false;
-    for (int a = 0; a < 3; ++a) c->ldims[a] = c->fdims[a];
-    c->lch = nc;
-    c->has_lic = true;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!c->has_field) return fail(c, VV_ESTATE, "no vector field uploaded");
+        LicLaunch L;
+        vv_status st = fill_lic_launch(c, L, nullptr);
+        if (st) return st;
+        const int nc = c->nch;
+        HIPCHK(c, c->lic.ensure(sizeof(float) * nc * (size_t)repeat_elems(c->fdims)));
+        L.out = make_repeat_desc(c->lic.p, c->fdims);
+        HIPCHK(c, launch_lic(L, c->stream));  // also writes the REPEAT halo planes
+        // the opacity scale lives on the LIC grid: drop it when that grid changes
+        if (c->ldims[0] != c->fdims[0] || c->ldims[1] != c->fdims[1] || c->ldims[2] != c->fdims[2])
+            c->has_sigma = false;
+        for (int a = 0; a < 3; ++a) c->ldims[a] = c->fdims[a];
+        c->lch = nc;
+        c->has_lic = true;
+        return VV_OK;
+    });
 }
 
 vv_status vv_lic_info(const vv_ctx* c, int dims[3], int* channels)
 {
-    if (!c || !c->has_lic) return VV_ESTATE;
-    if (dims)
-        for (int a = 0; a < 3; ++a) dims[a] = c->ldims[a];
-    if (channels) *channels = c->lch;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !c->has_lic) return VV_ESTATE;
+        if (dims)
+            for (int a = 0; a < 3; ++a) dims[a] = c->ldims[a];
+        if (channels) *channels = c->lch;
+        return VV_OK;
+    });
 }
 
 vv_status vv_download_lic(vv_ctx* c, float* out, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!c->has_lic) return fail(c, VV_ESTATE, "no LIC volume");
-    if (!out) return fail(c, VV_EINVAL, "out is NULL");
-    const size_t bytes = sizeof(float) * c->lch * (size_t)nvox(c->ldims);
-    float* dst = out;
-    if (!on_device) {
-        HIPCHK(c, c->stage.ensure(bytes));
-        dst = (float*)c->stage.p;
-    }
-    HIPCHK(c, launch_unpad_repeat_f32((const float*)c->lic.p, c->lch, c->ldims, dst, c->stream));
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!c->has_lic) return fail(c, VV_ESTATE, "no LIC volume");
+        if (!out) return fail(c, VV_EINVAL, "out is NULL");
+        const size_t bytes = sizeof(float) * c->lch * (size_t)nvox(c->ldims);
+        float* dst = out;
+        if (!on_device) {
+            HIPCHK(c, c->stage.ensure(bytes));
+            dst = (float*)c->stage.p;
+        }
+        HIPCHK(c, launch_unpad_repeat_f32((const float*)c->lic.p, c->lch, c->ldims, dst, c->stream));
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(out, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
 }
 
 vv_status vv_upload_lic(vv_ctx* c, const float* lic, int channels, const int dims[3], int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!lic) return fail(c, VV_EINVAL, "lic is NULL");
-    if (channels != 1 && channels != 4) return fail(c, VV_EINVAL, "channels must be 1 or 4");
-    if (!dims_ok(dims)) return fail(c, VV_EINVAL, "lic dims %s", kDimsRule);
-    HIPCHK(c, hipSetDevice(c->device));
-    const size_t bytes = sizeof(float) * channels * (size_t)nvox(dims);
-    vv_status st = upload(c, c->stage, lic, bytes, on_device);
-    if (st) return st;
-    HIPCHK(c, c->lic.ensure(sizeof(float) * channels * (size_t)repeat_elems(dims)));
-    HIPCHK(c, launch_pad_repeat_f32((const float*)c->stage.p, channels, dims, (float*)c->lic.p, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int a = 0; a < 3; ++a) c->ldims[a] = dims[a];
-    c->lch = channels;
-    c->has_lic = true;
-    c->has_sigma = false;  // opacity scale lives on the LIC grid
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!lic) return fail(c, VV_EINVAL, "lic is NULL");
+        if (channels != 1 && channels != 4) return fail(c, VV_EINVAL, "channels must be 1 or 4");
+        if (!dims_ok(dims)) return fail(c, VV_EINVAL, "lic dims %s", kDimsRule);
+        HIPCHK(c, hipSetDevice(c->device));
+        const size_t bytes = sizeof(float) * channels * (size_t)nvox(dims);
+        vv_status st = upload(c, c->stage, lic, bytes, on_device);
+        if (st) return st;
+        HIPCHK(c, c->lic.ensure(sizeof(float) * channels * (size_t)repeat_elems(dims)));
+        HIPCHK(c, launch_pad_repeat_f32((const float*)c->stage.p, channels, dims, (float*)c->lic.p, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int a = 0; a < 3; ++a) c->ldims[a] = dims[a];
+        c->lch = channels;
+        c->has_lic = true;
+        c->has_sigma = false;  // opacity scale lives on the LIC grid
+        return VV_OK;
+    });
 }
 
 // ---- tuning ------------------------------------------------------------------
 vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
 {
-    if (!c) return VV_EINVAL;
-    if (!key) return fail(c, VV_EINVAL, "key is NULL");
-    const std::string k(key);
-    int* slot = nullptr;
-    if (k == "lic.tile_x") slot = &c->lic_tile[0];
-    else if (k == "lic.tile_y") slot = &c->lic_tile[1];
-    else if (k == "lic.tile_z") slot = &c->lic_tile[2];
-    else if (k == "lic.cache") slot = &c->lic_cache;
-    else if (k == "lic.super_x") slot = &c->lic_super[0];
-    else if (k == "lic.super_y") slot = &c->lic_super[1];
-    else if (k == "lic.super_z") slot = &c->lic_super[2];
-    else if (k == "adjoint.window") slot = &c->adj_window;
-    else if (k == "render.block_x") slot = &c->render_block[0];
-    else if (k == "render.block_y") slot = &c->render_block[1];
-    else if (k == "render.ray_group") slot = &c->ray_group;
-    else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
-    if (k == "lic.cache" || k == "adjoint.window") {
-        if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
-    } else if (k == "render.ray_group") {
-        if (value < 0 || value > 64 || (value & (value - 1)))
-            return fail(c, VV_EINVAL, "render.ray_group must be 0 (automatic) or a power of two up to 64");
-    } else if (value < 1 || value > 256 || (value & (value - 1))) {
-        return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
-    }
-    *slot = value;  // render block pairs are checked when a frame is rendered
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!key) return fail(c, VV_EINVAL, "key is NULL");
+        const std::string k(key);
+        int* slot = nullptr;
+        if (k == "lic.tile_x") slot = &c->lic_tile[0];
+        else if (k == "lic.tile_y") slot = &c->lic_tile[1];
+        else if (k == "lic.tile_z") slot = &c->lic_tile[2];
+        else if (k == "lic.cache") slot = &c->lic_cache;
+        else if (k == "lic.super_x") slot = &c->lic_super[0];
+        else if (k == "lic.super_y") slot = &c->lic_super[1];
+        else if (k == "lic.super_z") slot = &c->lic_super[2];
+        else if (k == "adjoint.window") slot = &c->adj_window;
+        else if (k == "render.block_x") slot = &c->render_block[0];
+        else if (k == "render.block_y") slot = &c->render_block[1];
+        else if (k == "render.ray_group") slot = &c->ray_group;
+        else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
+        if (k == "lic.cache" || k == "adjoint.window") {
+            if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
+        } else if (k == "render.ray_group") {
+            if (value < 0 || value > 64 || (value & (value - 1)))
+                return fail(c, VV_EINVAL, "render.ray_group must be 0 (automatic) or a power of two up to 64");
+        } else if (value < 1 || value > 256 || (value & (value - 1))) {
+            return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
+        }
+        *slot = value;  // render block pairs are checked when a frame is rendered
+        return VV_OK;
+    });
 }
 
 vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
 {
-    if (!c || !key || !value) return VV_EINVAL;
-    const std::string k(key);
-    if (k == "lic.tile_x") *value = c->lic_tile[0];
-    else if (k == "lic.tile_y") *value = c->lic_tile[1];
-    else if (k == "lic.tile_z") *value = c->lic_tile[2];
-    else if (k == "lic.cache") *value = c->lic_cache;
-    else if (k == "lic.super_x") *value = c->lic_super[0];
-    else if (k == "lic.super_y") *value = c->lic_super[1];
-    else if (k == "lic.super_z") *value = c->lic_super[2];
-    else if (k == "adjoint.window") *value = c->adj_window;
-    else if (k == "render.block_x") *value = c->render_block[0];
-    else if (k == "render.block_y") *value = c->render_block[1];
-    else if (k == "render.ray_group") *value = c->ray_group;
-    else return VV_EINVAL;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !key || !value) return VV_EINVAL;
+        const std::string k(key);
+        if (k == "lic.tile_x") *value = c->lic_tile[0];
+        else if (k == "lic.tile_y") *value = c->lic_tile[1];
+        else if (k == "lic.tile_z") *value = c->lic_tile[2];
+        else if (k == "lic.cache") *value = c->lic_cache;
+        else if (k == "lic.super_x") *value = c->lic_super[0];
+        else if (k == "lic.super_y") *value = c->lic_super[1];
+        else if (k == "lic.super_z") *value = c->lic_super[2];
+        else if (k == "adjoint.window") *value = c->adj_window;
+        else if (k == "render.block_x") *value = c->render_block[0];
+        else if (k == "render.block_y") *value = c->render_block[1];
+        else if (k == "render.ray_group") *value = c->ray_group;
+        else return VV_EINVAL;
+        return VV_OK;
+    });
 }
 
 // ---- render ------------------------------------------------------------------
 vv_status vv_view_block(const vv_ctx* c, const vv_camera* cam, float out24[24])
 {
-    if (!c || !cam || !out24) return VV_EINVAL;
-    ViewBlock v = view_block(c, cam);
-    float* o = out24;
-    for (int a = 0; a < 3; ++a) o[a] = v.cam[a];
-    for (int a = 0; a < 3; ++a) o[3 + a] = v.du[a];
-    for (int a = 0; a < 3; ++a) o[6 + a] = v.dv[a];
-    for (int a = 0; a < 3; ++a) o[9 + a] = v.dw[a];
-    o[12] = v.inv_w2;
-    o[13] = v.inv_h2;
-    for (int a = 0; a < 3; ++a) o[14 + a] = v.extent[a];
-    for (int a = 0; a < 3; ++a) o[17 + a] = v.scale[a];
-    for (int a = 0; a < 3; ++a) o[20 + a] = v.tex_max[a];
-    o[23] = 0.0f;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !cam || !out24) return VV_EINVAL;
+        ViewBlock v = view_block(c, cam);
+        float* o = out24;
+        for (int a = 0; a < 3; ++a) o[a] = v.cam[a];
+        for (int a = 0; a < 3; ++a) o[3 + a] = v.du[a];
+        for (int a = 0; a < 3; ++a) o[6 + a] = v.dv[a];
+        for (int a = 0; a < 3; ++a) o[9 + a] = v.dw[a];
+        o[12] = v.inv_w2;
+        o[13] = v.inv_h2;
+        for (int a = 0; a < 3; ++a) o[14 + a] = v.extent[a];
+        for (int a = 0; a < 3; ++a) o[17 + a] = v.scale[a];
+        for (int a = 0; a < 3; ++a) o[20 + a] = v.tex_max[a];
+        o[23] = 0.0f;
+        return VV_OK;
+    });
 }
 
 vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
-    if (!c) return VV_EINVAL;
-    if (!image) return fail(c, VV_EINVAL, "image is NULL");
-    HIPCHK(c, hipSetDevice(c->device));
-    RenderLaunch R;
-    vv_status st = fill_render_launch(c, cam, R);
-    if (st) return st;
-    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
-    float* dst = image;
-    if (!on_device) {
-        HIPCHK(c, c->image.ensure(bytes));
-        dst = (float*)c->image.p;
-    }
-    R.image = dst;
-    if (samples_out) {
-        HIPCHK(c, c->counter.ensure(64));
-        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
-        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
-    }
-    HIPCHK(c, launch_render(R, c->stream));
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (samples_out) {
-        unsigned long long t = 0;
-        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        *samples_out = t;
-    } else if (!on_device) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!image) return fail(c, VV_EINVAL, "image is NULL");
+        HIPCHK(c, hipSetDevice(c->device));
+        RenderLaunch R;
+        vv_status st = fill_render_launch(c, cam, R);
+        if (st) return st;
+        const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+        float* dst = image;
+        if (!on_device) {
+            HIPCHK(c, c->image.ensure(bytes));
+            dst = (float*)c->image.p;
+        }
+        R.image = dst;
+        if (samples_out) {
+            HIPCHK(c, c->counter.ensure(64));
+            R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+            HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+        }
+        HIPCHK(c, launch_render(R, c->stream));
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (samples_out) {
+            unsigned long long t = 0;
+            HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *samples_out = t;
+        } else if (!on_device) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return VV_OK;
+    });
 }
 
 // lanes per ray of the on-the-fly casters: the tuning value, or enough to
@@ -696,185 +756,193 @@ static int ray_group_for(const vv_ctx* c, const vv_camera* cam)
 
 vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
-    if (!c) return VV_EINVAL;
-    if (!image) return fail(c, VV_EINVAL, "image is NULL");
-    HIPCHK(c, hipSetDevice(c->device));
-    LicLaunch L;
-    vv_status st = fill_lic_launch(c, L, nullptr);
-    if (st) return st;
-    RenderLaunch R;
-    st = fill_render_launch(c, cam, R, false);
-    if (st) return st;
-    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
-    float* dst = image;
-    if (!on_device) {
-        HIPCHK(c, c->image.ensure(bytes));
-        dst = (float*)c->image.p;
-    }
-    R.image = dst;
-    if (samples_out) {
-        HIPCHK(c, c->counter.ensure(64));
-        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
-        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
-    }
-    const int G = ray_group_for(c, cam);
-    if (G > 1) {
-        SliceSetup S;
-        std::memset(&S, 0, sizeof(S));
-        HIPCHK(c, launch_render_group(R, L, S, 0, G, c->stream));
-    } else {
-        HIPCHK(c, launch_render_fused(R, L, c->stream));
-    }
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (samples_out) {
-        unsigned long long t = 0;
-        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        *samples_out = t;
-    } else if (!on_device) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!image) return fail(c, VV_EINVAL, "image is NULL");
+        HIPCHK(c, hipSetDevice(c->device));
+        LicLaunch L;
+        vv_status st = fill_lic_launch(c, L, nullptr);
+        if (st) return st;
+        RenderLaunch R;
+        st = fill_render_launch(c, cam, R, false);
+        if (st) return st;
+        const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+        float* dst = image;
+        if (!on_device) {
+            HIPCHK(c, c->image.ensure(bytes));
+            dst = (float*)c->image.p;
+        }
+        R.image = dst;
+        if (samples_out) {
+            HIPCHK(c, c->counter.ensure(64));
+            R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+            HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+        }
+        const int G = ray_group_for(c, cam);
+        if (G > 1) {
+            SliceSetup S;
+            std::memset(&S, 0, sizeof(S));
+            HIPCHK(c, launch_render_group(R, L, S, 0, G, c->stream));
+        } else {
+            HIPCHK(c, launch_render_fused(R, L, c->stream));
+        }
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (samples_out) {
+            unsigned long long t = 0;
+            HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *samples_out = t;
+        } else if (!on_device) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return VV_OK;
+    });
 }
 
 vv_status vv_render_slices(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
-    if (!c) return VV_EINVAL;
-    if (!image) return fail(c, VV_EINVAL, "image is NULL");
-    HIPCHK(c, hipSetDevice(c->device));
-    LicLaunch L;
-    vv_status st = fill_lic_launch(c, L, nullptr);
-    if (st) return st;
-    RenderLaunch R;
-    st = fill_render_launch(c, cam, R, false);
-    if (st) return st;
-    // ViewSlicing::setupSlicing (VV/slicing.cpp:42-100): the view vector, the
-    // depth range D of the box along it and n = D / stepSizeVol + 1 slices
-    SliceSetup S;
-    float inv = 1.0f / std::sqrt(R.view.dw[0] * R.view.dw[0] + R.view.dw[1] * R.view.dw[1] +
-                                 R.view.dw[2] * R.view.dw[2]);
-    float half[3], dv = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-        S.v[a] = R.view.dw[a] * inv;
-        if (std::fabs(S.v[a]) < 1.0e-8f) S.v[a] = 0.0f;  // VS_EPS (VV/slicing.cpp:129-134)
-        S.center[a] = c->center[a];
-        half[a] = 0.5f * c->extent[a];
-    }
-    dv = std::fabs(S.v[0]) * half[0] + std::fabs(S.v[1]) * half[1] + std::fabs(S.v[2]) * half[2];
-    S.D = 2.0f * dv;
-    const float nslices = S.D / c->params.step_size_vol;
-    if (!(nslices < 1.0e6f)) return fail(c, VV_EINVAL, "too many slices (%g)", (double)nslices);
-    S.n = (int)nslices + 1;
-    const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
-    float* dst = image;
-    if (!on_device) {
-        HIPCHK(c, c->image.ensure(bytes));
-        dst = (float*)c->image.p;
-    }
-    R.image = dst;
-    if (samples_out) {
-        HIPCHK(c, c->counter.ensure(64));
-        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
-        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
-    }
-    const int G = ray_group_for(c, cam);
-    if (G > 1)
-        HIPCHK(c, launch_render_group(R, L, S, 1, G, c->stream));
-    else
-        HIPCHK(c, launch_render_slices(R, L, S, c->stream));
-    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-    if (samples_out) {
-        unsigned long long t = 0;
-        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        *samples_out = t;
-    } else if (!on_device) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-    }
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!image) return fail(c, VV_EINVAL, "image is NULL");
+        HIPCHK(c, hipSetDevice(c->device));
+        LicLaunch L;
+        vv_status st = fill_lic_launch(c, L, nullptr);
+        if (st) return st;
+        RenderLaunch R;
+        st = fill_render_launch(c, cam, R, false);
+        if (st) return st;
+        // ViewSlicing::setupSlicing (VV/slicing.cpp:42-100): the view vector, the
+        // depth range D of the box along it and n = D / stepSizeVol + 1 slices
+        SliceSetup S;
+        float inv = 1.0f / std::sqrt(R.view.dw[0] * R.view.dw[0] + R.view.dw[1] * R.view.dw[1] +
+                                     R.view.dw[2] * R.view.dw[2]);
+        float half[3], dv = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            S.v[a] = R.view.dw[a] * inv;
+            if (std::fabs(S.v[a]) < 1.0e-8f) S.v[a] = 0.0f;  // VS_EPS (VV/slicing.cpp:129-134)
+            S.center[a] = c->center[a];
+            half[a] = 0.5f * c->extent[a];
+        }
+        dv = std::fabs(S.v[0]) * half[0] + std::fabs(S.v[1]) * half[1] + std::fabs(S.v[2]) * half[2];
+        S.D = 2.0f * dv;
+        const float nslices = S.D / c->params.step_size_vol;
+        if (!(nslices < 1.0e6f)) return fail(c, VV_EINVAL, "too many slices (%g)", (double)nslices);
+        S.n = (int)nslices + 1;
+        const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
+        float* dst = image;
+        if (!on_device) {
+            HIPCHK(c, c->image.ensure(bytes));
+            dst = (float*)c->image.p;
+        }
+        R.image = dst;
+        if (samples_out) {
+            HIPCHK(c, c->counter.ensure(64));
+            R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+            HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+        }
+        const int G = ray_group_for(c, cam);
+        if (G > 1)
+            HIPCHK(c, launch_render_group(R, L, S, 1, G, c->stream));
+        else
+            HIPCHK(c, launch_render_slices(R, L, S, c->stream));
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (samples_out) {
+            unsigned long long t = 0;
+            HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *samples_out = t;
+        } else if (!on_device) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        return VV_OK;
+    });
 }
 
 // ---- differentiation -----------------------------------------------------------
 vv_status vv_set_opacity_scale(vv_ctx* c, const float* sigma, int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!sigma) {
-        c->has_sigma = false;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!sigma) {
+            c->has_sigma = false;
+            return VV_OK;
+        }
+        if (!c->has_lic) return fail(c, VV_ESTATE, "opacity scale lives on the LIC grid: no LIC volume");
+        HIPCHK(c, hipSetDevice(c->device));
+        const long long n = nvox(c->ldims);
+        vv_status st = upload(c, c->stage, sigma, sizeof(float) * (size_t)n, on_device);
+        if (st) return st;
+        HIPCHK(c, c->sigma.ensure(sizeof(float) * (size_t)clamp_elems(c->ldims)));
+        HIPCHK(c, launch_pad_clamp_f32((const float*)c->stage.p, c->ldims, (float*)c->sigma.p, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->has_sigma = true;
         return VV_OK;
-    }
-    if (!c->has_lic) return fail(c, VV_ESTATE, "opacity scale lives on the LIC grid: no LIC volume");
-    HIPCHK(c, hipSetDevice(c->device));
-    const long long n = nvox(c->ldims);
-    vv_status st = upload(c, c->stage, sigma, sizeof(float) * (size_t)n, on_device);
-    if (st) return st;
-    HIPCHK(c, c->sigma.ensure(sizeof(float) * (size_t)clamp_elems(c->ldims)));
-    HIPCHK(c, launch_pad_clamp_f32((const float*)c->stage.p, c->ldims, (float*)c->sigma.p, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->has_sigma = true;
-    return VV_OK;
+    });
 }
 
 vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf, float* dsigma,
                             int on_device)
 {
-    if (!c) return VV_EINVAL;
-    if (!dimage || !dtf) return fail(c, VV_EINVAL, "dimage/dtf NULL");
-    HIPCHK(c, hipSetDevice(c->device));
-    AdjointLaunch A;
-    std::memset(&A, 0, sizeof(A));
-    vv_status st = fill_render_launch(c, cam, A.r);
-    if (st) return st;
-    const long long nrays = (long long)cam->width * cam->height;
-    // samples per ray are bounded by the box diagonal in texture space over
-    // the shortest per-step displacement (|dir| >= min scale)
-    double diag = 0.0, smin = 1e30;
-    for (int a = 0; a < 3; ++a) {
-        diag += (double)A.r.view.tex_max[a] * A.r.view.tex_max[a];
-        if (c->scale[a] < smin) smin = c->scale[a];
-    }
-    diag = std::sqrt(diag);
-    long long geo = (long long)(diag / (smin * (double)c->params.step_size_vol) * 1.01) + 4;
-    long long maxs = A.r.max_samples < geo ? A.r.max_samples : geo;
-    const int max_ckpt = (int)((maxs + 63) / 64);
-    const size_t ck_bytes = sizeof(float) * 4 * (size_t)(2 * max_ckpt + 2 * 64) * (size_t)nrays;
-    HIPCHK(c, c->adj_ckpt.ensure(ck_bytes));
-    const long long nv = nvox(c->ldims);
-    const size_t acc_bytes = sizeof(long long) * (size_t)(1024 + (dsigma ? nv : 0));
-    HIPCHK(c, c->adj_acc.ensure(acc_bytes));
-    const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
-    const float* d_dimg = dimage;
-    if (!on_device) {
-        st = upload(c, c->adj_dimg, dimage, img_bytes, 0);
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!dimage || !dtf) return fail(c, VV_EINVAL, "dimage/dtf NULL");
+        HIPCHK(c, hipSetDevice(c->device));
+        AdjointLaunch A;
+        std::memset(&A, 0, sizeof(A));
+        vv_status st = fill_render_launch(c, cam, A.r);
         if (st) return st;
-        d_dimg = (const float*)c->adj_dimg.p;
-    }
-    long long* acc = (long long*)c->adj_acc.p;
-    HIPCHK(c, launch_fill_i64(acc, 0, (long long)(acc_bytes / 8), c->stream));
-    A.dimage = d_dimg;
-    A.ckpt = (float*)c->adj_ckpt.p;
-    A.max_ckpt = max_ckpt;
-    A.r.max_samples = maxs;
-    A.dtf_fx = acc;
-    A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
-    A.fx_scale = 4294967296.0;  // 2^32
-    A.use_window = c->adj_window;
-    HIPCHK(c, launch_render_adjoint(A, c->stream));
-    const size_t out_bytes = sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0));
-    float* dout = nullptr;
-    if (on_device) {
-        HIPCHK(c, launch_fixed_to_float(acc, dtf, 1024, 1.0 / A.fx_scale, c->stream));
-        if (dsigma) HIPCHK(c, launch_fixed_to_float(acc + 1024, dsigma, nv, 1.0 / A.fx_scale, c->stream));
-    } else {
-        HIPCHK(c, c->adj_out.ensure(out_bytes));
-        dout = (float*)c->adj_out.p;
-        HIPCHK(c, launch_fixed_to_float(acc, dout, (long long)(out_bytes / 4), 1.0 / A.fx_scale, c->stream));
-        HIPCHK(c, hipMemcpyAsync(dtf, dout, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));
-        if (dsigma)
-            HIPCHK(c, hipMemcpyAsync(dsigma, dout + 1024, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost,
-                                     c->stream));
-    }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return VV_OK;
+        const long long nrays = (long long)cam->width * cam->height;
+        // samples per ray are bounded by the box diagonal in texture space over
+        // the shortest per-step displacement (|dir| >= min scale)
+        double diag = 0.0, smin = 1e30;
+        for (int a = 0; a < 3; ++a) {
+            diag += (double)A.r.view.tex_max[a] * A.r.view.tex_max[a];
+            if (c->scale[a] < smin) smin = c->scale[a];
+        }
+        diag = std::sqrt(diag);
+        long long geo = (long long)(diag / (smin * (double)c->params.step_size_vol) * 1.01) + 4;
+        long long maxs = A.r.max_samples < geo ? A.r.max_samples : geo;
+        const int max_ckpt = (int)((maxs + 63) / 64);
+        const size_t ck_bytes = sizeof(float) * 4 * (size_t)(2 * max_ckpt + 2 * 64) * (size_t)nrays;
+        HIPCHK(c, c->adj_ckpt.ensure(ck_bytes));
+        const long long nv = nvox(c->ldims);
+        const size_t acc_bytes = sizeof(long long) * (size_t)(1024 + (dsigma ? nv : 0));
+        HIPCHK(c, c->adj_acc.ensure(acc_bytes));
+        const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
+        const float* d_dimg = dimage;
+        if (!on_device) {
+            st = upload(c, c->adj_dimg, dimage, img_bytes, 0);
+            if (st) return st;
+            d_dimg = (const float*)c->adj_dimg.p;
+        }
+        long long* acc = (long long*)c->adj_acc.p;
+        HIPCHK(c, launch_fill_i64(acc, 0, (long long)(acc_bytes / 8), c->stream));
+        A.dimage = d_dimg;
+        A.ckpt = (float*)c->adj_ckpt.p;
+        A.max_ckpt = max_ckpt;
+        A.r.max_samples = maxs;
+        A.dtf_fx = acc;
+        A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
+        A.fx_scale = 4294967296.0;  // 2^32
+        A.use_window = c->adj_window;
+        HIPCHK(c, launch_render_adjoint(A, c->stream));
+        const size_t out_bytes = sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0));
+        float* dout = nullptr;
+        if (on_device) {
+            HIPCHK(c, launch_fixed_to_float(acc, dtf, 1024, 1.0 / A.fx_scale, c->stream));
+            if (dsigma) HIPCHK(c, launch_fixed_to_float(acc + 1024, dsigma, nv, 1.0 / A.fx_scale, c->stream));
+        } else {
+            HIPCHK(c, c->adj_out.ensure(out_bytes));
+            dout = (float*)c->adj_out.p;
+            HIPCHK(c, launch_fixed_to_float(acc, dout, (long long)(out_bytes / 4), 1.0 / A.fx_scale, c->stream));
+            HIPCHK(c, hipMemcpyAsync(dtf, dout, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));
+            if (dsigma)
+                HIPCHK(c, hipMemcpyAsync(dsigma, dout + 1024, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost,
+                                         c->stream));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
 }
 
 }  // extern "C"

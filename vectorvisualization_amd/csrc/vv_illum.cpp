@@ -105,61 +105,67 @@ void vv_default_line_material(vv_line_material* m)
 
 vv_status vv_make_illum_tables(const vv_line_material* m, float* zoeckler, float* mallo_diff, float* mallo_spec)
 {
-    if (!m || !zoeckler || !mallo_diff || !mallo_spec) return fail(nullptr, VV_EINVAL, "NULL argument");
-    for (int i = 0; i < 4; ++i)
-        if (!std::isfinite(m->light_color[i]) || !std::isfinite(m->ambient[i]) || !std::isfinite(m->diffuse[i]) ||
-            !std::isfinite(m->specular[i]))
-            return fail(nullptr, VV_EINVAL, "line material not finite");
-    if (!(m->spec_exp >= 0.0f && m->spec_exp <= 1024.0f) || !(m->diff_exp >= 0.0f && m->diff_exp <= 1024.0f))
-        return fail(nullptr, VV_EINVAL, "line material exponents must be in [0, 1024]");
-    const double pi = 3.14159265358979323846;
-    const double irx = 1.0 / 255.0, iry = 1.0 / 255.0;
-    for (int y = 0; y < 256; ++y)
-        for (int x = 0; x < 256; ++x) {
-            const size_t i = (size_t)y * 256 + x;
-            const double lt = 2.0 * ((double)x * irx) - 1.0;
-            const double vt = 2.0 * ((double)y * iry) - 1.0;
-            const double dif = std::pow(std::sqrt(1.0 - lt * lt), (double)m->diff_exp);
-            double dot = lt * vt - std::sqrt(1.0 - lt * lt) * std::sqrt(1.0 - vt * vt);
-            dot = dot < -1.0 ? -1.0 : (dot > 1.0 ? 1.0 : dot);
-            // ambient * lightColor is a float product in the reference
-            const float amb = m->ambient[0] * m->light_color[0];
-            const double dz = clamp01d((amb + dif * m->diffuse[0] * m->light_color[0]) * 0.5);
-            const double sz =
-                clamp01d(std::pow(std::fabs(dot), (double)m->spec_exp) * m->specular[0] * m->light_color[0] * 0.5);
-            zoeckler[2 * i] = half_round((float)dz);
-            zoeckler[2 * i + 1] = half_round((float)sz * 0.9f);
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!m || !zoeckler || !mallo_diff || !mallo_spec) return fail(nullptr, VV_EINVAL, "NULL argument");
+        for (int i = 0; i < 4; ++i)
+            if (!std::isfinite(m->light_color[i]) || !std::isfinite(m->ambient[i]) || !std::isfinite(m->diffuse[i]) ||
+                !std::isfinite(m->specular[i]))
+                return fail(nullptr, VV_EINVAL, "line material not finite");
+        if (!(m->spec_exp >= 0.0f && m->spec_exp <= 1024.0f) || !(m->diff_exp >= 0.0f && m->diff_exp <= 1024.0f))
+            return fail(nullptr, VV_EINVAL, "line material exponents must be in [0, 1024]");
+        const double pi = 3.14159265358979323846;
+        const double irx = 1.0 / 255.0, iry = 1.0 / 255.0;
+        for (int y = 0; y < 256; ++y)
+            for (int x = 0; x < 256; ++x) {
+                const size_t i = (size_t)y * 256 + x;
+                const double lt = 2.0 * ((double)x * irx) - 1.0;
+                const double vt = 2.0 * ((double)y * iry) - 1.0;
+                const double dif = std::pow(std::sqrt(1.0 - lt * lt), (double)m->diff_exp);
+                double dot = lt * vt - std::sqrt(1.0 - lt * lt) * std::sqrt(1.0 - vt * vt);
+                dot = dot < -1.0 ? -1.0 : (dot > 1.0 ? 1.0 : dot);
+                // ambient * lightColor is a float product in the reference
+                const float amb = m->ambient[0] * m->light_color[0];
+                const double dz = clamp01d((amb + dif * m->diffuse[0] * m->light_color[0]) * 0.5);
+                const double sz =
+                    clamp01d(std::pow(std::fabs(dot), (double)m->spec_exp) * m->specular[0] * m->light_color[0] * 0.5);
+                zoeckler[2 * i] = half_round((float)dz);
+                zoeckler[2 * i + 1] = half_round((float)sz * 0.9f);
 
-            const double s = ((double)x + 0.5) / 256.0, t = ((double)y + 0.5) / 256.0;
-            const double alpha = std::acos(2.0 * s - 1.0), beta = std::acos(2.0 * t - 1.0);
-            const double l2 = 2.0 * t - 1.0;
-            const double fd = std::sqrt(1.0 - l2 * l2) * (std::sin(alpha) + (pi - alpha) * std::cos(alpha)) * 0.25;
-            const double fs = 3.5 * mallo_spec_term(alpha, beta, (double)m->spec_exp);
-            for (int ch = 0; ch < 3; ++ch) {
-                mallo_diff[3 * i + ch] = half_round((float)clamp01d(fd * m->diffuse[ch] * m->light_color[ch]));
-                mallo_spec[3 * i + ch] = half_round((float)clamp01d(fs * m->specular[ch] * m->light_color[ch]));
+                const double s = ((double)x + 0.5) / 256.0, t = ((double)y + 0.5) / 256.0;
+                const double alpha = std::acos(2.0 * s - 1.0), beta = std::acos(2.0 * t - 1.0);
+                const double l2 = 2.0 * t - 1.0;
+                const double fd = std::sqrt(1.0 - l2 * l2) * (std::sin(alpha) + (pi - alpha) * std::cos(alpha)) * 0.25;
+                const double fs = 3.5 * mallo_spec_term(alpha, beta, (double)m->spec_exp);
+                for (int ch = 0; ch < 3; ++ch) {
+                    mallo_diff[3 * i + ch] = half_round((float)clamp01d(fd * m->diffuse[ch] * m->light_color[ch]));
+                    mallo_spec[3 * i + ch] = half_round((float)clamp01d(fs * m->specular[ch] * m->light_color[ch]));
+                }
             }
-        }
-    return VV_OK;
+        return VV_OK;
+    });
 }
 
 vv_status vv_set_line_material(vv_ctx* c, const vv_line_material* m)
 {
-    if (!c) return VV_EINVAL;
-    if (!m) return fail(c, VV_EINVAL, "material is NULL");
-    std::vector<float> probe(8 * 256 * 256);  // validates the material
-    vv_status s = vv_make_illum_tables(m, probe.data(), probe.data() + 2 * 65536, probe.data() + 5 * 65536);
-    if (s) return fail(c, s, "%s", g_create_error.c_str());
-    c->line_mat = *m;
-    c->illum_dirty = true;
-    return VV_OK;
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!m) return fail(c, VV_EINVAL, "material is NULL");
+        std::vector<float> probe(8 * 256 * 256);  // validates the material
+        vv_status s = vv_make_illum_tables(m, probe.data(), probe.data() + 2 * 65536, probe.data() + 5 * 65536);
+        if (s) return fail(c, s, "%s", g_create_error.c_str());
+        c->line_mat = *m;
+        c->illum_dirty = true;
+        return VV_OK;
+    });
 }
 
 vv_status vv_get_line_material(const vv_ctx* c, vv_line_material* m)
 {
-    if (!c || !m) return VV_EINVAL;
-    *m = c->line_mat;
-    return VV_OK;
+    return guarded(nullptr, [&]() -> vv_status {
+        if (!c || !m) return VV_EINVAL;
+        *m = c->line_mat;
+        return VV_OK;
+    });
 }
 
 }  // extern "C"
