@@ -111,3 +111,19 @@ def test_binding_refuses_short_or_mistyped_buffers():
         _lib._buf(t, "uint8", 12, "noise")
     with pytest.raises(ValueError):
         _lib._buf(np.zeros((4, 4), np.float32)[:, ::2], "float32", 8, "field")  # not contiguous
+
+
+def test_every_entry_point_keeps_exceptions_inside():
+    # the C ABI lets no C++ exception through: every vv_status entry point
+    # runs its body inside guarded(), which maps std::bad_alloc to VV_ENOMEM
+    csrc = Path(__file__).resolve().parent.parent / "vectorvisualization_amd" / "csrc"
+    bare = []
+    n = 0
+    for f in sorted(csrc.glob("*.cpp")):
+        text = f.read_text()
+        for m in re.finditer(r"^vv_status (vv_\w+)\([^;{]*?\)\n\{\n(.*)$", text, re.M):
+            n += 1
+            if not m.group(2).strip().startswith("return guarded("):
+                bare.append(f"{f.name}:{m.group(1)}")
+    assert n >= 60, n  # 69 entry points return vv_status
+    assert not bare, bare
