@@ -6,7 +6,9 @@ density, frequency scale, gradient (RGBA) noise or scalar gate, filter,
 forward and backward step counts, LIC step, camera, image size, sampling step,
 TF index, transfer function and opacity scale, so paths the hand-written cases
 pin one at a time are also met in combination. Eight more draws check the TF
-and opacity-scale adjoint against the FP64 oracle adjoint."""
+and opacity-scale adjoint against the FP64 oracle adjoint. Four random batches
+(time steps of random size and spacing, views of random size and pose) must
+equal single calls bit for bit, before and after autotune."""
 import numpy as np
 import pytest
 
@@ -173,3 +175,67 @@ def test_random_adjoint(ctx, seed):
     e_sg = np.max(np.abs(dsig - rdsig)) / np.max(np.abs(rdsig))
     print(f"adjoint rel err: dTF {e_tf:.2e}  dsigma {e_sg:.2e}", flush=True)
     assert e_tf <= 2e-3 and e_sg <= 2e-3
+
+
+@pytest.mark.parametrize("seed", range(200, 204))
+def test_random_batch_matches_single(ctx, seed):
+    # random time steps (dims, spacing) and views (size, orbit pose) of one
+    # storage: every LIC volume and image of the batch equals the single-call
+    # result bit for bit, before and after autotune changes the tiles
+    import torch
+
+    rng = np.random.default_rng(seed)
+    storage = int(rng.choice([vv.STORE_FP32, vv.STORE_FP16, vv.STORE_UNORM8]))
+    L = int(rng.integers(2, 16))
+    lut, inv = vv.make_filter(int(rng.integers(0, 4)), L)
+    ctx.set_filter(lut, inv)
+    p = vv.default_lic_params()
+    p.steps_forward, p.steps_backward = L, L
+    p.step_size_lic = float(np.float32(rng.uniform(0.004, 0.03)))
+    p.step_size_vol = float(rng.choice([1.0 / 64.0, 1.0 / 128.0]))
+    p.gradient_scale = 3.0
+    p.tf_index = vv.TF_INDEX_MAG
+    ctx.set_lic_params(p)
+    ctx.set_tf(*fields.tf_magnitude(0.05, 0.3))
+    steps = []
+    for i in range(int(rng.integers(2, 5))):
+        dims = tuple(int(x) for x in rng.integers(8, 37, 3))
+        sp = tuple(float(x) for x in rng.choice([0.5, 1.0, 2.0], 3))
+        v = rng.normal(size=(dims[2], dims[1], dims[0], 3)).astype(np.float32)
+        steps.append((dims, sp, v, int(rng.integers(1, 1000))))
+    views = [(int(rng.integers(0, len(steps))), int(rng.integers(16, 97)), int(rng.integers(16, 97)),
+              float(rng.uniform(-3.1, 3.1)), float(rng.uniform(-1.0, 1.0))) for _ in range(int(rng.integers(3, 7)))]
+    print(f"batch case {seed}: storage {storage}, L {L}, steps {[s[:2] for s in steps]}, views {views}", flush=True)
+
+    lics, centers, imgs, total = [], [], [None] * len(views), 0
+    for i, (dims, sp, v, nseed) in enumerate(steps):
+        ctx.upload_field(v, dims, spacing=sp, storage=storage)
+        ctx.generate_noise(nseed, vv.NOISE_UNIFORM, 0.0, dims)
+        ctx.compute_lic()
+        lics.append(ctx.download_lic())
+        centers.append(ctx.field_geometry()[2])
+        for k, (t, w, h, yaw, pitch) in enumerate(views):
+            if t == i:
+                imgs[k], n = ctx.render(fields.orbit_camera(w, h, centers[i], yaw, pitch), count_samples=True)
+                total += n
+    b = vv.Batch(ctx)
+    for dims, sp, v, nseed in steps:
+        b.add_timestep(v, dims, spacing=sp, storage=storage, noise_seed=nseed, noise_kind=vv.NOISE_UNIFORM)
+    outs = []
+    for t, w, h, yaw, pitch in views:
+        img = torch.zeros(w * h * 4, dtype=torch.float32, device="cuda")
+        b.add_view(t, fields.orbit_camera(w, h, centers[t], yaw, pitch), img)
+        outs.append(img)
+    for tuned in (False, True):
+        if tuned:
+            b.autotune()
+            for o in outs:
+                o.zero_()
+        b.run()
+        torch.cuda.synchronize()
+        for i, (dims, _, _, _) in enumerate(steps):
+            np.testing.assert_array_equal(b.download_lic(i, dims), lics[i])
+        for k, img in enumerate(outs):
+            np.testing.assert_array_equal(img.cpu().numpy(), imgs[k])
+        assert b.samples() == total
+    b.close()
