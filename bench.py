@@ -82,8 +82,8 @@ MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+        print(json.dumps({"error": "the engine runs on one GPU; --gpus must be 1 "
+                          "(z-slab sharding of the LIC and image-tile sharding of the ray cast are not run)"}))
         sys.exit(2)
 
     import numpy as np
