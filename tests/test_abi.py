@@ -127,3 +127,52 @@ def test_every_entry_point_keeps_exceptions_inside():
                 bare.append(f"{f.name}:{m.group(1)}")
     assert n >= 60, n  # 69 entry points return vv_status
     assert not bare, bare
+
+
+class _FakeDeviceTensor:
+    """Stands in for a CUDA tensor on a machine without a GPU: the binding
+    reads only these attributes before it reaches the C ABI."""
+
+    is_cuda = True
+    dtype = "torch.float32"
+
+    def __init__(self, n):
+        self._n = n
+
+    def numel(self):
+        return self._n
+
+    def data_ptr(self):
+        return 0x1000
+
+    def is_contiguous(self):
+        return True
+
+
+class _NoLib:
+    def __getattr__(self, name):
+        raise AssertionError(f"{name} reached the C ABI with mixed host and device buffers")
+
+
+def test_adjoint_and_batch_refuse_mixed_host_and_device_buffers(monkeypatch):
+    # one on_device flag covers every buffer of vv_render_adjoint and
+    # vv_batch_set_field: a host buffer beside a device one must be refused
+    # before any library call
+    monkeypatch.setattr(_lib, "LIB", _NoLib())
+    ctx = object.__new__(_lib.Context)
+    ctx._h = None
+    ctx._lic_voxels = lambda: 8
+    cam = _lib.Camera()
+    cam.width, cam.height = 2, 2
+    dev_img, host_img = _FakeDeviceTensor(16), np.zeros(16, np.float32)
+    for dimage, dtf, dsigma in [(dev_img, np.zeros(1024, np.float32), None),
+                                (dev_img, _FakeDeviceTensor(1024), np.zeros(8, np.float32)),
+                                (host_img, _FakeDeviceTensor(1024), None)]:
+        with pytest.raises(ValueError, match="all be host memory or all device memory"):
+            ctx.render_adjoint(cam, dimage, dtf, dsigma)
+    b = object.__new__(_lib.Batch)
+    b._ctx, b._dims, b._h = ctx, [(2, 2, 2)], None
+    with pytest.raises(ValueError, match="v0 on device, v1 on host"):
+        b.set_field(0, _FakeDeviceTensor(24), np.zeros(24, np.float32), 0.5)
+    with pytest.raises(ValueError, match="v0 on host, v1 on device"):
+        b.set_field(0, np.zeros(24, np.float32), _FakeDeviceTensor(24), 0.5)

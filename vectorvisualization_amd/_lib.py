@@ -266,6 +266,18 @@ def _buf(a, dtype: str, count: int, what: str) -> tuple[int, int]:
     return _ptr(a)
 
 
+def _one_side(named, what: str) -> None:
+    """Refuse a call whose buffers mix host memory and device memory.  The C
+    entry points take one on_device flag for all their buffers, so a host
+    buffer passed beside a device one would be dereferenced on the wrong side
+    (a GPU page fault or a host segfault).  `named` holds (name, buffer)
+    pairs; None buffers are skipped."""
+    sides = [(n, _ptr(b)[1]) for n, b in named if b is not None]
+    if len({d for _, d in sides}) > 1:
+        desc = ", ".join(f"{n} on {'device' if d else 'host'}" for n, d in sides)
+        raise ValueError(f"{what}: buffers must all be host memory or all device memory ({desc})")
+
+
 def _nvox(dims) -> int:
     return int(dims[0]) * int(dims[1]) * int(dims[2])
 
@@ -699,6 +711,7 @@ class Context:
             self._chk(LIB.vv_set_opacity_scale(self._h, p, dev))
 
     def render_adjoint(self, cam: Camera, dimage, dtf=None, dsigma=None, want_sigma: bool = True):
+        _one_side([("dimage", dimage), ("dtf", dtf), ("dsigma", dsigma)], "render_adjoint")
         host = isinstance(dimage, np.ndarray)
         if host:
             dimage = np.ascontiguousarray(dimage, np.float32)
@@ -785,6 +798,7 @@ class Batch:
 
     def set_field(self, timestep: int, v0, v1=None, lerp_t: float = 0.0) -> None:
         n = 3 * _nvox(self._dims[timestep]) if 0 <= timestep < len(self._dims) else 0
+        _one_side([("v0", v0), ("v1", v1)], "Batch.set_field")
         p0, dev = _buf(v0, "float32", n, "time step field")
         p1 = _buf(v1, "float32", n, "second time step field")[0] if v1 is not None else None
         with self._ctx.ordered(v0, v1):
