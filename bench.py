@@ -65,18 +65,80 @@ def _profile_value(name: str, key: str):
     for line in p.read_text().splitlines():
         if line.startswith(key + ":"):
             try:
-                return float(line.split(":", 1)[1].split()[0])
+                return float(line.split(":", 1)[1].split()[0].rstrip(","))
             except (ValueError, IndexError):
                 return None
     return None
 
 
-def load_traffic(name: str):
-    """HBM bytes per LIC launch from the committed rocprof PMC summary, if any."""
-    return _profile_value(name, "traffic_bytes_per_launch")
-
-
 MI355X_CUS, MI355X_CLOCK_HZ = 256, 2.4e9
+FP32_PEAK_TFLOPS = 157.3  # published MI355X FP32 figure (counts packed FMA)
+
+
+def storage_kernel(vv, storage) -> str:
+    return {vv.STORE_FP32: "k_lic_cached<float4>", vv.STORE_FP16: "k_lic_cached<half4_t>",
+            vv.STORE_UNORM8: "k_lic_cached<u8x4_t>"}[storage]
+
+
+def roofline(tag: str, kernel: str, ms: float, fetch_bytes: float) -> dict:
+    """Roofline of the LIC kernel from its launch time `ms` (HIP events on the
+    library's stream, in this run) and the per-launch counts of the committed
+    rocprof counter summary of the same kernel and workload
+    (profiles/<tag>_pmc.txt).
+
+    The bound that holds is FP32 VALU issue: the fma/add/mul wave64
+    instructions of a launch (SQ_INSTS_VALU_{FMA,ADD,MUL}_F32) issue at most
+    once per clock per CU (the 157.3 TFLOP/s FP32 figure is 256 CUs x 2.4 GHz
+    x 64 lanes x 2 flops x 2 for packed FP32, which this build does not
+    emit), so `frac` = instructions / (launch clocks x 256 CUs).  Beside it:
+    the FP32 flop rate against 157.3 TFLOP/s, HBM bytes (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy of known size) against 8 TB/s, L1
+    cache-line accesses per clock per CU, L2 request bandwidth, and the
+    texture-fetch bytes the reference's shader requests: those are served by
+    the per-lane cell caches, L1 and L2, so they are a gather volume, not an
+    HBM fraction."""
+    name = f"{tag}_pmc.txt"
+    sec = ms / 1000.0
+    clocks = sec * MI355X_CUS * MI355X_CLOCK_HZ
+    valu = _profile_value(name, "valu_f32_insts_per_launch")
+    fma = _profile_value(name, "SQ_INSTS_VALU_FMA_F32 per launch")
+    add = _profile_value(name, "SQ_INSTS_VALU_ADD_F32 per launch")
+    mul = _profile_value(name, "SQ_INSTS_VALU_MUL_F32 per launch")
+    traffic = _profile_value(name, "traffic_bytes_per_launch")
+    l1 = _profile_value(name, "TCP_TOTAL_CACHE_ACCESSES per launch")
+    l2 = _profile_value(name, "TCC requests per launch")
+    rate = None if valu is None else valu / clocks
+    r = {
+        "bound": "valu_f32",
+        "achieved": rate,
+        "peak": 1.0,
+        "unit": "wave64 FP32 fma/add/mul instructions per clock per CU",
+        "frac": None if rate is None else min(rate, 1.0),
+        "traffic": traffic,
+        "kernel": kernel,
+        "launch_ms": ms,
+        "counts_from": f"profiles/{name}",
+    }
+    if None not in (fma, add, mul):
+        tflops = (2.0 * fma + add + mul) * 64.0 / sec / 1e12
+        r["flops"] = {"achieved": tflops, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": tflops / FP32_PEAK_TFLOPS}
+    if traffic is not None:
+        gbs = traffic / sec / 1e9
+        r["hbm"] = {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                    "bytes_per_launch": traffic}
+    if l1 is not None:
+        r["l1_gather"] = {"accesses_per_launch": l1, "achieved": l1 / clocks, "peak": 1.0,
+                          "unit": "cache-line accesses per clock per CU", "frac": min(l1 / clocks, 1.0)}
+    if l2 is not None:
+        l2_gbs = l2 * 128.0 / sec / 1e9
+        l2_peak = 8 * 16 * 128.0 * MI355X_CLOCK_HZ / 1e9
+        r["l2"] = {"requests_per_launch": l2, "achieved": l2_gbs, "peak": l2_peak, "unit": "GB/s",
+                   "frac": l2_gbs / l2_peak, "peak_basis": "16 channels x 128 B/clk per XCD, 8 XCDs"}
+    r["texture_fetch"] = {"bytes_per_launch": fetch_bytes, "rate_gbs": fetch_bytes / sec / 1e9,
+                          "served_by": "per-lane cell caches, L1 and L2 (not HBM)"}
+    return r
+
 
 
 def main():
@@ -175,10 +237,7 @@ def main():
     value = samples / (step_ms / 1000.0)
     # bytes of the three direction channels of one texel
     bpv = lic_bytes_per_voxel(L, L, {vv.STORE_FP32: 12, vv.STORE_FP16: 6, vv.STORE_UNORM8: 3}[storage])
-    lic_bytes = bpv * vox
-    achieved = lic_bytes / (lic_ms / 1000.0) / 1e9
     tag = f"lic_rankine{n}_gauss{L}_{args.storage}"
-    traffic = load_traffic(f"{tag}_pmc.txt")
 
     out = {
         "metric": "LIC voxel-samples/s + ms/frame (256^3, 1024^2 view); fwd+adjoint vs plain GPU",
@@ -207,45 +266,8 @@ def main():
         "lic_voxel_samples_per_s": samples / (lic_ms / 1000.0),
         "ray_samples_per_frame": int(ray_samples),
         "ray_samples_per_s": ray_samples / (frame_ms / 1000.0),
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": {vv.STORE_FP32: "k_lic_cached<float4>", vv.STORE_FP16: "k_lic_cached<half4_t>",
-                       vv.STORE_UNORM8: "k_lic_cached<u8x4_t>"}[storage],
-            "algorithmic_bytes_per_voxel": bpv,
-        },
+        "roofline": roofline(tag, storage_kernel(vv, storage), lic_ms, bpv * vox),
     }
-
-    # the bound that applies: L1 cache-line accesses per clock per CU (at most
-    # about one), from the committed counter summary of the same kernel
-    l1 = _profile_value(f"{tag}_pmc.txt", "TCP_TOTAL_CACHE_ACCESSES per launch")
-    if l1 is not None:
-        rate = l1 / (lic_ms / 1000.0) / (MI355X_CUS * MI355X_CLOCK_HZ)
-        out["roofline"]["l1_gather"] = {"accesses_per_launch": l1, "per_clock_per_cu": rate, "peak": 1.0,
-                                        "frac": rate, "hbm_traffic_gbs": (traffic or 0) / (lic_ms / 1000.0) / 1e9}
-    # L2 (TCC) requests of the same kernel at 128 B per line, against 16
-    # channels x 128 B per clock per XCD (an assumed CDNA L2 figure)
-    l2 = _profile_value(f"{tag}_pmc.txt", "TCC requests per launch")
-    if l2 is not None:
-        l2_gbs = l2 * 128.0 / (lic_ms / 1000.0) / 1e9
-        l2_peak = 8 * 16 * 128.0 * MI355X_CLOCK_HZ / 1e9
-        out["roofline"]["l2"] = {"requests_per_launch": l2, "achieved": l2_gbs, "peak": l2_peak, "unit": "GB/s",
-                                 "frac": l2_gbs / l2_peak, "peak_basis": "16 channels x 128 B/clk per XCD, 8 XCDs"}
-
-    # the bound that holds the LIC kernel: FP32 fma/add/sub/mul wave64
-    # instructions (SQ_INSTS_VALU_{FMA,ADD,MUL}_F32 of the same kernel), which
-    # issue once per clock per CU (157.3 TFLOP/s FP32 counts packed FMAs)
-    valu = _profile_value(f"{tag}_pmc.txt", "valu_f32_insts_per_launch")
-    if valu is not None:
-        rate = valu / (lic_ms / 1000.0) / (MI355X_CUS * MI355X_CLOCK_HZ)
-        out["roofline"]["valu_f32"] = {"insts_per_launch": valu, "per_clock_per_cu": rate, "peak": 1.0,
-                                       "frac": rate,
-                                       "peak_basis": "one unpacked wave64 FP32 fma/add/mul per clock per CU "
-                                                     "(157.3 TFLOP/s = 256 CUs x 2.4 GHz x 64 lanes x 2 packed x 2)"}
 
     if not args.no_extras:
         def timed(fn, reps=3):
@@ -273,21 +295,24 @@ def main():
             a_ms = timed(lambda: ctx.render_adjoint(cam, dimg, dtf, dsig))
             legs[name] = dict(lic_ms=l_ms, frame_ms=f_ms, adjoint_ms=a_ms,
                               pipeline_ms=l_ms + f_ms + a_ms)
-        # the straightforward FP32 kernel: no per-lane cell caches
+        # the one-lane-per-voxel FP32 kernel without cell caches (information
+        # only: the library runs the cached kernel for this workload)
         ctx.upload_field(field, dims, storage=vv.STORE_FP32)
         ctx.set_tuning("lic.cache", 0)
-        plain_lic = timed(ctx.compute_lic)
+        uncached_lic = timed(ctx.compute_lic)
         ctx.set_tuning("lic.cache", 1)
-        plain = plain_lic + legs["fp32"]["frame_ms"] + legs["fp32"]["adjoint_ms"]
         out["pipeline"] = {
-            "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms)",
+            "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms); speedups against the "
+                    "FP32-stored path the library runs",
             "fp32": legs["fp32"],
             "fp16": legs["fp16"],
             "unorm8": legs["unorm8"],
-            "fp32_plain_lic_ms": plain_lic,
             "fp16_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["fp16"]["pipeline_ms"],
-            "fp16_speedup_vs_fp32_plain_kernel": plain / legs["fp16"]["pipeline_ms"],
+            "fp16_lic_speedup_vs_fp32": legs["fp32"]["lic_ms"] / legs["fp16"]["lic_ms"],
             "unorm8_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["unorm8"]["pipeline_ms"],
+            "fwd_plus_adjoint_over_frame": {k: (legs[k]["frame_ms"] + legs[k]["adjoint_ms"]) / legs[k]["frame_ms"]
+                                            for k in legs},
+            "fp32_uncached_lic_ms": uncached_lic,
         }
         out["fwd_plus_adjoint_ms"] = legs[args.storage]["frame_ms"] + legs[args.storage]["adjoint_ms"]
         # the fused caster (lic3d_fragment.glsl): LIC of every ray sample on the fly
@@ -767,39 +792,71 @@ def tf_fit_leg(dev, field, L, W=1024, steps=200, seed_target=3, seed_start=5):
     }
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(field_np, dims, L, lut, inv, p, seconds, samples_per_voxel):
-    """The oracle's FP32 LIC on a z-slab of the same workload, all host threads."""
+    """The oracle's FP32 LIC (its -O3 AVX2/FMA build, bit-identical to the
+    checker) on a central z-slab of the same workload: one warm-up run, then
+    the median of 5 timed runs, on all the threads this process may use
+    (the CPU share of one GPU: the affinity mask, limited by OMP_NUM_THREADS)
+    and on 1 thread.  The slab is sized so that the 12 runs take about
+    `seconds` in all."""
     import numpy as np
 
     from oracle import oracle as orc
 
     n = dims[0]
     try:
-        cores = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
-    print(f"[bench] cpu baseline on {cores} threads", file=sys.stderr, flush=True)
+        avail = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    cores = max(1, min(avail, int(omp))) if omp.isdigit() else avail
     f4 = orc.encode_field(field_np, dims)
     noise = orc.gen_noise(1, 1, 0.0, dims)
     op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=0.005, inv_filter_area=inv,
                     gradient_scale=3.0)
-    z0 = 0
-    t = time.perf_counter()
-    orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + 1, nthreads=cores)
-    one = time.perf_counter() - t
-    k = int(max(1, min(n - z0, round(seconds / max(one, 1e-3)))))
-    t = time.perf_counter()
-    orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + k, nthreads=cores)
-    dt = time.perf_counter() - t
-    vox = k * n * n
+    per_run = seconds / 12.0
+
+    def run(threads, k):
+        z0 = max(0, n // 2 - k // 2)
+        t = time.perf_counter()
+        orc.lic(f4, dims, noise, 1, dims, lut, op, z0=z0, z1=z0 + k, nthreads=threads, fast=True)
+        return time.perf_counter() - t, z0
+
+    res = {}
+    for threads in (cores, 1):
+        print(f"[bench] cpu baseline on {threads} thread(s)", file=sys.stderr, flush=True)
+        one, _ = run(threads, 1)  # warm-up, and the slab size
+        k = int(max(1, min(n, round(per_run / max(one, 1e-4)))))
+        times = []
+        for _ in range(5):
+            dt, z0 = run(threads, k)
+            times.append(dt)
+        med = float(np.median(times))
+        vox = k * n * n
+        res[threads] = {"value": vox * samples_per_voxel / med, "median_s": med, "runs_s": times,
+                        "slab": f"z {z0}..{z0 + k - 1} ({vox} voxels)"}
+    allc = res[cores]
     return {
-        "value": vox * samples_per_voxel / dt,
+        "value": allc["value"],
         "unit": "voxel-samples/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"oracle FP32 LIC of z-slices {z0}..{z0 + k - 1} ({vox} voxels) of the same "
-                  f"{n}^3 workload, {dt:.1f} s",
+        "sample": f"oracle FP32 LIC (-O3 x86-64-v3 build) of the central z-slab {allc['slab']} of the same "
+                  f"{n}^3 workload; median of 5 runs after a warm-up",
+        "cpu_model": _cpu_model(),
+        "host_threads_available": avail,
+        "one_core": {"value": res[1]["value"], "median_s": res[1]["median_s"], "slab": res[1]["slab"]},
+        "all_cores_runs_s": allc["runs_s"],
     }
 
 

@@ -40,7 +40,7 @@
 extern "C" {
 #endif
 
-#define VVLIC_ABI_VERSION 1
+#define VVLIC_ABI_VERSION 2
 
 typedef enum vv_status {
     VV_OK = 0,
@@ -109,6 +109,10 @@ typedef struct vv_lic_params {
     int   gate_enable;     /* scalar gate of freqSampling (needs vv_upload_scalar) */
     float gate_lo;         /* 0.1 */
     float gate_hi;         /* 0.3 */
+    int   speed_of_flow;   /* SPEED_OF_FLOW (inc_lic.glsl:107-110,120-122, key '6' in
+                              VV/3DLIC.cpp:429-432): both Heun displacements of a
+                              step scale with the field magnitude |v|/max at the
+                              step's start point.  Default 0 */
 } vv_lic_params;
 
 /* Mirrors Camera (VV/camera.cpp:42-68) and the light set by
@@ -261,8 +265,15 @@ vv_status vv_render_slices(vv_ctx* ctx, const vv_camera* cam, float* image, int 
 vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 /* Gradient of a loss L w.r.t. the 256x4 RGBA transfer function and the
  * opacity scale, given dL/dimage (width*height*4).  dtf: 256*4 floats,
- * dsigma: N floats or NULL.  Checkpoints every 64 samples; reductions are
- * deterministic (fixed-point integer accumulation). */
+ * dsigma: N floats or NULL.  Checkpoints every 64 samples.  Reductions are
+ * deterministic: contributions are rounded to a fixed point 2^-k and summed
+ * as 64-bit integers.  k is chosen per call on the device from a bound on
+ * every contribution (from max |dL/dimage| and the forward pass) and the
+ * number of ray samples, so that no sum can overflow; scaling dL/dimage by
+ * a power of two scales every gradient by exactly that power.  A non-finite
+ * dL/dimage entry on a pixel the volume covers makes every gradient NaN.
+ * The reference has no adjoint; this differentiates the compositing loop
+ * of raycast_lic3d_fragment.glsl:35-69. */
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
@@ -273,9 +284,13 @@ vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dima
  * each into a caller-owned device image (width*height*4 float32).
  * vv_batch_run computes every LIC volume (one block per tile) and every view
  * (a persistent kernel over pixel tiles), launches captured once into a
- * hipGraph and replayed on later runs; it
- * returns without waiting.  Replaces the reference's per-frame re-encode +
- * LIC recompute of unsteady data (VV/3DLIC.cpp:129-172). */
+ * hipGraph and replayed on later runs.  When the views, LIC parameters,
+ * filter and transfer function are those of its previous run it only
+ * enqueues the graph and returns without waiting; otherwise it first waits
+ * for the context's stream (the previous run reads the device descriptors),
+ * uploads the new descriptors and waits for that upload.  Replaces the
+ * reference's per-frame re-encode + LIC recompute of unsteady data
+ * (VV/3DLIC.cpp:129-172). */
 typedef struct vv_batch vv_batch;
 vv_status vv_batch_create(vv_ctx* ctx, vv_batch** out);
 vv_status vv_batch_destroy(vv_batch* b);

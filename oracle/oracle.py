@@ -15,6 +15,7 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
+FAST_LIB_PATH = HERE / "liboracle_fast.so"
 
 
 def build() -> Path:
@@ -53,6 +54,7 @@ class Params(C.Structure):
         ("lut_mallo_diff", C.c_void_p),
         ("lut_mallo_spec", C.c_void_p),
         ("field_unorm8", C.c_int),
+        ("speed_of_flow", C.c_int),
     ]
 
 
@@ -110,10 +112,10 @@ class View(C.Structure):
         return np.array(vals, np.float32)
 
 
-def _load() -> C.CDLL:
-    if not LIB_PATH.exists():
+def _load(path: Path = LIB_PATH) -> C.CDLL:
+    if not path.exists():
         build()
-    lib = C.CDLL(str(LIB_PATH))
+    lib = C.CDLL(str(path))
     P, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
     L = C.c_long
     sig = {
@@ -167,6 +169,15 @@ def _load() -> C.CDLL:
 
 
 LIB = _load()
+_FAST = None
+
+
+def fast_lib() -> C.CDLL:
+    """The -O3 AVX2/FMA build of the same source (the timed CPU baseline)."""
+    global _FAST
+    if _FAST is None:
+        _FAST = _load(FAST_LIB_PATH)
+    return _FAST
 
 
 def _p(a: np.ndarray | None):
@@ -331,8 +342,9 @@ def lic_weights(lut: np.ndarray, F: int, B: int, f64=False) -> np.ndarray:
 
 def lic(field4: np.ndarray, dims, noise: np.ndarray, noise_ch: int, ndims, lut: np.ndarray,
         p: Params, scale=(1, 1, 1), scalar=None, sdims=None, z0=0, z1=None, nthreads=0,
-        f64=None) -> np.ndarray:
-    """LIC volume (slab z0..z1) of an encoded field; precision follows field4's dtype."""
+        f64=None, fast=False) -> np.ndarray:
+    """LIC volume (slab z0..z1) of an encoded field; precision follows field4's dtype.
+    fast: run the -O3 AVX2/FMA build (same results)."""
     nx, ny, nz = dims
     if f64 is None:
         f64 = field4.dtype == np.float64
@@ -345,7 +357,8 @@ def lic(field4: np.ndarray, dims, noise: np.ndarray, noise_ch: int, ndims, lut: 
     nd = _i3(ndims)
     sd = _i3(sdims) if sdims is not None else None
     scalar = None if scalar is None else np.ascontiguousarray(scalar, np.uint8)
-    fn = LIB.orc_lic_f64 if f64 else LIB.orc_lic_f32
+    lib = fast_lib() if fast else LIB
+    fn = lib.orc_lic_f64 if f64 else lib.orc_lic_f32
     fn(_p(field4), nx, ny, nz, _p(sc), _p(noise), noise_ch, _p(nd), _p(scalar), _p(sd), _p(lut),
        lut.size, C.byref(p), z0, z1, nthreads, _p(out))
     return out

@@ -63,6 +63,9 @@ typedef struct orc_params {
     /* 1: the field array holds UNORM8 byte values (0..255, orc_encode_field_u8);
      * blended field values are scaled by 1/255 like every UNORM8 texture */
     int   field_unorm8;
+    /* SPEED_OF_FLOW (inc_lic.glsl:107-110,120-122): both displacements of a
+     * Heun step scaled by the field's .a at the step's start point */
+    int   speed_of_flow;
 } orc_params;
 
 /* Line material of the illumination tables (LineMat, VV/illumination.h:40-62). */

@@ -227,13 +227,17 @@ typedef struct {
     const orc_params* p;
 } FN(lic_ctx);
 
-static inline void FN(field_rgb)(const FN(lic_ctx)* c, const R x[3], R s[3])
+/* the field's direction (s[0..2]) and, with SPEED_OF_FLOW, its magnitude
+ * .a (s[3]; 1 otherwise) */
+static inline void FN(field_rgb)(const FN(lic_ctx)* c, const R x[3], R s[4])
 {
     FN(cell) cl;
     FN(cell_clamp)(x, c->fd, &cl);
     ORC_FIELD_TRILIN(c->p, c->field, 0, c->fd, cl, s[0]);
     ORC_FIELD_TRILIN(c->p, c->field, 1, c->fd, cl, s[1]);
     ORC_FIELD_TRILIN(c->p, c->field, 2, c->fd, cl, s[2]);
+    s[3] = K(1);
+    if (c->p->speed_of_flow) ORC_FIELD_TRILIN(c->p, c->field, 3, c->fd, cl, s[3]);
 }
 
 /* freqSampling (inc_lic.glsl:71-90) / freqSamplingGrad (inc_lic.glsl:61-68) */
@@ -269,7 +273,8 @@ static void FN(lic_voxel)(const FN(lic_ctx)* c, const R* w, const R seed[3], R o
     const orc_params* p = c->p;
     const int nc = c->nc;
     const R he = (R)p->step_size_lic * (R)p->eye_factor;
-    R s0[3], n[4];
+    const int speed = p->speed_of_flow;
+    R s0[4], n[4];
     R acc[4] = {0, 0, 0, 0};
     FN(field_rgb)(c, seed, s0);
     FN(noise_at)(c, seed, n);
@@ -279,16 +284,20 @@ static void FN(lic_voxel)(const FN(lic_ctx)* c, const R* w, const R seed[3], R o
         const R h = dirn == 0 ? -he : he;
         const R* wk = dirn == 0 ? w + 1 : w + 1 + p->steps_backward;
         R x[3] = {seed[0], seed[1], seed[2]};
-        R s[3] = {s0[0], s0[1], s0[2]};
+        R s[4] = {s0[0], s0[1], s0[2], s0[3]};
         for (int i = 0; i < steps; ++i) {
-            R d1[3], x2[3], s2[3], d2[3];
+            R d1[3], x2[3], s2[4], d2[3];
             for (int a = 0; a < 3; ++a) {
-                d1[a] = (K(2) * s[a] - K(1)) * h;          /* inc_lic.glsl:114 */
+                R t = K(2) * s[a] - K(1);
+                if (speed) t = t * s[3];                     /* :107-110 licdir *= step.a */
+                d1[a] = t * h;                               /* :114 */
                 x2[a] = x[a] + d1[a];                        /* :115 */
             }
             FN(field_rgb)(c, x2, s2);                        /* :116 */
             for (int a = 0; a < 3; ++a) {
-                d2[a] = (K(2) * s2[a] - K(1)) * h;          /* :117-123 */
+                R t = K(2) * s2[a] - K(1);
+                if (speed) t = t * s[3];                     /* :120-122 licdir2 *= step.a (old step) */
+                d2[a] = t * h;                               /* :123 */
                 x[a] = x[a] + K(0.5) * (d1[a] + d2[a]);      /* :125 */
             }
             FN(field_rgb)(c, x, s);                          /* :128 */

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version():
-    assert vv.LIB.vv_abi_version() == 1
+    assert vv.LIB.vv_abi_version() == 2  # vv_lic_params gained speed_of_flow
 
 
 def test_default_params_mirror_licparams():

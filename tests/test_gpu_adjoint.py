@@ -178,3 +178,50 @@ def test_adam_fits_transfer_function(ctx):
         losses.append(loss.item())
     print(f"Adam 200 steps: loss {losses[0]:.4g} -> {losses[-1]:.4g}", flush=True)
     assert losses[-1] < 0.05 * losses[0]
+
+
+@pytest.mark.parametrize("factor", [1.0 / (48 * 40 * 4), 1e6])
+def test_adjoint_of_mean_and_scaled_losses_matches_oracle(ctx, factor):
+    """dL/dimage of a .mean() loss (entries ~1e-4 here, ~1e-7 at 1024^2) and
+    of a loss scaled by 1e6: the fixed-point scale follows the gradient's
+    magnitude, so both keep the accuracy of the unscaled case."""
+    c, cam, rgba, op, sigma, oparams, view = _scene(ctx, sigma_seed=7)
+    W, H = cam.width, cam.height
+    dimg = (np.random.default_rng(11).normal(size=W * H * 4) * factor).astype(np.float32)
+    dtf, dsig = ctx.render_adjoint(cam, dimg)
+    ldims, lch = ctx.lic_info()
+    rdtf, rdsig = orc.raycast_adjoint(c["f4"].astype(np.float64), c["dims"], ctx.download_lic(), lch, ldims,
+                                      rgba, op, oparams, view, dimg, sigma=sigma)
+    e_tf = np.max(np.abs(dtf - rdtf)) / np.max(np.abs(rdtf))
+    e_sg = np.max(np.abs(dsig - rdsig)) / np.max(np.abs(rdsig))
+    print(f"factor {factor:.3g}: dTF {e_tf:.2e}  dsigma {e_sg:.2e}", flush=True)
+    assert np.max(np.abs(rdtf)) > 0 and np.max(np.abs(rdsig)) > 0
+    assert e_tf <= 2e-3 and e_sg <= 2e-3
+
+
+def test_adjoint_power_of_two_scaling_is_exact(ctx):
+    """Multiplying dL/dimage by 2^e multiplies every gradient by exactly 2^e:
+    the scale exponent moves by -e and the integer sums do not change."""
+    c, cam, rgba, op, sigma, oparams, view = _scene(ctx, n=20, W=64, H=56, sigma_seed=3)
+    dimg = np.random.default_rng(2).normal(size=64 * 56 * 4).astype(np.float32)
+    base_tf, base_sg = ctx.render_adjoint(cam, dimg)
+    assert np.max(np.abs(base_tf)) > 0
+    for e in (20, -40, 60):
+        s = np.float32(2.0 ** e)
+        tf_e, sg_e = ctx.render_adjoint(cam, dimg * s)
+        np.testing.assert_array_equal(tf_e, base_tf * s)
+        np.testing.assert_array_equal(sg_e, base_sg * s)
+
+
+def test_adjoint_of_zero_and_nonfinite_image_gradients(ctx):
+    c, cam, rgba, op, sigma, oparams, view = _scene(ctx, sigma_seed=7)
+    n = cam.width * cam.height * 4
+    dtf, dsig = ctx.render_adjoint(cam, np.zeros(n, np.float32))
+    assert not np.any(dtf) and not np.any(dsig)
+    g = np.random.default_rng(1).normal(size=n).astype(np.float32)
+    # a non-finite entry on a pixel the box covers makes every gradient NaN
+    img = ctx.render(cam).reshape(-1, 4)
+    hit = int(np.flatnonzero(img[:, 3] > 0)[0])
+    g[4 * hit + 1] = np.inf
+    dtf, dsig = ctx.render_adjoint(cam, g)
+    assert np.all(np.isnan(dtf)) and np.all(np.isnan(dsig))

@@ -210,3 +210,31 @@ def test_batch_views_finish_with_their_timestep(ctx):
         b.add_view(0, fields.orbit_camera(8, 8, (0.5, 0.5, 0.5), 0.0), torch.zeros(256, device="cuda"))
         b.view_latency_ms()  # the batch changed since its last run
     b.close()
+
+
+def test_batch_runs_follow_changed_settings(ctx):
+    """A run whose descriptors equal the previous run's only enqueues the
+    graph.  A new transfer function (same device table, same descriptors)
+    and new LIC parameters (new descriptors, uploaded) must both show in the
+    next run, equal to single calls with the same settings."""
+    _setup(ctx)
+    lics, centers, imgs, total = _single(ctx, vv.STORE_FP32)
+    b, outs = _batch(ctx, vv.STORE_FP32, centers)
+    b.run()
+    b.run()
+    _check(b, outs, lics, imgs, total)
+    rgba, op = fields.tf_magnitude(0.1, 0.5)
+    ctx.set_tf(rgba, op)
+    lics2, _, imgs2, total2 = _single(ctx, vv.STORE_FP32)
+    assert any(not np.array_equal(a, c) for a, c in zip(imgs, imgs2))
+    b.run()
+    _check(b, outs, lics2, imgs2, total2)
+    p = ctx.get_lic_params()
+    p.step_size_lic = 0.007
+    ctx.set_lic_params(p)
+    lics3, _, imgs3, total3 = _single(ctx, vv.STORE_FP32)
+    assert not np.array_equal(lics3[0], lics2[0])
+    b.run()
+    b.run()
+    _check(b, outs, lics3, imgs3, total3)
+    b.close()

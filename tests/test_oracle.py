@@ -644,3 +644,58 @@ def test_slicing_skips_low_tf_alpha():
     f4, noise, lut, p, rgba, op, view, n = _slicing_case(0.05)
     img, ns, total = orc.render_slices(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
     assert total == 0 and np.all(img == 0)
+
+
+def test_fast_build_equals_the_checker():
+    """The timed CPU baseline runs liboracle_fast.so (-O3, AVX2/FMA code
+    generation, -ffp-contract=off): its LIC must equal the checker's bit for
+    bit, in FP32 and FP64."""
+    n = 12
+    lut, inv, p = _case(n, 9, 11, 0.02, kind=3)
+    v = np.random.default_rng(31).normal(size=(n, n, n, 3)).astype(np.float32)
+    noise = orc.gen_noise(5, 1, 0.0, (n, n, n))
+    for f64 in (False, True):
+        f4 = orc.encode_field(v, (n, n, n), f64=f64)
+        a = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p)
+        b = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p, fast=True)
+        np.testing.assert_array_equal(a, b)
+
+
+def test_lic_speed_of_flow_scales_each_step_by_the_magnitude():
+    """SPEED_OF_FLOW (inc_lic.glsl:107-110,120-122): v = (m(z), 0, 0) with
+    m(z) = (z + 1)/n, so |v|/max = m(z) on voxel plane z.  A streamline seeded
+    on a voxel centre stays in its z plane, where the magnitude channel
+    samples exactly m(z), and each Heun step moves x by m(z) h_e (by h_e
+    without SPEED_OF_FLOW)."""
+    n, F, B, h = 16, 9, 6, 0.02
+    lut, inv, p = _case(n, F, B, h)
+    p.speed_of_flow = 1
+    m = (np.arange(n) + 1.0) / n
+    v = np.zeros((n, n, n, 3), np.float32)
+    v[..., 0] = m[:, None, None]
+    f4 = orc.encode_field(v, (n, n, n), f64=True)
+    rng = np.random.default_rng(6)
+    line = rng.integers(0, 256, n).astype(np.uint8)
+    noise = np.broadcast_to(line, (n, n, n)).astype(np.uint8).ravel()
+    out = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p).reshape(n, n, n)
+    he = float(np.float64(np.float32(h)) * np.float64(np.float32(0.3)))
+    xs = (np.arange(n) + 0.5) / n
+    lp = np.concatenate([line, line[:1]]).astype(np.float64)
+
+    def N(u):  # REPEAT linear interpolation along x, independent of the oracle
+        x = u * n - 0.5
+        i = np.floor(x)
+        f = x - i
+        i = np.mod(i, n).astype(int)
+        return ((1 - f) * lp[i] + f * lp[i + 1]) / 255.0
+
+    ks = np.concatenate([[0], -np.arange(1, B + 1), np.arange(1, F + 1)])
+    for z in (0, 5, 15):
+        ref = sum(N(xs + k * m[z] * he) for k in ks) * inv / (F + B) * 30.0
+        for y in (2, 11):
+            np.testing.assert_allclose(out[z, y], ref, rtol=1e-9)
+    p.speed_of_flow = 0
+    plain = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p).reshape(n, n, n)
+    ref = sum(N(xs + k * he) for k in ks) * inv / (F + B) * 30.0
+    np.testing.assert_allclose(plain[5, 2], ref, rtol=1e-9)
+    assert np.max(np.abs(plain[5] - out[5])) > 1e-3

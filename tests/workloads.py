@@ -57,7 +57,7 @@ def float_field_params(op):
 
 def lic_setup(ctx: vv.Context, *, n=32, kind="abc", noise_kind=vv.NOISE_BINARY, density=0.05,
               grad=False, filt=vv.FILTER_BOX, L=20, h=0.01, storage=vv.STORE_FP32, seed=1,
-              gradient_scale=30.0, freq=1.0, gate=None):
+              gradient_scale=30.0, freq=1.0, gate=None, speed=False):
     """Loads a case into ctx and returns what the oracle needs for the same case."""
     dims = (n, n, n)
     v = field(kind, n, seed)
@@ -77,13 +77,14 @@ def lic_setup(ctx: vv.Context, *, n=32, kind="abc", noise_kind=vv.NOISE_BINARY, 
         scalar = gate
         ctx.upload_scalar(scalar, dims)
         p.gate_enable = 1
+    p.speed_of_flow = 1 if speed else 0
     ctx.set_lic_params(p)
     f4 = oracle_field(v, dims, storage)
     noise = oracle_noise(seed, noise_kind, density, dims, grad)
     op = orc.params(steps_forward=L, steps_backward=L, step_size_lic=h, inv_filter_area=inv,
                     gradient_scale=gradient_scale, freq_scale=freq,
                     illum_mode=1 if grad else 0, gate_enable=1 if gate is not None else 0,
-                    field_unorm8=1 if storage == vv.STORE_UNORM8 else 0)
+                    field_unorm8=1 if storage == vv.STORE_UNORM8 else 0, speed_of_flow=1 if speed else 0)
     return dict(v=v, dims=dims, f4=f4, noise=noise, nch=4 if grad else 1, lut=lut, inv=inv,
                 op=op, params=p, scalar=scalar)
 

@@ -43,6 +43,7 @@ class LicParams(C.Structure):
         ("gate_enable", C.c_int),
         ("gate_lo", C.c_float),
         ("gate_hi", C.c_float),
+        ("speed_of_flow", C.c_int),
     ]
 
 

@@ -58,7 +58,7 @@ vv_status vv_destroy(vv_ctx* c)
         (void)hipStreamSynchronize(c->stream);
         DevBuf* bufs[] = {&c->field, &c->noise, &c->scalar, &c->tf_dev, &c->lic, &c->weights,
                           &c->sigma, &c->stage, &c->stage2, &c->image, &c->counter, &c->adj_ckpt,
-                          &c->adj_acc, &c->adj_dimg, &c->adj_out};
+                          &c->adj_dimg, &c->adj_out, &c->lic_field, &c->illum_luts};
         for (DevBuf* b : bufs) b->release();
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -523,8 +523,11 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     L.field_bytes = (unsigned long long)clamp_elems(c->fdims) * field_texel_bytes(L.field_type);
     L.noise_bytes = (unsigned long long)repeat_elems(c->ndims) * (unsigned long long)nc;
     // the cached kernels address through 32-bit byte offsets
-    // and 24-bit index multiplies (plane strides below 2^24 elements)
-    L.cache = c->lic_cache && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
+    // and 24-bit index multiplies (plane strides below 2^24 elements); with
+    // SPEED_OF_FLOW the LIC reads the field's magnitude channel, which the
+    // cached kernels do not keep: the one-lane-per-voxel kernels run
+    L.speed = p.speed_of_flow;
+    L.cache = c->lic_cache && !L.speed && L.field_bytes < (1ull << 32) && L.noise_bytes < (1ull << 32) &&
               (long long)L.field.sz < (1ll << 24) && (long long)L.noise.sz < (1ll << 24);
     // scalar noise on the field's grid at freq 1 (the cached SAME kernels):
     // the LIC reads the field copy that carries the noise bytes, built once per
@@ -880,6 +883,51 @@ vv_status vv_set_opacity_scale(vv_ctx* c, const float* sigma, int on_device)
     });
 }
 
+// Launch descriptor and checkpoint count of an adjoint call.  Samples per ray
+// are bounded by the box diagonal in texture space over the shortest
+// per-step displacement (|dir| >= min scale).
+static vv_status adjoint_setup(vv_ctx* c, const vv_camera* cam, AdjointLaunch& A)
+{
+    std::memset(&A, 0, sizeof(A));
+    vv_status st = fill_render_launch(c, cam, A.r);
+    if (st) return st;
+    double diag = 0.0, smin = 1e30;
+    for (int a = 0; a < 3; ++a) {
+        diag += (double)A.r.view.tex_max[a] * A.r.view.tex_max[a];
+        if (c->scale[a] < smin) smin = c->scale[a];
+    }
+    diag = std::sqrt(diag);
+    const long long geo = (long long)(diag / (smin * (double)c->params.step_size_vol) * 1.01) + 4;
+    A.r.max_samples = A.r.max_samples < geo ? A.r.max_samples : geo;
+    A.max_ckpt = (int)((A.r.max_samples + kAdjSeg - 1) / kAdjSeg);
+    A.use_window = c->adj_window;
+    return VV_OK;
+}
+
+// The adjoint kernels and the fixed-point conversion on device buffers:
+// dimage in, dtf (1024) and dsigma (nv, or null) out; scratch holds
+// adj_scratch_layout(nrays, A.max_ckpt, nv) bytes.
+static vv_status adjoint_enqueue(vv_ctx* c, AdjointLaunch& A, char* scratch, const float* dimage, float* dtf,
+                                 float* dsigma, long long nv)
+{
+    const long long nrays = (long long)A.r.view.width * A.r.view.height;
+    const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
+    unsigned long long* hdr = reinterpret_cast<unsigned long long*>(scratch);
+    long long* acc = reinterpret_cast<long long*>(scratch + SL.acc);
+    const long long zero_words = (long long)(SL.acc / 8) + 1024 + (dsigma ? nv : 0);
+    HIPCHK(c, launch_fill_i64(reinterpret_cast<long long*>(scratch), 0, zero_words, c->stream));
+    A.dimage = dimage;
+    A.ckpt = reinterpret_cast<float*>(scratch + SL.ckpt);
+    A.nsamp = reinterpret_cast<int*>(scratch + SL.nsamp);
+    A.hdr = hdr;
+    A.dtf_fx = acc;
+    A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
+    HIPCHK(c, launch_render_adjoint(A, c->stream));
+    HIPCHK(c, launch_fixed_to_float(hdr, acc, dtf, 1024, c->stream));
+    if (dsigma) HIPCHK(c, launch_fixed_to_float(hdr, acc + 1024, dsigma, nv, c->stream));
+    return VV_OK;
+}
+
 vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf, float* dsigma,
                             int on_device)
 {
@@ -888,56 +936,30 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
         if (!dimage || !dtf) return fail(c, VV_EINVAL, "dimage/dtf NULL");
         HIPCHK(c, hipSetDevice(c->device));
         AdjointLaunch A;
-        std::memset(&A, 0, sizeof(A));
-        vv_status st = fill_render_launch(c, cam, A.r);
+        vv_status st = adjoint_setup(c, cam, A);
         if (st) return st;
         const long long nrays = (long long)cam->width * cam->height;
-        // samples per ray are bounded by the box diagonal in texture space over
-        // the shortest per-step displacement (|dir| >= min scale)
-        double diag = 0.0, smin = 1e30;
-        for (int a = 0; a < 3; ++a) {
-            diag += (double)A.r.view.tex_max[a] * A.r.view.tex_max[a];
-            if (c->scale[a] < smin) smin = c->scale[a];
-        }
-        diag = std::sqrt(diag);
-        long long geo = (long long)(diag / (smin * (double)c->params.step_size_vol) * 1.01) + 4;
-        long long maxs = A.r.max_samples < geo ? A.r.max_samples : geo;
-        const int max_ckpt = (int)((maxs + 63) / 64);
-        const size_t ck_bytes = sizeof(float) * 4 * (size_t)(2 * max_ckpt + 2 * 64) * (size_t)nrays;
-        HIPCHK(c, c->adj_ckpt.ensure(ck_bytes));
         const long long nv = nvox(c->ldims);
-        const size_t acc_bytes = sizeof(long long) * (size_t)(1024 + (dsigma ? nv : 0));
-        HIPCHK(c, c->adj_acc.ensure(acc_bytes));
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
+        HIPCHK(c, c->adj_ckpt.ensure(SL.total));
         const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
         const float* d_dimg = dimage;
+        float* d_dtf = dtf;
+        float* d_dsigma = dsigma;
         if (!on_device) {
             st = upload(c, c->adj_dimg, dimage, img_bytes, 0);
             if (st) return st;
             d_dimg = (const float*)c->adj_dimg.p;
+            HIPCHK(c, c->adj_out.ensure(sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0))));
+            d_dtf = (float*)c->adj_out.p;
+            d_dsigma = dsigma ? d_dtf + 1024 : nullptr;
         }
-        long long* acc = (long long*)c->adj_acc.p;
-        HIPCHK(c, launch_fill_i64(acc, 0, (long long)(acc_bytes / 8), c->stream));
-        A.dimage = d_dimg;
-        A.ckpt = (float*)c->adj_ckpt.p;
-        A.max_ckpt = max_ckpt;
-        A.r.max_samples = maxs;
-        A.dtf_fx = acc;
-        A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
-        A.fx_scale = 4294967296.0;  // 2^32
-        A.use_window = c->adj_window;
-        HIPCHK(c, launch_render_adjoint(A, c->stream));
-        const size_t out_bytes = sizeof(float) * (size_t)(1024 + (dsigma ? nv : 0));
-        float* dout = nullptr;
-        if (on_device) {
-            HIPCHK(c, launch_fixed_to_float(acc, dtf, 1024, 1.0 / A.fx_scale, c->stream));
-            if (dsigma) HIPCHK(c, launch_fixed_to_float(acc + 1024, dsigma, nv, 1.0 / A.fx_scale, c->stream));
-        } else {
-            HIPCHK(c, c->adj_out.ensure(out_bytes));
-            dout = (float*)c->adj_out.p;
-            HIPCHK(c, launch_fixed_to_float(acc, dout, (long long)(out_bytes / 4), 1.0 / A.fx_scale, c->stream));
-            HIPCHK(c, hipMemcpyAsync(dtf, dout, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));
+        st = adjoint_enqueue(c, A, (char*)c->adj_ckpt.p, d_dimg, d_dtf, d_dsigma, nv);
+        if (st) return st;
+        if (!on_device) {
+            HIPCHK(c, hipMemcpyAsync(dtf, d_dtf, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));
             if (dsigma)
-                HIPCHK(c, hipMemcpyAsync(dsigma, dout + 1024, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost,
+                HIPCHK(c, hipMemcpyAsync(dsigma, d_dsigma, sizeof(float) * (size_t)nv, hipMemcpyDeviceToHost,
                                          c->stream));
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));

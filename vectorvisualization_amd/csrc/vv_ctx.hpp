@@ -118,7 +118,7 @@ struct vv_ctx {
     DevBuf sigma;
 
     // staging / scratch
-    DevBuf stage, stage2, image, counter, adj_ckpt, adj_acc, adj_dimg, adj_out;
+    DevBuf stage, stage2, image, counter, adj_ckpt, adj_dimg, adj_out;
 };
 
 namespace vv_host {
@@ -205,6 +205,7 @@ inline void default_params(vv_lic_params* p)
     p->gate_enable = 0;
     p->gate_lo = 0.1f;
     p->gate_hi = 0.3f;
+    p->speed_of_flow = 0;
 }
 
 inline void default_tf5(uint8_t* t)
@@ -363,6 +364,7 @@ inline vv_status check_params(vv_ctx* c, const vv_lic_params* p)
         return fail(c, VV_EINVAL, "illum_mode invalid");
     if (p->tf_index != VV_TF_INDEX_B && p->tf_index != VV_TF_INDEX_MAG)
         return fail(c, VV_EINVAL, "tf_index invalid");
+    if (p->speed_of_flow != 0 && p->speed_of_flow != 1) return fail(c, VV_EINVAL, "speed_of_flow must be 0 or 1");
     return VV_OK;
 }
 

@@ -179,6 +179,27 @@ __device__ __forceinline__ float3 tri_rgb(const Vol<FT>& v, const Cell& c)
     return r;
 }
 
+// all four channels of a field (direction and magnitude)
+template <typename FT>
+__device__ __forceinline__ float4 tri_rgba(const Vol<FT>& v, const Cell& c)
+{
+    const int b = c.base, sy = v.sy, sz = v.sz;
+    float4 a000 = ld4(v.p, b);
+    float4 a100 = ld4(v.p, b + 1);
+    float4 a010 = ld4(v.p, b + sy);
+    float4 a110 = ld4(v.p, b + sy + 1);
+    float4 a001 = ld4(v.p, b + sz);
+    float4 a101 = ld4(v.p, b + sz + 1);
+    float4 a011 = ld4(v.p, b + sz + sy);
+    float4 a111 = ld4(v.p, b + sz + sy + 1);
+    float4 r;
+    r.x = FieldScale<FT>::apply(tri(a000.x, a100.x, a010.x, a110.x, a001.x, a101.x, a011.x, a111.x, c));
+    r.y = FieldScale<FT>::apply(tri(a000.y, a100.y, a010.y, a110.y, a001.y, a101.y, a011.y, a111.y, c));
+    r.z = FieldScale<FT>::apply(tri(a000.z, a100.z, a010.z, a110.z, a001.z, a101.z, a011.z, a111.z, c));
+    r.w = FieldScale<FT>::apply(tri(a000.w, a100.w, a010.w, a110.w, a001.w, a101.w, a011.w, a111.w, c));
+    return r;
+}
+
 // one channel of a 4-channel field
 template <typename FT>
 __device__ __forceinline__ float tri_ch(const Vol<FT>& v, const Cell& c, int ch)

@@ -61,6 +61,7 @@ struct LicLaunch {
     int cache;         // 1: per-lane cell caches (scalar noise, no gate)
     unsigned long long field_bytes, noise_bytes;  // padded buffer sizes
     int field_has_noise; // field is the LIC copy: channel 3 = noise byte (same grid)
+    int speed;         // SPEED_OF_FLOW: Heun displacements scaled by the field's channel 3
 };
 
 struct ViewBlock {
@@ -114,16 +115,46 @@ struct LicBatchLaunch {
     int field_type, same;         // shared by all volumes
 };
 
+// Scratch of one adjoint call, carved from one device buffer (the caller of
+// launch_render_adjoint owns it):
+//   header  2 x u64: [0] total samples of all rays, [1] low 32 bits: the
+//           float bits of the largest bound on one sample's gradient
+//           contribution (atomicMax; NaN bits when a bound is not finite)
+//   acc     1024 (+ sigma voxels) int64 fixed-point accumulators
+//   ckpt    per ray 2*max_ckpt float4 checkpoints, then the segment column
+//           (kAdjSeg + kAdjSeg/16 float4)
+//   nsamp   per ray sample count (int)
+constexpr int kAdjSeg = 64;          // checkpoint interval (samples)
+constexpr int kAdjColumn = kAdjSeg + kAdjSeg / 16;  // float4 per ray in the segment column
+
 struct AdjointLaunch {
     RenderLaunch r;
-    const float* dimage;          // width*height*4
-    float* ckpt;                  // scratch: per ray, per checkpoint 8 floats
+    const float* dimage;          // width*height*4 (device)
+    float* ckpt;                  // per ray: checkpoints, then the segment column
     int max_ckpt;                 // checkpoints per ray
-    long long* dtf_fx;            // 256*4 fixed-point accumulators
-    long long* dsigma_fx;         // sigma padded-clamp element count accumulators
-    double fx_scale;              // fixed-point scale (2^k)
+    int* nsamp;                   // per ray sample count
+    unsigned long long* hdr;      // the two header words (zero before launch)
+    long long* dtf_fx;            // 256*4 fixed-point accumulators (zero before launch)
+    long long* dsigma_fx;         // sigma voxel accumulators, or null
     int use_window;               // accumulate dsigma in an LDS window per segment
 };
+
+// Bytes of adjoint scratch for nrays rays, max_ckpt checkpoints and nv sigma
+// voxels (0 without sigma gradients), and the offsets of its parts.
+struct AdjScratchLayout {
+    size_t acc, ckpt, nsamp, total;
+};
+inline AdjScratchLayout adj_scratch_layout(long long nrays, int max_ckpt, long long nv)
+{
+    AdjScratchLayout L;
+    const size_t align = 256;
+    auto up = [&](size_t x) { return (x + align - 1) / align * align; };
+    L.acc = 16;
+    L.ckpt = up(L.acc + sizeof(long long) * (size_t)(1024 + nv));
+    L.nsamp = up(L.ckpt + 16 * (size_t)(2 * max_ckpt + kAdjColumn) * (size_t)nrays);
+    L.total = up(L.nsamp + sizeof(int) * (size_t)nrays);
+    return L;
+}
 
 // ---- launchers (vv_lic.hip / vv_render.hip) ----
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
@@ -186,10 +217,10 @@ hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const 
 // unused), L as for launch_lic (its out fields unused)
 hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s);
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s);
-hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
+// dst[i] = src[i] / 2^k for the fixed-point scale 2^k the adjoint derived from
+// its header words (NaN when the header holds a non-finite bound)
+hipError_t launch_fixed_to_float(const unsigned long long* hdr, const long long* src, float* dst, long long n,
                                  hipStream_t s);
-hipError_t launch_unpad_clamp_fixed(const long long* src, const int dims[3], float* dst,
-                                    double inv_scale, hipStream_t s);
 hipError_t launch_fill_i64(long long* p, long long v, long long n, hipStream_t s);
 
 }  // namespace vv

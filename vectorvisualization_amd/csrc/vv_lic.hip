@@ -259,8 +259,26 @@ struct W27 {
     float w[27];
 };
 
+// quantize8 (VV/gradient.cpp:462-478): (unsigned char)(((double)f + 1) / 2 * 255)
+// in float arithmetic.  The double expression equals floor(127.5 (f + 1))
+// for every float f (no float is within its rounding of an integer
+// boundary).  The float fma's rounding can cross a boundary, so the floor is
+// corrected by exact sign tests: fma(f, 255, 255 - 2k) has the sign of
+// 255 f + 255 - 2k (a nonzero exact value is a multiple of 2^-149, so it
+// never rounds to zero).  Compared with the double expression for every
+// float in [-1 - 4 ulp, 1 + 4 ulp] on the CPU: no mismatch.
+__device__ __forceinline__ uint8_t quantize8(float f)
+{
+    float k = floorf(fmaf(f, 127.5f, 127.5f));
+    if (fmaf(f, 255.0f, 255.0f - 2.0f * k) < 0.0f)
+        k -= 1.0f;
+    else if (fmaf(f, 255.0f, 255.0f - 2.0f * (k + 1.0f)) >= 0.0f)
+        k += 1.0f;
+    return (uint8_t)fminf(fmaxf(k, 0.0f), 255.0f);
+}
+
 // filterGradients (VV/gradient.cpp:377-459: window [-fw, fw-2], fw shrinking
-// at the border) followed by quantize8 (:462-478, double arithmetic)
+// at the border) followed by quantize8 (:462-478)
 __global__ void __launch_bounds__(kThreads) k_noise_grad_filter(const float* g, int nx, int ny, int nz,
                                                                 W27 fw27, uint8_t* rgb)
 {
@@ -289,7 +307,7 @@ __global__ void __launch_bounds__(kThreads) k_noise_grad_filter(const float* g, 
         f[1] /= len;
         f[2] /= len;
     }
-    for (int c = 0; c < 3; ++c) rgb[3 * q + c] = (uint8_t)(((double)f[c] + 1.0) / 2.0 * 255.0);
+    for (int c = 0; c < 3; ++c) rgb[3 * q + c] = quantize8(f[c]);
 }
 
 // raw noise (+ quantised gradients) -> REPEAT-padded texture
@@ -640,6 +658,7 @@ hipError_t make_lic_args(const LicLaunch& L, LicArgs& a)
     a.F = L.F;
     a.B = L.B;
     a.he = L.he;
+    a.speed = L.speed;
     a.scl = L.scl;
     a.freq = L.freq;
     a.gate_lo = L.gate_lo;

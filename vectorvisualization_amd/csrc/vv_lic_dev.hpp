@@ -22,6 +22,7 @@ struct LicArgs {
     int per_xcd;         // tiles handled by each of the 8 XCDs
     int sx, sy, sz;      // super-tile size in tiles (3D-blocked tile order)
     int sgx, sgy;        // super-tiles per axis (x, y)
+    int speed;           // SPEED_OF_FLOW (uncached path only)
 };
 
 // logical tile t -> tile coordinates: super-tiles in raster order, tiles
@@ -84,6 +85,16 @@ __device__ __forceinline__ float3 field_rgb(const LicArgs& a, float x, float y, 
                      a.field.sz};
     Cell c = cell_clamp(fv, x, y, z);
     return tri_rgb(fv, c);
+}
+
+// direction and magnitude (.a = |v|/max) of the field
+template <typename FT>
+__device__ __forceinline__ float4 field_rgba(const LicArgs& a, float x, float y, float z)
+{
+    const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
+                     a.field.sz};
+    Cell c = cell_clamp(fv, x, y, z);
+    return tri_rgba(fv, c);
 }
 
 // Stores one LIC voxel into the REPEAT-padded output and into the copies
@@ -541,11 +552,16 @@ __device__ __forceinline__ CachedArgs cached_args(const LicArgs& a)
 }
 
 // LIC of one seed, any noise (scalar or RGBA) and the optional scalar gate:
-// the weighted noise sums, before the final scaling
+// the weighted noise sums, before the final scaling.  With a.speed
+// (SPEED_OF_FLOW, inc_lic.glsl:107-110,120-122) both displacements of a step
+// are scaled by the field magnitude .a at the step's start point (the shader
+// scales licdir2 by step.a before step is re-read), in the shader's order:
+// ((2s - 1) * a) * h.
 template <typename FT, int NC, bool GATE>
 __device__ __forceinline__ float4 lic_point(const LicArgs& a, float sx, float sy, float sz)
 {
-    const float3 s0 = field_rgb<FT>(a, sx, sy, sz);
+    const bool speed = a.speed != 0;
+    const float4 s0 = speed ? field_rgba<FT>(a, sx, sy, sz) : make_float4(0.f, 0.f, 0.f, 1.f);
     float4 n = noise_at<NC, GATE>(a, sx, sy, sz);
     const float w0 = a.w[0];
     float4 acc;
@@ -553,6 +569,7 @@ __device__ __forceinline__ float4 lic_point(const LicArgs& a, float sx, float sy
     acc.y = n.y * w0;
     acc.z = n.z * w0;
     acc.w = n.w * w0;
+    const float3 s0rgb = speed ? make_float3(s0.x, s0.y, s0.z) : field_rgb<FT>(a, sx, sy, sz);
 
 #pragma unroll 1
     for (int dirn = 0; dirn < 2; ++dirn) {
@@ -560,21 +577,40 @@ __device__ __forceinline__ float4 lic_point(const LicArgs& a, float sx, float sy
         const float h = dirn == 0 ? -a.he : a.he;
         const float* wk = dirn == 0 ? a.w + 1 : a.w + 1 + a.B;
         float px = sx, py = sy, pz = sz;
-        float3 s = s0;
+        float3 s = s0rgb;
+        float sa = s0.w;
 #pragma unroll 1
         for (int i = 0; i < steps; ++i) {
             // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector
-            const float d1x = (2.0f * s.x - 1.0f) * h;
-            const float d1y = (2.0f * s.y - 1.0f) * h;
-            const float d1z = (2.0f * s.z - 1.0f) * h;
+            float t1x = 2.0f * s.x - 1.0f, t1y = 2.0f * s.y - 1.0f, t1z = 2.0f * s.z - 1.0f;
+            if (speed) {
+                t1x = t1x * sa;
+                t1y = t1y * sa;
+                t1z = t1z * sa;
+            }
+            const float d1x = t1x * h;
+            const float d1y = t1y * h;
+            const float d1z = t1z * h;
             const float3 s2 = field_rgb<FT>(a, px + d1x, py + d1y, pz + d1z);
-            const float d2x = (2.0f * s2.x - 1.0f) * h;
-            const float d2y = (2.0f * s2.y - 1.0f) * h;
-            const float d2z = (2.0f * s2.z - 1.0f) * h;
+            float t2x = 2.0f * s2.x - 1.0f, t2y = 2.0f * s2.y - 1.0f, t2z = 2.0f * s2.z - 1.0f;
+            if (speed) {
+                t2x = t2x * sa;
+                t2y = t2y * sa;
+                t2z = t2z * sa;
+            }
+            const float d2x = t2x * h;
+            const float d2y = t2y * h;
+            const float d2z = t2z * h;
             px = px + 0.5f * (d1x + d2x);
             py = py + 0.5f * (d1y + d2y);
             pz = pz + 0.5f * (d1z + d2z);
-            s = field_rgb<FT>(a, px, py, pz);
+            if (speed) {
+                const float4 q = field_rgba<FT>(a, px, py, pz);
+                s = make_float3(q.x, q.y, q.z);
+                sa = q.w;
+            } else {
+                s = field_rgb<FT>(a, px, py, pz);
+            }
             n = noise_at<NC, GATE>(a, px, py, pz);
             const float wi = wk[i];
             acc.x = acc.x + n.x * wi;

@@ -20,7 +20,7 @@ namespace {
 #include "vv_lic_dev.hpp"
 
 constexpr int kBX = 16, kBY = 16, kRThreads = kBX * kBY;
-constexpr int kSeg = 64;  // adjoint checkpoint interval (samples)
+constexpr int kSeg = kAdjSeg;  // adjoint checkpoint interval (samples)
 
 struct Sample {
     float tfu;     // transfer-function coordinate
@@ -62,13 +62,17 @@ __device__ __forceinline__ float ipow(float x, int e)
     return r;
 }
 
+// b^c for b in [0, 1], c > 0, as exp2(c log2 b) in FP32 (within a few ulp of
+// powf; the math library's powf extracts exponents through FP64 conversions)
+__device__ __forceinline__ float pow01(float b, float c) { return exp2f(c * log2f(b)); }
+
 __device__ __forceinline__ float pow_corr(float b, float c, int mode)
 {
     switch (mode) {
     case 0: return b;
     case 1: return sqrtf(b);
     case 2: return b * b;
-    default: return powf(b, c);
+    default: return pow01(b, c);
     }
 }
 
@@ -391,19 +395,62 @@ __global__ void k_stamp(unsigned long long* t0) { *t0 = (unsigned long long)wall
 // Forward  D' = clamp(D + (1 - D.a) s, 0, 1),  s = (rgb a, a).
 // Backward G -> G masked where the clamp was active;
 //          dL/ds = (1 - D.a) G;  dL/dD = G - (G . s) e_a.
-// Phase 1 runs the ray once and stores (pos, D) every kSeg samples.
-// Phase 2 walks the segments last to first: it re-runs a segment from its
-// checkpoint, saving (pos, mask) and D per sample in a per-ray scratch
-// column, then walks it backwards, re-evaluating each sample.  Gradient
-// contributions are rounded to fixed point (fx_scale) and accumulated with
-// 64-bit integer atomics, so the sums do not depend on thread order.
+// Phase 1 (k_adjoint_forward) runs the ray once and stores (pos, D) every
+// kSeg samples.  Phase 2 (k_render_adjoint) walks the segments last to
+// first: it re-runs a segment from its checkpoint, saving D per sample in a
+// per-ray scratch column, then walks it backwards, re-evaluating each sample.
+//
+// Gradient contributions are accumulated as 64-bit integers, so the sums do
+// not depend on thread order.  A contribution v becomes rint(v * 2^k).  k is
+// chosen per call, on the device, so that no sum can leave the int64 range:
+// phase 1 bounds every contribution a ray will make (below) and counts the
+// samples of all rays, Ns.  One sample adds at most |v| + 4 quanta to any
+// accumulator (its two TF corners, or its eight sigma corners, have weights
+// summing to 1), so with C the largest bound and 2^k * C <= 2^61 / Ns, every
+// sum stays within Ns (C 2^k + 4) <= 2^62.  The scale follows the gradient's
+// own magnitude: multiplying dL/dimage by a power of two shifts k and leaves
+// the integer sums unchanged.
+//
+// The bound of one ray (G = the pixel's dL/dimage, |G|max its largest entry).
+// Along the backward walk the rgb entries of G keep their value or become 0,
+// and G.a changes by -sum_c G.c src.c - G.a src.a with src.a in [0,1], so
+// |G.a| <= |G|max * Gamma, Gamma = 1 + sum over the ray's samples of
+// sum_c |src.c|.  Per sample, with t = 1 - D.a in [0,1]:
+//   TF rgb       |t G.c a m.c|                      <= |G|max a |m.c|
+//   TF alpha     |da dadraw ao sg|,  |da| <= |G|max (Gamma + sum_c |rgb.c|)
+//   sigma        |da dadraw ao tf.a|
+// so the ray's bound is |G|max * max(max a|m|, Gamma max q + max q R) with
+// q = dadraw ao max(sg, tf.a) and R = sum_c |rgb.c|.  Rounding in these float
+// sums is far inside the factor 2 between 2^62 and the int64 limit.
 
-__device__ __forceinline__ long long to_fx(float v, double scale)
+// d(corrected alpha) / d(raw alpha): 1 - (1 - a)^c differentiated
+__device__ __forceinline__ float dalpha_draw(const RenderLaunch& a, float araw)
 {
-    return (long long)llrint((double)v * scale);
+    const float b = 1.0f - araw;
+    if (a.corr_mode == 0) return 1.0f;
+    if (b > 0.0f) return a.alpha_corr * pow01(b, a.alpha_corr - 1.0f);
+    return 0.0f;
 }
 
-__device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float val, double scale)
+// exponent k of the fixed-point scale 2^k from the header words; false when
+// the bound is not finite (the gradients are then NaN) or zero (all
+// contributions are zero)
+__device__ __forceinline__ bool adj_scale_exp(const unsigned long long* hdr, int& k)
+{
+    const unsigned cb_bits = (unsigned)hdr[1];
+    const unsigned long long ns = hdr[0];
+    const float cb = __uint_as_float(cb_bits);
+    if (!(cb > 0.0f) || cb_bits >= 0x7f800000u || ns == 0) return false;
+    int ec;
+    (void)frexpf(cb, &ec);  // cb < 2^ec
+    const int en = ns <= 1 ? 0 : 64 - __clzll((long long)(ns - 1));  // ns <= 2^en
+    k = 61 - en - ec;
+    return true;
+}
+
+__device__ __forceinline__ long long to_fx(float v, int k) { return __float2ll_rn(ldexpf(v, k)); }
+
+__device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float val, int k)
 {
     const float x = fmaf(u, 256.0f, -0.5f);
     float fl = floorf(x);
@@ -412,10 +459,80 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
     const int i = (int)fl;
     const int i0 = i < 0 ? 0 : i;
     const int i1 = i + 1 > 255 ? 255 : i + 1;
-    const long long a0 = to_fx((1.0f - f) * val, scale);
-    const long long a1 = to_fx(f * val, scale);
+    const long long a0 = to_fx((1.0f - f) * val, k);
+    const long long a1 = to_fx(f * val, k);
     if (a0) atomicAdd((unsigned long long*)&acc[i0 * 4 + ch], (unsigned long long)a0);
     if (a1) atomicAdd((unsigned long long*)&acc[i1 * 4 + ch], (unsigned long long)a1);
+}
+
+// Phase 1: the forward ray with a checkpoint (pos, D) every kSeg samples, its
+// sample count, and the contribution bound and sample total of the header.
+template <typename FT, int LC>
+__global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
+{
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    __shared__ unsigned bmax;
+    const RenderLaunch& a = A.r;
+    const int tid = threadIdx.y * kBX + threadIdx.x;
+    if (tid == 0) bmax = 0u;
+    load_tables(a.tf, tf, op);
+    const int i = blockIdx.x * kBX + threadIdx.x;
+    const int j = blockIdx.y * kBY + threadIdx.y;
+    const long long W = a.view.width, H = a.view.height;
+    const long long nrays = W * H;
+    const long long pix = (long long)j * W + i;
+    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
+    Ray r;
+    const bool inside = i < W && j < H;
+    const bool active = inside && ray_setup(a.view, i, j, r);
+    long long n = 0;
+    float bound = 0.0f;
+    if (active) {
+        float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
+        float dest[4] = {0.f, 0.f, 0.f, 0.f};
+        float max_am = 0.0f, max_q = 0.0f, max_qr = 0.0f, gamma = 1.0f;
+        for (long long k = 0; k < a.max_samples; ++k) {
+            if ((k % kSeg) == 0) {
+                const long long c = k / kSeg;
+                ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+                ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+            }
+            Sample s;
+            eval_sample<FT, LC>(a, tf, op, r, pos, s);
+            composite(dest, s);
+            ++n;
+            const float rsum = fabsf(s.rgb[0]) + fabsf(s.rgb[1]) + fabsf(s.rgb[2]);
+            const float mmax = fmaxf(fabsf(s.m[0]), fmaxf(fabsf(s.m[1]), fabsf(s.m[2])));
+            const float q = s.aclamp ? 0.0f : dalpha_draw(a, s.araw) * s.ao * fmaxf(s.sg, s.tfa);
+            max_am = fmaxf(max_am, s.a * mmax);
+            max_q = fmaxf(max_q, q);
+            max_qr = fmaxf(max_qr, q * rsum);
+            gamma = fmaf(s.a, rsum, gamma);
+            for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
+            if (outside(pos, a.view) || dest[3] > 0.95f) break;
+        }
+        const float4 g = reinterpret_cast<const float4*>(A.dimage)[pix];
+        const float gm = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w)));
+        const bool finite_g = isfinite(g.x) && isfinite(g.y) && isfinite(g.z) && isfinite(g.w);
+        bound = gm * fmaxf(max_am, fmaf(gamma, max_q, max_qr));
+        if (!finite_g || !isfinite(bound)) bound = __uint_as_float(0x7fc00000u);
+    }
+    if (inside) A.nsamp[pix] = (int)n;
+    // the bound is >= 0 (or NaN, whose bits exceed every finite one), so its
+    // bits order as unsigned integers
+    atomicMax(&bmax, __float_as_uint(bound));
+    unsigned long long ns = (unsigned long long)n;
+    for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = ns;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kRThreads / 64; ++w) t += red[w];
+        if (t) atomicAdd(&A.hdr[0], t);
+        if (bmax) atomicMax(&A.hdr[1], (unsigned long long)bmax);
+    }
 }
 
 // dsigma accumulation window in LDS: entries (int64 fixed point), one
@@ -427,7 +544,10 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
 constexpr int kWin = 3072;
 constexpr int kSub = 16;
 constexpr int kNSub = kSeg / kSub;
+static_assert(kSeg + kNSub == kAdjColumn, "segment column layout");
 
+// Phase 2: the backward walk, segments last to first, all rays of the block
+// in lockstep.
 template <typename FT, int LC>
 __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4))) k_render_adjoint(AdjointLaunch A)
 {
@@ -437,12 +557,13 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     __shared__ long long win[kWin];
     __shared__ int box[kNSub][6];
     __shared__ int seg_max;
+    int kexp;
+    if (!adj_scale_exp(A.hdr, kexp)) return;  // zero or non-finite bound: nothing to add
     const RenderLaunch& a = A.r;
-    load_tables(a.tf, tf, op);
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    for (int q = tid; q < 1024; q += kRThreads) dtf[q] = 0;
     if (tid == 0) seg_max = 0;
-    __syncthreads();
+    for (int q = tid; q < 1024; q += kRThreads) dtf[q] = 0;
+    load_tables(a.tf, tf, op);
 
     const int i = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
@@ -450,37 +571,19 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     const long long nrays = W * H;
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
-    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
+    const float4* ck = reinterpret_cast<const float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
     // scratch column of one segment: [k][ray] D before sample k, with the
     // sample's clamp mask in the sign bits (D lies in [0,1]); then
     // [kSeg + m][ray] the position at the start of part m
     float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;
 
-    // phase 1: forward with a checkpoint (pos, dest) every kSeg samples
     Ray r;
     const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
-    long long n = 0;
-    if (active) {
-        float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
-        float dest[4] = {0.f, 0.f, 0.f, 0.f};
-        for (long long k = 0; k < a.max_samples; ++k) {
-            if ((k % kSeg) == 0) {
-                const long long c = k / kSeg;
-                ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
-                ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
-            }
-            Sample s;
-            eval_sample<FT, LC>(a, tf, op, r, pos, s);
-            composite(dest, s);
-            ++n;
-            for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
-            if (outside(pos, a.view) || dest[3] > 0.95f) break;
-        }
-        atomicMax(&seg_max, (int)((n + kSeg - 1) / kSeg));
-    }
+    const long long n = active ? A.nsamp[pix] : 0;
+    const int nseg = (int)((n + kSeg - 1) / kSeg);
+    if (nseg) atomicMax(&seg_max, nseg);
     __syncthreads();
 
-    // phase 2: segments last to first, all rays of the block in lockstep
     float G[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
         const float4 gi = reinterpret_cast<const float4*>(A.dimage)[pix];
@@ -489,7 +592,6 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
         G[2] = gi.z;
         G[3] = gi.w;
     }
-    const int nseg = (int)((n + kSeg - 1) / kSeg);
     const int smax = seg_max;
     for (int c = smax - 1; c >= 0; --c) {
         const bool mine = active && c < nseg;
@@ -559,8 +661,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
                                             __float_as_uint(dm.w)};
                     int mask = 0;
                     for (int q = 0; q < 4; ++q) mask |= (int)(db[q] >> 31) << q;
-                    const float4 dk = make_float4(__uint_as_float(db[0] & 0x7fffffffu), __uint_as_float(db[1] & 0x7fffffffu),
-                                                  __uint_as_float(db[2] & 0x7fffffffu), __uint_as_float(db[3] & 0x7fffffffu));
+                    const float dka = __uint_as_float(db[3] & 0x7fffffffu);
                     // the sample position: the forward fma chain from the part's start
                     float pk[3] = {ps.x, ps.y, ps.z};
                     for (long long t = ka; t < k; ++t)
@@ -569,7 +670,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
                     eval_sample<FT, LC>(a, tf, op, r, pk, s);
                     for (int q = 0; q < 4; ++q)
                         if (mask & (1 << q)) G[q] = 0.0f;
-                    const float t = 1.0f - dk.w;
+                    const float t = 1.0f - dka;
                     const float al = s.a;
                     const float src[4] = {s.rgb[0] * al, s.rgb[1] * al, s.rgb[2] * al, al};
                     float dsrc[4];
@@ -582,17 +683,9 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
                         drgb[q] = dsrc[q] * al;
                         da = da + dsrc[q] * s.rgb[q];
                     }
-                    const float b = 1.0f - s.araw;
-                    float dadraw;
-                    if (a.corr_mode == 0)
-                        dadraw = 1.0f;
-                    else if (b > 0.0f)
-                        dadraw = a.alpha_corr * powf(b, a.alpha_corr - 1.0f);
-                    else
-                        dadraw = 0.0f;
-                    const float draw = s.aclamp ? 0.0f : da * dadraw;
-                    for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m[q], A.fx_scale);
-                    splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, A.fx_scale);
+                    const float draw = s.aclamp ? 0.0f : da * dalpha_draw(a, s.araw);
+                    for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m[q], kexp);
+                    splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, kexp);
                     if (A.dsigma_fx) {
                         const float dsg = draw * s.ao * s.tfa;
                         float fx, fy, fz;
@@ -606,7 +699,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
                         for (int cz = 0; cz < 2; ++cz)
                             for (int cy = 0; cy < 2; ++cy)
                                 for (int cx = 0; cx < 2; ++cx) {
-                                    const long long v = to_fx(wxs[cx] * wys[cy] * wzs[cz] * dsg, A.fx_scale);
+                                    const long long v = to_fx(wxs[cx] * wys[cy] * wzs[cz] * dsg, kexp);
                                     if (!v) continue;
                                     if (use_win) {
                                         const int o = ((zs[cz] - bz0) * wy + (ys[cy] - by0)) * wx + (xs[cx] - bx0);
@@ -634,16 +727,24 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
             __syncthreads();
         }
     }
+    __syncthreads();
     for (int q = tid; q < 1024; q += kRThreads) {
         const long long v = dtf[q];
         if (v) atomicAdd((unsigned long long*)&A.dtf_fx[q], (unsigned long long)v);
     }
 }
 
-__global__ void k_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale)
+// fixed point back to float: one rounding (int64 -> float), then the exact
+// power-of-two scale
+__global__ void k_fixed_to_float(const unsigned long long* hdr, const long long* src, float* dst, long long n)
 {
-    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (float)((double)src[i] * inv_scale);
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int k;
+    if (adj_scale_exp(hdr, k))
+        dst[i] = ldexpf(__ll2float_rn(src[i]), -k);
+    else
+        dst[i] = (unsigned)hdr[1] >= 0x7f800000u ? __uint_as_float(0x7fc00000u) : 0.0f;
 }
 
 // ---- fused on-the-fly LIC ray caster --------------------------------------
@@ -1121,10 +1222,13 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
     return with_field_type(A.r.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
-        if (A.r.lc == 4)
+        if (A.r.lc == 4) {
+            hipLaunchKernelGGL((k_adjoint_forward<FT, 4>), grid, block, 0, s, A);
             hipLaunchKernelGGL((k_render_adjoint<FT, 4>), grid, block, 0, s, A);
-        else
+        } else {
+            hipLaunchKernelGGL((k_adjoint_forward<FT, 1>), grid, block, 0, s, A);
             hipLaunchKernelGGL((k_render_adjoint<FT, 1>), grid, block, 0, s, A);
+        }
     });
 }
 
@@ -1151,19 +1255,12 @@ hipError_t launch_stamp(unsigned long long* t0, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_fixed_to_float(const long long* src, float* dst, long long n, double inv_scale,
+hipError_t launch_fixed_to_float(const unsigned long long* hdr, const long long* src, float* dst, long long n,
                                  hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fixed_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n,
-                       inv_scale);
+    hipLaunchKernelGGL(k_fixed_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, hdr, src, dst, n);
     return hipGetLastError();
-}
-
-hipError_t launch_unpad_clamp_fixed(const long long* src, const int d[3], float* dst, double inv_scale,
-                                    hipStream_t s)
-{
-    return launch_fixed_to_float(src, dst, (long long)d[0] * d[1] * d[2], inv_scale, s);
 }
 
 }  // namespace vv
