@@ -277,6 +277,25 @@ vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
+/* vv_render_adjoint without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  dimage,
+ * dtf and dsigma (or NULL) are device pointers.  The kernels' scratch
+ * (checkpoints, segment columns, sample counts, fixed-point accumulators)
+ * lives in a static device array of 128 MiB that every GPU the library runs
+ * on holds until the process ends; a call whose view and opacity-scale grid
+ * need more returns VV_EINVAL (about 1.35 KB per pixel for rays of up to
+ * 512 samples, plus 8 B per opacity-scale voxel: a 256x256 view at sample
+ * distance 1/256 with an opacity scale on a 128^3 grid fits).  The results
+ * are those of vv_render_adjoint bit for bit.  The call is stream-ordered
+ * and returns without waiting, except that a transfer function changed
+ * since the last call is first uploaded with a blocking copy; line-
+ * illumination tables must already be on the device (VV_ESTATE otherwise:
+ * render once with the material).  Calls that overlap on one GPU, from any
+ * context, race on its array. */
+vv_status vv_render_adjoint_noalloc(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
+                                    float* dtf, float* dsigma);
+
 /* ---- batching: many timesteps and views in one captured launch --------
  * A batch belongs to a context and uses its LIC parameters, filter and
  * transfer function at run time.  Timesteps are resident volumes (field +

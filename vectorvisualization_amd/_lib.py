@@ -156,6 +156,7 @@ def _load() -> C.CDLL:
         "vv_render_slices": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_set_opacity_scale": (I, [P, P, I]),
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
+        "vv_render_adjoint_noalloc": (I, [P, C.POINTER(Camera), P, P, P]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
         "vv_batch_create": (I, [P, C.POINTER(P)]),
@@ -224,7 +225,7 @@ EXPORTED = [
     "vv_load_filter_png", "vv_tf_load_png", "vv_tf_save_png", "vv_batch_set_field",
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
-    "vv_make_illum_tables", "vv_render_fused", "vv_render_slices",
+    "vv_make_illum_tables", "vv_render_fused", "vv_render_slices", "vv_render_adjoint_noalloc",
 ]
 
 
@@ -726,6 +727,19 @@ class Context:
         ps = _buf(dsigma, "float32", self._lic_voxels(), "opacity-scale gradient")[0] if dsigma is not None else None
         with self.ordered(dimage, dtf, dsigma):
             self._chk(LIB.vv_render_adjoint(self._h, C.byref(cam), pi, pt, ps, dev))
+        return dtf, dsigma
+
+
+    def render_adjoint_noalloc(self, cam: Camera, dimage, dtf, dsigma=None):
+        """vv_render_adjoint_noalloc: device tensors only; no device allocation."""
+        _one_side([("dimage", dimage), ("dtf", dtf), ("dsigma", dsigma)], "render_adjoint_noalloc")
+        pi, dev = _buf(dimage, "float32", cam.width * cam.height * 4, "image gradient")
+        if not dev:
+            raise ValueError("render_adjoint_noalloc takes device tensors")
+        pt = _buf(dtf, "float32", 1024, "TF gradient")[0]
+        ps = _buf(dsigma, "float32", self._lic_voxels(), "opacity-scale gradient")[0] if dsigma is not None else None
+        with self.ordered(dimage, dtf, dsigma):
+            self._chk(LIB.vv_render_adjoint_noalloc(self._h, C.byref(cam), pi, pt, ps))
         return dtf, dsigma
 
 

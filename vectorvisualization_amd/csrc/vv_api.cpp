@@ -37,6 +37,16 @@ vv_status vv_create(vv_ctx** out, int device)
             delete c;
             return fail(nullptr, VV_EDEVICE, "hipStreamCreate: %s", hipGetErrorString(e));
         }
+        // the allocation-free adjoint's scratch on this device, and the TF
+        // table's device buffer, so that entry point never allocates
+        e = noalloc_scratch(device, &c->noalloc_scratch);
+        if (e == hipSuccess) e = c->tf_dev.ensure(sizeof(c->tf));
+        if (e != hipSuccess) {
+            c->tf_dev.release();
+            (void)hipStreamDestroy(c->stream);
+            delete c;
+            return fail(nullptr, VV_EDEVICE, "device set-up: %s", hipGetErrorString(e));
+        }
         default_params(&c->params);
         vv_default_line_material(&c->line_mat);
         uint8_t t5[256 * 5];
@@ -964,6 +974,33 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
         }
         HIPCHK(c, hipStreamSynchronize(c->stream));
         return VV_OK;
+    });
+}
+
+vv_status vv_render_adjoint_noalloc(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf,
+                                    float* dsigma)
+{
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!dimage || !dtf) return fail(c, VV_EINVAL, "dimage/dtf NULL");
+        if (!c->noalloc_scratch) return fail(c, VV_ESTATE, "no scratch array on this device");
+        if (c->params.illum_mode >= VV_ILLUM_MALLO && (c->illum_dirty || !c->illum_luts.p))
+            return fail(c, VV_ESTATE,
+                        "line-illumination tables are not on the device: render once with this material first");
+        HIPCHK(c, hipSetDevice(c->device));
+        AdjointLaunch A;
+        vv_status st = adjoint_setup(c, cam, A);
+        if (st) return st;
+        const long long nrays = (long long)cam->width * cam->height;
+        const long long nv = nvox(c->ldims);
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
+        if (SL.total > kNoAllocScratchBytes)
+            return fail(c, VV_EINVAL,
+                        "the allocation-free adjoint needs %zu bytes of scratch for a %dx%d view (%d checkpoints per "
+                        "ray%s); its array holds %zu",
+                        SL.total, cam->width, cam->height, A.max_ckpt, dsigma ? ", opacity-scale gradient" : "",
+                        kNoAllocScratchBytes);
+        return adjoint_enqueue(c, A, static_cast<char*>(c->noalloc_scratch), dimage, dtf, dsigma, nv);
     });
 }
 

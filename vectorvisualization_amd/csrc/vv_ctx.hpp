@@ -119,6 +119,7 @@ struct vv_ctx {
 
     // staging / scratch
     DevBuf stage, stage2, image, counter, adj_ckpt, adj_dimg, adj_out;
+    void* noalloc_scratch = nullptr;  // this device's scratch array (vv_render_adjoint_noalloc)
 };
 
 namespace vv_host {

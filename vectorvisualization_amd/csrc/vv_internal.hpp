@@ -139,6 +139,13 @@ struct AdjointLaunch {
     int use_window;               // accumulate dsigma in an LDS window per segment
 };
 
+// Scratch of vv_render_adjoint_noalloc: one static device array per GPU
+// (vv_render.hip).  noalloc_scratch() returns the current device's address of
+// it (hipGetSymbolAddress, looked up once per device and kept), to be called
+// with that device current.
+constexpr size_t kNoAllocScratchBytes = 128ull << 20;
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // Bytes of adjoint scratch for nrays rays, max_ckpt checkpoints and nv sigma
 // voxels (0 without sigma gradients), and the offsets of its parts.
 struct AdjScratchLayout {

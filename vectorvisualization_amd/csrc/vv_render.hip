@@ -10,6 +10,8 @@
 // One thread per pixel; the 256x4 RGBA transfer function and the 256-entry
 // opacity table live in LDS.  The build is -ffp-contract=off; fmaf marks
 // every fused operation of the shared formulation.
+#include <mutex>
+
 #include "vv_device.hpp"
 #include "vv_internal.hpp"
 
@@ -1214,6 +1216,35 @@ hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const 
             VV_GROUP_LAUNCH(FT, 1, false, 0);
     });
 #undef VV_GROUP_LAUNCH
+}
+
+// ---- scratch of the allocation-free adjoint ---------------------------------
+// vv_render_adjoint_noalloc takes its checkpoints, segment columns, sample
+// counts and fixed-point accumulators from this array instead of device
+// allocations.  Every GPU the library runs on holds its own copy until the
+// process ends.  On the host the array's name is only a stand-in: host code
+// uses the address noalloc_scratch() reports for the device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return hipSuccess;
 }
 
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
