@@ -47,6 +47,8 @@ def parse():
                     help="only 3 LIC launches and 3 copies of 256 MiB (rocprof counter runs)")
     ap.add_argument("--pmc-probe-fused", action="store_true",
                     help="only 3 fused on-the-fly LIC ray casts of the image (rocprof counter runs)")
+    ap.add_argument("--pmc-probe-frame", action="store_true",
+                    help="only 3 frames and 3 TF/opacity adjoints of the image (rocprof counter runs)")
     return ap.parse_args()
 
 
@@ -205,6 +207,23 @@ def main():
         ctx.synchronize()
         ctx.close()
         print("[bench] fused pmc probe done", file=sys.stderr, flush=True)
+        return
+
+    if args.pmc_probe_frame:
+        # counter runs of the ray caster and its adjoint on this step's LIC
+        # volume, with the adjoint's dL/dimage and opacity-scale gradient of
+        # the pipeline leg
+        ctx.compute_lic()
+        dimg = torch.randn(W * W * 4, dtype=torch.float32, device=dev) * 1e-3
+        dtf = torch.empty(1024, dtype=torch.float32, device=dev)
+        dsig = torch.empty(n ** 3, dtype=torch.float32, device=dev)
+        for _ in range(3):
+            ctx.render(cam, out=image)
+        for _ in range(3):
+            ctx.render_adjoint(cam, dimg, dtf, dsig)
+        ctx.synchronize()
+        ctx.close()
+        print("[bench] frame pmc probe done", file=sys.stderr, flush=True)
         return
 
     print("[bench] warmup", file=sys.stderr, flush=True)

@@ -254,9 +254,24 @@ vv_status vv_render_fused(vv_ctx* ctx, const vv_camera* cam, float* image, int o
  * the slice plane inside the box.  It is skipped when the accumulated alpha has reached
  * 0.95 or its TF alpha is at most 0.05 (no LIC computed); otherwise the LIC of its position
  * is computed, shaded and composited, and the result rounded to FP16 (the RGBA16F render
- * target) before the next slice.  samples: the composited samples. */
+ * target) before the next slice.  The TF is indexed by the field's .a (|v|/max), as the
+ * slicing shaders do (lic3d_slicing_fragment.glsl:43), whatever tf_index says.
+ * samples: the composited samples. */
 vv_status vv_render_slices(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
                            uint64_t* samples);
+
+/* Renderer::sliceVolume without render targets, the reference's default
+ * (_useFBO(false), VV/renderer.cpp:33,1149-1157,1233-1258), with
+ * lic3d_slicingblend_fragment.glsl: the same slices, each fragment's colour
+ * (rgb premultiplied; 0 where the TF alpha is at most 0.05, with no LIC
+ * computed) blended into an 8-bit RGBA framebuffer cleared to 0 with
+ * glBlendFunc(GL_ONE_MINUS_DST_ALPHA, GL_ONE), i.e. dst = src (1 - dst.a) +
+ * dst, no 0.95 test; then a white quad blended the same way (the
+ * background).  The framebuffer is fixed point: the fragment colour is
+ * clamped to [0,1] and every blend result stored as round(255 x).  image:
+ * width*height*4 bytes (RGBA).  samples: the blended fragments. */
+vv_status vv_render_slices_blend(vv_ctx* ctx, const vv_camera* cam, uint8_t* image, int on_device,
+                                 uint64_t* samples);
 
 /* ---- differentiation ---------------------------------------------------
  * Per-voxel opacity scale sigma (N float32 at the LIC-volume resolution,
@@ -277,6 +292,19 @@ vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
+/* vv_render that also keeps the adjoint's forward pass (checkpoints, sample
+ * counts, per-ray bounds) in the context.  No counterpart in the reference:
+ * a training step that renders and then differentiates the same view calls
+ * this instead of vv_render, and the following vv_render_adjoint with the
+ * same camera skips its own forward pass.  The image and the sample count
+ * are vv_render's bit for bit.  Any change to the scene (field, noise, LIC
+ * volume or parameters, transfer function, opacity scale, line material)
+ * or another camera makes the next vv_render_adjoint run the forward pass
+ * again, as does a vv_render_adjoint in between (its backward walk reuses
+ * the buffer); the gradients are the same either way. */
+vv_status vv_render_record(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
+                           uint64_t* samples);
+
 /* vv_render_adjoint without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  dimage,
@@ -284,8 +312,8 @@ vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dima
  * (checkpoints, segment columns, sample counts, fixed-point accumulators)
  * lives in a static device array of 128 MiB that every GPU the library runs
  * on holds until the process ends; a call whose view and opacity-scale grid
- * need more returns VV_EINVAL (about 1.35 KB per pixel for rays of up to
- * 512 samples, plus 8 B per opacity-scale voxel: a 256x256 view at sample
+ * need more returns VV_EINVAL (about 0.6 KB per pixel for rays of up to
+ * 512 samples, plus 8 B per opacity-scale voxel: a 384x384 view at sample
  * distance 1/256 with an opacity scale on a 128^3 grid fits).  The results
  * are those of vv_render_adjoint bit for bit.  The call is stream-ordered
  * and returns without waiting, except that a transfer function changed

@@ -154,6 +154,10 @@ def _load(path: Path = LIB_PATH) -> C.CDLL:
                                             C.POINTER(View), I, P, P]),
         "orc_render_slices_f64": (C.c_long, [P, P, P, I, P, P, P, P, I, P, P, P, P, C.POINTER(Params),
                                             C.POINTER(View), I, P, P]),
+        "orc_render_slices_blend_f32": (C.c_long, [P, P, P, I, P, P, P, P, I, P, P, P, P, C.POINTER(Params),
+                                                  C.POINTER(View), I, P, P]),
+        "orc_render_slices_blend_f64": (C.c_long, [P, P, P, I, P, P, P, P, I, P, P, P, P, C.POINTER(Params),
+                                                  C.POINTER(View), I, P, P]),
         "orc_raycast_f32": (C.c_long, [P, P, P, I, P, P, P, P, C.POINTER(Params), C.POINTER(View),
                                        I, P, P]),
         "orc_raycast_f64": (C.c_long, [P, P, P, I, P, P, P, P, C.POINTER(Params), C.POINTER(View),
@@ -443,6 +447,32 @@ def render_slices(field4, fdims, noise, noise_ch, ndims, lut, tf_rgba, tf_opac, 
     img = np.zeros(v.width * v.height * 4, dt)
     ns = np.zeros(v.width * v.height, np.int32)
     fn = LIB.orc_render_slices_f64 if f64 else LIB.orc_render_slices_f32
+    fd, nd = _i3(fdims), _i3(ndims)
+    sd = _i3(sdims) if sdims is not None else None
+    gd = _i3(sigma_dims) if sigma_dims is not None else None
+    total = fn(_p(field4), _p(fd), _p(noise), noise_ch, _p(nd), _p(scalar), _p(sd), _p(lut), lut.size,
+               _p(tf_rgba), _p(tf_opac), _p(sigma), _p(gd), C.byref(p), C.byref(v), nthreads, _p(img),
+               _p(ns))
+    return img, ns, int(total)
+
+
+def render_slices_blend(field4, fdims, noise, noise_ch, ndims, lut, tf_rgba, tf_opac, p: Params, v: View,
+                        scalar=None, sdims=None, sigma=None, sigma_dims=None, nthreads=0, f64=None):
+    """The same slices blended into an 8-bit RGBA framebuffer over a white
+    background (lic3d_slicingblend_fragment.glsl): (uint8 image, samples, total)."""
+    if f64 is None:
+        f64 = field4.dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    field4 = np.ascontiguousarray(field4, dt)
+    noise = np.ascontiguousarray(noise, np.uint8)
+    lut = np.ascontiguousarray(lut, np.uint8)
+    scalar = None if scalar is None else np.ascontiguousarray(scalar, np.uint8)
+    tf_rgba = np.ascontiguousarray(tf_rgba, dt)
+    tf_opac = np.ascontiguousarray(tf_opac, dt)
+    sigma = None if sigma is None else np.ascontiguousarray(sigma, dt)
+    img = np.zeros(v.width * v.height * 4, np.uint8)
+    ns = np.zeros(v.width * v.height, np.int32)
+    fn = LIB.orc_render_slices_blend_f64 if f64 else LIB.orc_render_slices_blend_f32
     fd, nd = _i3(fdims), _i3(ndims)
     sd = _i3(sdims) if sdims is not None else None
     gd = _i3(sigma_dims) if sigma_dims is not None else None

@@ -215,6 +215,25 @@ long orc_render_slices_f64(const double* field4, const int fdims[3],
                            const double* sigma, const int sigma_dims[3],
                            const orc_params* p, const orc_view* v,
                            int nthreads, double* image, int* nsamples);
+/* The same slices blended into an 8-bit RGBA framebuffer (the reference's
+ * default output, lic3d_slicingblend_fragment.glsl): image holds 4 bytes per
+ * pixel. */
+long orc_render_slices_blend_f32(const float* field4, const int fdims[3],
+                                 const uint8_t* noise, int noise_ch, const int ndims[3],
+                                 const uint8_t* scalar, const int sdims[3],
+                                 const uint8_t* lut, int lut_width,
+                                 const float* tf_rgba, const float* tf_opac,
+                                 const float* sigma, const int sigma_dims[3],
+                                 const orc_params* p, const orc_view* v,
+                                 int nthreads, uint8_t* image, int* nsamples);
+long orc_render_slices_blend_f64(const double* field4, const int fdims[3],
+                                 const uint8_t* noise, int noise_ch, const int ndims[3],
+                                 const uint8_t* scalar, const int sdims[3],
+                                 const uint8_t* lut, int lut_width,
+                                 const double* tf_rgba, const double* tf_opac,
+                                 const double* sigma, const int sigma_dims[3],
+                                 const orc_params* p, const orc_view* v,
+                                 int nthreads, uint8_t* image, int* nsamples);
 
 /* Reverse-mode derivative of orc_raycast_* with respect to tf_rgba (256x4)
  * and the per-voxel opacity scale sigma (ldims voxels), for a loss whose

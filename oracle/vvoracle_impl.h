@@ -703,14 +703,38 @@ long FN(orc_raycast_fused)(const R* field4, const int fdims[3],
 /* ------------------------------------------------------------------ */
 static inline R FN(half_rn)(R x) { return (R)orc_f16_to_f32(orc_f32_to_f16((float)x)); }
 
-long FN(orc_render_slices)(const R* field4, const int fdims[3],
-                           const uint8_t* noise, int noise_ch, const int ndims[3],
-                           const uint8_t* scalar, const int sdims[3],
-                           const uint8_t* lut, int lut_width,
-                           const R* tf_rgba, const R* tf_opac,
-                           const R* sigma, const int sigma_dims[3],
-                           const orc_params* p, const orc_view* v,
-                           int nthreads, R* image, int* nsamples)
+/* The reference's default output (no render targets, VV/renderer.cpp:
+ * 1149-1157,1233-1258; lic3d_slicingblend_fragment.glsl): the slices' colours
+ * blended into an 8-bit RGBA framebuffer cleared to 0 with
+ * glBlendFunc(GL_ONE_MINUS_DST_ALPHA, GL_ONE): dst = src (1 - dst.a) + dst,
+ * the fragment colour clamped to [0,1] and each result stored as
+ * round(255 x); no 0.95 test; then a white quad blended the same way.  The
+ * blend arithmetic is float (the byte values read as b/255). */
+static inline float FN(unorm8_rn)(float x)
+{
+    x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+    return floorf(fmaf(x, 255.0f, 0.5f));
+}
+
+static inline void FN(blend_u8)(float dst[4], const R src[4])
+{
+    const float t = 1.0f - dst[3] * (1.0f / 255.0f);
+    for (int a = 0; a < 4; ++a) {
+        float sc = (float)src[a];
+        sc = sc < 0.0f ? 0.0f : (sc > 1.0f ? 1.0f : sc);
+        dst[a] = FN(unorm8_rn)(fmaf(t, sc, dst[a] * (1.0f / 255.0f)));
+    }
+}
+
+/* blend == 0: RGBA16F target into image (R); 1: framebuffer bytes into img8 */
+static long FN(render_slices_impl)(const R* field4, const int fdims[3],
+                                   const uint8_t* noise, int noise_ch, const int ndims[3],
+                                   const uint8_t* scalar, const int sdims[3],
+                                   const uint8_t* lut, int lut_width,
+                                   const R* tf_rgba, const R* tf_opac,
+                                   const R* sigma, const int sigma_dims[3],
+                                   const orc_params* p, const orc_view* v,
+                                   int nthreads, int blend, R* image, uint8_t* img8, int* nsamples)
 {
     /* slicing setup (VV/slicing.cpp:42-100), in float as the reference */
     float sv[3], half[3], ctr[3];
@@ -730,10 +754,14 @@ long FN(orc_render_slices)(const R* field4, const int fdims[3],
     const int nw = 1 + p->steps_forward + p->steps_backward;
     R* w = (R*)malloc(sizeof(R) * nw);
     FN(orc_lic_weights)(lut, lut_width, p->steps_forward, p->steps_backward, w);
+    /* both slicing shaders index the TF with vectorData.a
+     * (lic3d_slicing_fragment.glsl:43, lic3d_slicingblend_fragment.glsl:40) */
+    orc_params ps = *p;
+    ps.tf_index = 1;
     FN(rc_ctx) c;
     c.field = field4; c.lic = NULL; c.lc = noise_ch;
     for (int a = 0; a < 3; ++a) { c.fd[a] = fdims[a]; c.ld[a] = sigma_dims ? sigma_dims[a] : fdims[a]; }
-    c.tf = tf_rgba; c.op = tf_opac; c.sigma = sigma; c.p = p;
+    c.tf = tf_rgba; c.op = tf_opac; c.sigma = sigma; c.p = &ps;
     R lraw[3] = {(R)p->light[0], (R)p->light[1], (R)p->light[2]};
     FN(normalize3)(lraw, c.light);
     long total = 0;
@@ -754,8 +782,10 @@ long FN(orc_render_slices)(const R* field4, const int fdims[3],
             const R vdir = (R)sv[0] * d[0] + (R)sv[1] * d[1] + (R)sv[2] * d[2];
             const R vcam = (R)sv[0] * rc[0] + (R)sv[1] * rc[1] + (R)sv[2] * rc[2];
             int ns = 0;
+            float fb[4] = {0.0f, 0.0f, 0.0f, 0.0f};  /* framebuffer bytes (blend) */
             for (int k = 0; k < nsl && vdir != K(0); ++k) {
-                if (!(dest[3] < K(0.95))) break;
+                if (!blend && !(dest[3] < K(0.95))) break;
+                if (blend && fb[3] == 255.0f) break;  /* dst.a = 1: later slices add nothing */
                 const float dkf = -0.5f * D + ((float)k + 0.5f) * D / (float)nsl;
                 const R sk = ((R)dkf - vcam) / vdir;
                 if (!(sk > K(0))) continue;
@@ -775,23 +805,59 @@ long FN(orc_render_slices)(const R* field4, const int fdims[3],
                 FN(cell) cl;
                 R tfu;
                 FN(cell_clamp)(pos, fdims, &cl);
-                ORC_FIELD_TRILIN(p, field4, p->tf_index ? 3 : 2, fdims, cl, tfu);
+                ORC_FIELD_TRILIN(p, field4, 3, fdims, cl, tfu);
                 if (!(FN(lut1d)(tf_rgba, 256, 4, 3, tfu) > K(0.05))) continue;
                 R l4[4];
                 FN(lic_voxel)(&lc, w, pos, l4);
                 FN(rc_sample) s;
                 FN(rc_shade)(&c, pos, dir, l4, &s);
                 R src[4] = {s.rgb[0] * s.a, s.rgb[1] * s.a, s.rgb[2] * s.a, s.a};
-                R t = K(1) - dest[3];
-                for (int a = 0; a < 4; ++a) dest[a] = FN(half_rn)(FN(clamp01)(FMA(t, src[a], dest[a])));
+                if (blend) {
+                    FN(blend_u8)(fb, src);
+                } else {
+                    R t = K(1) - dest[3];
+                    for (int a = 0; a < 4; ++a) dest[a] = FN(half_rn)(FN(clamp01)(FMA(t, src[a], dest[a])));
+                }
                 ++ns;
             }
-            for (int a = 0; a < 4; ++a) image[4 * pix + a] = dest[a];
+            if (blend) {
+                const R white[4] = {K(1), K(1), K(1), K(1)};
+                FN(blend_u8)(fb, white);
+                for (int a = 0; a < 4; ++a) img8[4 * pix + a] = (uint8_t)fb[a];
+            } else {
+                for (int a = 0; a < 4; ++a) image[4 * pix + a] = dest[a];
+            }
             if (nsamples) nsamples[pix] = ns;
             total += ns;
         }
     free(w);
     return total;
+}
+
+long FN(orc_render_slices)(const R* field4, const int fdims[3],
+                           const uint8_t* noise, int noise_ch, const int ndims[3],
+                           const uint8_t* scalar, const int sdims[3],
+                           const uint8_t* lut, int lut_width,
+                           const R* tf_rgba, const R* tf_opac,
+                           const R* sigma, const int sigma_dims[3],
+                           const orc_params* p, const orc_view* v,
+                           int nthreads, R* image, int* nsamples)
+{
+    return FN(render_slices_impl)(field4, fdims, noise, noise_ch, ndims, scalar, sdims, lut, lut_width, tf_rgba,
+                                  tf_opac, sigma, sigma_dims, p, v, nthreads, 0, image, NULL, nsamples);
+}
+
+long FN(orc_render_slices_blend)(const R* field4, const int fdims[3],
+                                 const uint8_t* noise, int noise_ch, const int ndims[3],
+                                 const uint8_t* scalar, const int sdims[3],
+                                 const uint8_t* lut, int lut_width,
+                                 const R* tf_rgba, const R* tf_opac,
+                                 const R* sigma, const int sigma_dims[3],
+                                 const orc_params* p, const orc_view* v,
+                                 int nthreads, uint8_t* image, int* nsamples)
+{
+    return FN(render_slices_impl)(field4, fdims, noise, noise_ch, ndims, scalar, sdims, lut, lut_width, tf_rgba,
+                                  tf_opac, sigma, sigma_dims, p, v, nthreads, 1, NULL, image, nsamples);
 }
 
 #if ORC_DOUBLE

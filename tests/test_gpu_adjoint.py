@@ -225,3 +225,89 @@ def test_adjoint_of_zero_and_nonfinite_image_gradients(ctx):
     g[4 * hit + 1] = np.inf
     dtf, dsig = ctx.render_adjoint(cam, g)
     assert np.all(np.isnan(dtf)) and np.all(np.isnan(dsig))
+
+
+@pytest.mark.parametrize("grad,illum", [(False, None), (True, vv.ILLUM_ZOECKLER)])
+def test_render_record_image_and_reused_forward_pass(ctx, grad, illum):
+    """vv_render_record: its image and sample count are vv_render's bit for
+    bit, and the adjoint that reuses its forward pass gives the bits of an
+    adjoint that runs its own, with and without the opacity-scale gradient."""
+    c, cam, rgba, op, sigma, oparams, view = _scene(ctx, n=24, W=96, H=80, sigma_seed=4, grad=grad,
+                                                    step=1.0 / 256.0, illum=illum)
+    dimg = np.random.default_rng(8).normal(size=96 * 80 * 4).astype(np.float32)
+    ref_img, ref_ns = ctx.render(cam, count_samples=True)
+    ref_tf, ref_sg = ctx.render_adjoint(cam, dimg)
+    ref_tf_only, _ = ctx.render_adjoint(cam, dimg, want_sigma=False)
+    img, ns = ctx.render_record(cam, count_samples=True)
+    np.testing.assert_array_equal(img, ref_img)
+    assert ns == ref_ns and ns > 0
+    tf, sg = ctx.render_adjoint(cam, dimg)
+    np.testing.assert_array_equal(tf, ref_tf)
+    np.testing.assert_array_equal(sg, ref_sg)
+    # device image, then the TF-only gradient from the record
+    dev = torch.empty(96 * 80 * 4, dtype=torch.float32, device="cuda")
+    ctx.render_record(cam, out=dev)
+    ctx.synchronize()
+    np.testing.assert_array_equal(dev.cpu().numpy(), ref_img)
+    tf2, _ = ctx.render_adjoint(cam, dimg, want_sigma=False)
+    np.testing.assert_array_equal(tf2, ref_tf_only)
+    # a second adjoint after one that consumed the record runs its own forward pass
+    tf3, sg3 = ctx.render_adjoint(cam, dimg)
+    np.testing.assert_array_equal(tf3, ref_tf)
+    np.testing.assert_array_equal(sg3, ref_sg)
+
+
+def test_render_record_is_invalidated_by_scene_and_camera_changes(ctx):
+    """After a record, a changed transfer function, opacity scale, LIC volume or
+    camera makes the adjoint run its own forward pass: every result equals a
+    plain adjoint of the new state (a stale record would give the old
+    scene's gradients)."""
+    c, cam, rgba, op, sigma, oparams, view = _scene(ctx, n=24, W=64, H=56, sigma_seed=4)
+    dimg = np.random.default_rng(9).normal(size=64 * 56 * 4).astype(np.float32)
+    rgba2, op2 = fields.tf_random(5)
+
+    def plain():
+        return ctx.render_adjoint(cam, dimg)
+
+    def recorded(change):
+        ctx.render_record(cam)
+        change()
+        return ctx.render_adjoint(cam, dimg)
+
+    stale_tf, stale_sg = plain()
+    cases = [
+        ("tf", lambda: ctx.set_tf(rgba2, op2), lambda: ctx.set_tf(rgba, op)),
+        ("sigma", lambda: ctx.set_opacity_scale(sigma * np.float32(0.5)), lambda: ctx.set_opacity_scale(sigma)),
+    ]
+    for name, change, restore in cases:
+        tf, sg = recorded(change)
+        want_tf, want_sg = plain()
+        assert not np.array_equal(want_tf, stale_tf), name
+        np.testing.assert_array_equal(tf, want_tf, err_msg=name)
+        np.testing.assert_array_equal(sg, want_sg, err_msg=name)
+        restore()
+    # another camera: the record of the first does not apply
+    cam2 = vv.default_camera(64, 56, ctx.field_geometry()[2])
+    ctx.render_record(cam)
+    tf, sg = ctx.render_adjoint(cam2, dimg)
+    want_tf, want_sg = ctx.render_adjoint(cam2, dimg)
+    assert not np.array_equal(want_tf, stale_tf)
+    np.testing.assert_array_equal(tf, want_tf)
+    np.testing.assert_array_equal(sg, want_sg)
+    # unchanged TF tables set again keep the record
+    ctx.render_record(cam)
+    ctx.set_tf(rgba, op)
+    tf, sg = ctx.render_adjoint(cam, dimg)
+    np.testing.assert_array_equal(tf, stale_tf)
+    np.testing.assert_array_equal(sg, stale_sg)
+    # a new LIC volume (which also drops the opacity scale)
+    ldims, lch = ctx.lic_info()
+    lic = ctx.download_lic()
+    ctx.set_opacity_scale(None)
+    base_tf, _ = ctx.render_adjoint(cam, dimg, want_sigma=False)
+    ctx.render_record(cam)
+    ctx.upload_lic(lic * np.float32(0.75), lch, ldims)
+    tf, _ = ctx.render_adjoint(cam, dimg, want_sigma=False)
+    want_tf, _ = ctx.render_adjoint(cam, dimg, want_sigma=False)
+    assert not np.array_equal(want_tf, base_tf)
+    np.testing.assert_array_equal(tf, want_tf)

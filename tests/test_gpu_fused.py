@@ -178,3 +178,59 @@ def test_slices_gradient_illumination(ctx, group):
     assert nsamp == total and total > 0
     assert np.all(np.isfinite(img))
     assert rel_err(img, ref) <= 1e-5
+
+
+def _blend_case(ctx, c, w, h, quat=(0.1, 0.3, -0.05, 0.95), illum=None, luts=None, step=1.0 / 64.0):
+    """vv_render_slices_blend against the oracle's blended slices (bytes)."""
+    p = c["params"]
+    p.step_size_vol = step
+    if illum is not None:
+        p.illum_mode = illum
+    ctx.set_lic_params(p)
+    ext, sc, cen = ctx.field_geometry()
+    cam = vv.default_camera(w, h, cen)
+    for i in range(4):
+        cam.quat[i] = quat[i]
+    rgba, op = vv.fields.tf_magnitude(0.02, 0.5)
+    ctx.set_tf(rgba, op)
+    img, nsamp = ctx.render_slices_blend(cam, count_samples=True)
+    view = orc.view_setup(w, h, quat=[float(np.float32(q)) for q in quat], center=cen, extent=ext, scale=sc)
+    op_ = c["op"]
+    op_.step_size_vol = step
+    op_.alpha_correction = float(np.float32(step) * np.float32(128.0))
+    op_.light[0], op_.light[1], op_.light[2] = list(cam.light_pos)
+    if illum is not None:
+        op_.illum_mode = illum
+    if luts is not None:
+        orc.attach_luts(op_, luts)
+    ref, ns, total = orc.render_slices_blend(c["f4"], c["dims"], c["noise"], c["nch"], c["dims"], c["lut"], rgba,
+                                             op, op_, view, scalar=c["scalar"],
+                                             sdims=c["dims"] if c["scalar"] is not None else None)
+    d = np.abs(img.astype(np.int32) - ref.astype(np.int32))
+    print(f"blend: bytes differing {np.count_nonzero(d)} of {d.size}, max {d.max()}; samples {nsamp} "
+          f"(oracle {total})", flush=True)
+    return img, ref, d, nsamp, total
+
+
+@pytest.mark.parametrize("storage,group", [(vv.STORE_FP32, 0), (vv.STORE_FP32, 1), (vv.STORE_FP16, 0),
+                                           (vv.STORE_UNORM8, 0)])
+def test_slices_blend_into_8bit_framebuffer(ctx, storage, group):
+    """The reference's default slicing output: bytes equal to the oracle's.
+    GPU and oracle LIC values agree to about 1e-6 relative, so a byte may
+    differ by one step where a blend result lies within that of a rounding
+    boundary; the bound allows one step on 0.1 % of the bytes."""
+    c = lic_setup(ctx, n=16, kind="rankine", L=6, noise_kind=vv.NOISE_UNIFORM, storage=storage,
+                  gradient_scale=3.0)
+    ctx.set_tuning("render.ray_group", group)
+    img, ref, d, nsamp, total = _blend_case(ctx, c, 40, 32)
+    ctx.set_tuning("render.ray_group", 0)
+    assert total > 0 and np.any(ref[3::4] == 255) and np.any(ref[0::4] < 250)
+    assert d.max() <= 1 and np.count_nonzero(d) <= 0.001 * d.size
+    assert nsamp == total or abs(nsamp - total) <= 0.001 * total
+
+
+def test_slices_blend_line_illumination(ctx):
+    c = lic_setup(ctx, n=14, kind="abc", L=5, freq=1.5, gradient_scale=3.0)
+    img, ref, d, nsamp, total = _blend_case(ctx, c, 36, 30, illum=vv.ILLUM_MALLO, luts=orc.illum_luts())
+    assert total > 0
+    assert d.max() <= 1 and np.count_nonzero(d) <= 0.001 * d.size

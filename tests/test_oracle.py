@@ -694,8 +694,213 @@ def test_lic_speed_of_flow_scales_each_step_by_the_magnitude():
         ref = sum(N(xs + k * m[z] * he) for k in ks) * inv / (F + B) * 30.0
         for y in (2, 11):
             np.testing.assert_allclose(out[z, y], ref, rtol=1e-9)
+    # magnitude varying along x: each step moves by M(x_k) h_e, both Heun
+    # displacements scaled by the magnitude at the step's start x_k
+    # (licdir2 *= step.a with step not yet re-read), M the linear CLAMP
+    # interpolation of |v|/max along x
+    mx = 0.3 + 0.7 * np.arange(n) / (n - 1)
+    v2 = np.zeros((n, n, n, 3), np.float32)
+    v2[..., 0] = mx[None, None, :]
+    f4b = orc.encode_field(v2, (n, n, n), f64=True)
+    outb = orc.lic(f4b, (n, n, n), noise, 1, (n, n, n), lut, p).reshape(n, n, n)
+    amag = f4b.reshape(n, n, n, 4)[0, 0, :, 3]
+
+    def M(u):
+        return np.interp(u * n - 0.5, np.arange(n), amag)
+
+    refb = N(xs) * 1.0
+    for sgn, steps in ((-1, B), (1, F)):
+        x = xs.copy()
+        for _ in range(steps):
+            d = sgn * M(x) * he
+            x = x + 0.5 * (d + d)
+            refb = refb + N(x)
+    refb = refb * inv / (F + B) * 30.0
+    for z in (0, 9):
+        np.testing.assert_allclose(outb[z, 4], refb, rtol=1e-9)
     p.speed_of_flow = 0
     plain = orc.lic(f4, (n, n, n), noise, 1, (n, n, n), lut, p).reshape(n, n, n)
     ref = sum(N(xs + k * he) for k in ks) * inv / (F + B) * 30.0
     np.testing.assert_allclose(plain[5, 2], ref, rtol=1e-9)
     assert np.max(np.abs(plain[5] - out[5])) > 1e-3
+
+
+def _blend_bytes(src, count):
+    """glBlendFunc(GL_ONE_MINUS_DST_ALPHA, GL_ONE) into an 8-bit RGBA
+    framebuffer cleared to 0: `count` equal fragments src (premultiplied,
+    clamped to [0,1]), then the white background quad; every result stored
+    as round(255 x).  Double precision, written from the GL blend equation.
+    Returns the pixel and the number of fragments before its alpha reached
+    255 (after which a fragment changes nothing)."""
+    d = np.zeros(4)
+    s = np.clip(np.asarray(src, np.float64), 0.0, 1.0)
+    used = 0
+    for _ in range(count):
+        if d[3] == 255.0:
+            break
+        d = np.floor(np.clip(s * (1.0 - d[3] / 255.0) + d / 255.0, 0.0, 1.0) * 255.0 + 0.5)
+        used += 1
+    d = np.floor(np.clip(1.0 * (1.0 - d[3] / 255.0) + d / 255.0, 0.0, 1.0) * 255.0 + 0.5)
+    return d.astype(np.uint8), used
+
+
+@pytest.mark.parametrize("alpha,rgb", [(0.5, (1.0, 1.0, 1.0)), (0.06, (0.9, 0.4, 0.2)), (0.3, (8.0, 0.5, 0.1))])
+def test_slicing_blend_under_operator_closed_form(alpha, rgb):
+    """lic3d_slicingblend_fragment.glsl blended with GL_ONE_MINUS_DST_ALPHA,
+    GL_ONE (VV/renderer.cpp:1149-1157), then the white quad (:1233-1258).
+    The scene's LIC is constant, so every slice adds the same fragment: the
+    pixel is the closed-form sequence above.  Per-sample alpha 0.5 shows
+    there is no 0.95 test (the framebuffer alpha keeps rising to 255); an rgb
+    above 1 shows the clamp of the fragment colour."""
+    f4, noise, lut, p, rgba, op, view, n = _slicing_case(alpha)
+    rgba = np.tile(np.array([*rgb, alpha], np.float32), 256)
+    img, ns, total = orc.render_slices_blend(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    # illumLIC: rgb = l * tf.rgb * illum_scale, l = 255/255 * (F+B+1) * 0.5 / (F+B) * 1.0
+    lic = 1.0 * 5 * 0.5 / 4 * 1.0
+    a = alpha  # opacity table 1, alpha correction 1
+    src = [lic * rgb[0] * a, lic * rgb[1] * a, lic * rgb[2] * a, a]
+    img = img.reshape(-1, 4)
+    # the central ray crosses all 129 slices (no 0.95 test: alpha 0.5 goes on
+    # until the framebuffer alpha is 255)
+    c = 7 * 15 + 7
+    want, used = _blend_bytes(src, 129)
+    np.testing.assert_array_equal(img[c], want)
+    assert ns[c] == used
+    assert used > 5 if alpha == 0.5 else used == 129
+    if rgb[0] > 1.0:
+        assert src[0] > 1.0  # the fragment colour is clamped before blending
+    for q in range(img.shape[0]):
+        np.testing.assert_array_equal(img[q], _blend_bytes(src, int(ns[q]))[0])
+    assert np.all(img[:, 3] == 255)
+
+
+def test_slicing_blend_skipped_fragments_leave_the_white_background():
+    f4, noise, lut, p, rgba, op, view, n = _slicing_case(0.05)
+    img, ns, total = orc.render_slices_blend(f4, (n,) * 3, noise, 1, (n,) * 3, lut, rgba, op, p, view)
+    assert total == 0 and np.all(img == 255)
+
+
+@pytest.mark.parametrize("sg", [0.4, 1.7, 40.0])
+def test_raycast_opacity_scale_closed_form(sg):
+    """Opacity scale sigma (the adjoint's second input): each sample's alpha
+    is ao * tf.a * sigma, clamped to [0, 1], before the alpha correction.  A
+    constant volume and a constant sigma give dest.a = 1 - (1 - a sigma)^k,
+    or 1 after one sample when a sigma >= 1."""
+    n = 8
+    tfv = (0.8, 0.6, 0.4, 0.1)
+    f4, lic, rgba, op, p, v = _const_scene(n, 0.5, tfv, 0.4)
+    sigma = np.full(n ** 3, sg, np.float64)
+    img, ns, _ = orc.raycast(f4, (n, n, n), lic, 1, (n, n, n), rgba, op, p, v, sigma=sigma, f64=True)
+    a = min(float(np.float32(0.4)) * float(np.float32(0.1)) * sg, 1.0)
+    k = ns.reshape(33, 33)[16, 16]
+    want_k = 129
+    if a >= 1.0:
+        want_k = 1
+    else:
+        for j in range(1, 130):  # early termination once dest.a > 0.95
+            if 1 - (1 - a) ** j > 0.95:
+                want_k = j
+                break
+    assert k == want_k
+    px = img.reshape(33, 33, 4)[16, 16]
+    np.testing.assert_allclose(px[3], 1 - (1 - a) ** k, rtol=1e-9)
+    col = np.float32(tfv[:3]).astype(np.float64) * 0.5  # tf.rgb * l, premultiplied by the same alpha
+    np.testing.assert_allclose(px[:3], col * (1 - (1 - a) ** k), rtol=1e-9)
+
+
+def test_lic_gate_and_frequency_match_scipy_brute_force():
+    """freqSampling (inc_lic.glsl:71-90): the noise at x * freqScale, and 0
+    unless the scalar volume's value at x lies in (0.1, 0.3); the scalar
+    volume is sampled with CLAMP_TO_EDGE, the noise with REPEAT.  Brute force
+    on scipy.ndimage.map_coordinates, independent of the oracle."""
+    n, F, B, h = 8, 5, 6, 0.03
+    freq = float(np.float32(1.7))  # the parameter block holds floats
+    lo, hi = float(np.float32(0.1)), float(np.float32(0.3))
+    lut, inv, p = _case(n, F, B, h, kind=1)
+    p.freq_scale = freq
+    p.gate_enable = 1
+    rng = np.random.default_rng(12)
+    v = rng.normal(size=(n, n, n, 3)).astype(np.float32)
+    f4 = orc.encode_field(v, (n, n, n), f64=True).reshape(n, n, n, 4)
+    nd = (6, 7, 5)
+    noise = rng.integers(0, 256, (nd[2], nd[1], nd[0]), dtype=np.uint8)
+    scalar = rng.integers(0, 120, (n, n, n), dtype=np.uint8)  # values around the gate
+    out = orc.lic(f4.ravel(), (n, n, n), noise.ravel(), 1, nd, lut, p, scalar=scalar.ravel(),
+                  sdims=(n, n, n))
+
+    def samp(vol, pts, dims, mode):
+        c = np.stack([pts[:, 2] * dims[2], pts[:, 1] * dims[1], pts[:, 0] * dims[0]]) - 0.5
+        return ndimage.map_coordinates(vol, c, order=1, mode=mode)
+
+    def F3(pts):
+        return np.stack([samp(f4[..., ch], pts, (n, n, n), "nearest") for ch in range(3)], 1)
+
+    def Nz(pts):
+        s = samp(scalar.astype(np.float64), pts, (n, n, n), "nearest") / 255.0
+        val = samp(noise.astype(np.float64), pts * freq, nd, "grid-wrap") / 255.0
+        return np.where((s > lo) & (s < hi), val, 0.0)
+
+    g = (np.indices((n, n, n)).reshape(3, -1)[::-1].T + 0.5) / n
+    he = float(np.float64(np.float32(h)) * np.float64(np.float32(0.3)))
+    w = orc.lic_weights(lut, F, B, f64=True)
+    acc = Nz(g) * w[0]
+    for sgn, steps, off in ((-1, B, 1), (1, F, 1 + B)):
+        x = g.copy()
+        s = F3(x)
+        for i in range(steps):
+            d1 = (2 * s - 1) * sgn * he
+            d2 = (2 * F3(x + d1) - 1) * sgn * he
+            x = x + 0.5 * (d1 + d2)
+            s = F3(x)
+            acc = acc + Nz(x) * w[off + i]
+    ref = acc * inv / (F + B) * 30.0
+    np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-12)
+    # the gate is neither always open nor always shut here
+    assert 0.05 < np.mean(ref > 0) < 0.999
+
+
+def test_slicing_blend_depth_varying_tf_central_ray():
+    """Blended slices along the central ray of a scene whose TF colour
+    changes with depth: |v| grows from the front (z = 1) to the back, the
+    slicing shaders index the TF with .a = |v|/max, the front half of the TF
+    is dim (tf.a 0.1) and the back half bright (rgb up to 40, tf.a 10).  The
+    bright fragments' colour exceeds 1 and is clamped before blending, and
+    the first one is opaque, so its clamped colour is the final one.  Slice
+    k of 129 crosses the central ray at z_k = 1 - (k + 1/2)/129.  Numpy from
+    the GL blend equation, within one byte step (the oracle's slice depths
+    are computed in float)."""
+    n = 8
+    m = 1.0 - 0.9 * np.arange(n) / (n - 1)  # per z plane, larger at the back (low z)
+    v = np.zeros((n, n, n, 3), np.float32)
+    v[...] = m[:, None, None, None]
+    f4 = orc.encode_field(v, (n, n, n), f64=True)
+    amag = f4.reshape(n, n, n, 4)[:, 0, 0, 3]
+    noise = np.full(n ** 3, 255, np.uint8)
+    lut = np.full(256, 255, np.uint8)
+    p = orc.params(steps_forward=2, steps_backward=2, inv_filter_area=0.5, gradient_scale=1.0,
+                   step_size_vol=1.0 / 128.0)
+    tf = np.zeros((256, 4), np.float64)
+    tf[:128] = (0.2, 0.1, 0.05, 0.1)
+    tf[128:] = (40.0, 1.0, 0.2, 10.0)  # alpha 0.1 * 10: the first bright fragment is opaque
+    op = np.full(256, 0.1, np.float64)
+    view = orc.view_setup(15, 15)
+    img, ns, _ = orc.render_slices_blend(f4, (n,) * 3, noise, 1, (n,) * 3, lut, tf.ravel(), op, p, view)
+    px = img.reshape(-1, 4)[7 * 15 + 7].astype(int)
+    lic = 1.0 * 5 * 0.5 / 4 * 1.0
+    centres = (np.arange(256) + 0.5) / 256
+    d = np.zeros(4)
+    clamped_bright = False
+    for k in range(129):
+        z = 1.0 - (k + 0.5) / 129
+        u = np.interp(z * n - 0.5, np.arange(n), amag)
+        t4 = np.array([np.interp(u, centres, tf[:, c]) for c in range(4)])
+        if not t4[3] > 0.05 or d[3] == 255:
+            continue
+        a = np.interp(1.3 * lic, centres, op) * t4[3]
+        src = np.concatenate([lic * t4[:3] * a, [a]])
+        clamped_bright |= src[0] > 1.0 and d[0] < d[3]
+        s = np.clip(src, 0.0, 1.0)
+        d = np.floor(np.clip(s * (1.0 - d[3] / 255.0) + d / 255.0, 0.0, 1.0) * 255.0 + 0.5)
+    d = np.floor(np.clip(1.0 - d[3] / 255.0 + d / 255.0, 0.0, 1.0) * 255.0 + 0.5)
+    assert clamped_bright  # the case exercises the fragment clamp
+    assert np.max(np.abs(px - d.astype(int))) <= 1, (px, d)

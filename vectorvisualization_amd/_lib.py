@@ -154,9 +154,11 @@ def _load() -> C.CDLL:
         "vv_render": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_render_fused": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_render_slices": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
+        "vv_render_slices_blend": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_set_opacity_scale": (I, [P, P, I]),
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
         "vv_render_adjoint_noalloc": (I, [P, C.POINTER(Camera), P, P, P]),
+        "vv_render_record": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
         "vv_batch_create": (I, [P, C.POINTER(P)]),
@@ -226,6 +228,7 @@ EXPORTED = [
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
     "vv_make_illum_tables", "vv_render_fused", "vv_render_slices", "vv_render_adjoint_noalloc",
+    "vv_render_slices_blend", "vv_render_record",
 ]
 
 
@@ -463,6 +466,7 @@ class Context:
         self._h = h
         self.device = int(device)
         self._ext = None
+        self.sigma_serial = 0  # count of set_opacity_scale calls
 
     def _torch_stream(self):
         """The context's stream as a torch stream object."""
@@ -677,6 +681,18 @@ class Context:
                                     C.byref(ns) if count_samples else None))
         return (out, ns.value) if count_samples else out
 
+    def render_record(self, cam: Camera, out=None, count_samples: bool = False):
+        """vv_render that keeps the adjoint's forward pass for a following
+        render_adjoint with the same camera and scene."""
+        if out is None:
+            out = np.empty(cam.width * cam.height * 4, np.float32)
+        p, dev = _buf(out, "float32", cam.width * cam.height * 4, "image")
+        ns = C.c_uint64()
+        with self.ordered(out):
+            self._chk(LIB.vv_render_record(self._h, C.byref(cam), p, dev,
+                                           C.byref(ns) if count_samples else None))
+        return (out, ns.value) if count_samples else out
+
     def render_fused(self, cam: Camera, out=None, count_samples: bool = False):
         """Fused ray cast: the LIC of every sample computed on the fly."""
         if out is None:
@@ -699,12 +715,25 @@ class Context:
                                            C.byref(ns) if count_samples else None))
         return (out, ns.value) if count_samples else out
 
+    def render_slices_blend(self, cam: Camera, out=None, count_samples: bool = False):
+        """The reference's default slicing output: the slices blended into an
+        8-bit RGBA framebuffer over a white background (uint8, 4 per pixel)."""
+        if out is None:
+            out = np.empty(cam.width * cam.height * 4, np.uint8)
+        p, dev = _buf(out, "uint8", cam.width * cam.height * 4, "image")
+        ns = C.c_uint64()
+        with self.ordered(out):
+            self._chk(LIB.vv_render_slices_blend(self._h, C.byref(cam), p, dev,
+                                                 C.byref(ns) if count_samples else None))
+        return (out, ns.value) if count_samples else out
+
     def view_block(self, cam: Camera) -> np.ndarray:
         out = (C.c_float * 24)()
         self._chk(LIB.vv_view_block(self._h, C.byref(cam), out))
         return np.array(out, np.float32)
 
     def set_opacity_scale(self, sigma) -> None:
+        self.sigma_serial += 1
         if sigma is None:
             self._chk(LIB.vv_set_opacity_scale(self._h, None, 0))
             return

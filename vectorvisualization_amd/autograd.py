@@ -10,6 +10,11 @@ The backward pass sets the transfer function, opacity table and opacity scale
 of its own forward pass again before the adjoint, so several renders may share
 one engine before a single backward.  The field, the LIC volume and the LIC
 parameters must stay as they were at the forward pass.
+
+The forward pass renders with ``vv_render_record``: when nothing changed the
+engine's scene in between, the adjoint reuses its checkpoints instead of
+running the forward walk again (a changed transfer function or opacity scale
+is detected by the library and only costs that walk).
 """
 from __future__ import annotations
 
@@ -29,19 +34,23 @@ class _RenderTF(torch.autograd.Function):
         sig = None if sigma is None else sigma.detach().to(torch.float32).contiguous().clone()
         _RenderTF._apply_state(engine, rgba, opacity, sig)
         img = torch.empty(cam.height * cam.width * 4, dtype=torch.float32, device=tf_rgba.device)
-        engine.render(cam, out=img)
+        engine.render_record(cam, out=img)
         engine.synchronize()
         fctx.engine = engine
         fctx.cam = cam
         fctx.state = (rgba, opacity, sig)
+        fctx.sigma_serial = engine.sigma_serial
         fctx.has_sigma = sigma is not None
         fctx.sigma_shape = None if sigma is None else sigma.shape
         return img.view(cam.height, cam.width, 4)
 
     @staticmethod
-    def _apply_state(engine: Context, rgba: np.ndarray, opacity: np.ndarray, sigma) -> None:
+    def _apply_state(engine: Context, rgba: np.ndarray, opacity: np.ndarray, sigma, serial=None) -> None:
+        # set_tf with unchanged tables leaves the scene as it is; the opacity
+        # scale is set again only if anything set it since (serial)
         engine.set_tf(rgba, opacity)
-        engine.set_opacity_scale(sigma)
+        if serial is None or engine.sigma_serial != serial:
+            engine.set_opacity_scale(sigma)
 
     @staticmethod
     def backward(fctx, grad_img):
@@ -53,7 +62,7 @@ class _RenderTF(torch.autograd.Function):
             dsig = torch.empty(int(np.prod(fctx.sigma_shape)), dtype=torch.float32, device=g.device)
         torch.cuda.current_stream().synchronize()
         # the state of this forward pass (another forward may have run since)
-        _RenderTF._apply_state(engine, *fctx.state)
+        _RenderTF._apply_state(engine, *fctx.state, serial=fctx.sigma_serial)
         engine.render_adjoint(cam, g, dtf, dsig)
         engine.synchronize()
         dsig_out = None if dsig is None else dsig.view(fctx.sigma_shape)

@@ -194,6 +194,7 @@ vv_status vv_upload_field_ex(vv_ctx* c, const float* v0, const float* v1, float 
         c->storage = storage;
         c->has_field = true;
         c->lic_field_ok = false;
+        ++c->scene_version;
         return VV_OK;
     });
 }
@@ -410,6 +411,8 @@ vv_status vv_set_tf(vv_ctx* c, const float* rgba, const float* opacity)
             if (!std::isfinite(rgba[i])) return fail(c, VV_EINVAL, "tf rgba not finite");
         for (int i = 0; i < 256; ++i)
             if (!std::isfinite(opacity[i])) return fail(c, VV_EINVAL, "tf opacity not finite");
+        if (std::memcmp(c->tf, rgba, sizeof(float) * 1024) || std::memcmp(c->tf + 1024, opacity, sizeof(float) * 256))
+            ++c->scene_version;
         std::memcpy(c->tf, rgba, sizeof(float) * 1024);
         std::memcpy(c->tf + 1024, opacity, sizeof(float) * 256);
         c->tf_dirty = true;
@@ -422,7 +425,10 @@ vv_status vv_set_tf_u8(vv_ctx* c, const uint8_t* tf5)
     return guarded(c, [&]() -> vv_status {
         if (!c) return VV_EINVAL;
         if (!tf5) return fail(c, VV_EINVAL, "tf NULL");
-        tf5_to_float(tf5, c->tf);
+        float t[1280];
+        tf5_to_float(tf5, t);
+        if (std::memcmp(c->tf, t, sizeof(t))) ++c->scene_version;
+        std::memcpy(c->tf, t, sizeof(t));
         c->tf_dirty = true;
         return VV_OK;
     });
@@ -464,6 +470,7 @@ vv_status vv_set_lic_params(vv_ctx* c, const vv_lic_params* p)
         vv_status st = check_params(c, p);
         if (st) return st;
         c->params = *p;
+        ++c->scene_version;
         return VV_OK;
     });
 }
@@ -583,6 +590,7 @@ vv_status vv_compute_lic(vv_ctx* c)
         for (int a = 0; a < 3; ++a) c->ldims[a] = c->fdims[a];
         c->lch = nc;
         c->has_lic = true;
+        ++c->scene_version;
         return VV_OK;
     });
 }
@@ -635,6 +643,7 @@ vv_status vv_upload_lic(vv_ctx* c, const float* lic, int channels, const int dim
         c->lch = channels;
         c->has_lic = true;
         c->has_sigma = false;  // opacity scale lives on the LIC grid
+        ++c->scene_version;
         return VV_OK;
     });
 }
@@ -812,63 +821,79 @@ vv_status vv_render_fused(vv_ctx* c, const vv_camera* cam, float* image, int on_
     });
 }
 
+// View-aligned slicing into the RGBA16F render target (blend 0, float4
+// pixels) or blended into the 8-bit window framebuffer (blend 1, 4 bytes per
+// pixel).  Both slicing shaders index the TF with the field's .a
+// (lic3d_slicing_fragment.glsl:43, lic3d_slicingblend_fragment.glsl:40).
+static vv_status render_slices(vv_ctx* c, const vv_camera* cam, void* image, int on_device, uint64_t* samples_out,
+                               int blend)
+{
+    if (!c) return VV_EINVAL;
+    if (!image) return fail(c, VV_EINVAL, "image is NULL");
+    HIPCHK(c, hipSetDevice(c->device));
+    LicLaunch L;
+    vv_status st = fill_lic_launch(c, L, nullptr);
+    if (st) return st;
+    RenderLaunch R;
+    st = fill_render_launch(c, cam, R, false);
+    if (st) return st;
+    R.tf_ch = 3;
+    // ViewSlicing::setupSlicing (VV/slicing.cpp:42-100): the view vector, the
+    // depth range D of the box along it and n = D / stepSizeVol + 1 slices
+    SliceSetup S;
+    float inv = 1.0f / std::sqrt(R.view.dw[0] * R.view.dw[0] + R.view.dw[1] * R.view.dw[1] +
+                                 R.view.dw[2] * R.view.dw[2]);
+    float half[3], dv = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+        S.v[a] = R.view.dw[a] * inv;
+        if (std::fabs(S.v[a]) < 1.0e-8f) S.v[a] = 0.0f;  // VS_EPS (VV/slicing.cpp:129-134)
+        S.center[a] = c->center[a];
+        half[a] = 0.5f * c->extent[a];
+    }
+    dv = std::fabs(S.v[0]) * half[0] + std::fabs(S.v[1]) * half[1] + std::fabs(S.v[2]) * half[2];
+    S.D = 2.0f * dv;
+    const float nslices = S.D / c->params.step_size_vol;
+    if (!(nslices < 1.0e6f)) return fail(c, VV_EINVAL, "too many slices (%g)", (double)nslices);
+    S.n = (int)nslices + 1;
+    S.blend = blend;
+    const size_t bytes = (blend ? 4 : sizeof(float) * 4) * (size_t)cam->width * cam->height;
+    void* dst = image;
+    if (!on_device) {
+        HIPCHK(c, c->image.ensure(bytes));
+        dst = c->image.p;
+    }
+    R.image = static_cast<float*>(dst);
+    if (samples_out) {
+        HIPCHK(c, c->counter.ensure(64));
+        R.counter = (unsigned long long*)((char*)c->counter.p + 8);
+        HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
+    }
+    const int G = ray_group_for(c, cam);
+    if (G > 1)
+        HIPCHK(c, launch_render_group(R, L, S, 1, G, c->stream));
+    else
+        HIPCHK(c, launch_render_slices(R, L, S, c->stream));
+    if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+    if (samples_out) {
+        unsigned long long t = 0;
+        HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        *samples_out = t;
+    } else if (!on_device) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return VV_OK;
+}
+
 vv_status vv_render_slices(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
-    return guarded(c, [&]() -> vv_status {
-        if (!c) return VV_EINVAL;
-        if (!image) return fail(c, VV_EINVAL, "image is NULL");
-        HIPCHK(c, hipSetDevice(c->device));
-        LicLaunch L;
-        vv_status st = fill_lic_launch(c, L, nullptr);
-        if (st) return st;
-        RenderLaunch R;
-        st = fill_render_launch(c, cam, R, false);
-        if (st) return st;
-        // ViewSlicing::setupSlicing (VV/slicing.cpp:42-100): the view vector, the
-        // depth range D of the box along it and n = D / stepSizeVol + 1 slices
-        SliceSetup S;
-        float inv = 1.0f / std::sqrt(R.view.dw[0] * R.view.dw[0] + R.view.dw[1] * R.view.dw[1] +
-                                     R.view.dw[2] * R.view.dw[2]);
-        float half[3], dv = 0.0f;
-        for (int a = 0; a < 3; ++a) {
-            S.v[a] = R.view.dw[a] * inv;
-            if (std::fabs(S.v[a]) < 1.0e-8f) S.v[a] = 0.0f;  // VS_EPS (VV/slicing.cpp:129-134)
-            S.center[a] = c->center[a];
-            half[a] = 0.5f * c->extent[a];
-        }
-        dv = std::fabs(S.v[0]) * half[0] + std::fabs(S.v[1]) * half[1] + std::fabs(S.v[2]) * half[2];
-        S.D = 2.0f * dv;
-        const float nslices = S.D / c->params.step_size_vol;
-        if (!(nslices < 1.0e6f)) return fail(c, VV_EINVAL, "too many slices (%g)", (double)nslices);
-        S.n = (int)nslices + 1;
-        const size_t bytes = sizeof(float) * 4 * (size_t)cam->width * cam->height;
-        float* dst = image;
-        if (!on_device) {
-            HIPCHK(c, c->image.ensure(bytes));
-            dst = (float*)c->image.p;
-        }
-        R.image = dst;
-        if (samples_out) {
-            HIPCHK(c, c->counter.ensure(64));
-            R.counter = (unsigned long long*)((char*)c->counter.p + 8);
-            HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
-        }
-        const int G = ray_group_for(c, cam);
-        if (G > 1)
-            HIPCHK(c, launch_render_group(R, L, S, 1, G, c->stream));
-        else
-            HIPCHK(c, launch_render_slices(R, L, S, c->stream));
-        if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
-        if (samples_out) {
-            unsigned long long t = 0;
-            HIPCHK(c, hipMemcpyAsync(&t, R.counter, 8, hipMemcpyDeviceToHost, c->stream));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            *samples_out = t;
-        } else if (!on_device) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-        }
-        return VV_OK;
-    });
+    return guarded(c, [&]() -> vv_status { return render_slices(c, cam, image, on_device, samples_out, 0); });
+}
+
+vv_status vv_render_slices_blend(vv_ctx* c, const vv_camera* cam, uint8_t* image, int on_device,
+                                 uint64_t* samples_out)
+{
+    return guarded(c, [&]() -> vv_status { return render_slices(c, cam, image, on_device, samples_out, 1); });
 }
 
 // ---- differentiation -----------------------------------------------------------
@@ -876,6 +901,7 @@ vv_status vv_set_opacity_scale(vv_ctx* c, const float* sigma, int on_device)
 {
     return guarded(c, [&]() -> vv_status {
         if (!c) return VV_EINVAL;
+        ++c->scene_version;
         if (!sigma) {
             c->has_sigma = false;
             return VV_OK;
@@ -914,28 +940,97 @@ static vv_status adjoint_setup(vv_ctx* c, const vv_camera* cam, AdjointLaunch& A
     return VV_OK;
 }
 
-// The adjoint kernels and the fixed-point conversion on device buffers:
-// dimage in, dtf (1024) and dsigma (nv, or null) out; scratch holds
-// adj_scratch_layout(nrays, A.max_ckpt, nv) bytes.
+// Pointers of A into an adjoint scratch buffer laid out by SL.
+static void adjoint_bind(AdjointLaunch& A, char* scratch, const AdjScratchLayout& SL, bool with_sigma)
+{
+    A.hdr = reinterpret_cast<unsigned long long*>(scratch);
+    A.ckpt = reinterpret_cast<float*>(scratch + SL.ckpt);
+    A.nsamp = reinterpret_cast<int*>(scratch + SL.nsamp);
+    A.ray_bound = reinterpret_cast<float*>(scratch + SL.bound);
+    long long* acc = reinterpret_cast<long long*>(scratch + SL.acc);
+    A.dtf_fx = acc;
+    A.dsigma_fx = with_sigma ? acc + 1024 : nullptr;
+}
+
+// The adjoint's forward pass (header word 0 cleared first).
+static vv_status adjoint_forward(vv_ctx* c, AdjointLaunch& A)
+{
+    HIPCHK(c, launch_fill_i64(reinterpret_cast<long long*>(A.hdr), 0, 1, c->stream));
+    HIPCHK(c, launch_adjoint_forward(A, c->stream));
+    return VV_OK;
+}
+
+// The backward walk and the fixed-point conversion on device buffers:
+// dimage in, dtf (1024) and dsigma (nv, or null) out.
+static vv_status adjoint_backward(vv_ctx* c, AdjointLaunch& A, const float* dimage, float* dtf, float* dsigma,
+                                  long long nv)
+{
+    HIPCHK(c, launch_fill_i64(reinterpret_cast<long long*>(A.hdr) + 1, 0, 1, c->stream));
+    HIPCHK(c, launch_fill_i64(A.dtf_fx, 0, 1024 + (dsigma ? nv : 0), c->stream));
+    A.dimage = dimage;
+    HIPCHK(c, launch_render_adjoint(A, c->stream));
+    HIPCHK(c, launch_fixed_to_float(A.hdr, A.dtf_fx, dtf, 1024, c->stream));
+    if (dsigma) HIPCHK(c, launch_fixed_to_float(A.hdr, A.dsigma_fx, dsigma, nv, c->stream));
+    return VV_OK;
+}
+
+// Both passes on a scratch buffer of adj_scratch_layout(nrays, A.max_ckpt,
+// dsigma ? nv : 0) bytes.
 static vv_status adjoint_enqueue(vv_ctx* c, AdjointLaunch& A, char* scratch, const float* dimage, float* dtf,
                                  float* dsigma, long long nv)
 {
     const long long nrays = (long long)A.r.view.width * A.r.view.height;
-    const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
-    unsigned long long* hdr = reinterpret_cast<unsigned long long*>(scratch);
-    long long* acc = reinterpret_cast<long long*>(scratch + SL.acc);
-    const long long zero_words = (long long)(SL.acc / 8) + 1024 + (dsigma ? nv : 0);
-    HIPCHK(c, launch_fill_i64(reinterpret_cast<long long*>(scratch), 0, zero_words, c->stream));
-    A.dimage = dimage;
-    A.ckpt = reinterpret_cast<float*>(scratch + SL.ckpt);
-    A.nsamp = reinterpret_cast<int*>(scratch + SL.nsamp);
-    A.hdr = hdr;
-    A.dtf_fx = acc;
-    A.dsigma_fx = dsigma ? acc + 1024 : nullptr;
-    HIPCHK(c, launch_render_adjoint(A, c->stream));
-    HIPCHK(c, launch_fixed_to_float(hdr, acc, dtf, 1024, c->stream));
-    if (dsigma) HIPCHK(c, launch_fixed_to_float(hdr, acc + 1024, dsigma, nv, c->stream));
-    return VV_OK;
+    adjoint_bind(A, scratch, adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0), dsigma != nullptr);
+    vv_status st = adjoint_forward(c, A);
+    if (st) return st;
+    return adjoint_backward(c, A, dimage, dtf, dsigma, nv);
+}
+
+vv_status vv_render_record(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
+{
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!image) return fail(c, VV_EINVAL, "image is NULL");
+        HIPCHK(c, hipSetDevice(c->device));
+        AdjointLaunch A;
+        vv_status st = adjoint_setup(c, cam, A);
+        if (st) return st;
+        c->adj_rec.valid = false;
+        const long long nrays = (long long)cam->width * cam->height;
+        // room for the opacity-scale gradient, so the adjoint can keep the buffer
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, nvox(c->ldims));
+        HIPCHK(c, c->adj_ckpt.ensure(SL.total));
+        const size_t bytes = sizeof(float) * 4 * (size_t)nrays;
+        float* dst = image;
+        if (!on_device) {
+            HIPCHK(c, c->image.ensure(bytes));
+            dst = (float*)c->image.p;
+        }
+        A.r.image = dst;
+        if (samples_out) {
+            HIPCHK(c, c->counter.ensure(64));
+            A.r.counter = (unsigned long long*)((char*)c->counter.p + 8);
+            HIPCHK(c, launch_fill_u32((uint32_t*)A.r.counter, 0u, 2, c->stream));
+        }
+        adjoint_bind(A, (char*)c->adj_ckpt.p, SL, false);
+        st = adjoint_forward(c, A);
+        if (st) return st;
+        if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
+        if (samples_out) {
+            unsigned long long t = 0;
+            HIPCHK(c, hipMemcpyAsync(&t, A.r.counter, 8, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            *samples_out = t;
+        } else if (!on_device) {
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+        c->adj_rec.valid = true;
+        c->adj_rec.cam = *cam;
+        c->adj_rec.version = c->scene_version;
+        c->adj_rec.max_ckpt = A.max_ckpt;
+        c->adj_rec.buffer = c->adj_ckpt.p;
+        return VV_OK;
+    });
 }
 
 vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf, float* dsigma,
@@ -952,6 +1047,11 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
         const long long nv = nvox(c->ldims);
         const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
         HIPCHK(c, c->adj_ckpt.ensure(SL.total));
+        // the forward pass vv_render_record left for this camera and scene
+        const bool reuse = c->adj_rec.valid && c->adj_rec.version == c->scene_version &&
+                           c->adj_rec.buffer == c->adj_ckpt.p && c->adj_rec.max_ckpt == A.max_ckpt &&
+                           std::memcmp(&c->adj_rec.cam, cam, sizeof(vv_camera)) == 0;
+        c->adj_rec.valid = false;  // the backward walk rewrites the segment column
         const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
         const float* d_dimg = dimage;
         float* d_dtf = dtf;
@@ -964,7 +1064,12 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
             d_dtf = (float*)c->adj_out.p;
             d_dsigma = dsigma ? d_dtf + 1024 : nullptr;
         }
-        st = adjoint_enqueue(c, A, (char*)c->adj_ckpt.p, d_dimg, d_dtf, d_dsigma, nv);
+        adjoint_bind(A, (char*)c->adj_ckpt.p, SL, dsigma != nullptr);
+        if (!reuse) {
+            st = adjoint_forward(c, A);
+            if (st) return st;
+        }
+        st = adjoint_backward(c, A, d_dimg, d_dtf, d_dsigma, nv);
         if (st) return st;
         if (!on_device) {
             HIPCHK(c, hipMemcpyAsync(dtf, d_dtf, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->stream));

@@ -105,6 +105,18 @@ struct vv_ctx {
     int lic_super[3] = {2, 16, 16};
     int lic_cache = 1;
     int adj_window = 1;
+    // Version of what a frame depends on (field, LIC volume, opacity scale,
+    // transfer function, LIC parameters, line material); every change bumps it.
+    unsigned long long scene_version = 0;
+    // The forward pass vv_render_record left in adj_ckpt for the adjoint of
+    // the same camera and scene version (valid until an adjoint uses it).
+    struct {
+        bool valid = false;
+        vv_camera cam;
+        unsigned long long version = 0;
+        int max_ckpt = 0;
+        void* buffer = nullptr;
+    } adj_rec;
     int render_block[2] = {16, 16};
     int ray_group = 0;  // lanes per ray of the on-the-fly casters, 0: automatic
 

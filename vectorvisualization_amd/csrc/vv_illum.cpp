@@ -155,6 +155,7 @@ vv_status vv_set_line_material(vv_ctx* c, const vv_line_material* m)
         if (s) return fail(c, s, "%s", g_create_error.c_str());
         c->line_mat = *m;
         c->illum_dirty = true;
+        ++c->scene_version;
         return VV_OK;
     });
 }

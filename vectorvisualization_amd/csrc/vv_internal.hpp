@@ -116,24 +116,32 @@ struct LicBatchLaunch {
 };
 
 // Scratch of one adjoint call, carved from one device buffer (the caller of
-// launch_render_adjoint owns it):
-//   header  2 x u64: [0] total samples of all rays, [1] low 32 bits: the
-//           float bits of the largest bound on one sample's gradient
-//           contribution (atomicMax; NaN bits when a bound is not finite)
-//   acc     1024 (+ sigma voxels) int64 fixed-point accumulators
-//   ckpt    per ray 2*max_ckpt float4 checkpoints, then the segment column
-//           (kAdjSeg + kAdjSeg/16 float4)
+// launch_render_adjoint owns it).  The parts a forward pass writes come
+// first, so a recorded forward pass (vv_render_record) stays valid whatever
+// size of accumulators a later adjoint call asks for:
+//   header  2 x u64: [0] total samples of all rays (forward pass), [1] low
+//           32 bits: the float bits of the largest bound on one sample's
+//           gradient contribution (atomicMax; NaN bits when a bound is not
+//           finite)
+//   ckpt    per ray 2*max_ckpt float4 checkpoints, then the segment column:
+//           kAdjSeg floats (D.a before each sample), kAdjSeg/16 u64 (the
+//           clamp masks of 16 samples, 4 bits each), kAdjSeg/16 float4 (the
+//           position at the start of each 16-sample part)
 //   nsamp   per ray sample count (int)
+//   bound   per ray bound on its contributions per unit of max |dL/dimage|
+//           (float)
+//   acc     1024 (+ sigma voxels) int64 fixed-point accumulators
 constexpr int kAdjSeg = 64;          // checkpoint interval (samples)
-constexpr int kAdjColumn = kAdjSeg + kAdjSeg / 16;  // float4 per ray in the segment column
+constexpr int kAdjColumnBytes = kAdjSeg * 4 + (kAdjSeg / 16) * (8 + 16);  // per ray
 
 struct AdjointLaunch {
-    RenderLaunch r;
+    RenderLaunch r;               // r.image: the forward pass also writes the image (or null)
     const float* dimage;          // width*height*4 (device)
     float* ckpt;                  // per ray: checkpoints, then the segment column
     int max_ckpt;                 // checkpoints per ray
     int* nsamp;                   // per ray sample count
-    unsigned long long* hdr;      // the two header words (zero before launch)
+    float* ray_bound;             // per ray bound (per unit of max |dL/dimage|)
+    unsigned long long* hdr;      // the two header words
     long long* dtf_fx;            // 256*4 fixed-point accumulators (zero before launch)
     long long* dsigma_fx;         // sigma voxel accumulators, or null
     int use_window;               // accumulate dsigma in an LDS window per segment
@@ -149,17 +157,18 @@ hipError_t noalloc_scratch(int device, void** ptr);
 // Bytes of adjoint scratch for nrays rays, max_ckpt checkpoints and nv sigma
 // voxels (0 without sigma gradients), and the offsets of its parts.
 struct AdjScratchLayout {
-    size_t acc, ckpt, nsamp, total;
+    size_t ckpt, nsamp, bound, acc, total;
 };
 inline AdjScratchLayout adj_scratch_layout(long long nrays, int max_ckpt, long long nv)
 {
     AdjScratchLayout L;
     const size_t align = 256;
     auto up = [&](size_t x) { return (x + align - 1) / align * align; };
-    L.acc = 16;
-    L.ckpt = up(L.acc + sizeof(long long) * (size_t)(1024 + nv));
-    L.nsamp = up(L.ckpt + 16 * (size_t)(2 * max_ckpt + kAdjColumn) * (size_t)nrays);
-    L.total = up(L.nsamp + sizeof(int) * (size_t)nrays);
+    L.ckpt = align;
+    L.nsamp = up(L.ckpt + (16 * (size_t)(2 * max_ckpt) + kAdjColumnBytes) * (size_t)nrays);
+    L.bound = up(L.nsamp + sizeof(int) * (size_t)nrays);
+    L.acc = up(L.bound + sizeof(float) * (size_t)nrays);
+    L.total = up(L.acc + sizeof(long long) * (size_t)(1024 + nv));
     return L;
 }
 
@@ -211,6 +220,9 @@ struct SliceSetup {
     float center[3];  // volume centre (extent / 2)
     float D;          // depth range
     int n;            // number of slices
+    int blend;        // 0: RGBA16F render target (lic3d_slicing_fragment.glsl);
+                      // 1: blended into an 8-bit RGBA window framebuffer
+                      //    (lic3d_slicingblend_fragment.glsl), image = uchar4
 };
 
 hipError_t launch_render_slices(const RenderLaunch& R, const LicLaunch& L, const SliceSetup& S, hipStream_t s);
@@ -223,6 +235,10 @@ hipError_t launch_render_group(const RenderLaunch& R, const LicLaunch& L, const 
 // fused on-the-fly LIC ray caster: R as for launch_render (its lic fields
 // unused), L as for launch_lic (its out fields unused)
 hipError_t launch_render_fused(const RenderLaunch& R, const LicLaunch& L, hipStream_t s);
+// the forward pass of the adjoint (checkpoints, column, sample counts, ray
+// bounds, header word 0; the image when A.r.image is set)
+hipError_t launch_adjoint_forward(const AdjointLaunch& A, hipStream_t s);
+// the contribution bound of dL/dimage (header word 1), then the backward walk
 hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s);
 // dst[i] = src[i] / 2^k for the fixed-point scale 2^k the adjoint derived from
 // its header words (NaN when the header holds a non-finite bound)

@@ -467,18 +467,49 @@ __device__ __forceinline__ void splat_tf(long long* acc, int ch, float u, float 
     if (a1) atomicAdd((unsigned long long*)&acc[i1 * 4 + ch], (unsigned long long)a1);
 }
 
-// Phase 1: the forward ray with a checkpoint (pos, D) every kSeg samples, its
-// sample count, and the contribution bound and sample total of the header.
+// dsigma accumulation window in LDS: entries (int64 fixed point), one
+// window per kSub samples of a segment.  Sized for occupancy: 24 KB of
+// window (38 KB of LDS per block) lets four blocks share a CU, matching the
+// kernel's 4 waves per SIMD (at most 128 VGPRs), and the voxel box of 16
+// samples of a 16x16-ray tile fits it at 256^3 / 1024^2 (a box that does
+// not fit goes to global atomics, with the same sums).
+constexpr int kWin = 3072;
+constexpr int kSub = 16;
+constexpr int kNSub = kSeg / kSub;
+static_assert(kSub == 16 && kSeg * 4 + kNSub * 24 == kAdjColumnBytes, "segment column layout");
+
+// The segment column of a ray: [k][ray] D.a before sample k of the segment;
+// [m][ray] the clamp masks of part m's samples (4 bits each, sample j of the
+// part at bit 4j); [m][ray] the position at the start of part m.  Phase 1
+// fills it for every segment in turn, so it holds each ray's last segment
+// when phase 2 starts; phase 2 re-runs the earlier segments into it.
+struct AdjColumn {
+    float* da;
+    unsigned long long* mask;
+    float4* pos;
+};
+
+__device__ __forceinline__ AdjColumn adj_column(const AdjointLaunch& A, long long nrays)
+{
+    AdjColumn c;
+    c.da = reinterpret_cast<float*>(reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays);
+    c.mask = reinterpret_cast<unsigned long long*>(c.da + (long long)kSeg * nrays);
+    c.pos = reinterpret_cast<float4*>(c.mask + (long long)kNSub * nrays);
+    return c;
+}
+
+// Phase 1: the forward ray with a checkpoint (pos, D) every kSeg samples, the
+// segment column, its sample count and its contribution bound per unit of
+// max |dL/dimage|, the sample total of the header, and, when a.image is set,
+// the image (the same samples and arithmetic as k_render).
 template <typename FT, int LC>
 __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
     __shared__ unsigned long long red[kRThreads / 64];
-    __shared__ unsigned bmax;
     const RenderLaunch& a = A.r;
     const int tid = threadIdx.y * kBX + threadIdx.x;
-    if (tid == 0) bmax = 0u;
     load_tables(a.tf, tf, op);
     const int i = blockIdx.x * kBX + threadIdx.x;
     const int j = blockIdx.y * kBY + threadIdx.y;
@@ -486,24 +517,33 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
     const long long nrays = W * H;
     const long long pix = (long long)j * W + i;
     float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
+    const AdjColumn col = adj_column(A, nrays);
     Ray r;
     const bool inside = i < W && j < H;
     const bool active = inside && ray_setup(a.view, i, j, r);
     long long n = 0;
     float bound = 0.0f;
+    float dest[4] = {0.f, 0.f, 0.f, 0.f};
     if (active) {
         float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
-        float dest[4] = {0.f, 0.f, 0.f, 0.f};
         float max_am = 0.0f, max_q = 0.0f, max_qr = 0.0f, gamma = 1.0f;
+        unsigned long long mbits = 0;
         for (long long k = 0; k < a.max_samples; ++k) {
-            if ((k % kSeg) == 0) {
+            const int kk = (int)(k % kSeg);
+            if (kk == 0) {
                 const long long c = k / kSeg;
                 ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
                 ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
             }
+            if (kk % kSub == 0) col.pos[(long long)(kk / kSub) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
             Sample s;
             eval_sample<FT, LC>(a, tf, op, r, pos, s);
-            composite(dest, s);
+            col.da[(long long)kk * nrays + pix] = dest[3];
+            mbits |= (unsigned long long)composite(dest, s) << (4 * (kk % kSub));
+            if (kk % kSub == kSub - 1) {
+                col.mask[(long long)(kk / kSub) * nrays + pix] = mbits;
+                mbits = 0;
+            }
             ++n;
             const float rsum = fabsf(s.rgb[0]) + fabsf(s.rgb[1]) + fabsf(s.rgb[2]);
             const float mmax = fmaxf(fabsf(s.m[0]), fmaxf(fabsf(s.m[1]), fabsf(s.m[2])));
@@ -515,16 +555,17 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
             for (int c = 0; c < 3; ++c) pos[c] = fmaf(r.dir[c], a.step, pos[c]);
             if (outside(pos, a.view) || dest[3] > 0.95f) break;
         }
-        const float4 g = reinterpret_cast<const float4*>(A.dimage)[pix];
-        const float gm = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w)));
-        const bool finite_g = isfinite(g.x) && isfinite(g.y) && isfinite(g.z) && isfinite(g.w);
-        bound = gm * fmaxf(max_am, fmaf(gamma, max_q, max_qr));
-        if (!finite_g || !isfinite(bound)) bound = __uint_as_float(0x7fc00000u);
+        // the masks of a last part the loop left unfinished
+        const int kl = (int)((n - 1) % kSeg);
+        if (kl % kSub != kSub - 1) col.mask[(long long)(kl / kSub) * nrays + pix] = mbits;
+        bound = fmaxf(max_am, fmaf(gamma, max_q, max_qr));
+        if (!isfinite(bound)) bound = __uint_as_float(0x7fc00000u);
     }
-    if (inside) A.nsamp[pix] = (int)n;
-    // the bound is >= 0 (or NaN, whose bits exceed every finite one), so its
-    // bits order as unsigned integers
-    atomicMax(&bmax, __float_as_uint(bound));
+    if (inside) {
+        A.nsamp[pix] = (int)n;
+        A.ray_bound[pix] = bound;
+        if (a.image) reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
     unsigned long long ns = (unsigned long long)n;
     for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
     if ((tid & 63) == 0) red[tid >> 6] = ns;
@@ -532,21 +573,41 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
     if (tid == 0) {
         unsigned long long t = 0;
         for (int w = 0; w < kRThreads / 64; ++w) t += red[w];
-        if (t) atomicAdd(&A.hdr[0], t);
-        if (bmax) atomicMax(&A.hdr[1], (unsigned long long)bmax);
+        if (t) {
+            atomicAdd(&A.hdr[0], t);
+            if (a.counter) atomicAdd(a.counter, t);
+        }
     }
 }
 
-// dsigma accumulation window in LDS: entries (int64 fixed point), one
-// window per kSub samples of a segment.  Sized for occupancy: 24 KB of
-// window (38 KB of LDS per block) lets four blocks share a CU, matching the
-// kernel's 4 waves per SIMD (at most 128 VGPRs), and the voxel box of 16
-// samples of a 16x16-ray tile fits it at 256^3 / 1024^2 (a box that does
-// not fit goes to global atomics, with the same sums).
-constexpr int kWin = 3072;
-constexpr int kSub = 16;
-constexpr int kNSub = kSeg / kSub;
-static_assert(kSeg + kNSub == kAdjColumn, "segment column layout");
+// The largest contribution bound of the call: each ray's bound per unit of
+// max |dL/dimage| times its pixel's max |dL/dimage| (NaN when that pixel's
+// gradient is not finite on a ray that has samples), into header word 1.
+// The bound is >= 0 or NaN, whose bits exceed every finite one, so its bits
+// order as unsigned integers.
+__global__ void __launch_bounds__(256) k_adjoint_bound(AdjointLaunch A)
+{
+    __shared__ unsigned bmax;
+    if (threadIdx.x == 0) bmax = 0u;
+    __syncthreads();
+    const long long nrays = (long long)A.r.view.width * A.r.view.height;
+    const long long pix = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float bound = 0.0f;
+    if (pix < nrays) {
+        const float rb = A.ray_bound[pix];
+        if (rb != 0.0f) {
+            const float4 g = reinterpret_cast<const float4*>(A.dimage)[pix];
+            const float gm = fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w)));
+            const bool finite_g = isfinite(g.x) && isfinite(g.y) && isfinite(g.z) && isfinite(g.w);
+            bound = gm * rb;
+            if (!finite_g || !isfinite(bound)) bound = __uint_as_float(0x7fc00000u);
+        }
+    }
+    if (bound != 0.0f || isnan(bound)) atomicMax(&bmax, __float_as_uint(bound));
+    __syncthreads();
+    if (threadIdx.x == 0 && bmax) atomicMax(&A.hdr[1], (unsigned long long)bmax);
+}
+
 
 // Phase 2: the backward walk, segments last to first, all rays of the block
 // in lockstep.
@@ -574,10 +635,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
     const float4* ck = reinterpret_cast<const float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
-    // scratch column of one segment: [k][ray] D before sample k, with the
-    // sample's clamp mask in the sign bits (D lies in [0,1]); then
-    // [kSeg + m][ray] the position at the start of part m
-    float4* seg = reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays;
+    const AdjColumn col = adj_column(A, nrays);
 
     Ray r;
     const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
@@ -604,43 +662,48 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
             box[tid][3] = box[tid][4] = box[tid][5] = -1;
         }
         __syncthreads();
-        // (a) re-run the segment from its checkpoint into the scratch column;
-        //     collect the voxel box of each kSub-sample part
-        if (mine) {
+        // (a) re-run the segment from its checkpoint into the column (phase 1
+        //     left each ray's last segment there)
+        if (mine && c < nseg - 1) {
             const float4 cp = ck[(2 * c) * nrays + pix];
             const float4 cd = ck[(2 * c + 1) * nrays + pix];
             float p[3] = {cp.x, cp.y, cp.z};
             float d[4] = {cd.x, cd.y, cd.z, cd.w};
-            int lo[3] = {0x7fffffff, 0x7fffffff, 0x7fffffff}, hi[3] = {-1, -1, -1};
+            unsigned long long mbits = 0;
             for (long long k = k0; k < k1; ++k) {
-                if ((k - k0) % kSub == 0)
-                    seg[(kSeg + (k - k0) / kSub) * nrays + pix] = make_float4(p[0], p[1], p[2], 0.f);
+                const int kk = (int)(k - k0);
+                if (kk % kSub == 0) col.pos[(long long)(kk / kSub) * nrays + pix] = make_float4(p[0], p[1], p[2], 0.f);
                 Sample s;
                 eval_sample<FT, LC>(a, tf, op, r, p, s);
-                unsigned dbits[4];
-                for (int q = 0; q < 4; ++q) dbits[q] = __float_as_uint(d[q]);
-                const int mask = composite(d, s);
-                for (int q = 0; q < 4; ++q) dbits[q] |= (unsigned)((mask >> q) & 1) << 31;
-                seg[(k - k0) * nrays + pix] = make_float4(__uint_as_float(dbits[0]), __uint_as_float(dbits[1]),
-                                                          __uint_as_float(dbits[2]), __uint_as_float(dbits[3]));
-                if (A.dsigma_fx) {
-                    for (int q = 0; q < 3; ++q) {
-                        float f;
-                        const int c0 = tc_clamp(p[q], n3[q], f) - 1;
-                        lo[q] = min(lo[q], max(c0, 0));
-                        hi[q] = max(hi[q], min(c0 + 1, n3[q] - 1));
-                    }
-                    if ((k - k0) % kSub == kSub - 1 || k == k1 - 1) {
-                        const int m = (int)((k - k0) / kSub);
-                        for (int q = 0; q < 3; ++q) {
-                            atomicMin(&box[m][q], lo[q]);
-                            atomicMax(&box[m][3 + q], hi[q]);
-                            lo[q] = 0x7fffffff;
-                            hi[q] = -1;
-                        }
-                    }
+                col.da[(long long)kk * nrays + pix] = d[3];
+                mbits |= (unsigned long long)composite(d, s) << (4 * (kk % kSub));
+                if (kk % kSub == kSub - 1 || k == k1 - 1) {
+                    col.mask[(long long)(kk / kSub) * nrays + pix] = mbits;
+                    mbits = 0;
                 }
                 for (int q = 0; q < 3; ++q) p[q] = fmaf(r.dir[q], a.step, p[q]);
+            }
+        }
+        // (b) the voxel box of each part's sigma splats: along a ray every
+        //     axis coordinate is monotone (the same fma step each sample), so
+        //     the part's first and last samples bound its cells
+        if (mine && A.dsigma_fx) {
+            for (int m = 0; m < kNSub; ++m) {
+                const long long ka = k0 + (long long)m * kSub;
+                const long long kb = min(k1, ka + kSub);
+                if (ka >= kb) break;
+                const float4 ps = col.pos[(long long)m * nrays + pix];
+                float pe[3] = {ps.x, ps.y, ps.z};
+                for (long long t = ka + 1; t < kb; ++t)
+                    for (int q = 0; q < 3; ++q) pe[q] = fmaf(r.dir[q], a.step, pe[q]);
+                const float pf[3] = {ps.x, ps.y, ps.z};
+                for (int q = 0; q < 3; ++q) {
+                    float f;
+                    const int c0 = tc_clamp(pf[q], n3[q], f) - 1;
+                    const int c1 = tc_clamp(pe[q], n3[q], f) - 1;
+                    atomicMin(&box[m][q], max(min(c0, c1), 0));
+                    atomicMax(&box[m][3 + q], min(max(c0, c1) + 1, n3[q] - 1));
+                }
             }
         }
         __syncthreads();
@@ -656,14 +719,11 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
             const long long ka = k0 + (long long)m * kSub;
             const long long kb = min(k1, ka + kSub);
             if (mine) {
-                const float4 ps = seg[(kSeg + m) * nrays + pix];
+                const float4 ps = col.pos[(long long)m * nrays + pix];
+                const unsigned long long mbits = ka < kb ? col.mask[(long long)m * nrays + pix] : 0ull;
                 for (long long k = kb - 1; k >= ka; --k) {
-                    const float4 dm = seg[(k - k0) * nrays + pix];
-                    const unsigned db[4] = {__float_as_uint(dm.x), __float_as_uint(dm.y), __float_as_uint(dm.z),
-                                            __float_as_uint(dm.w)};
-                    int mask = 0;
-                    for (int q = 0; q < 4; ++q) mask |= (int)(db[q] >> 31) << q;
-                    const float dka = __uint_as_float(db[3] & 0x7fffffffu);
+                    const int mask = (int)((mbits >> (4 * (int)(k - ka))) & 15ull);
+                    const float dka = col.da[(k - k0) * nrays + pix];
                     // the sample position: the forward fma chain from the part's start
                     float pk[3] = {ps.x, ps.y, ps.z};
                     for (long long t = ka; t < k; ++t)
@@ -821,6 +881,33 @@ k_render_fused(RenderLaunch a, LicArgs L)
 // target, i.e. rounded to FP16, before the next slice reads it.
 __device__ __forceinline__ float half_rn(float x) { return __half2float(__float2half_rn(x)); }
 
+// The reference's default slicing output (Renderer::sliceVolume without
+// render targets, VV/renderer.cpp:1149-1157, with
+// lic3d_slicingblend_fragment.glsl): each slice's fragment src (rgb
+// premultiplied, 0 where the TF alpha is at most 0.05) is blended into the
+// 8-bit RGBA window framebuffer with glBlendFunc(GL_ONE_MINUS_DST_ALPHA,
+// GL_ONE), dst = src (1 - dst.a) + dst: front to back "under", with no 0.95
+// test.  The framebuffer is fixed point: the fragment colour is clamped to
+// [0,1], the blend result clamped and stored as round(255 x).  After the
+// slices a white screen quad is blended the same way (the background).
+// dst holds byte values; blend_u8 blends one slice's src into them.
+__device__ __forceinline__ float unorm8_rn(float x) { return floorf(fmaf(clamp01(x), 255.0f, 0.5f)); }
+
+__device__ __forceinline__ void blend_u8(float dst[4], const float src[4])
+{
+    const float t = 1.0f - dst[3] * (1.0f / 255.0f);
+    for (int c = 0; c < 4; ++c) dst[c] = unorm8_rn(fmaf(t, clamp01(src[c]), dst[c] * (1.0f / 255.0f)));
+}
+
+// the white background quad, then the pixel's bytes
+__device__ __forceinline__ void store_blend(float* image, long long pix, float dst[4])
+{
+    const float white[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    blend_u8(dst, white);
+    reinterpret_cast<uchar4*>(image)[pix] =
+        make_uchar4((unsigned char)dst[0], (unsigned char)dst[1], (unsigned char)dst[2], (unsigned char)dst[3]);
+}
+
 template <typename FT, int NC, bool GATE, int MODE>
 __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
@@ -852,7 +939,9 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
         nc.reset();
         float dest[4] = {0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < S.n && vdir != 0.0f; ++k) {
-            if (!(dest[3] < 0.95f)) break;
+            // the blend adds nothing once dst.a = 1; the render-target path
+            // stops at 0.95
+            if (S.blend ? !(dest[3] < 255.0f) : !(dest[3] < 0.95f)) break;
             const float dk = -0.5f * S.D + ((float)k + 0.5f) * S.D / (float)S.n;
             const float sk = (dk - vcam) / vdir;
             if (!(sk > 0.0f)) continue;
@@ -889,12 +978,20 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
             }
             Sample s;
             shade_sample<FT, NC>(a, tf, op, r, pos, fv, cf, l4, s);
-            composite(dest, s);
-            for (int q = 0; q < 4; ++q) dest[q] = half_rn(dest[q]);
+            if (S.blend) {
+                const float src[4] = {s.rgb[0] * s.a, s.rgb[1] * s.a, s.rgb[2] * s.a, s.a};
+                blend_u8(dest, src);
+            } else {
+                composite(dest, s);
+                for (int q = 0; q < 4; ++q) dest[q] = half_rn(dest[q]);
+            }
             ++ns;
         }
         const long long pix = (long long)j * a.view.width + i;
-        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+        if (S.blend)
+            store_blend(a.image, pix, dest);
+        else
+            reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
     }
     if (a.counter) add_samples(a.counter, ns, red);
 }
@@ -1043,7 +1140,7 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
         for (int g = 0; g < G; ++g) {
             const int sl = gl0 + g;
             if constexpr (SLICES) {
-                if (bk + g >= S.n || !(dest[3] < 0.95f)) {
+                if (bk + g >= S.n || (S.blend ? !(dest[3] < 255.0f) : !(dest[3] < 0.95f))) {
                     alive = false;
                     break;
                 }
@@ -1056,10 +1153,14 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
             }
             float sv[4];
             for (int q = 0; q < 4; ++q) sv[q] = __shfl(src[q], sl);
-            const float t = 1.0f - dest[3];
-            for (int q = 0; q < 4; ++q) {
-                dest[q] = clamp01(fmaf(t, sv[q], dest[q]));
-                if constexpr (SLICES) dest[q] = half_rn(dest[q]);
+            if (SLICES && S.blend) {
+                blend_u8(dest, sv);
+            } else {
+                const float t = 1.0f - dest[3];
+                for (int q = 0; q < 4; ++q) {
+                    dest[q] = clamp01(fmaf(t, sv[q], dest[q]));
+                    if constexpr (SLICES) dest[q] = half_rn(dest[q]);
+                }
             }
             ++ns;
             if constexpr (!SLICES) {
@@ -1080,7 +1181,10 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
     }
     if (has_pix && sub == 0) {
         const long long pix = (long long)j * W + i;
-        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+        if (SLICES && S.blend)
+            store_blend(a.image, pix, dest);
+        else
+            reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
     }
     if (a.counter) add_samples(a.counter, sub == 0 ? ns : 0ull, red);
 }
@@ -1247,19 +1351,31 @@ hipError_t noalloc_scratch(int device, void** ptr)
     return hipSuccess;
 }
 
-hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
+hipError_t launch_adjoint_forward(const AdjointLaunch& A, hipStream_t s)
 {
     dim3 block(kBX, kBY);
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
     return with_field_type(A.r.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
-        if (A.r.lc == 4) {
+        if (A.r.lc == 4)
             hipLaunchKernelGGL((k_adjoint_forward<FT, 4>), grid, block, 0, s, A);
-            hipLaunchKernelGGL((k_render_adjoint<FT, 4>), grid, block, 0, s, A);
-        } else {
+        else
             hipLaunchKernelGGL((k_adjoint_forward<FT, 1>), grid, block, 0, s, A);
+    });
+}
+
+hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
+{
+    const long long nrays = (long long)A.r.view.width * A.r.view.height;
+    hipLaunchKernelGGL(k_adjoint_bound, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, A);
+    dim3 block(kBX, kBY);
+    dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
+    return with_field_type(A.r.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (A.r.lc == 4)
+            hipLaunchKernelGGL((k_render_adjoint<FT, 4>), grid, block, 0, s, A);
+        else
             hipLaunchKernelGGL((k_render_adjoint<FT, 1>), grid, block, 0, s, A);
-        }
     });
 }
 
