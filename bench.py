@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-target", action="store_true",
                     help="skip the FP64 check of the 256^3 LIC + 1024^2 image + adjoint")
     ap.add_argument("--no-precision", action="store_true", help="skip the 512^3 FP16-storage leg")
+    ap.add_argument("--lic-brick", action="store_true",
+                    help="FP32: the LIC volume kernel reads the 8^3-brick copy of the field (tuning lic.brick)")
     ap.add_argument("--batch-reps", type=int, default=5)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="only 3 LIC launches and 3 copies of 256 MiB (rocprof counter runs)")
@@ -164,6 +166,8 @@ def main():
     print(f"[bench] generating {n}^3 Rankine field on {dev}", file=sys.stderr, flush=True)
     field = fields.rankine_vortices_torch(n, device=dev).contiguous()
     ctx = vv.Context(0)
+    if args.lic_brick:
+        ctx.set_tuning("lic.brick", 1)
     ctx.upload_field(field, dims, storage=storage)
     ctx.generate_noise(1, vv.NOISE_UNIFORM, 0.0, dims)
     lut, inv = vv.make_filter(vv.FILTER_GAUSSIAN, L)

@@ -79,6 +79,11 @@ def config2():
     print(f"\n[config 2] {n}^3 Rankine, Gaussian L={L}, {W}^2 frame", flush=True)
     e = _engine(n, L, vv.FILTER_GAUSSIAN, vv.NOISE_UNIFORM, 0.0, W)
     out = {s: _run(e, s) for s in (vv.STORE_FP32, vv.STORE_FP16)}
+    # the FP32 LIC copy in 8^3 bricks (lic.brick)
+    e["ctx"].upload_field(e["field"], e["dims"], storage=vv.STORE_FP32)
+    e["ctx"].set_tuning("lic.brick", 1 - e["ctx"].get_tuning("lic.brick"))
+    e["ctx"].compute_lic()
+    e["lic_other_layout"] = e["ctx"].download_lic()
     e["host"] = e["field"].cpu().numpy()
     e["ctx"].close()
     del e["field"]
@@ -107,6 +112,7 @@ def test_config2_fp32_parity(config2):
     assert e_lic <= 1e-5
     assert e_img <= 1e-5
     assert ns == total
+    np.testing.assert_array_equal(e["lic_other_layout"], lic)
 
 
 def test_config2_fp16_image_within_1e3_of_fp64(config2):

@@ -206,6 +206,11 @@ def test_lic_streamlines_leaving_the_volume(ctx, storage):
     ctx.compute_lic()
     ctx.set_tuning("lic.cache", 1)
     assert rel_err(ctx.download_lic(), ref) <= TOL
+    # the FP32 LIC copy in 8^3 bricks: the same bits
+    ctx.set_tuning("lic.brick", 1)
+    ctx.compute_lic()
+    ctx.set_tuning("lic.brick", 0)
+    np.testing.assert_array_equal(ctx.download_lic(), got)
 
 
 def test_lic_scalar_gate(ctx):

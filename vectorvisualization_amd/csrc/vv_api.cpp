@@ -551,20 +551,31 @@ vv_status fill_lic_launch(vv_ctx* c, LicLaunch& L, void* out)
     // field/noise pair.  Without memory for the copy it reads the noise volume.
     if (L.cache && nc == 1 && !L.gate && L.freq == 1.0f && c->ndims[0] == c->fdims[0] &&
         c->ndims[1] == c->fdims[1] && c->ndims[2] == c->fdims[2]) {
+        long long nb[3];
+        brick_counts(c->fdims, nb);
+        const size_t brick_bytes = (size_t)(nb[0] * nb[1] * nb[2] * 512) * 16;
+        const bool brick = c->lic_brick && c->storage == VV_STORE_FP32 && brick_bytes < (1ull << 32) &&
+                           nb[0] * nb[1] * 512 < (1ll << 24);
+        if (c->lic_field_ok && c->lic_field_brick != brick) c->lic_field_ok = false;
         if (!c->lic_field_ok) {
-            const hipError_t e = c->lic_field.ensure((size_t)L.field_bytes);
+            const hipError_t e = c->lic_field.ensure((size_t)L.field_bytes + (brick ? brick_bytes : 0));
             if (e == hipErrorOutOfMemory) {
                 (void)hipGetLastError();
             } else {
                 HIPCHK(c, e);
                 HIPCHK(c, launch_pack_lic_field(c->field.p, c->storage, (const uint8_t*)c->noise.p, c->fdims,
                                                 c->lic_field.p, c->stream));
+                if (brick)
+                    HIPCHK(c, launch_pack_lic_field_brick(c->lic_field.p, c->fdims,
+                                                          (char*)c->lic_field.p + L.field_bytes, c->stream));
                 c->lic_field_ok = true;
+                c->lic_field_brick = brick;
             }
         }
         if (c->lic_field_ok) {
             L.field.p = c->lic_field.p;
             L.field_has_noise = 1;
+            if (c->lic_field_brick) L.field_brick = (const char*)c->lic_field.p + L.field_bytes;
         }
     }
     return VV_OK;
@@ -660,6 +671,7 @@ vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
         else if (k == "lic.tile_y") slot = &c->lic_tile[1];
         else if (k == "lic.tile_z") slot = &c->lic_tile[2];
         else if (k == "lic.cache") slot = &c->lic_cache;
+        else if (k == "lic.brick") slot = &c->lic_brick;
         else if (k == "lic.super_x") slot = &c->lic_super[0];
         else if (k == "lic.super_y") slot = &c->lic_super[1];
         else if (k == "lic.super_z") slot = &c->lic_super[2];
@@ -668,7 +680,7 @@ vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
         else if (k == "render.block_y") slot = &c->render_block[1];
         else if (k == "render.ray_group") slot = &c->ray_group;
         else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
-        if (k == "lic.cache" || k == "adjoint.window") {
+        if (k == "lic.cache" || k == "lic.brick" || k == "adjoint.window") {
             if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
         } else if (k == "render.ray_group") {
             if (value < 0 || value > 64 || (value & (value - 1)))
@@ -690,6 +702,7 @@ vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
         else if (k == "lic.tile_y") *value = c->lic_tile[1];
         else if (k == "lic.tile_z") *value = c->lic_tile[2];
         else if (k == "lic.cache") *value = c->lic_cache;
+        else if (k == "lic.brick") *value = c->lic_brick;
         else if (k == "lic.super_x") *value = c->lic_super[0];
         else if (k == "lic.super_y") *value = c->lic_super[1];
         else if (k == "lic.super_z") *value = c->lic_super[2];

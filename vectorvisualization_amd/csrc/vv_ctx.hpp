@@ -104,6 +104,8 @@ struct vv_ctx {
     int lic_tile[3] = {16, 4, 4};
     int lic_super[3] = {2, 16, 16};
     int lic_cache = 1;
+    int lic_brick = 0;  // FP32: the LIC volume kernel reads an 8^3-brick copy (lic.brick)
+    bool lic_field_brick = false;  // lic_field holds the brick copy after the linear one
     int adj_window = 1;
     // Version of what a frame depends on (field, LIC volume, opacity scale,
     // transfer function, LIC parameters, line material); every change bumps it.

@@ -62,7 +62,16 @@ struct LicLaunch {
     unsigned long long field_bytes, noise_bytes;  // padded buffer sizes
     int field_has_noise; // field is the LIC copy: channel 3 = noise byte (same grid)
     int speed;         // SPEED_OF_FLOW: Heun displacements scaled by the field's channel 3
+    const void* field_brick;  // float4 LIC copy in 8^3 bricks (launch_pack_lic_field_brick), or null
 };
+
+// 8^3-brick order of a CLAMP-padded volume of d + 2 texels per axis: texel
+// (x, y, z) at element ((bz * nby + by) * nbx + bx) * 512 + (lz * 8 + ly) * 8
+// + lx, b = coordinate / 8, l = coordinate % 8, nb = ceil((d + 2) / 8)
+inline void brick_counts(const int d[3], long long nb[3])
+{
+    for (int a = 0; a < 3; ++a) nb[a] = (d[a] + 2 + 7) / 8;
+}
 
 struct ViewBlock {
     float cam[3], du[3], dv[3], dw[3];
@@ -173,6 +182,8 @@ inline AdjScratchLayout adj_scratch_layout(long long nrays, int max_ckpt, long l
 }
 
 // ---- launchers (vv_lic.hip / vv_render.hip) ----
+// the float4 LIC copy (CLAMP-padded, d = field dims) reordered into 8^3 bricks
+hipError_t launch_pack_lic_field_brick(const void* copy, const int d[3], void* brick, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, uint32_t v, long long n, hipStream_t s);
 hipError_t launch_field_maxlen(const float* v0, const float* v1, float t, long long n,
                                uint32_t* maxbits, hipStream_t s);

@@ -196,6 +196,22 @@ __global__ void __launch_bounds__(kThreads) k_pack_lic_field(const T* field, con
     }
 }
 
+// the float4 LIC copy reordered into 8^3 bricks (brick_counts): texel (x, y,
+// z) of the padded copy to element ((bz * nby + by) * nbx + bx) * 512 +
+// (lz * 8 + ly) * 8 + lx; elements of bricks past the padded volume are not
+// written (no fill reads them)
+__global__ void __launch_bounds__(kThreads) k_pack_lic_field_brick(const float4* copy, int px, int py, int pz,
+                                                                   int nbx, int nby, float4* out)
+{
+    const long long np = (long long)px * py * pz;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int x = (int)(q % px), y = (int)((q / px) % py), z = (int)(q / ((long long)px * py));
+    const long long e = (((long long)(z >> 3) * nby + (y >> 3)) * nbx + (x >> 3)) * 512 +
+                        (((z & 7) * 8 + (y & 7)) * 8 + (x & 7));
+    out[e] = copy[q];
+}
+
 // ---- noise -----------------------------------------------------------------
 // noise[i] = (i+1)-th output of SplitMix64 seeded with `seed`
 __device__ __forceinline__ uint64_t splitmix64_at(uint64_t seed, uint64_t i)
@@ -411,7 +427,7 @@ __global__ void __launch_bounds__(kThreads) k_repeat_halo_f32(float* vol, int nc
 // ---- LIC ---------------------------------------------------------------------
 #include "vv_lic_dev.hpp"
 
-template <typename FT, bool SAME>
+template <typename FT, bool SAME, bool BRICK = false>
 __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
 {
     int bx, by, bz;
@@ -433,21 +449,22 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     fc.reset();
     NoiseCache ncache;
     ncache.reset();
-    const float acc = lic_point_cached<FT, SAME>(a, c, fc, ncache, sx, sy, sz);
-    store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
+    const float acc = lic_point_cached<FT, SAME, BRICK, true>(a, c, fc, ncache, sx, sy, sz);
+    store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * noise_scale(a));
 }
 
 // SAME: at most 80 VGPRs (6 waves per SIMD instead of 5, 1-2 registers
 // spilled), measured 1.5 % faster at 256^3 (FP32 5.71 vs 5.80 ms, FP16 4.79
 // vs 4.86 ms); the separate-noise variants keep their 86 registers
-template <typename FT, bool SAME>
+// BRICK: the float4 LIC copy in 8^3 bricks (a.field.p, strides a.bsy/bsz)
+template <typename FT, bool SAME, bool BRICK = false>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(SAME ? 6 : 1)))
 k_lic_cached(LicArgs a)
 {
     const int b = blockIdx.x;
     const int tile = (b & 7) * a.per_xcd + (b >> 3);
     if (tile >= a.nblocks) return;
-    lic_cached_tile<FT, SAME>(a, tile);
+    lic_cached_tile<FT, SAME, BRICK>(a, tile);
 }
 
 // LIC over the tiles of several volumes (one LicArgs per volume, same
@@ -557,6 +574,16 @@ hipError_t launch_pack_lic_field(const void* field, int field_type, const uint8_
         hipLaunchKernelGGL(k_pack_lic_field<FT>, dim3(blocks_for(np)), dim3(kThreads), 0, s, (const FT*)field, noise,
                            d[0], d[1], d[2], (FT*)out);
     });
+}
+
+hipError_t launch_pack_lic_field_brick(const void* copy, const int d[3], void* brick, hipStream_t s)
+{
+    long long nb[3];
+    brick_counts(d, nb);
+    const long long np = (long long)(d[0] + 2) * (d[1] + 2) * (d[2] + 2);
+    hipLaunchKernelGGL(k_pack_lic_field_brick, dim3(blocks_for(np)), dim3(kThreads), 0, s, (const float4*)copy,
+                       d[0] + 2, d[1] + 2, d[2] + 2, (int)nb[0], (int)nb[1], (float4*)brick);
+    return hipGetLastError();
 }
 
 hipError_t launch_noise_gen(uint64_t seed, int kind, uint64_t thr, const int d[3], uint8_t* out,
@@ -707,6 +734,20 @@ hipError_t launch_lic(const LicLaunch& L, hipStream_t s)
                       L.noise.n[1] == L.field.n[1] && L.noise.n[2] == L.field.n[2];
     return with_field_type(L.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
+        if constexpr (std::is_same<FT, float4>::value) {
+            if (cached && same && L.field_brick) {
+                // the LIC volume kernel alone reads the brick copy
+                long long nb[3];
+                const int d[3] = {L.field.n[0], L.field.n[1], L.field.n[2]};
+                brick_counts(d, nb);
+                LicArgs ab = a;
+                ab.field.p = L.field_brick;
+                ab.bsy = (int)(nb[0] * 512);
+                ab.bsz = (int)(nb[0] * nb[1] * 512);
+                hipLaunchKernelGGL((k_lic_cached<FT, true, true>), grid, block, 0, s, ab);
+                return;
+            }
+        }
         if (cached && same)
             hipLaunchKernelGGL((k_lic_cached<FT, true>), grid, block, 0, s, a);
         else if (cached)

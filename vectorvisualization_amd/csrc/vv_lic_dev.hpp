@@ -23,6 +23,8 @@ struct LicArgs {
     int sx, sy, sz;      // super-tile size in tiles (3D-blocked tile order)
     int sgx, sgy;        // super-tiles per axis (x, y)
     int speed;           // SPEED_OF_FLOW (uncached path only)
+    int bsy, bsz;        // BRICK kernels: field.p is the 8^3-brick copy with these brick-row and
+                         // brick-plane strides (elements)
 };
 
 // logical tile t -> tile coordinates: super-tiles in raster order, tiles
@@ -281,6 +283,7 @@ __device__ __forceinline__ unsigned noise_pair(const unsigned char* base, unsign
 
 struct CachedArgs {
     const char* fr;           // field texels
+    unsigned bsy, bsz;        // BRICK: brick-row and brick-plane strides (elements)
     const unsigned char* nr;  // noise bytes
     float fnm1[3];      // field n-1 per axis
     unsigned fsy, fsz;  // field element strides
@@ -302,7 +305,7 @@ __device__ __forceinline__ bool key_hit(float kx, float ky, float kz, const Ax& 
 // arithmetic are those of noise_from_axes.  field_from_axes sets fc.same
 // for a cell whose three floors lie in that range.
 
-template <typename FT, int NCH>
+template <typename FT, int NCH, bool BRICK = false>
 __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<NCH>& fc, const Ax& ax,
                                                   const Ax& ay, const Ax& az)
 {
@@ -315,17 +318,45 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
         const float my = __builtin_amdgcn_fmed3f(ay.fl, -1.0f, c.fnm1[1]);
         const float mz = __builtin_amdgcn_fmed3f(az.fl, -1.0f, c.fnm1[2]);
         if constexpr (NCH == 4) fc.same = mx == ax.fl && my == ay.fl && mz == az.fl;
-        const unsigned base = __umul24((unsigned)((int)mz + 1), c.fsz) + __umul24((unsigned)((int)my + 1), c.fsy) +
-                              (unsigned)((int)mx + 1);
-        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+        if constexpr (BRICK) {
+            // float4 texels in 8^3 bricks (c.fr): the element of padded
+            // texel (x, y, z) is xb(x) + yb(y) + zb(z)
+            const unsigned x0 = (unsigned)((int)mx + 1), y0 = (unsigned)((int)my + 1), z0 = (unsigned)((int)mz + 1);
+            const unsigned xb[2] = {((x0 >> 3) << 9) | (x0 & 7u), (((x0 + 1) >> 3) << 9) | ((x0 + 1) & 7u)};
+            const unsigned yb[2] = {__umul24(y0 >> 3, c.bsy) + ((y0 & 7u) << 3),
+                                    __umul24((y0 + 1) >> 3, c.bsy) + (((y0 + 1) & 7u) << 3)};
+            const unsigned zb[2] = {__umul24(z0 >> 3, c.bsz) + ((z0 & 7u) << 6),
+                                    __umul24((z0 + 1) >> 3, c.bsz) + (((z0 + 1) & 7u) << 6)};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            float t0[NCH], t1[NCH];
-            FieldLoad<FT>::template pair<NCH>(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+            for (int k = 0; k < 4; ++k) {
+                const unsigned row = yb[k & 1] + zb[k >> 1];
+                const gfloat* q0 = (const gfloat*)((gchar)c.fr + ((row + xb[0]) << 4));
+                const gfloat* q1 = (const gfloat*)((gchar)c.fr + ((row + xb[1]) << 4));
+                float t0[NCH], t1[NCH];
 #pragma unroll
-            for (int ch = 0; ch < NCH; ++ch) {
-                fc.v0[ch][k] = t0[ch];
-                fc.dx[ch][k] = t1[ch] - t0[ch];
+                for (int ch = 0; ch < NCH; ++ch) {
+                    t0[ch] = q0[ch];
+                    t1[ch] = q1[ch];
+                }
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) {
+                    fc.v0[ch][k] = t0[ch];
+                    fc.dx[ch][k] = t1[ch] - t0[ch];
+                }
+            }
+        } else {
+            const unsigned base = __umul24((unsigned)((int)mz + 1), c.fsz) + __umul24((unsigned)((int)my + 1), c.fsy) +
+                                  (unsigned)((int)mx + 1);
+            const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                float t0[NCH], t1[NCH];
+                FieldLoad<FT>::template pair<NCH>(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+#pragma unroll
+                for (int ch = 0; ch < NCH; ++ch) {
+                    fc.v0[ch][k] = t0[ch];
+                    fc.dx[ch][k] = t1[ch] - t0[ch];
+                }
             }
         }
     }
@@ -391,7 +422,7 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
             nc.dx[k] = t1 - t0;
         }
     }
-    noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f) * kInv255;
+    noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f);
     float3 r;
     r.x = FieldScale<FT>::apply(tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f));
     r.y = FieldScale<FT>::apply(tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f));
@@ -418,7 +449,7 @@ __device__ __forceinline__ float noise_from_axes(const LicArgs& a, const CachedA
             nc.dx[k] = t1 - t0;
         }
     }
-    return tri_cached(nc.v0, nc.dx, ax.f, ay.f, az.f) * kInv255;
+    return tri_cached(nc.v0, nc.dx, ax.f, ay.f, az.f);
 }
 
 // REPEAT corner index (floor mod n) with float arithmetic instead of an
@@ -451,7 +482,7 @@ __device__ __forceinline__ float noise_direct(const LicArgs& a, const CachedArgs
         v0[k] = (float)(u & 0xffu);
         dx[k] = (float)(u >> 8) - v0[k];
     }
-    return tri_cached(v0, dx, ax.f, ay.f, az.f) * kInv255;
+    return tri_cached(v0, dx, ax.f, ay.f, az.f);
 }
 
 // noise of the cell just fetched into fc (SAME): channel 3 of the cache, or
@@ -460,12 +491,17 @@ template <typename FT>
 __device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& c, const FieldCache<4>& fc,
                                             const Ax& ax, const Ax& ay, const Ax& az)
 {
-    if (fc.same) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f) * kInv255;
+    if (fc.same) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f);
     return noise_direct(a, c, ax, ay, az);
 }
 
-// LIC of one seed with the per-lane cell caches (scalar noise without gate):
-// the weighted noise sum, before the final scaling.  SAME: the noise grid
+// LIC of one seed with the per-lane cell caches (scalar noise without gate).
+// The noise helpers of the cached path (noise_same, noise_from_axes,
+// noise_direct, field_and_noise) return byte units (0..255).
+// FOLD (the LIC-volume kernels): the sum of noise * weight in byte units,
+// each term one fma; the caller scales it by scl / 255 (noise_scale).
+// Otherwise (the ray casters, whose FP16 slicing target rounds every
+// slice): the shader's sum of (noise / 255) * weight, scaled by scl.  SAME: the noise grid
 // equals the field grid and freq == 1, so the noise cell of the corrector
 // position reuses the corrector's floors and fractions.  The caches may hold
 // cells of an earlier seed (the ray caster keeps them along a ray): they only
@@ -473,7 +509,18 @@ __device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& 
 template <bool SAME>
 using LicFieldCache = FieldCache<SAME ? 4 : 3>;
 
-template <typename FT, bool SAME>
+// Step of one axis from an encoded field value s: (2s - 1) * h with the
+// shader's two roundings.  2s is exact, so fma(2, s, -1) rounds 2s - 1
+// once, as 2.0*s - 1.0 does.  (The one-rounding fma(s, 2h, -h) equals it
+// only where 2s - 1 is exact, s >= 0.25; elsewhere a one-ulp step
+// difference grows along a streamline through a rough field to several
+// 1e-5 of the LIC range, so the literal form stays.)
+__device__ __forceinline__ float decode_step(float s, float h)
+{
+    return fmaf(2.0f, s, -1.0f) * h;
+}
+
+template <typename FT, bool SAME, bool BRICK = false, bool FOLD = false>
 __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, LicFieldCache<SAME>& fc,
                                                   NoiseCache& ncache, float sx, float sy, float sz)
 {
@@ -488,11 +535,12 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
         }
     };
     Ax ax = axis(sx, nx), ay = axis(sy, ny), az = axis(sz, nz);
-    const float3 s0 = field_from_axes<FT>(c, fc, ax, ay, az);
+    const float3 s0 = field_from_axes<FT, SAME ? 4 : 3, BRICK>(c, fc, ax, ay, az);
     // the weights are global memory; the cast keeps the batch kernel, which
     // reads this pointer from a descriptor, on scalar loads
     const gfloat* w = (const gfloat*)a.w;
-    float acc = noise(sx, sy, sz, ax, ay, az) * w[0];
+    const float n0 = noise(sx, sy, sz, ax, ay, az);
+    float acc = FOLD ? n0 * w[0] : (n0 * kInv255) * w[0];
 
 #pragma unroll 1
     for (int dirn = 0; dirn < 2; ++dirn) {
@@ -504,17 +552,17 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
 #pragma unroll 1
         for (int i = 0; i < steps; ++i) {
             // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector.
-            // 2s is exact, so fma(2, s, -1) rounds 2s - 1 exactly once, and
-            // 0.5*(d1 + d2) is exact, so fma(0.5, d1 + d2, p) rounds once:
-            // both equal the two-operation forms bit for bit.
-            const float d1x = fmaf(2.0f, s.x, -1.0f) * h;
-            const float d1y = fmaf(2.0f, s.y, -1.0f) * h;
-            const float d1z = fmaf(2.0f, s.z, -1.0f) * h;
+            // 0.5*(d1 + d2) is exact, so fma(0.5, d1 + d2, p) rounds once,
+            // as p + 0.5*(d1 + d2) does.
+            const float d1x = decode_step(s.x, h);
+            const float d1y = decode_step(s.y, h);
+            const float d1z = decode_step(s.z, h);
             const float qx = px + d1x, qy = py + d1y, qz = pz + d1z;
-            const float3 s2 = field_from_axes<FT>(c, fc, axis(qx, nx), axis(qy, ny), axis(qz, nz));
-            const float d2x = fmaf(2.0f, s2.x, -1.0f) * h;
-            const float d2y = fmaf(2.0f, s2.y, -1.0f) * h;
-            const float d2z = fmaf(2.0f, s2.z, -1.0f) * h;
+            const float3 s2 =
+                field_from_axes<FT, SAME ? 4 : 3, BRICK>(c, fc, axis(qx, nx), axis(qy, ny), axis(qz, nz));
+            const float d2x = decode_step(s2.x, h);
+            const float d2y = decode_step(s2.y, h);
+            const float d2z = decode_step(s2.z, h);
             px = fmaf(0.5f, d1x + d2x, px);
             py = fmaf(0.5f, d1y + d2y, py);
             pz = fmaf(0.5f, d1z + d2z, pz);
@@ -523,23 +571,34 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
             az = axis(pz, nz);
             float nval;
             if constexpr (SAME) {
-                s = field_from_axes<FT>(c, fc, ax, ay, az);
+                s = field_from_axes<FT, 4, BRICK>(c, fc, ax, ay, az);
                 nval = noise_same<FT>(a, c, fc, ax, ay, az);
             } else {
                 s = field_and_noise<FT>(a, c, fc, ncache, ax, ay, az, axis(px * a.freq, a.noise.n[0]),
                                         axis(py * a.freq, a.noise.n[1]), axis(pz * a.freq, a.noise.n[2]),
                                         nval);
             }
-            acc = acc + nval * wk[i];
+            if constexpr (FOLD)
+                acc = fmaf(nval, wk[i], acc);
+            else
+                acc = acc + (nval * kInv255) * wk[i];
         }
     }
     return acc;
+}
+
+// factor from lic_point_cached's FOLD sum to the LIC value
+__device__ __forceinline__ float noise_scale(const LicArgs& a)
+{
+    return a.scl * kInv255;
 }
 
 __device__ __forceinline__ CachedArgs cached_args(const LicArgs& a)
 {
     CachedArgs c;
     c.fr = (const char*)a.field.p;
+    c.bsy = (unsigned)a.bsy;
+    c.bsz = (unsigned)a.bsz;
     c.nr = (const unsigned char*)a.noise.p;
     c.fnm1[0] = (float)(a.field.n[0] - 1);
     c.fnm1[1] = (float)(a.field.n[1] - 1);
@@ -557,6 +616,13 @@ __device__ __forceinline__ CachedArgs cached_args(const LicArgs& a)
 // are scaled by the field magnitude .a at the step's start point (the shader
 // scales licdir2 by step.a before step is re-read), in the shader's order:
 // ((2s - 1) * a) * h.
+// SPEED_OF_FLOW step in the shader's order: ((2s - 1) * a) * h
+__device__ __forceinline__ float3 speed_step(const float3& s, float sa, float h)
+{
+    return make_float3(((2.0f * s.x - 1.0f) * sa) * h, ((2.0f * s.y - 1.0f) * sa) * h,
+                       ((2.0f * s.z - 1.0f) * sa) * h);
+}
+
 template <typename FT, int NC, bool GATE>
 __device__ __forceinline__ float4 lic_point(const LicArgs& a, float sx, float sy, float sz)
 {
@@ -582,25 +648,13 @@ __device__ __forceinline__ float4 lic_point(const LicArgs& a, float sx, float sy
 #pragma unroll 1
         for (int i = 0; i < steps; ++i) {
             // singleLICstep (inc_lic.glsl:93-145), Heun predictor/corrector
-            float t1x = 2.0f * s.x - 1.0f, t1y = 2.0f * s.y - 1.0f, t1z = 2.0f * s.z - 1.0f;
-            if (speed) {
-                t1x = t1x * sa;
-                t1y = t1y * sa;
-                t1z = t1z * sa;
-            }
-            const float d1x = t1x * h;
-            const float d1y = t1y * h;
-            const float d1z = t1z * h;
+            const float3 d1 = speed ? speed_step(s, sa, h) : make_float3(decode_step(s.x, h), decode_step(s.y, h),
+                                                                         decode_step(s.z, h));
+            const float d1x = d1.x, d1y = d1.y, d1z = d1.z;
             const float3 s2 = field_rgb<FT>(a, px + d1x, py + d1y, pz + d1z);
-            float t2x = 2.0f * s2.x - 1.0f, t2y = 2.0f * s2.y - 1.0f, t2z = 2.0f * s2.z - 1.0f;
-            if (speed) {
-                t2x = t2x * sa;
-                t2y = t2y * sa;
-                t2z = t2z * sa;
-            }
-            const float d2x = t2x * h;
-            const float d2y = t2y * h;
-            const float d2z = t2z * h;
+            const float3 d2 = speed ? speed_step(s2, sa, h) : make_float3(decode_step(s2.x, h), decode_step(s2.y, h),
+                                                                          decode_step(s2.z, h));
+            const float d2x = d2.x, d2y = d2.y, d2z = d2.z;
             px = px + 0.5f * (d1x + d2x);
             py = py + 0.5f * (d1y + d2y);
             pz = pz + 0.5f * (d1z + d2z);
