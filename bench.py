@@ -316,8 +316,16 @@ def main():
             l_ms = timed(ctx.compute_lic)
             f_ms = timed(lambda: ctx.render(cam, out=image))
             a_ms = timed(lambda: ctx.render_adjoint(cam, dimg, dtf, dsig))
-            legs[name] = dict(lic_ms=l_ms, frame_ms=f_ms, adjoint_ms=a_ms,
-                              pipeline_ms=l_ms + f_ms + a_ms)
+
+            # a training step's render + backward: vv_render_record, then the
+            # adjoint reusing its forward pass (what the autograd shim runs)
+            def rec(ds):
+                ctx.render_record(cam, out=image)
+                ctx.render_adjoint(cam, dimg, dtf, ds)
+            r_ms = timed(lambda: rec(dsig))
+            r_tf_ms = timed(lambda: rec(None))
+            legs[name] = dict(lic_ms=l_ms, frame_ms=f_ms, adjoint_ms=a_ms, fwd_adjoint_recorded_ms=r_ms,
+                              fwd_adjoint_recorded_tf_only_ms=r_tf_ms, pipeline_ms=l_ms + r_ms)
         # the one-lane-per-voxel FP32 kernel without cell caches (information
         # only: the library runs the cached kernel for this workload)
         ctx.upload_field(field, dims, storage=vv.STORE_FP32)
@@ -325,19 +333,23 @@ def main():
         uncached_lic = timed(ctx.compute_lic)
         ctx.set_tuning("lic.cache", 1)
         out["pipeline"] = {
-            "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms); speedups against the "
-                    "FP32-stored path the library runs",
+            "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms); adjoint_ms is a stand-alone "
+                    "adjoint (its own forward walk), fwd_adjoint_recorded_ms a vv_render_record frame followed by "
+                    "the adjoint that reuses it (pipeline_ms = lic_ms + that); speedups against the FP32-stored "
+                    "path the library runs",
             "fp32": legs["fp32"],
             "fp16": legs["fp16"],
             "unorm8": legs["unorm8"],
             "fp16_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["fp16"]["pipeline_ms"],
             "fp16_lic_speedup_vs_fp32": legs["fp32"]["lic_ms"] / legs["fp16"]["lic_ms"],
             "unorm8_speedup_vs_fp32": legs["fp32"]["pipeline_ms"] / legs["unorm8"]["pipeline_ms"],
-            "fwd_plus_adjoint_over_frame": {k: (legs[k]["frame_ms"] + legs[k]["adjoint_ms"]) / legs[k]["frame_ms"]
+            "fwd_plus_adjoint_over_frame": {k: legs[k]["fwd_adjoint_recorded_ms"] / legs[k]["frame_ms"]
                                             for k in legs},
+            "fwd_plus_tf_adjoint_over_frame": {k: legs[k]["fwd_adjoint_recorded_tf_only_ms"] / legs[k]["frame_ms"]
+                                               for k in legs},
             "fp32_uncached_lic_ms": uncached_lic,
         }
-        out["fwd_plus_adjoint_ms"] = legs[args.storage]["frame_ms"] + legs[args.storage]["adjoint_ms"]
+        out["fwd_plus_adjoint_ms"] = legs[args.storage]["fwd_adjoint_recorded_ms"]
         # the fused caster (lic3d_fragment.glsl): LIC of every ray sample on the fly
         fused = {}
         for name, st in (("fp32", vv.STORE_FP32), ("fp16", vv.STORE_FP16)):
@@ -706,8 +718,10 @@ def tf_fit_leg(dev, field, L, W=1024, steps=200, seed_target=3, seed_start=5):
     of some of the million rays when it changes by h, a jump that the
     difference quotient sees and the adjoint (which holds the index fixed)
     does not; RGB entries leave every alpha unchanged.  The alpha entries are
-    checked a second time with the opacity table scaled by 0.01, where no ray
-    terminates early and the loss is smooth."""
+    checked again on the loss restricted to the rays whose termination index
+    stays the same (vv_record_ray_samples of the +h, 0 and -h passes), and a
+    third time with the opacity table scaled by 0.01, where no ray terminates
+    early and the loss is smooth."""
     import numpy as np
     import torch
 
@@ -766,6 +780,35 @@ def tf_fit_leg(dev, field, L, W=1024, steps=200, seed_target=3, seed_start=5):
     n_rgb, worst_rgb = fd_worst(np.flatnonzero(big & (ch < 3)), g)
     n_a, worst_a = fd_worst(np.flatnonzero(big & (ch == 3)), g)
 
+    # alpha entries at the configuration's own settings, on the rays whose
+    # early-termination index is the same at tf0 - h, tf0 and tf0 + h (the
+    # per-ray sample counts of the recorded forward passes): central
+    # differences of that masked loss against its adjoint
+    H_, W_ = cam.height, cam.width
+
+    def counts_of(tf_):
+        with torch.no_grad():
+            img_ = render_tf(ctx, cam, tf_, op)
+        return img_, ctx.record_ray_samples(cam).copy()
+
+    _, c0 = counts_of(tf0.detach())
+    idx_m = rng.choice(np.flatnonzero(big & (ch == 3)), min(32, int((big & (ch == 3)).sum())), replace=False)
+    worst_m, excluded = 0.0, []
+    for k in idx_m:
+        e = torch.zeros(1024, device=dev)
+        e[k] = h
+        ip, cp = counts_of(tf0.detach() + e.view(256, 4))
+        im, cm = counts_of(tf0.detach() - e.view(256, 4))
+        keep = (cp == c0) & (cm == c0)
+        excluded.append(int((~keep).sum()))
+        mt = torch.from_numpy(keep).to(dev).view(H_, W_, 1).double()
+        fd = float(((((ip.double() - target.double()) ** 2) - ((im.double() - target.double()) ** 2)) * mt).sum()
+                   / (2 * h))
+        tfk = tf0.detach().clone().requires_grad_(True)
+        (((render_tf(ctx, cam, tfk, op).double() - target.double()) ** 2) * mt).sum().backward()
+        gk = float(tfk.grad.view(-1)[k])
+        worst_m = max(worst_m, abs(fd - gk) / max(abs(fd), 1e-3 * np.abs(g).max()))
+
     # alpha entries again with no early termination: the opacity table scaled
     # by 0.01 keeps every sample's alpha <= 0.01 (<= 0.005 after the 1/256
     # step correction), so no ray of <= 443 samples reaches dest.a > 0.95 and
@@ -806,6 +849,9 @@ def tf_fit_leg(dev, field, L, W=1024, steps=200, seed_target=3, seed_start=5):
         "fd_rgb_worst_rel_err": worst_rgb,
         "fd_alpha_entries": n_a,
         "fd_alpha_worst_rel_err": worst_a,
+        "fd_alpha_masked": {"entries": len(idx_m), "worst_rel_err": worst_m,
+                            "rays_excluded_mean": float(np.mean(excluded)), "rays_excluded_max": max(excluded),
+                            "rule": "loss over the rays whose termination index is equal at tf0-h, tf0, tf0+h"},
         "fd_alpha_no_termination": {"opacity_table_scale": 0.01, "entries": n_a_lo,
                                     "worst_rel_err": worst_a_lo, "image_alpha_max": alpha_max},
         "adam_steps": steps,

@@ -305,6 +305,16 @@ vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dima
 vv_status vv_render_record(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
                            uint64_t* samples);
 
+/* Per-ray sample counts of the forward pass vv_render_record keeps:
+ * width*height int32 (row-major, as the image), the number of samples each
+ * ray composited before it left the volume or its opacity passed 0.95 (its
+ * early-termination index).  No counterpart in the reference: a diagnostic
+ * (ray cost map) and the check of a finite difference, which is only
+ * comparable with the adjoint on rays whose index does not change.
+ * VV_ESTATE when there is no recorded pass (none yet, or an adjoint has
+ * consumed it) or it is not of this camera. */
+vv_status vv_record_ray_samples(vv_ctx* ctx, const vv_camera* cam, int32_t* counts, int on_device);
+
 /* vv_render_adjoint without any device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  dimage,

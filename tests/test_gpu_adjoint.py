@@ -241,9 +241,20 @@ def test_render_record_image_and_reused_forward_pass(ctx, grad, illum):
     img, ns = ctx.render_record(cam, count_samples=True)
     np.testing.assert_array_equal(img, ref_img)
     assert ns == ref_ns and ns > 0
+    # per-ray sample counts of the record: the oracle's termination indices
+    counts = ctx.record_ray_samples(cam)
+    ldims, lch = ctx.lic_info()
+    _, ref_counts, ref_total = orc.raycast(c["f4"], c["dims"], ctx.download_lic(), lch, ldims, rgba, op,
+                                           oparams, view, sigma=sigma)
+    assert int(counts.sum()) == ns == ref_total
+    np.testing.assert_array_equal(counts, ref_counts)
+    with pytest.raises(vv.VVError):
+        ctx.record_ray_samples(vv.default_camera(96, 80, ctx.field_geometry()[2]))  # another camera
     tf, sg = ctx.render_adjoint(cam, dimg)
     np.testing.assert_array_equal(tf, ref_tf)
     np.testing.assert_array_equal(sg, ref_sg)
+    with pytest.raises(vv.VVError):
+        ctx.record_ray_samples(cam)  # the adjoint consumed the record
     # device image, then the TF-only gradient from the record
     dev = torch.empty(96 * 80 * 4, dtype=torch.float32, device="cuda")
     ctx.render_record(cam, out=dev)

@@ -159,6 +159,7 @@ def _load() -> C.CDLL:
         "vv_render_adjoint": (I, [P, C.POINTER(Camera), P, P, P, I]),
         "vv_render_adjoint_noalloc": (I, [P, C.POINTER(Camera), P, P, P]),
         "vv_render_record": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
+        "vv_record_ray_samples": (I, [P, C.POINTER(Camera), P, I]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
         "vv_batch_create": (I, [P, C.POINTER(P)]),
@@ -228,7 +229,7 @@ EXPORTED = [
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
     "vv_make_illum_tables", "vv_render_fused", "vv_render_slices", "vv_render_adjoint_noalloc",
-    "vv_render_slices_blend", "vv_render_record",
+    "vv_render_slices_blend", "vv_render_record", "vv_record_ray_samples",
 ]
 
 
@@ -692,6 +693,16 @@ class Context:
             self._chk(LIB.vv_render_record(self._h, C.byref(cam), p, dev,
                                            C.byref(ns) if count_samples else None))
         return (out, ns.value) if count_samples else out
+
+    def record_ray_samples(self, cam: Camera, out=None):
+        """vv_record_ray_samples: per-ray sample counts (int32, height*width)
+        of the last render_record of this camera."""
+        if out is None:
+            out = np.empty(cam.width * cam.height, np.int32)
+        p, dev = _buf(out, "int32", cam.width * cam.height, "ray sample counts")
+        with self.ordered(out):
+            self._chk(LIB.vv_record_ray_samples(self._h, C.byref(cam), p, dev))
+        return out
 
     def render_fused(self, cam: Camera, out=None, count_samples: bool = False):
         """Fused ray cast: the LIC of every sample computed on the fly."""

@@ -1046,6 +1046,25 @@ vv_status vv_render_record(vv_ctx* c, const vv_camera* cam, float* image, int on
     });
 }
 
+vv_status vv_record_ray_samples(vv_ctx* c, const vv_camera* cam, int32_t* counts, int on_device)
+{
+    return guarded(c, [&]() -> vv_status {
+        if (!c) return VV_EINVAL;
+        if (!cam || !counts) return fail(c, VV_EINVAL, "cam/counts is NULL");
+        if (!c->adj_rec.valid || c->adj_rec.buffer != c->adj_ckpt.p)
+            return fail(c, VV_ESTATE, "no recorded forward pass (vv_render_record; an adjoint consumes it)");
+        if (std::memcmp(&c->adj_rec.cam, cam, sizeof(vv_camera)) != 0)
+            return fail(c, VV_ESTATE, "the recorded forward pass is of another camera");
+        HIPCHK(c, hipSetDevice(c->device));
+        const long long nrays = (long long)c->adj_rec.cam.width * c->adj_rec.cam.height;
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, c->adj_rec.max_ckpt, 0);
+        HIPCHK(c, hipMemcpyAsync(counts, (const char*)c->adj_ckpt.p + SL.nsamp, sizeof(int32_t) * (size_t)nrays,
+                                 on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
+        if (!on_device) HIPCHK(c, hipStreamSynchronize(c->stream));
+        return VV_OK;
+    });
+}
+
 vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage, float* dtf, float* dsigma,
                             int on_device)
 {
