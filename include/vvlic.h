@@ -292,16 +292,20 @@ vv_status vv_set_opacity_scale(vv_ctx* ctx, const float* sigma, int on_device);
 vv_status vv_render_adjoint(vv_ctx* ctx, const vv_camera* cam, const float* dimage,
                             float* dtf, float* dsigma, int on_device);
 
-/* vv_render that also keeps the adjoint's forward pass (checkpoints, sample
- * counts, per-ray bounds) in the context.  No counterpart in the reference:
- * a training step that renders and then differentiates the same view calls
- * this instead of vv_render, and the following vv_render_adjoint with the
- * same camera skips its own forward pass.  The image and the sample count
- * are vv_render's bit for bit.  Any change to the scene (field, noise, LIC
- * volume or parameters, transfer function, opacity scale, line material)
- * or another camera makes the next vv_render_adjoint run the forward pass
- * again, as does a vv_render_adjoint in between (its backward walk reuses
- * the buffer); the gradients are the same either way. */
+/* vv_render that also keeps the adjoint's forward pass (per-segment state
+ * of every ray, sample counts, per-ray bounds) in the context.  No
+ * counterpart in the reference: a training step that renders and then
+ * differentiates the same view calls this instead of vv_render, and the
+ * following vv_render_adjoint with the same camera skips its own forward
+ * pass.  The image and the sample count are vv_render's bit for bit.  Any
+ * change to the scene (field, noise, LIC volume or parameters, transfer
+ * function, opacity scale, line material) or another camera makes the next
+ * vv_render_adjoint run the forward pass again; the gradients are the same
+ * either way.  The pass keeps 352 bytes per ray and 64 samples of its
+ * longest possible ray (about 2.5 GB at 1024^2 and sample distance 1/256);
+ * above 8 GiB it keeps checkpoints every 64 samples instead (about 0.6 KB
+ * per ray), the adjoint re-runs the segments, and a vv_render_adjoint uses
+ * the recorded pass up. */
 vv_status vv_render_record(vv_ctx* ctx, const vv_camera* cam, float* image, int on_device,
                            uint64_t* samples);
 
@@ -312,7 +316,7 @@ vv_status vv_render_record(vv_ctx* ctx, const vv_camera* cam, float* image, int 
  * (ray cost map) and the check of a finite difference, which is only
  * comparable with the adjoint on rays whose index does not change.
  * VV_ESTATE when there is no recorded pass (none yet, or an adjoint has
- * consumed it) or it is not of this camera. */
+ * used it up) or it is not of this camera. */
 vv_status vv_record_ray_samples(vv_ctx* ctx, const vv_camera* cam, int32_t* counts, int on_device);
 
 /* vv_render_adjoint without any device allocation.  No counterpart in the
@@ -324,7 +328,9 @@ vv_status vv_record_ray_samples(vv_ctx* ctx, const vv_camera* cam, int32_t* coun
  * on holds until the process ends; a call whose view and opacity-scale grid
  * need more returns VV_EINVAL (about 0.6 KB per pixel for rays of up to
  * 512 samples, plus 8 B per opacity-scale voxel: a 384x384 view at sample
- * distance 1/256 with an opacity scale on a 128^3 grid fits).  The results
+ * distance 1/256 with an opacity scale on a 128^3 grid fits; views small
+ * enough for a column of 352 B per 64 samples of every ray keep those
+ * instead of checkpoints, as vv_render_adjoint does).  The results
  * are those of vv_render_adjoint bit for bit.  The call is stream-ordered
  * and returns without waiting, except that a transfer function changed
  * since the last call is first uploaded with a blocking copy; line-

@@ -987,13 +987,22 @@ static vv_status adjoint_backward(vv_ctx* c, AdjointLaunch& A, const float* dima
     return VV_OK;
 }
 
+// The full layout (a column per segment, no re-run in the backward walk)
+// when its columns take at most this many bytes; the compact one otherwise.
+constexpr size_t kFullColumnCap = 8ull << 30;
+static bool full_columns_fit(long long nrays, int max_ckpt, size_t cap)
+{
+    return (size_t)kAdjColumnBytes * (size_t)max_ckpt * (size_t)nrays <= cap;
+}
+
 // Both passes on a scratch buffer of adj_scratch_layout(nrays, A.max_ckpt,
-// dsigma ? nv : 0) bytes.
+// dsigma ? nv : 0, A.full_col) bytes.
 static vv_status adjoint_enqueue(vv_ctx* c, AdjointLaunch& A, char* scratch, const float* dimage, float* dtf,
                                  float* dsigma, long long nv)
 {
     const long long nrays = (long long)A.r.view.width * A.r.view.height;
-    adjoint_bind(A, scratch, adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0), dsigma != nullptr);
+    adjoint_bind(A, scratch, adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0, A.full_col != 0),
+                 dsigma != nullptr);
     vv_status st = adjoint_forward(c, A);
     if (st) return st;
     return adjoint_backward(c, A, dimage, dtf, dsigma, nv);
@@ -1010,8 +1019,9 @@ vv_status vv_render_record(vv_ctx* c, const vv_camera* cam, float* image, int on
         if (st) return st;
         c->adj_rec.valid = false;
         const long long nrays = (long long)cam->width * cam->height;
+        A.full_col = full_columns_fit(nrays, A.max_ckpt, kFullColumnCap);
         // room for the opacity-scale gradient, so the adjoint can keep the buffer
-        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, nvox(c->ldims));
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, nvox(c->ldims), A.full_col != 0);
         HIPCHK(c, c->adj_ckpt.ensure(SL.total));
         const size_t bytes = sizeof(float) * 4 * (size_t)nrays;
         float* dst = image;
@@ -1041,6 +1051,7 @@ vv_status vv_render_record(vv_ctx* c, const vv_camera* cam, float* image, int on
         c->adj_rec.cam = *cam;
         c->adj_rec.version = c->scene_version;
         c->adj_rec.max_ckpt = A.max_ckpt;
+        c->adj_rec.full_col = A.full_col;
         c->adj_rec.buffer = c->adj_ckpt.p;
         return VV_OK;
     });
@@ -1057,7 +1068,7 @@ vv_status vv_record_ray_samples(vv_ctx* c, const vv_camera* cam, int32_t* counts
             return fail(c, VV_ESTATE, "the recorded forward pass is of another camera");
         HIPCHK(c, hipSetDevice(c->device));
         const long long nrays = (long long)c->adj_rec.cam.width * c->adj_rec.cam.height;
-        const AdjScratchLayout SL = adj_scratch_layout(nrays, c->adj_rec.max_ckpt, 0);
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, c->adj_rec.max_ckpt, 0, c->adj_rec.full_col != 0);
         HIPCHK(c, hipMemcpyAsync(counts, (const char*)c->adj_ckpt.p + SL.nsamp, sizeof(int32_t) * (size_t)nrays,
                                  on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
         if (!on_device) HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1077,13 +1088,17 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
         if (st) return st;
         const long long nrays = (long long)cam->width * cam->height;
         const long long nv = nvox(c->ldims);
-        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
+        A.full_col = full_columns_fit(nrays, A.max_ckpt, kFullColumnCap);
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0, A.full_col != 0);
         HIPCHK(c, c->adj_ckpt.ensure(SL.total));
         // the forward pass vv_render_record left for this camera and scene
         const bool reuse = c->adj_rec.valid && c->adj_rec.version == c->scene_version &&
                            c->adj_rec.buffer == c->adj_ckpt.p && c->adj_rec.max_ckpt == A.max_ckpt &&
+                           c->adj_rec.full_col == A.full_col &&
                            std::memcmp(&c->adj_rec.cam, cam, sizeof(vv_camera)) == 0;
-        c->adj_rec.valid = false;  // the backward walk rewrites the segment column
+        // the compact layout's backward walk rewrites the segment column; the
+        // full layout leaves the recorded pass as it was
+        if (!A.full_col) c->adj_rec.valid = false;
         const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
         const float* d_dimg = dimage;
         float* d_dtf = dtf;
@@ -1130,7 +1145,8 @@ vv_status vv_render_adjoint_noalloc(vv_ctx* c, const vv_camera* cam, const float
         if (st) return st;
         const long long nrays = (long long)cam->width * cam->height;
         const long long nv = nvox(c->ldims);
-        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0);
+        A.full_col = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0, true).total <= kNoAllocScratchBytes;
+        const AdjScratchLayout SL = adj_scratch_layout(nrays, A.max_ckpt, dsigma ? nv : 0, A.full_col != 0);
         if (SL.total > kNoAllocScratchBytes)
             return fail(c, VV_EINVAL,
                         "the allocation-free adjoint needs %zu bytes of scratch for a %dx%d view (%d checkpoints per "

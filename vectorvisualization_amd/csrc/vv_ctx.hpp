@@ -111,12 +111,14 @@ struct vv_ctx {
     // transfer function, LIC parameters, line material); every change bumps it.
     unsigned long long scene_version = 0;
     // The forward pass vv_render_record left in adj_ckpt for the adjoint of
-    // the same camera and scene version (valid until an adjoint uses it).
+    // the same camera and scene version (the compact layout's adjoint uses
+    // it up; the full layout's keeps it).
     struct {
         bool valid = false;
         vv_camera cam;
         unsigned long long version = 0;
         int max_ckpt = 0;
+        int full_col = 0;
         void* buffer = nullptr;
     } adj_rec;
     int render_block[2] = {16, 16};

@@ -132,10 +132,17 @@ struct LicBatchLaunch {
 //           32 bits: the float bits of the largest bound on one sample's
 //           gradient contribution (atomicMax; NaN bits when a bound is not
 //           finite)
-//   ckpt    per ray 2*max_ckpt float4 checkpoints, then the segment column:
-//           kAdjSeg floats (D.a before each sample), kAdjSeg/16 u64 (the
-//           clamp masks of 16 samples, 4 bits each), kAdjSeg/16 float4 (the
-//           position at the start of each 16-sample part)
+//   ckpt    one of two layouts, per ray:
+//           compact (full_col 0): 2*max_ckpt float4 checkpoints (position,
+//             D) every kAdjSeg samples, then one segment column (below)
+//             that the forward pass leaves holding the last segment and the
+//             backward walk refills from the checkpoints;
+//           full (full_col 1): max_ckpt segment columns, one per segment,
+//             all written by the forward pass (no checkpoints, no re-run)
+//           A segment column: kAdjSeg floats (D.a before each sample),
+//           kAdjSeg/16 u64 (the clamp masks of 16 samples, 4 bits each),
+//           kAdjSeg/16 float4 (the position at the start of each 16-sample
+//           part), each array [index][ray]
 //   nsamp   per ray sample count (int)
 //   bound   per ray bound on its contributions per unit of max |dL/dimage|
 //           (float)
@@ -154,6 +161,7 @@ struct AdjointLaunch {
     long long* dtf_fx;            // 256*4 fixed-point accumulators (zero before launch)
     long long* dsigma_fx;         // sigma voxel accumulators, or null
     int use_window;               // accumulate dsigma in an LDS window per segment
+    int full_col;                 // the full layout: a column per segment, no checkpoints
 };
 
 // Scratch of vv_render_adjoint_noalloc: one static device array per GPU
@@ -168,13 +176,15 @@ hipError_t noalloc_scratch(int device, void** ptr);
 struct AdjScratchLayout {
     size_t ckpt, nsamp, bound, acc, total;
 };
-inline AdjScratchLayout adj_scratch_layout(long long nrays, int max_ckpt, long long nv)
+inline AdjScratchLayout adj_scratch_layout(long long nrays, int max_ckpt, long long nv, bool full_col = false)
 {
     AdjScratchLayout L;
     const size_t align = 256;
     auto up = [&](size_t x) { return (x + align - 1) / align * align; };
     L.ckpt = align;
-    L.nsamp = up(L.ckpt + (16 * (size_t)(2 * max_ckpt) + kAdjColumnBytes) * (size_t)nrays);
+    const size_t per_ray = full_col ? (size_t)kAdjColumnBytes * (size_t)max_ckpt
+                                    : 16 * (size_t)(2 * max_ckpt) + kAdjColumnBytes;
+    L.nsamp = up(L.ckpt + per_ray * (size_t)nrays);
     L.bound = up(L.nsamp + sizeof(int) * (size_t)nrays);
     L.acc = up(L.bound + sizeof(float) * (size_t)nrays);
     L.total = up(L.acc + sizeof(long long) * (size_t)(1024 + nv));

@@ -480,19 +480,24 @@ static_assert(kSub == 16 && kSeg * 4 + kNSub * 24 == kAdjColumnBytes, "segment c
 
 // The segment column of a ray: [k][ray] D.a before sample k of the segment;
 // [m][ray] the clamp masks of part m's samples (4 bits each, sample j of the
-// part at bit 4j); [m][ray] the position at the start of part m.  Phase 1
-// fills it for every segment in turn, so it holds each ray's last segment
-// when phase 2 starts; phase 2 re-runs the earlier segments into it.
+// part at bit 4j); [m][ray] the position at the start of part m.  Compact
+// layout: one column, which phase 1 fills for every segment in turn, so it
+// holds each ray's last segment when phase 2 starts, and phase 2 re-runs the
+// earlier segments into it from their checkpoints.  Full layout (A.full_col):
+// a column per segment, all filled by phase 1.
 struct AdjColumn {
     float* da;
     unsigned long long* mask;
     float4* pos;
 };
 
-__device__ __forceinline__ AdjColumn adj_column(const AdjointLaunch& A, long long nrays)
+__device__ __forceinline__ AdjColumn adj_column(const AdjointLaunch& A, long long nrays, long long seg)
 {
     AdjColumn c;
-    c.da = reinterpret_cast<float*>(reinterpret_cast<float4*>(A.ckpt) + (long long)2 * A.max_ckpt * nrays);
+    char* base = A.full_col ? reinterpret_cast<char*>(A.ckpt) + seg * (long long)kAdjColumnBytes * nrays
+                            : reinterpret_cast<char*>(reinterpret_cast<float4*>(A.ckpt) +
+                                                      (long long)2 * A.max_ckpt * nrays);
+    c.da = reinterpret_cast<float*>(base);
     c.mask = reinterpret_cast<unsigned long long*>(c.da + (long long)kSeg * nrays);
     c.pos = reinterpret_cast<float4*>(c.mask + (long long)kNSub * nrays);
     return c;
@@ -516,8 +521,8 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
     const long long W = a.view.width, H = a.view.height;
     const long long nrays = W * H;
     const long long pix = (long long)j * W + i;
-    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
-    const AdjColumn col = adj_column(A, nrays);
+    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // compact: [c][2][ray] checkpoint (pos, D)
+    AdjColumn col = adj_column(A, nrays, 0);
     Ray r;
     const bool inside = i < W && j < H;
     const bool active = inside && ray_setup(a.view, i, j, r);
@@ -532,8 +537,12 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
             const int kk = (int)(k % kSeg);
             if (kk == 0) {
                 const long long c = k / kSeg;
-                ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
-                ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+                if (A.full_col) {
+                    col = adj_column(A, nrays, c);
+                } else {
+                    ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+                    ck[(2 * c + 1) * nrays + pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+                }
             }
             if (kk % kSub == 0) col.pos[(long long)(kk / kSub) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
             Sample s;
@@ -634,8 +643,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     const long long nrays = W * H;
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
-    const float4* ck = reinterpret_cast<const float4*>(A.ckpt);  // [c][2][ray]: checkpoint (pos, D)
-    const AdjColumn col = adj_column(A, nrays);
+    const float4* ck = reinterpret_cast<const float4*>(A.ckpt);  // compact: [c][2][ray] checkpoint (pos, D)
 
     Ray r;
     const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
@@ -655,6 +663,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     const int smax = seg_max;
     for (int c = smax - 1; c >= 0; --c) {
         const bool mine = active && c < nseg;
+        const AdjColumn col = adj_column(A, nrays, c);
         const long long k0 = (long long)c * kSeg;
         const long long k1 = min(n, k0 + kSeg);
         if (tid < kNSub) {
@@ -662,9 +671,9 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
             box[tid][3] = box[tid][4] = box[tid][5] = -1;
         }
         __syncthreads();
-        // (a) re-run the segment from its checkpoint into the column (phase 1
-        //     left each ray's last segment there)
-        if (mine && c < nseg - 1) {
+        // (a) compact layout: re-run the segment from its checkpoint into the
+        //     column (phase 1 left each ray's last segment there)
+        if (mine && c < nseg - 1 && !A.full_col) {
             const float4 cp = ck[(2 * c) * nrays + pix];
             const float4 cd = ck[(2 * c + 1) * nrays + pix];
             float p[3] = {cp.x, cp.y, cp.z};
