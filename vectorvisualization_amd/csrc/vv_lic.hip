@@ -445,7 +445,7 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     const float sz = ((float)z + 0.5f) / (float)nz * a.sc2;
 
     const CachedArgs c = cached_args(a);
-    LicFieldCache<SAME> fc;
+    LicFieldCache<FT, SAME> fc;
     fc.reset();
     NoiseCache ncache;
     ncache.reset();

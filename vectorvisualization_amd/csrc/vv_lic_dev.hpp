@@ -135,14 +135,97 @@ __device__ __forceinline__ void store_repeat(T* out, int osy, int osz, int x, in
 // costs three compares and no index arithmetic; NaN keys never hit.  (Cells
 // outside the volume with different floors but the same clamped texels are
 // reloaded, with the same values.)
-template <int NCH>
+// v_fma_mix_f32 with one half operand, which the instruction widens exactly
+// (every half is a float), so each form is one rounding of the exact result:
+//   fma_half<HI>(a, b, w) = fmaf(a, b, (float)h), h the low (HI 0) or high
+//   (HI 1) half of w;
+//   sub_half<HI>(w1, w0)  = (float)h1 - (float)h0.
+// They let the FP16 cell cache keep its x0 texels packed as loaded and skip
+// every half-to-float conversion of a cell fill.
+template <int HI>
+__device__ __forceinline__ float fma_half(float a, float b, unsigned w)
+{
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(r) : "v"(a), "v"(b), "v"(w));
+    else
+        asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[0,0,1]" : "=v"(r) : "v"(a), "v"(b), "v"(w));
+    return r;
+}
+
+template <int HI>
+__device__ __forceinline__ float sub_half(unsigned w1, unsigned w0)
+{
+    float r;
+    if constexpr (HI)
+        asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "=v"(r) : "v"(w1), "v"(w0));
+    else
+        asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(w1), "v"(w0));
+    return r;
+}
+
+// FP32 and UNORM8 texels: x0 texels and differences as floats
+template <typename FT, int NCH>
 struct FieldCache {
     float kx, ky, kz;   // floors of the cached cell
     bool same;          // NCH 4: the cell's noise bytes are those of the REPEAT cell
     float v0[NCH][4];   // [channel][(y,z) corner]: texel at x0
     float dx[NCH][4];   // texel(x1) - texel(x0)
     __device__ __forceinline__ void reset() { kx = ky = kz = __builtin_nanf(""); same = false; }
+    // x-lerp of channel CH on corner row k: fma(fx, x1 - x0, x0)
+    template <int CH>
+    __device__ __forceinline__ float xlerp(int k, float fx) const { return fmaf(fx, dx[CH][k], v0[CH][k]); }
+    __device__ __forceinline__ void set_row(int k, const float* t0, const float* t1)
+    {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            v0[ch][k] = t0[ch];
+            dx[ch][k] = t1[ch] - t0[ch];
+        }
+    }
 };
+
+// FP16 texels: the x0 texel of each corner row stays as the two words the
+// row load returned (channels 0|1 and 2|3, low half first), the differences
+// are floats; x-lerps and differences read the halves with fma_half and
+// sub_half.  Values and operation order are those of the float cache on the
+// widened halves, so results are the same bits.
+template <int NCH>
+struct FieldCache<half4_t, NCH> {
+    float kx, ky, kz;
+    bool same;
+    unsigned v0h[4][2];  // [(y,z) corner][word]
+    float dx[NCH][4];
+    __device__ __forceinline__ void reset() { kx = ky = kz = __builtin_nanf(""); same = false; }
+    template <int CH>
+    __device__ __forceinline__ float xlerp(int k, float fx) const
+    {
+        return fma_half<CH & 1>(fx, dx[CH][k], v0h[k][CH >> 1]);
+    }
+    // a = the row's 16 bytes: texel x0 in .x/.y, texel x1 in .z/.w
+    __device__ __forceinline__ void set_row(int k, const uint4& a)
+    {
+        v0h[k][0] = a.x;
+        v0h[k][1] = a.y;
+        dx[0][k] = sub_half<0>(a.z, a.x);
+        dx[1][k] = sub_half<1>(a.z, a.x);
+        dx[2][k] = sub_half<0>(a.w, a.y);
+        if constexpr (NCH == 4) dx[3][k] = sub_half<1>(a.w, a.y);
+    }
+};
+
+// trilinear value of channel CH of a cached cell: tri_cached's arithmetic
+template <int CH, typename FC>
+__device__ __forceinline__ float tri_fc(const FC& fc, float fx, float fy, float fz)
+{
+    const float c00 = fc.template xlerp<CH>(0, fx);
+    const float c10 = fc.template xlerp<CH>(1, fx);
+    const float c01 = fc.template xlerp<CH>(2, fx);
+    const float c11 = fc.template xlerp<CH>(3, fx);
+    const float c0 = lerpf(c00, c10, fy);
+    const float c1 = lerpf(c01, c11, fy);
+    return lerpf(c0, c1, fz);
+}
 
 struct NoiseCache {
     float kx, ky, kz;
@@ -225,10 +308,21 @@ struct FieldLoad<float4> {
     }
 };
 
-// half4 texels: one 16-byte load holds x0 and x1
+// half4 texels: one 16-byte load holds x0 and x1 (row() for the FP16 cell
+// cache, pair() converted)
 template <>
 struct FieldLoad<half4_t> {
     static constexpr unsigned kShift = 3;
+    __device__ static __forceinline__ uint4 row(const char* base, unsigned off)
+    {
+        const gu32* q = (const gu32*)((gchar)base + off);
+        uint4 a;
+        a.x = q[0];
+        a.y = q[1];
+        a.z = q[2];
+        a.w = q[3];
+        return a;
+    }
     template <int NCH>
     __device__ static __forceinline__ void pair(const char* base, unsigned off, float* t0, float* t1)
     {
@@ -273,6 +367,25 @@ struct FieldLoad<u8x4_t> {
     }
 };
 
+// One corner row of a cell into a cache, in two steps so a caller can issue
+// several loads before it uses any: RowLoad<FT, NCH>::load, then store.
+template <typename FT, int NCH>
+struct RowLoad {
+    float t0[NCH], t1[NCH];
+    __device__ __forceinline__ void load(const char* base, unsigned off)
+    {
+        FieldLoad<FT>::template pair<NCH>(base, off, t0, t1);
+    }
+    __device__ __forceinline__ void store(FieldCache<FT, NCH>& fc, int k) const { fc.set_row(k, t0, t1); }
+};
+
+template <int NCH>
+struct RowLoad<half4_t, NCH> {
+    uint4 a;
+    __device__ __forceinline__ void load(const char* base, unsigned off) { a = FieldLoad<half4_t>::row(base, off); }
+    __device__ __forceinline__ void store(FieldCache<half4_t, NCH>& fc, int k) const { fc.set_row(k, a); }
+};
+
 // two adjacent noise bytes (x0, x1) in one 16-bit load; the address may be odd
 typedef unsigned short __attribute__((aligned(1))) u16_unaligned;
 __device__ __forceinline__ unsigned noise_pair(const unsigned char* base, unsigned off)
@@ -306,7 +419,7 @@ __device__ __forceinline__ bool key_hit(float kx, float ky, float kz, const Ax& 
 // for a cell whose three floors lie in that range.
 
 template <typename FT, int NCH, bool BRICK = false>
-__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<NCH>& fc, const Ax& ax,
+__device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCache<FT, NCH>& fc, const Ax& ax,
                                                   const Ax& ay, const Ax& az)
 {
     if (!key_hit(fc.kx, fc.ky, fc.kz, ax, ay, az)) {
@@ -338,11 +451,7 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
                     t0[ch] = q0[ch];
                     t1[ch] = q1[ch];
                 }
-#pragma unroll
-                for (int ch = 0; ch < NCH; ++ch) {
-                    fc.v0[ch][k] = t0[ch];
-                    fc.dx[ch][k] = t1[ch] - t0[ch];
-                }
+                fc.set_row(k, t0, t1);
             }
         } else {
             const unsigned base = __umul24((unsigned)((int)mz + 1), c.fsz) + __umul24((unsigned)((int)my + 1), c.fsy) +
@@ -350,20 +459,16 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
             const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                float t0[NCH], t1[NCH];
-                FieldLoad<FT>::template pair<NCH>(c.fr, (base + off[k]) << FieldLoad<FT>::kShift, t0, t1);
-#pragma unroll
-                for (int ch = 0; ch < NCH; ++ch) {
-                    fc.v0[ch][k] = t0[ch];
-                    fc.dx[ch][k] = t1[ch] - t0[ch];
-                }
+                RowLoad<FT, NCH> row;
+                row.load(c.fr, (base + off[k]) << FieldLoad<FT>::kShift);
+                row.store(fc, k);
             }
         }
     }
     float3 r;
-    r.x = FieldScale<FT>::apply(tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f));
-    r.y = FieldScale<FT>::apply(tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f));
-    r.z = FieldScale<FT>::apply(tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f));
+    r.x = FieldScale<FT>::apply(tri_fc<0>(fc, ax.f, ay.f, az.f));
+    r.y = FieldScale<FT>::apply(tri_fc<1>(fc, ax.f, ay.f, az.f));
+    r.z = FieldScale<FT>::apply(tri_fc<2>(fc, ax.f, ay.f, az.f));
     return r;
 }
 
@@ -372,7 +477,7 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
 // overlap (one memory round trip instead of two); values and arithmetic
 // are those of field_from_axes and noise_from_axes.
 template <typename FT>
-__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache<3>& fc,
+__device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const CachedArgs& c, FieldCache<FT, 3>& fc,
                                                   NoiseCache& nc, const Ax& ax, const Ax& ay,
                                                   const Ax& az, const Ax& nx_, const Ax& ny_,
                                                   const Ax& nz_, float& noise)
@@ -380,18 +485,25 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
     const bool fmiss = !key_hit(fc.kx, fc.ky, fc.kz, ax, ay, az);
     const bool nmiss = !key_hit(nc.kx, nc.ky, nc.kz, nx_, ny_, nz_);
     unsigned short nraw[4];
+    constexpr bool kHalf = std::is_same<FT, half4_t>::value;
+    RowLoad<half4_t, 3> hrows[kHalf ? 4 : 1];  // FP16: the raw rows until both fills are in flight
     if (fmiss) {
         const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
                             __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
         const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            float t0[3], t1[3];
-            FieldLoad<FT>::template pair<3>(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+            if constexpr (kHalf) {
+                hrows[k].load(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift);
+            } else {
+                // float caches hold the raw x1 texel in dx until both fills are in flight
+                float t0[3], t1[3];
+                FieldLoad<FT>::template pair<3>(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                fc.v0[ch][k] = t0[ch];
-                fc.dx[ch][k] = t1[ch];  // raw x1 texel until both fills are in flight
+                for (int ch = 0; ch < 3; ++ch) {
+                    fc.v0[ch][k] = t0[ch];
+                    fc.dx[ch][k] = t1[ch];
+                }
             }
         }
     }
@@ -407,9 +519,14 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
         fc.ky = ay.fl;
         fc.kz = az.fl;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (kHalf) {
+                hrows[k].store(fc, k);
+            } else {
 #pragma unroll
-            for (int ch = 0; ch < 3; ++ch) fc.dx[ch][k] = fc.dx[ch][k] - fc.v0[ch][k];
+                for (int ch = 0; ch < 3; ++ch) fc.dx[ch][k] = fc.dx[ch][k] - fc.v0[ch][k];
+            }
+        }
     }
     if (nmiss) {
         nc.kx = nx_.fl;
@@ -424,9 +541,9 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
     }
     noise = tri_cached(nc.v0, nc.dx, nx_.f, ny_.f, nz_.f);
     float3 r;
-    r.x = FieldScale<FT>::apply(tri_cached(fc.v0[0], fc.dx[0], ax.f, ay.f, az.f));
-    r.y = FieldScale<FT>::apply(tri_cached(fc.v0[1], fc.dx[1], ax.f, ay.f, az.f));
-    r.z = FieldScale<FT>::apply(tri_cached(fc.v0[2], fc.dx[2], ax.f, ay.f, az.f));
+    r.x = FieldScale<FT>::apply(tri_fc<0>(fc, ax.f, ay.f, az.f));
+    r.y = FieldScale<FT>::apply(tri_fc<1>(fc, ax.f, ay.f, az.f));
+    r.z = FieldScale<FT>::apply(tri_fc<2>(fc, ax.f, ay.f, az.f));
     return r;
 }
 
@@ -488,10 +605,10 @@ __device__ __forceinline__ float noise_direct(const LicArgs& a, const CachedArgs
 // noise of the cell just fetched into fc (SAME): channel 3 of the cache, or
 // the REPEAT array for a cell outside the volume
 template <typename FT>
-__device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& c, const FieldCache<4>& fc,
+__device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& c, const FieldCache<FT, 4>& fc,
                                             const Ax& ax, const Ax& ay, const Ax& az)
 {
-    if (fc.same) return tri_cached(fc.v0[3], fc.dx[3], ax.f, ay.f, az.f);
+    if (fc.same) return tri_fc<3>(fc, ax.f, ay.f, az.f);
     return noise_direct(a, c, ax, ay, az);
 }
 
@@ -506,8 +623,8 @@ __device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& 
 // position reuses the corrector's floors and fractions.  The caches may hold
 // cells of an earlier seed (the ray caster keeps them along a ray): they only
 // decide whether a cell is reloaded, never its values.
-template <bool SAME>
-using LicFieldCache = FieldCache<SAME ? 4 : 3>;
+template <typename FT, bool SAME>
+using LicFieldCache = FieldCache<FT, SAME ? 4 : 3>;
 
 // Step of one axis from an encoded field value s: (2s - 1) * h with the
 // shader's two roundings.  2s is exact, so fma(2, s, -1) rounds 2s - 1
@@ -521,7 +638,7 @@ __device__ __forceinline__ float decode_step(float s, float h)
 }
 
 template <typename FT, bool SAME, bool BRICK = false, bool FOLD = false>
-__device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, LicFieldCache<SAME>& fc,
+__device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, LicFieldCache<FT, SAME>& fc,
                                                   NoiseCache& ncache, float sx, float sy, float sz)
 {
     const int nx = a.field.n[0], ny = a.field.n[1], nz = a.field.n[2];

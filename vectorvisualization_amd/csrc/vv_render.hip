@@ -846,7 +846,7 @@ k_render_fused(RenderLaunch a, LicArgs L)
             const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                              a.field.sz};
             const CachedArgs c = cached_args(L);
-            LicFieldCache<MODE == 2> fc;
+            LicFieldCache<FT, MODE == 2> fc;
             fc.reset();
             NoiseCache nc;
             nc.reset();
@@ -942,7 +942,7 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
         const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                          a.field.sz};
         const CachedArgs c = cached_args(L);
-        LicFieldCache<MODE == 2> fc;
+        LicFieldCache<FT, MODE == 2> fc;
         fc.reset();
         NoiseCache nc;
         nc.reset();
@@ -1048,7 +1048,7 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
     const ViewBlock& vb = a.view;
     const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy, a.field.sz};
     const CachedArgs c = cached_args(L);
-    LicFieldCache<MODE == 2> fc;
+    LicFieldCache<FT, MODE == 2> fc;
     fc.reset();
     NoiseCache nc;
     nc.reset();
