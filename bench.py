@@ -444,7 +444,7 @@ def batch_leg(args, dev, seed=7, nsteps=8, nviews=64, L=30):
     b.run()
     ctx.synchronize()
     stream = torch.cuda.ExternalStream(ctx.stream)
-    run_ms, lat = [], []
+    run_ms, lat, span = [], [], []
     for _ in range(max(1, args.batch_reps)):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -453,7 +453,9 @@ def batch_leg(args, dev, seed=7, nsteps=8, nviews=64, L=30):
         torch.cuda.synchronize()
         run_ms.append(e0.elapsed_time(e1))
         lat.append(b.view_latency_ms())
+        span.append(b.view_span_ms())
     lat = np.concatenate(lat)
+    span = np.concatenate(span)
     ray = b.samples()
     lic = sum(n ** 3 * (2 * L + 1) for n in sizes)
     batch_ms = float(np.median(run_ms))
@@ -488,6 +490,11 @@ def batch_leg(args, dev, seed=7, nsteps=8, nviews=64, L=30):
         "batch_ms": batch_ms,
         "view_latency_ms_p50": float(np.percentile(lat, 50)),
         "view_latency_ms_p99": float(np.percentile(lat, 99)),
+        "view_latency_note": "from the start of the batch to the view's last tile (includes the LIC volumes and "
+                             "views queued before it)",
+        "view_span_ms_p50": float(np.percentile(span, 50)),
+        "view_span_ms_p99": float(np.percentile(span, 99)),
+        "view_span_note": "the view's own render: its first tile's start to its last tile's end",
         "lic_voxel_samples": int(lic),
         "ray_samples": int(ray),
         "aggregate_samples_per_s": (lic + ray) / (batch_ms / 1000.0),

@@ -375,6 +375,10 @@ vv_status vv_batch_run(vv_batch* b);
 /* per view: milliseconds from the start of the last run to the end of that
  * view's last pixel tile (device wall clock); waits for the run */
 vv_status vv_batch_view_latency(vv_batch* b, double* ms);
+/* per view: milliseconds from the start of that view's first pixel tile to
+ * the end of its last (its own render span, without the time it waited in
+ * the queue behind LIC volumes and earlier views); waits for the run */
+vv_status vv_batch_view_span(vv_batch* b, double* ms);
 /* total compositing samples of the last run (waits) */
 vv_status vv_batch_samples(vv_batch* b, uint64_t* total);
 vv_status vv_batch_download_lic(vv_batch* b, int timestep, float* out, int on_device);

@@ -114,9 +114,12 @@ def test_batch_matches_single(ctx, storage):
         b.run()
     _check(b, outs, lics, imgs, total)
     lat = b.view_latency_ms()
-    print("view latency ms:", np.round(lat, 4), flush=True)
-    assert lat.shape == (len(VIEWS),)
+    span = b.view_span_ms()
+    print("view latency ms:", np.round(lat, 4), "span ms:", np.round(span, 4), flush=True)
+    assert lat.shape == (len(VIEWS),) and span.shape == (len(VIEWS),)
     assert np.all(lat > 0) and np.all(lat < 1e4)
+    # a view's own render span lies inside the batch, before its end stamp
+    assert np.all(span > 0) and np.all(span <= lat)
     b.close()
 
 

@@ -160,6 +160,7 @@ def _load() -> C.CDLL:
         "vv_render_adjoint_noalloc": (I, [P, C.POINTER(Camera), P, P, P]),
         "vv_render_record": (I, [P, C.POINTER(Camera), P, I, C.POINTER(C.c_uint64)]),
         "vv_record_ray_samples": (I, [P, C.POINTER(Camera), P, I]),
+        "vv_batch_view_span": (I, [P, C.POINTER(C.c_double)]),
         "vv_view_block": (I, [P, C.POINTER(Camera), FP]),
         "vv_lic_weights": (I, [P, FP]),
         "vv_batch_create": (I, [P, C.POINTER(P)]),
@@ -229,7 +230,7 @@ EXPORTED = [
     "vv_anim_create", "vv_anim_load_dat", "vv_anim_next", "vv_anim_seek", "vv_anim_destroy",
     "vv_default_line_material", "vv_set_line_material", "vv_get_line_material",
     "vv_make_illum_tables", "vv_render_fused", "vv_render_slices", "vv_render_adjoint_noalloc",
-    "vv_render_slices_blend", "vv_render_record", "vv_record_ray_samples",
+    "vv_render_slices_blend", "vv_render_record", "vv_record_ray_samples", "vv_batch_view_span",
 ]
 
 
@@ -844,6 +845,13 @@ class Batch:
         n = self.info()[1]
         out = (C.c_double * n)()
         self._ctx._chk(LIB.vv_batch_view_latency(self._h, out))
+        return np.array(out)
+
+    def view_span_ms(self) -> np.ndarray:
+        """Per view: its own render span, first tile start to last tile end."""
+        n = self.info()[1]
+        out = (C.c_double * n)()
+        self._ctx._chk(LIB.vv_batch_view_span(self._h, out))
         return np.array(out)
 
     def samples(self) -> int:

@@ -88,7 +88,7 @@ void release_graph(vv_batch* b)
 // (rounded up to an even count), per-view u64 sample counters, per-view u64
 // completion stamps, then u64 t0
 size_t ctr_head(size_t nsteps) { return (nsteps + 1) & ~(size_t)1; }
-size_t ctr_words(size_t nsteps, size_t nviews) { return ctr_head(nsteps) + 4 * nviews + 2; }
+size_t ctr_words(size_t nsteps, size_t nviews) { return ctr_head(nsteps) + 6 * nviews + 2; }
 
 vv_status build_lic_launch(vv_ctx* c, BatchStep* st, LicLaunch& L)
 {
@@ -444,6 +444,7 @@ vv_status vv_batch_run(vv_batch* b)
         unsigned long long* samples = (unsigned long long*)(ctr + head);
         unsigned long long* done = samples + nv;
         unsigned long long* t0 = done + nv;
+        unsigned long long* begun = t0 + 1;
         b->host_views.resize(nv);
         b->slot.assign(nv, 0);
         b->gview.assign(nsteps + 1, 0);
@@ -530,6 +531,7 @@ vv_status vv_batch_run(vv_batch* b)
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
             hipError_t e = launch_stamp(t0, c->stream);
             if (!e) e = launch_fill_u32(ctr, 0u, (long long)(head + 4 * nv), c->stream);
+            if (!e) e = launch_fill_u32((uint32_t*)begun, 0xffffffffu, (long long)(2 * nv), c->stream);
             for (size_t g = 0; g < nsteps && !e; ++g) {
                 const int t = b->order[g];
                 LicBatchLaunch LB;
@@ -549,6 +551,7 @@ vv_status vv_batch_run(vv_batch* b)
                 RB.total = gtiles[g];
                 RB.counter = ctr + g;
                 RB.done = done + b->gview[g];
+                RB.begun = begun + b->gview[g];
                 RB.block[0] = bx;
                 RB.block[1] = by;
                 RB.grid = render_grid;
@@ -589,6 +592,30 @@ vv_status vv_batch_view_latency(vv_batch* b, double* ms)
         if (khz <= 0) return fail(c, VV_EUNSUPPORTED, "no wall-clock rate");
         const unsigned long long t0 = h[nv];
         for (size_t v = 0; v < nv; ++v) ms[v] = (double)(h[b->slot[v]] - t0) / (double)khz;
+        return VV_OK;
+    });
+}
+
+vv_status vv_batch_view_span(vv_batch* b, double* ms)
+{
+    return guarded((b ? b->ctx : nullptr), [&]() -> vv_status {
+        if (!b || !ms) return VV_EINVAL;
+        vv_ctx* c = b->ctx;
+        if (!b->ran) return fail(c, VV_ESTATE, "batch has not run");
+        const size_t nv = b->views.size();
+        // done[nv], t0, begun[nv]
+        std::vector<unsigned long long> h(2 * nv + 1);
+        const uint32_t* ctr = (const uint32_t*)b->ctr.p;
+        HIPCHK(c, hipMemcpyAsync(h.data(), ctr + ctr_head(b->steps.size()) + 2 * nv,
+                                 sizeof(unsigned long long) * (2 * nv + 1), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        int khz = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+        if (khz <= 0) return fail(c, VV_EUNSUPPORTED, "no wall-clock rate");
+        for (size_t v = 0; v < nv; ++v) {
+            const size_t k = b->slot[v];
+            ms[v] = (double)(h[k] - h[nv + 1 + k]) / (double)khz;
+        }
         return VV_OK;
     });
 }

@@ -112,6 +112,7 @@ struct RenderBatchLaunch {
     int nviews, total;            // views, work items (pixel tiles)
     unsigned* counter;            // device work counter, zero before launch
     unsigned long long* done;     // device: completion stamp per view
+    unsigned long long* begun;    // device: start stamp per view (all ones before launch)
     int block[2];
     int grid;                     // persistent blocks
     int field_type, lc;           // shared by all views

@@ -352,12 +352,13 @@ __global__ void __launch_bounds__(kRThreads) k_render(RenderLaunch a)
 
 // Persistent ray caster over the pixel tiles of many views.  Blocks take
 // work items (view, tile) from a counter (items of different views cost
-// different amounts); the end of each view's last tile is recorded as the
-// largest wall-clock stamp written by that view's tiles.
+// different amounts); the start of each view's first tile and the end of its
+// last tile are recorded as the smallest and largest wall-clock stamps its
+// tiles write.
 template <typename FT, int LC>
 __global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* views, const int* first,
                                                            int nviews, int total, unsigned* counter,
-                                                           unsigned long long* done)
+                                                           unsigned long long* done, unsigned long long* begun)
 {
     __shared__ float tf[1024];
     __shared__ float op[256];
@@ -374,6 +375,7 @@ __global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* 
         int v = 0;
         while (v + 1 < nviews && item >= first[v + 1]) ++v;
         const RenderLaunch& a = views[v];
+        if (tid == 0) atomicMin(&begun[v], (unsigned long long)wall_clock64());
         if (v != cur) {
             load_tables(a.tf, tf, op);
             cur = v;
@@ -1398,10 +1400,10 @@ hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s)
         using FT = typename decltype(tag)::type;
         if (B.lc == 4)
             hipLaunchKernelGGL((k_render_batch<FT, 4>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
-                               B.counter, B.done);
+                               B.counter, B.done, B.begun);
         else
             hipLaunchKernelGGL((k_render_batch<FT, 1>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
-                               B.counter, B.done);
+                               B.counter, B.done, B.begun);
     });
 }
 
