@@ -299,7 +299,9 @@ def test_render_record_is_invalidated_by_scene_and_camera_changes(ctx):
         np.testing.assert_array_equal(tf, want_tf, err_msg=name)
         np.testing.assert_array_equal(sg, want_sg, err_msg=name)
         restore()
-    # another camera: the record of the first does not apply
+    # another camera: the record of the first does not apply, and the other
+    # camera's forward pass replaces it (the first camera's adjoint then runs
+    # its own again)
     cam2 = vv.default_camera(64, 56, ctx.field_geometry()[2])
     ctx.render_record(cam)
     tf, sg = ctx.render_adjoint(cam2, dimg)
@@ -307,6 +309,11 @@ def test_render_record_is_invalidated_by_scene_and_camera_changes(ctx):
     assert not np.array_equal(want_tf, stale_tf)
     np.testing.assert_array_equal(tf, want_tf)
     np.testing.assert_array_equal(sg, want_sg)
+    with pytest.raises(vv.VVError):
+        ctx.record_ray_samples(cam)
+    tf, sg = ctx.render_adjoint(cam, dimg)
+    np.testing.assert_array_equal(tf, stale_tf)
+    np.testing.assert_array_equal(sg, stale_sg)
     # unchanged TF tables set again keep the record
     ctx.render_record(cam)
     ctx.set_tf(rgba, op)

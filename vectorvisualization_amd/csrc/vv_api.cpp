@@ -1096,9 +1096,10 @@ vv_status vv_render_adjoint(vv_ctx* c, const vv_camera* cam, const float* dimage
                            c->adj_rec.buffer == c->adj_ckpt.p && c->adj_rec.max_ckpt == A.max_ckpt &&
                            c->adj_rec.full_col == A.full_col &&
                            std::memcmp(&c->adj_rec.cam, cam, sizeof(vv_camera)) == 0;
-        // the compact layout's backward walk rewrites the segment column; the
-        // full layout leaves the recorded pass as it was
-        if (!A.full_col) c->adj_rec.valid = false;
+        // a forward pass of this call overwrites the buffer, and the compact
+        // layout's backward walk rewrites its segment column; only a reused
+        // full-layout record stays as it was
+        if (!reuse || !A.full_col) c->adj_rec.valid = false;
         const size_t img_bytes = sizeof(float) * 4 * (size_t)nrays;
         const float* d_dimg = dimage;
         float* d_dtf = dtf;
