@@ -828,8 +828,8 @@ __global__ void k_fixed_to_float(const unsigned long long* hdr, const long long*
 // it.  A ray stops when it leaves the box or when the sample's own alpha
 // exceeds 0.95 (src.a, :91; the LIC-volume caster tests the accumulated
 // dest.a).  MODE 0: any noise / gate; 1: per-lane cell caches (scalar noise,
-// no gate), which persist along the ray; 2: the same with the noise grid
-// equal to the field grid at freq 1.
+// no gate), renewed for every sample; 2: the same with the noise grid equal
+// to the field grid at freq 1.
 template <typename FT, int NC, bool GATE, int MODE>
 __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(3)))
 k_render_fused(RenderLaunch a, LicArgs L)
@@ -848,14 +848,15 @@ k_render_fused(RenderLaunch a, LicArgs L)
             const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                              a.field.sz};
             const CachedArgs c = cached_args(L);
-            LicFieldCache<FT, MODE == 2> fc;
-            fc.reset();
-            NoiseCache nc;
-            nc.reset();
             float pos[3] = {r.pos[0], r.pos[1], r.pos[2]};
             for (long long k = 0; k < a.max_samples; ++k) {
                 float l4[4];
                 if constexpr (MODE > 0) {
+                    // cell caches per sample (their registers are free while shading)
+                    LicFieldCache<FT, MODE == 2> fc;
+                    fc.reset();
+                    NoiseCache nc;
+                    nc.reset();
                     const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
                     l4[0] = acc * L.scl;
                 } else {
@@ -944,10 +945,6 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
         const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy,
                          a.field.sz};
         const CachedArgs c = cached_args(L);
-        LicFieldCache<FT, MODE == 2> fc;
-        fc.reset();
-        NoiseCache nc;
-        nc.reset();
         float dest[4] = {0.f, 0.f, 0.f, 0.f};
         for (int k = 0; k < S.n && vdir != 0.0f; ++k) {
             // the blend adds nothing once dst.a = 1; the render-target path
@@ -978,6 +975,11 @@ k_render_slices(RenderLaunch a, LicArgs L, SliceSetup S)
             if (!(lut1d(tf, 256, 4, 3, tfu) > 0.05f)) continue;
             float l4[4];
             if constexpr (MODE > 0) {
+                // cell caches per sample (their registers are free while shading)
+                LicFieldCache<FT, MODE == 2> fc;
+                fc.reset();
+                NoiseCache nc;
+                nc.reset();
                 const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
                 l4[0] = acc * L.scl;
             } else {
@@ -1050,10 +1052,6 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
     const ViewBlock& vb = a.view;
     const Vol<FT> fv{(const FT*)a.field.p, {a.field.n[0], a.field.n[1], a.field.n[2]}, a.field.sy, a.field.sz};
     const CachedArgs c = cached_args(L);
-    LicFieldCache<FT, MODE == 2> fc;
-    fc.reset();
-    NoiseCache nc;
-    nc.reset();
     float dest[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned long long ns = 0;
     Ray r;
@@ -1130,6 +1128,13 @@ k_render_group(RenderLaunch a, LicArgs L, SliceSetup S, int G)
             if (visible) {
                 float l4[4];
                 if constexpr (MODE > 0) {
+                    // cell caches per sample: consecutive samples of a lane are G steps
+                    // apart, so the previous streamline's last cell rarely serves the
+                    // next one, and the caches' registers are free while shading
+                    LicFieldCache<FT, MODE == 2> fc;
+                    fc.reset();
+                    NoiseCache nc;
+                    nc.reset();
                     const float acc = lic_point_cached<FT, MODE == 2>(L, c, fc, nc, pos[0], pos[1], pos[2]);
                     l4[0] = acc * L.scl;
                 } else {
