@@ -449,8 +449,8 @@ __device__ __forceinline__ void lic_cached_tile(const LicArgs& a, int tile)
     fc.reset();
     NoiseCache ncache;
     ncache.reset();
-    const float acc = lic_point_cached<FT, SAME, BRICK, true>(a, c, fc, ncache, sx, sy, sz);
-    store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * noise_scale(a));
+    const float acc = lic_point_cached<FT, SAME, BRICK>(a, c, fc, ncache, sx, sy, sz);
+    store_repeat<float>(a.out, a.osy, a.osz, x, y, z, nx, ny, nz, acc * a.scl);
 }
 
 // SAME: at most 80 VGPRs (6 waves per SIMD instead of 5, 1-2 registers

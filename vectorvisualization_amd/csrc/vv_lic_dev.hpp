@@ -612,13 +612,10 @@ __device__ __forceinline__ float noise_same(const LicArgs& a, const CachedArgs& 
     return noise_direct(a, c, ax, ay, az);
 }
 
-// LIC of one seed with the per-lane cell caches (scalar noise without gate).
-// The noise helpers of the cached path (noise_same, noise_from_axes,
-// noise_direct, field_and_noise) return byte units (0..255).
-// FOLD (the LIC-volume kernels): the sum of noise * weight in byte units,
-// each term one fma; the caller scales it by scl / 255 (noise_scale).
-// Otherwise (the ray casters, whose FP16 slicing target rounds every
-// slice): the shader's sum of (noise / 255) * weight, scaled by scl.  SAME: the noise grid
+// LIC of one seed with the per-lane cell caches (scalar noise without gate):
+// the shader's sum of (noise / 255) * weight, before the final scaling by
+// scl.  The noise helpers of the cached path (noise_same, noise_from_axes,
+// noise_direct, field_and_noise) return byte units (0..255).  SAME: the noise grid
 // equals the field grid and freq == 1, so the noise cell of the corrector
 // position reuses the corrector's floors and fractions.  The caches may hold
 // cells of an earlier seed (the ray caster keeps them along a ray): they only
@@ -637,7 +634,7 @@ __device__ __forceinline__ float decode_step(float s, float h)
     return fmaf(2.0f, s, -1.0f) * h;
 }
 
-template <typename FT, bool SAME, bool BRICK = false, bool FOLD = false>
+template <typename FT, bool SAME, bool BRICK = false>
 __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const CachedArgs& c, LicFieldCache<FT, SAME>& fc,
                                                   NoiseCache& ncache, float sx, float sy, float sz)
 {
@@ -656,8 +653,7 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
     // the weights are global memory; the cast keeps the batch kernel, which
     // reads this pointer from a descriptor, on scalar loads
     const gfloat* w = (const gfloat*)a.w;
-    const float n0 = noise(sx, sy, sz, ax, ay, az);
-    float acc = FOLD ? n0 * w[0] : (n0 * kInv255) * w[0];
+    float acc = (noise(sx, sy, sz, ax, ay, az) * kInv255) * w[0];
 
 #pragma unroll 1
     for (int dirn = 0; dirn < 2; ++dirn) {
@@ -695,19 +691,10 @@ __device__ __forceinline__ float lic_point_cached(const LicArgs& a, const Cached
                                         axis(py * a.freq, a.noise.n[1]), axis(pz * a.freq, a.noise.n[2]),
                                         nval);
             }
-            if constexpr (FOLD)
-                acc = fmaf(nval, wk[i], acc);
-            else
-                acc = acc + (nval * kInv255) * wk[i];
+            acc = acc + (nval * kInv255) * wk[i];
         }
     }
     return acc;
-}
-
-// factor from lic_point_cached's FOLD sum to the LIC value
-__device__ __forceinline__ float noise_scale(const LicArgs& a)
-{
-    return a.scl * kInv255;
 }
 
 __device__ __forceinline__ CachedArgs cached_args(const LicArgs& a)
