@@ -93,7 +93,7 @@ def roofline(tag: str, kernel: str, ms: float, fetch_bytes: float) -> dict:
     The bound that holds is FP32 VALU issue: the fma/add/mul wave64
     instructions of a launch (SQ_INSTS_VALU_{FMA,ADD,MUL}_F32) issue at most
     once per clock per CU (the 157.3 TFLOP/s FP32 figure is 256 CUs x 2.4 GHz
-    x 64 lanes x 2 flops x 2 for packed FP32, which this build does not
+    x 64 lanes x 2 flops x 2 for packed FP32, which the library's kernels do not
     emit), so `frac` = instructions / (launch clocks x 256 CUs).  Beside it:
     the FP32 flop rate against 157.3 TFLOP/s, HBM bytes (FETCH_SIZE +
     WRITE_SIZE, calibrated on a copy of known size) against 8 TB/s, L1

@@ -47,7 +47,7 @@ typedef enum vv_status {
     VV_EINVAL = 1,        /* bad argument (size, pointer, NaN, range) */
     VV_ENOMEM = 2,        /* device or host allocation failed */
     VV_EDEVICE = 3,       /* HIP runtime error */
-    VV_EUNSUPPORTED = 4,  /* valid request this build does not implement */
+    VV_EUNSUPPORTED = 4,  /* valid request the library does not implement */
     VV_ESTATE = 5,        /* required input (field, noise, TF, ...) not loaded */
     VV_EIO = 6            /* file missing, unreadable or malformed */
 } vv_status;

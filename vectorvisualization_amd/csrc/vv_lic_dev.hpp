@@ -308,8 +308,8 @@ struct FieldLoad<float4> {
     }
 };
 
-// half4 texels: one 16-byte load holds x0 and x1 (row() for the FP16 cell
-// cache, pair() converted)
+// half4 texels: one 16-byte load holds x0 and x1, kept as the raw words for
+// the FP16 cell cache
 template <>
 struct FieldLoad<half4_t> {
     static constexpr unsigned kShift = 3;
@@ -322,26 +322,6 @@ struct FieldLoad<half4_t> {
         a.z = q[2];
         a.w = q[3];
         return a;
-    }
-    template <int NCH>
-    __device__ static __forceinline__ void pair(const char* base, unsigned off, float* t0, float* t1)
-    {
-        const gu32* q = (const gu32*)((gchar)base + off);
-        uint4 a;
-        a.x = q[0];
-        a.y = q[1];
-        a.z = q[2];
-        a.w = q[3];
-        t0[0] = __half2float(__ushort_as_half((unsigned short)(a.x & 0xffffu)));
-        t0[1] = __half2float(__ushort_as_half((unsigned short)(a.x >> 16)));
-        t0[2] = __half2float(__ushort_as_half((unsigned short)(a.y & 0xffffu)));
-        t1[0] = __half2float(__ushort_as_half((unsigned short)(a.z & 0xffffu)));
-        t1[1] = __half2float(__ushort_as_half((unsigned short)(a.z >> 16)));
-        t1[2] = __half2float(__ushort_as_half((unsigned short)(a.w & 0xffffu)));
-        if constexpr (NCH == 4) {
-            t0[3] = __half2float(__ushort_as_half((unsigned short)(a.y >> 16)));
-            t1[3] = __half2float(__ushort_as_half((unsigned short)(a.w >> 16)));
-        }
     }
 };
 
