@@ -492,7 +492,9 @@ vv_status vv_anim_destroy(vv_anim* a);
  * product is 256 threads per block), "lic.super_x", "lic.super_y",
  * "lic.super_z" (tiles per super-tile: the 3D-blocked order in which tiles
  * are handed to the GPU), "lic.cache" (0/1: per-lane trilinear cell
- * caches), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
+ * caches), "lic.brick" (0/1: with an FP32 field the LIC volume kernel reads
+ * a copy of the field in 8^3 bricks; measured no faster on the 256^3
+ * workload, off by default), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
  * an LDS window per ray segment before the global integer atomics),
  * "render.block_x", "render.block_y" (pixel tile of the ray caster, 64 to
  * 256 threads in multiples of 64), "render.ray_group" (lanes per ray of
