@@ -169,6 +169,38 @@ def test_ray_groups_give_identical_images(ctx, slices):
             assert n == nref, (G, n, nref)
 
 
+@pytest.mark.parametrize("slices", [False, True])
+def test_ray_groups_identical_for_off_centre_views(ctx, slices):
+    # panned views (the volume near the top or bottom edge, partly outside
+    # the image, or behind the camera): the ray-group kernels' pixel tiles
+    # must give the one-lane caster's image bit for bit
+    c = lic_setup(ctx, n=16, kind="rankine", L=5, noise_kind=vv.NOISE_UNIFORM, gradient_scale=3.0)
+    p = c["params"]
+    p.tf_index = vv.TF_INDEX_MAG
+    p.step_size_vol = 1.0 / 96.0
+    ctx.set_lic_params(p)
+    cen = ctx.field_geometry()[2]
+    ctx.set_tf(*vv.fields.tf_magnitude(0.1, 0.6))
+    render = ctx.render_slices if slices else ctx.render_fused
+    drawn = 0
+    for pos, dist in (((0.0, 1.1, 0.0), 4.0), ((0.0, -1.1, 0.0), 4.0), ((0.3, 1.6, 0.0), 4.0),
+                      ((0.0, -1.6, 0.0), 4.0), ((0.0, 0.0, 0.9), 0.3)):
+        cam = vv.default_camera(41, 67, cen)
+        cam.pos[0], cam.pos[1], cam.pos[2] = pos
+        cam.dist = dist
+        imgs = {}
+        for G in (1, 2, 8):
+            ctx.set_tuning("render.ray_group", G)
+            imgs[G] = render(cam, count_samples=True)
+        ctx.set_tuning("render.ray_group", 0)
+        ref, nref = imgs[1]
+        drawn += nref > 0
+        for G, (img, n) in imgs.items():
+            np.testing.assert_array_equal(img, ref, err_msg=f"pos={pos} G={G}")
+            assert n == nref, (pos, G, n, nref)
+    assert drawn == 4
+
+
 @pytest.mark.parametrize("group", [1, 0])
 def test_slices_gradient_illumination(ctx, group):
     c = lic_setup(ctx, n=14, kind="rankine", L=5, grad=True, density=0.3, gradient_scale=3.0)
