@@ -496,11 +496,14 @@ vv_status vv_anim_destroy(vv_anim* a);
  * a copy of the field in 8^3 bricks; measured no faster on the 256^3
  * workload, off by default), "adjoint.window" (0/1: accumulate the opacity-scale gradient in
  * an LDS window per ray segment before the global integer atomics),
- * "render.block_x", "render.block_y" (pixel tile of the ray caster, 64 to
- * 256 threads in multiples of 64), "render.ray_group" (lanes per ray of
- * vv_render_fused / vv_render_slices, a power of two up to 64; 0 = enough
- * lanes for about 2^21 in total with an FP32 field and 2^22 with FP16 or
- * UNORM8, at most 32 per ray; the image does not depend on it).
+ * "render.volume_group" (lanes per ray of vv_render, a power of two up to
+ * 64; 1 = one lane per pixel; 0 = 4 for views of 2^18 rays or more, about
+ * 2^20 lanes in total below), "render.block_x", "render.block_y" (pixel
+ * tile of vv_render with one lane per pixel, 64 to 256 threads in
+ * multiples of 64), "render.ray_group" (lanes per ray of vv_render_fused /
+ * vv_render_slices, a power of two up to 64; 0 = enough lanes for about
+ * 2^21 in total with an FP32 field and 2^22 with FP16 or UNORM8, at most 32
+ * per ray).  Images and sample counts do not depend on the lanes per ray.
  * Results do not depend on these settings. */
 vv_status vv_set_tuning(vv_ctx* ctx, const char* key, int value);
 vv_status vv_get_tuning(const vv_ctx* ctx, const char* key, int* value);

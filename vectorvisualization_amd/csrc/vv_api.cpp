@@ -679,12 +679,13 @@ vv_status vv_set_tuning(vv_ctx* c, const char* key, int value)
         else if (k == "render.block_x") slot = &c->render_block[0];
         else if (k == "render.block_y") slot = &c->render_block[1];
         else if (k == "render.ray_group") slot = &c->ray_group;
+        else if (k == "render.volume_group") slot = &c->volume_group;
         else return fail(c, VV_EINVAL, "unknown tuning key '%s'", key);
         if (k == "lic.cache" || k == "lic.brick" || k == "adjoint.window") {
             if (value != 0 && value != 1) return fail(c, VV_EINVAL, "%s must be 0 or 1", key);
-        } else if (k == "render.ray_group") {
+        } else if (k == "render.ray_group" || k == "render.volume_group") {
             if (value < 0 || value > 64 || (value & (value - 1)))
-                return fail(c, VV_EINVAL, "render.ray_group must be 0 (automatic) or a power of two up to 64");
+                return fail(c, VV_EINVAL, "%s must be 0 (automatic) or a power of two up to 64", key);
         } else if (value < 1 || value > 256 || (value & (value - 1))) {
             return fail(c, VV_EINVAL, "tile sizes must be powers of two in [1,256]");
         }
@@ -710,6 +711,7 @@ vv_status vv_get_tuning(const vv_ctx* c, const char* key, int* value)
         else if (k == "render.block_x") *value = c->render_block[0];
         else if (k == "render.block_y") *value = c->render_block[1];
         else if (k == "render.ray_group") *value = c->ray_group;
+        else if (k == "render.volume_group") *value = c->volume_group;
         else return VV_EINVAL;
         return VV_OK;
     });
@@ -736,6 +738,21 @@ vv_status vv_view_block(const vv_ctx* c, const vv_camera* cam, float out24[24])
     });
 }
 
+// Lanes per ray of the LIC-volume caster: the tuning value, else 4 for views
+// of 2^18 rays or more and about 2^20 lanes in all below (at most 64 per
+// ray).  With one lane per pixel the longest rays set the frame time while
+// most waves have finished.  Measured on MI355X (256^3 LIC volume, step
+// 1/256, mean of four views): 1024^2 0.70 -> 0.44 ms (G = 4), 512^2
+// 0.50 -> 0.23 ms (4), 256^2 0.35 -> 0.14 ms (16).
+static int volume_group_for(const vv_ctx* c, const vv_camera* cam)
+{
+    if (c->volume_group > 0) return c->volume_group;
+    const long long rays = (long long)cam->width * cam->height;
+    int G = 4;
+    while (G < 64 && rays < (1ll << 18) && rays * G < (1ll << 20)) G *= 2;
+    return G;
+}
+
 vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device, uint64_t* samples_out)
 {
     return guarded(c, [&]() -> vv_status {
@@ -757,7 +774,11 @@ vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device
             R.counter = (unsigned long long*)((char*)c->counter.p + 8);
             HIPCHK(c, launch_fill_u32((uint32_t*)R.counter, 0u, 2, c->stream));
         }
-        HIPCHK(c, launch_render(R, c->stream));
+        const int G = volume_group_for(c, cam);
+        if (G > 1)
+            HIPCHK(c, launch_render_rg(R, G, c->stream));
+        else
+            HIPCHK(c, launch_render(R, c->stream));
         if (!on_device) HIPCHK(c, hipMemcpyAsync(image, dst, bytes, hipMemcpyDeviceToHost, c->stream));
         if (samples_out) {
             unsigned long long t = 0;
@@ -771,9 +792,6 @@ vv_status vv_render(vv_ctx* c, const vv_camera* cam, float* image, int on_device
     });
 }
 
-// lanes per ray of the on-the-fly casters: the tuning value, or enough to
-// put about 2^21 lanes to work (measured on MI355X at 256^3, L = 40: 2 lanes
-// per ray at 1024^2, 8 at 512^2, 32 at 256^2, 64 below)
 // Lanes per ray of the on-the-fly casters: enough lanes in flight to cover
 // the serial chain of samples.  Measured at 256^3 L = 40 on 256^2..1024^2
 // views: about 2^21 lanes for an FP32 field (its LIC is bound by L1 gathers,

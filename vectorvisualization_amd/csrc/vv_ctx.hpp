@@ -123,6 +123,7 @@ struct vv_ctx {
     } adj_rec;
     int render_block[2] = {16, 16};
     int ray_group = 0;  // lanes per ray of the on-the-fly casters, 0: automatic
+    int volume_group = 0;  // lanes per ray of the LIC-volume caster, 0: automatic
 
     // line-illumination tables (built on first use of illum_mode 2 / 3)
     vv_line_material line_mat;

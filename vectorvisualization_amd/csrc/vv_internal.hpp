@@ -227,6 +227,8 @@ hipError_t launch_unpad_repeat_f32(const float* src, int nc, const int dims[3], 
 hipError_t launch_repeat_halo_f32(float* vol, int nc, const int dims[3], hipStream_t s);
 hipError_t launch_lic(const LicLaunch& L, hipStream_t s);
 hipError_t launch_render(const RenderLaunch& R, hipStream_t s);
+// the same frame with G lanes per ray (G a power of two, 2..64; 256/G-ray pixel tiles)
+hipError_t launch_render_rg(const RenderLaunch& R, int G, hipStream_t s);
 hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s);
 hipError_t launch_stamp(unsigned long long* t0, hipStream_t s);
 // fills the per-volume kernel arguments of a LIC batch (host memory, size

@@ -11,6 +11,7 @@
 // opacity table live in LDS.  The build is -ffp-contract=off; fmaf marks
 // every fused operation of the shared formulation.
 #include <mutex>
+#include <utility>
 
 #include "vv_device.hpp"
 #include "vv_internal.hpp"
@@ -348,6 +349,159 @@ __global__ void __launch_bounds__(kRThreads) k_render(RenderLaunch a)
     unsigned long long ns = 0;
     if (i < a.view.width && j < a.view.height) ns = render_pixel<FT, LC>(a, tf, op, i, j);
     if (a.counter) add_samples(a.counter, ns, red);
+}
+
+// Lane g of each G-lane group to every lane of the group, G = 2 or 4: the
+// groups are aligned to the quads of the wave, so one DPP quad permutation
+// ((g, g, g, g) for 4 lanes, (g, g, 2 + g, 2 + g) for 2) moves the value.
+template <int G, int g>
+__device__ __forceinline__ float group_bcast(float v)
+{
+    static_assert(G == 2 || G == 4, "quad groups only");
+    constexpr int ctrl = G == 4 ? g * 0x55 : (g | (g << 2) | ((2 + g) << 4) | ((2 + g) << 6));
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xF, 0xF, true));
+}
+
+// sv[g] = src of lane g of the caller's group, for every g < G
+template <int G, int... g>
+__device__ __forceinline__ void group_gather(const float src[4], float (*sv)[4], int gl0,
+                                             std::integer_sequence<int, g...>)
+{
+    if constexpr (G <= 4) {
+        ((sv[g][0] = group_bcast<G, g>(src[0]), sv[g][1] = group_bcast<G, g>(src[1]),
+          sv[g][2] = group_bcast<G, g>(src[2]), sv[g][3] = group_bcast<G, g>(src[3])),
+         ...);
+    } else {
+        ((sv[g][0] = __shfl(src[0], gl0 + g), sv[g][1] = __shfl(src[1], gl0 + g),
+          sv[g][2] = __shfl(src[2], gl0 + g), sv[g][3] = __shfl(src[3], gl0 + g)),
+         ...);
+    }
+}
+
+// The LIC-volume ray caster with G lanes per ray (G a power of two, 2..64),
+// for views whose longest rays would otherwise set the frame time: in each
+// round lane g of a ray's group evaluates sample base + g, stepping from the
+// round's first position with render_pixel's fma sequence, and the group
+// composites the G samples in order with composite()'s arithmetic and
+// render_pixel's stopping rules (a sample outside the box or past
+// max_samples ends the ray, so does dest.a > 0.95 after a sample).  A lane
+// tests only its own position: compositing stops at the first lane whose
+// sample is not valid, and that lane's test is the one render_pixel makes
+// at the same step.  Images and sample counts equal k_render's for every G.
+// A block's 256/G rays form a TW x TH pixel tile (TW = TH or 2 TH).  GT > 0
+// fixes G at compile time: the group's G samples then reach every lane
+// through shuffles issued back to back and one ballot before the ordered
+// compositing, instead of a shuffle round trip per sample (GT = 0: G at run
+// time, for 16 lanes and more).
+template <typename FT, int LC, int GT>
+__global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_rt)
+{
+    const int G = GT > 0 ? GT : G_rt;
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int tid = threadIdx.x;
+    const int sub = tid & (G - 1);
+    const int gl0 = (tid & 63) & ~(G - 1);  // first lane of the group in the wave
+    const int W = a.view.width, H = a.view.height;
+    const int nr = kRThreads / G;
+    const int lg = 31 - __builtin_clz(nr);
+    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
+    const int tiles_x = (W + TW - 1) / TW;
+    // each wave's 64/G rays form a square-ish WW x WH tile, the block's four
+    // waves a 2 x 2 arrangement of them (TW = 2 WW, TH = 2 WH)
+    const int r_in = tid / G;
+    const int rw = 64 / G;
+    const int lw = 31 - __builtin_clz(rw);
+    const int WW = 1 << ((lw + 1) >> 1), WH = rw / WW;
+    const int w = r_in / rw, lr = r_in & (rw - 1);
+    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
+    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    const bool has_pix = i < W && j < H;
+    float dest[4] = {0.f, 0.f, 0.f, 0.f};
+    unsigned long long ns = 0;
+    Ray r;
+    // every lane of a group follows the same ray, so alive is uniform per group
+    // and the shuffles below read lanes that run the loop with it
+    bool alive = has_pix && ray_setup(a.view, i, j, r);
+    float base[3] = {0.f, 0.f, 0.f};
+    if (alive)
+        for (int q = 0; q < 3; ++q) base[q] = r.pos[q];
+    long long bk = 0;
+    while (alive) {
+        float pos[3] = {base[0], base[1], base[2]};
+        for (int t = 0; t < sub; ++t)
+            for (int q = 0; q < 3; ++q) pos[q] = fmaf(r.dir[q], a.step, pos[q]);
+        const bool ok = bk + sub < a.max_samples && (sub == 0 || !outside(pos, a.view));
+        float src[4] = {0.f, 0.f, 0.f, 0.f};
+        if (ok) {
+            Sample s;
+            eval_sample<FT, LC>(a, tf, op, r, pos, s);
+            src[0] = s.rgb[0] * s.a;
+            src[1] = s.rgb[1] * s.a;
+            src[2] = s.rgb[2] * s.a;
+            src[3] = s.a;
+        }
+        // the position after this lane's sample: the next round's first one
+        // when this is the group's last lane
+        float nxt[3];
+        for (int q = 0; q < 3; ++q) nxt[q] = fmaf(r.dir[q], a.step, pos[q]);
+        const bool nxt_out = !ok || outside(nxt, a.view);
+        if constexpr (GT > 0) {
+            float sv[GT][4];
+            group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
+            // the group's ok bits; every lane of the group runs this code
+            const unsigned okg = (unsigned)(__ballot((int)ok) >> gl0) & ((1u << GT) - 1u);
+#pragma unroll
+            for (int g = 0; g < GT; ++g) {
+                if (!((okg >> g) & 1u)) {
+                    alive = false;
+                    break;
+                }
+                const float t = 1.0f - dest[3];
+                for (int q = 0; q < 4; ++q) dest[q] = clamp01(fmaf(t, sv[g][q], dest[q]));
+                ++ns;
+                if (dest[3] > 0.95f) {
+                    alive = false;
+                    break;
+                }
+            }
+        } else {
+            for (int g = 0; g < G; ++g) {
+                const int sl = gl0 + g;
+                if (__shfl((int)ok, sl) == 0) {
+                    alive = false;
+                    break;
+                }
+                float sv[4];
+                for (int q = 0; q < 4; ++q) sv[q] = __shfl(src[q], sl);
+                const float t = 1.0f - dest[3];
+                for (int q = 0; q < 4; ++q) dest[q] = clamp01(fmaf(t, sv[q], dest[q]));
+                ++ns;
+                if (dest[3] > 0.95f) {
+                    alive = false;
+                    break;
+                }
+            }
+        }
+        if (alive) {
+            const int last = gl0 + G - 1;
+            if constexpr (GT > 0 && GT <= 4) {
+                for (int q = 0; q < 3; ++q) base[q] = group_bcast<GT, GT - 1>(nxt[q]);
+            } else {
+                for (int q = 0; q < 3; ++q) base[q] = __shfl(nxt[q], last);
+            }
+            if ((__ballot((int)nxt_out) >> last) & 1ull) alive = false;
+        }
+        bk += G;
+    }
+    if (has_pix && sub == 0) {
+        const long long pix = (long long)j * W + i;
+        reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    if (a.counter) add_samples(a.counter, sub == 0 ? ns : 0ull, red);
 }
 
 // Persistent ray caster over the pixel tiles of many views.  Blocks take
@@ -1220,6 +1374,37 @@ hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
         else
             hipLaunchKernelGGL((k_render<FT, 1>), grid, block, 0, s, R);
     });
+}
+
+hipError_t launch_render_rg(const RenderLaunch& R, int G, hipStream_t s)
+{
+    if (G < 2 || G > 64 || (G & (G - 1))) return hipErrorInvalidValue;
+    const int nr = kRThreads / G;
+    int lg = 0;
+    while ((1 << (lg + 1)) <= nr) ++lg;
+    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
+    const long long nblocks = (long long)((R.view.width + TW - 1) / TW) * ((R.view.height + TH - 1) / TH);
+    if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    dim3 grid((unsigned)nblocks), block(kRThreads);
+#define VV_RG_LAUNCH(LC)                                                                    \
+    do {                                                                                    \
+        if (G == 2)                                                                         \
+            hipLaunchKernelGGL((k_render_rg<FT, LC, 2>), grid, block, 0, s, R, G);          \
+        else if (G == 4)                                                                    \
+            hipLaunchKernelGGL((k_render_rg<FT, LC, 4>), grid, block, 0, s, R, G);          \
+        else if (G == 8)                                                                    \
+            hipLaunchKernelGGL((k_render_rg<FT, LC, 8>), grid, block, 0, s, R, G);          \
+        else                                                                                \
+            hipLaunchKernelGGL((k_render_rg<FT, LC, 0>), grid, block, 0, s, R, G);          \
+    } while (0)
+    return with_field_type(R.field_type, [&](auto tag) {
+        using FT = typename decltype(tag)::type;
+        if (R.lc == 4)
+            VV_RG_LAUNCH(4);
+        else
+            VV_RG_LAUNCH(1);
+    });
+#undef VV_RG_LAUNCH
 }
 
 namespace {
