@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--pmc-probe-fused", action="store_true",
                     help="only 3 fused on-the-fly LIC ray casts of the image (rocprof counter runs)")
     ap.add_argument("--pmc-probe-frame", action="store_true",
-                    help="only 3 frames and 3 TF/opacity adjoints of the image (rocprof counter runs)")
+                    help="only 3 frames (automatic lanes per ray), 3 frames with one lane per pixel and 3 "
+                         "TF/opacity adjoints of the image (rocprof counter runs)")
     return ap.parse_args()
 
 
@@ -223,6 +224,10 @@ def main():
         dsig = torch.empty(n ** 3, dtype=torch.float32, device=dev)
         for _ in range(3):
             ctx.render(cam, out=image)
+        ctx.set_tuning("render.volume_group", 1)  # the one-lane-per-pixel caster for comparison
+        for _ in range(3):
+            ctx.render(cam, out=image)
+        ctx.set_tuning("render.volume_group", 0)
         for _ in range(3):
             ctx.render_adjoint(cam, dimg, dtf, dsig)
         ctx.synchronize()
@@ -332,6 +337,32 @@ def main():
         ctx.set_tuning("lic.cache", 0)
         uncached_lic = timed(ctx.compute_lic)
         ctx.set_tuning("lic.cache", 1)
+        # the LIC-volume caster on the default view and three seeded
+        # rotations: lanes per ray chosen automatically against one lane per
+        # pixel (same images and sample counts)
+        ctx.compute_lic()
+        rng = np.random.default_rng(7)
+        quats = [(0.0, 0.0, 0.0, 1.0)]
+        for _ in range(3):
+            q = rng.normal(size=4)
+            quats.append(tuple(float(x) for x in q / np.linalg.norm(q)))
+        views = []
+        for q in quats:
+            vcam = vv.default_camera(W, W, cen)
+            for k in range(4):
+                vcam.quat[k] = q[k]
+            _, vns = ctx.render(vcam, out=image, count_samples=True)
+            auto_ms = timed(lambda: ctx.render(vcam, out=image))
+            ctx.set_tuning("render.volume_group", 1)
+            one_ms = timed(lambda: ctx.render(vcam, out=image))
+            ctx.set_tuning("render.volume_group", 0)
+            views.append(dict(quat=[round(x, 4) for x in q], ray_samples=int(vns), ms=auto_ms,
+                              one_lane_per_pixel_ms=one_ms))
+        out["frame_views"] = {
+            "what": f"LIC-volume ray cast at {W}^2 (FP32 field): the default view, then three seeded "
+                    "rotations; ms with the automatic lanes per ray, and with one lane per pixel",
+            "views": views,
+        }
         out["pipeline"] = {
             "what": "LIC volume + 1024^2 forward frame + TF/opacity adjoint (ms); adjoint_ms is a stand-alone "
                     "adjoint (its own forward walk), fwd_adjoint_recorded_ms a vv_render_record frame followed by "
