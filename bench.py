@@ -352,15 +352,23 @@ def main():
             for k in range(4):
                 vcam.quat[k] = q[k]
             _, vns = ctx.render(vcam, out=image, count_samples=True)
-            auto_ms = timed(lambda: ctx.render(vcam, out=image))
-            ctx.set_tuning("render.volume_group", 1)
-            one_ms = timed(lambda: ctx.render(vcam, out=image))
+
+            def rec_tf():
+                ctx.render_record(vcam, out=image)
+                ctx.render_adjoint(vcam, dimg, dtf, None)
+            row = dict(quat=[round(x, 4) for x in q], ray_samples=int(vns))
+            for tag, G in (("", 0), ("one_lane_per_pixel_", 1)):
+                ctx.set_tuning("render.volume_group", G)
+                row[f"{tag}ms"] = timed(lambda: ctx.render(vcam, out=image))
+                row[f"{tag}record_ms"] = timed(lambda: ctx.render_record(vcam, out=image))
+                row[f"{tag}record_tf_adjoint_ms"] = timed(rec_tf)
             ctx.set_tuning("render.volume_group", 0)
-            views.append(dict(quat=[round(x, 4) for x in q], ray_samples=int(vns), ms=auto_ms,
-                              one_lane_per_pixel_ms=one_ms))
+            views.append(row)
         out["frame_views"] = {
             "what": f"LIC-volume ray cast at {W}^2 (FP32 field): the default view, then three seeded "
-                    "rotations; ms with the automatic lanes per ray, and with one lane per pixel",
+                    "rotations; frame, recorded frame (vv_render_record) and recorded frame + TF adjoint "
+                    "in ms, with the automatic lanes per ray and with one lane per pixel (the adjoint's "
+                    "backward walk keeps one lane per ray either way)",
             "views": views,
         }
         out["pipeline"] = {

@@ -1,10 +1,12 @@
-"""LIC-volume ray caster with several lanes per ray (tuning key
-render.volume_group): every lanes-per-ray setting must give the one-lane
-caster's image and sample count bit for bit, whatever ends the rays (leaving
-the box, dest.a > 0.95, the sample limit), for every field storage, RGBA LIC
-volumes with gradient shading, an opacity scale and panned views."""
+"""LIC-volume ray caster and the adjoint's forward pass with several lanes
+per ray (tuning key render.volume_group): every lanes-per-ray setting must
+give the one-lane kernels' images, sample counts and gradients bit for bit,
+whatever ends the rays (leaving the box, dest.a > 0.95, the sample limit),
+for every field storage, RGBA LIC volumes with gradient shading, an opacity
+scale, panned views and both adjoint scratch layouts."""
 import numpy as np
 import pytest
+import torch
 
 import vectorvisualization_amd as vv
 
@@ -109,3 +111,89 @@ def test_volume_group_key_is_checked(ctx):
     ctx.set_tuning("render.volume_group", 4)
     assert ctx.get_tuning("render.volume_group") == 4
     ctx.set_tuning("render.volume_group", 0)
+
+
+def _adjoint_scene(ctx, num_iterations=255, opaque=False):
+    lic_setup(ctx, n=20, kind="rankine", L=6, density=0.3)
+    p = ctx.get_lic_params()
+    p.step_size_vol = 1.0 / 256.0
+    p.num_iterations = num_iterations
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    rgba, op = vv.fields.tf_random(3)
+    if opaque:
+        op = np.minimum(op * 8.0, 1.0).astype(np.float32)
+    ctx.set_tf(rgba, op)
+    d, _ = ctx.lic_info()
+    nv = int(np.prod(d))
+    ctx.set_opacity_scale(np.random.default_rng(4).uniform(0.3, 1.2, nv).astype(np.float32))
+    return nv
+
+
+@pytest.mark.parametrize("case", ["default", "opaque", "limit_mid_part", "limit_part_end"])
+def test_adjoint_forward_groups_identical(ctx, case):
+    # the adjoint's forward pass (vv_render_record, and the pass an adjoint
+    # runs itself) with 2 or 4 lanes per ray: image, sample counts, and the
+    # gradients computed from its checkpoints and columns equal the
+    # one-lane pass's bits; sample limits of 25 (ends inside a 16-sample
+    # part) and 16 (ends on a part boundary)
+    iters = {"limit_mid_part": 5, "limit_part_end": 4}.get(case, 255)
+    nv = _adjoint_scene(ctx, num_iterations=iters, opaque=case == "opaque")
+    rgba, op = vv.fields.tf_random(3)
+    if case == "opaque":
+        op = np.minimum(op * 8.0, 1.0).astype(np.float32)
+    cam = _camera(ctx, 48, 40, quat=(0.2, -0.1, 0.05, 0.97))
+    dimg = torch.randn(48 * 40 * 4, dtype=torch.float32, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(11))
+    outs = {}
+    for G in (1, 2, 4):
+        ctx.set_tuning("render.volume_group", G)
+        ctx.set_tf(rgba, op)
+        img, ns = ctx.render_record(cam, count_samples=True)
+        counts = ctx.record_ray_samples(cam).copy()
+        dtf = torch.empty(1024, dtype=torch.float32, device="cuda")
+        dsg = torch.empty(nv, dtype=torch.float32, device="cuda")
+        ctx.render_adjoint(cam, dimg, dtf, dsg)  # reuses the recorded pass
+        # an adjoint running its own forward pass: a changed scene version
+        ctx.set_tf(*vv.fields.tf_random(5))
+        ctx.set_tf(rgba, op)
+        own_tf = torch.empty(1024, dtype=torch.float32, device="cuda")
+        own_sg = torch.empty(nv, dtype=torch.float32, device="cuda")
+        ctx.render_adjoint(cam, dimg, own_tf, own_sg)
+        ctx.synchronize()
+        outs[G] = (img, ns, counts, dtf.cpu().numpy(), dsg.cpu().numpy(), own_tf.cpu().numpy(),
+                   own_sg.cpu().numpy())
+    ctx.set_tuning("render.volume_group", 0)
+    ref = outs[1]
+    assert ref[1] > 0 and np.count_nonzero(ref[3]) > 0
+    if case == "limit_mid_part":
+        assert ref[2].max() == 25
+    if case == "limit_part_end":
+        assert ref[2].max() == 16
+    for G, got in outs.items():
+        for k, (x, y) in enumerate(zip(got, ref)):
+            if isinstance(x, np.ndarray):
+                np.testing.assert_array_equal(x, y, err_msg=f"G={G} output {k}")
+            else:
+                assert x == y, (G, k, x, y)
+
+
+def test_adjoint_forward_groups_identical_compact_layout(ctx):
+    # 320x288: the allocation-free adjoint keeps checkpoints and re-runs
+    # segments (the compact layout) whose checkpoints the grouped pass wrote
+    nv = _adjoint_scene(ctx)
+    cam = _camera(ctx, 320, 288, quat=(0.2, -0.1, 0.05, 0.97))
+    dimg = torch.randn(320 * 288 * 4, dtype=torch.float32, device="cuda",
+                       generator=torch.Generator(device="cuda").manual_seed(12))
+    outs = {}
+    for G in (1, 4):
+        ctx.set_tuning("render.volume_group", G)
+        dtf = torch.empty(1024, dtype=torch.float32, device="cuda")
+        dsg = torch.empty(nv, dtype=torch.float32, device="cuda")
+        ctx.render_adjoint_noalloc(cam, dimg, dtf, dsg)
+        ctx.synchronize()
+        outs[G] = (dtf.cpu().numpy(), dsg.cpu().numpy())
+    ctx.set_tuning("render.volume_group", 0)
+    assert np.count_nonzero(outs[1][0]) > 0 and np.count_nonzero(outs[1][1]) > 0
+    np.testing.assert_array_equal(outs[4][0], outs[1][0])
+    np.testing.assert_array_equal(outs[4][1], outs[1][1])

@@ -968,6 +968,9 @@ static vv_status adjoint_setup(vv_ctx* c, const vv_camera* cam, AdjointLaunch& A
     A.r.max_samples = A.r.max_samples < geo ? A.r.max_samples : geo;
     A.max_ckpt = (int)((A.r.max_samples + kAdjSeg - 1) / kAdjSeg);
     A.use_window = c->adj_window;
+    // the forward pass takes the frame's lanes per ray, at most 4
+    const int G = volume_group_for(c, cam);
+    A.group = G >= 4 ? 4 : G;
     return VV_OK;
 }
 

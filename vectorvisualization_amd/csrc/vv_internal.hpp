@@ -163,6 +163,7 @@ struct AdjointLaunch {
     long long* dsigma_fx;         // sigma voxel accumulators, or null
     int use_window;               // accumulate dsigma in an LDS window per segment
     int full_col;                 // the full layout: a column per segment, no checkpoints
+    int group;                    // lanes per ray of the forward pass: 2 or 4, else one lane per pixel
 };
 
 // Scratch of vv_render_adjoint_noalloc: one static device array per GPU

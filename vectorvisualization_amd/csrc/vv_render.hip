@@ -745,6 +745,181 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward(AdjointLaunch A)
     }
 }
 
+// Phase 1 with GT lanes per ray (2 or 4, the groups are quads), for the
+// views where one lane per pixel leaves the longest rays to set the time:
+// lanes evaluate consecutive samples as k_render_rg does, the group
+// composites them in order and sums gamma in order (every lane of the group
+// runs the same compositing), and each lane then writes its own sample's
+// column entries (D.a before it, the part's start position, the checkpoint)
+// while the part's clamp masks, the same in every lane, are written by the
+// group's first lane.  Columns, checkpoints, counts, bounds and image are
+// k_adjoint_forward's bits.
+template <typename FT, int LC, int GT>
+__global__ void __launch_bounds__(kRThreads) k_adjoint_forward_rg(AdjointLaunch A)
+{
+    static_assert(GT == 2 || GT == 4, "quad groups only");
+    static_assert(kSub % GT == 0, "a round lies in one part");
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    const RenderLaunch& a = A.r;
+    load_tables(a.tf, tf, op);
+    const int tid = threadIdx.x;
+    const int sub = tid & (GT - 1);
+    const int gl0 = (tid & 63) & ~(GT - 1);
+    const int W = a.view.width, H = a.view.height;
+    // k_render_rg's pixel tiles: each wave's 64/GT rays a square-ish WW x WH
+    // tile, the block's four waves 2 x 2 of them (TW = 2 WW, TH = 2 WH)
+    constexpr int rw = 64 / GT;
+    constexpr int WW = GT == 2 ? 8 : 4, WH = rw / WW;
+    constexpr int TW = 2 * WW, TH = 2 * WH;
+    const int tiles_x = (W + TW - 1) / TW;
+    const int r_in = tid / GT;
+    const int w = r_in / rw, lr = r_in & (rw - 1);
+    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
+    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    const long long nrays = (long long)W * H;
+    const long long pix = (long long)j * W + i;
+    float4* ck = reinterpret_cast<float4*>(A.ckpt);  // compact: [c][2][ray] checkpoint (pos, D)
+    Ray r;
+    const bool inside = i < W && j < H;
+    bool alive = inside && ray_setup(a.view, i, j, r);
+    const bool active = alive;
+    long long n = 0;
+    float bound = 0.0f;
+    float dest[4] = {0.f, 0.f, 0.f, 0.f};
+    float base[3] = {0.f, 0.f, 0.f};
+    if (alive)
+        for (int q = 0; q < 3; ++q) base[q] = r.pos[q];
+    float max_am = 0.0f, max_q = 0.0f, max_qr = 0.0f, gamma = 1.0f;
+    unsigned long long mbits = 0;
+    long long bk = 0;
+    while (alive) {
+        float pos[3] = {base[0], base[1], base[2]};
+        for (int t = 0; t < sub; ++t)
+            for (int q = 0; q < 3; ++q) pos[q] = fmaf(r.dir[q], a.step, pos[q]);
+        const long long k = bk + sub;
+        const bool ok = k < a.max_samples && (sub == 0 || !outside(pos, a.view));
+        float src[4] = {0.f, 0.f, 0.f, 0.f}, rsum = 0.f;
+        float am = 0.f, qv = 0.f;
+        if (ok) {
+            Sample s;
+            eval_sample<FT, LC>(a, tf, op, r, pos, s);
+            src[0] = s.rgb[0] * s.a;
+            src[1] = s.rgb[1] * s.a;
+            src[2] = s.rgb[2] * s.a;
+            src[3] = s.a;
+            rsum = fabsf(s.rgb[0]) + fabsf(s.rgb[1]) + fabsf(s.rgb[2]);
+            const float mmax = fmaxf(fabsf(s.m[0]), fmaxf(fabsf(s.m[1]), fabsf(s.m[2])));
+            qv = s.aclamp ? 0.0f : dalpha_draw(a, s.araw) * s.ao * fmaxf(s.sg, s.tfa);
+            am = s.a * mmax;
+        }
+        float nxt[3];
+        for (int q = 0; q < 3; ++q) nxt[q] = fmaf(r.dir[q], a.step, pos[q]);
+        const bool nxt_out = !ok || outside(nxt, a.view);
+        float sv[GT][4], rs[GT];
+        group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
+        {
+            float one[4] = {rsum, 0.f, 0.f, 0.f}, got[GT][4];
+            group_gather<GT>(one, got, gl0, std::make_integer_sequence<int, GT>{});
+            for (int g = 0; g < GT; ++g) rs[g] = got[g][0];
+        }
+        const unsigned okg = (unsigned)(__ballot((int)ok) >> gl0) & ((1u << GT) - 1u);
+        // compositing in sample order; this lane keeps D before its own sample
+        float mine_d[4] = {0.f, 0.f, 0.f, 0.f};
+        int ncomp = 0;
+#pragma unroll
+        for (int g = 0; g < GT; ++g) {
+            if (!((okg >> g) & 1u)) {
+                alive = false;
+                break;
+            }
+            if (g == sub)
+                for (int q = 0; q < 4; ++q) mine_d[q] = dest[q];
+            const float t = 1.0f - dest[3];
+            int mask = 0;
+            for (int q = 0; q < 4; ++q) {
+                const float u = fmaf(t, sv[g][q], dest[q]);
+                if (u < 0.0f || u > 1.0f) mask |= 1 << q;
+                dest[q] = clamp01(u);
+            }
+            const int kk = (int)((bk + g) % kSeg);
+            mbits |= (unsigned long long)mask << (4 * (kk % kSub));
+            gamma = fmaf(sv[g][3], rs[g], gamma);
+            ++ncomp;
+            if (dest[3] > 0.95f) {
+                alive = false;
+                break;
+            }
+        }
+        // this lane's sample, when composited: its column entries
+        if (sub < ncomp) {
+            const int kk = (int)(k % kSeg);
+            const long long c = k / kSeg;
+            const AdjColumn col = adj_column(A, nrays, A.full_col ? c : 0);
+            if (kk == 0 && !A.full_col) {
+                ck[(2 * c) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+                ck[(2 * c + 1) * nrays + pix] = make_float4(mine_d[0], mine_d[1], mine_d[2], mine_d[3]);
+            }
+            if (kk % kSub == 0)
+                col.pos[(long long)(kk / kSub) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+            col.da[(long long)kk * nrays + pix] = mine_d[3];
+            max_am = fmaxf(max_am, am);
+            max_q = fmaxf(max_q, qv);
+            max_qr = fmaxf(max_qr, qv * rsum);
+        }
+        n += ncomp;
+        // the part's masks once its last sample is composited (the loop's
+        // tail writes an unfinished last part)
+        const int kl = (int)((bk + GT - 1) % kSeg);
+        if (ncomp == GT && kl % kSub == kSub - 1) {
+            if (sub == 0) {
+                const AdjColumn col = adj_column(A, nrays, A.full_col ? (bk + GT - 1) / kSeg : 0);
+                col.mask[(long long)(kl / kSub) * nrays + pix] = mbits;
+            }
+            mbits = 0;
+        }
+        if (alive) {
+            for (int q = 0; q < 3; ++q) base[q] = group_bcast<GT, GT - 1>(nxt[q]);
+            if ((__ballot((int)nxt_out) >> (gl0 + GT - 1)) & 1ull) alive = false;
+        }
+        bk += GT;
+    }
+    if (active) {
+        // the masks of a last part the loop left unfinished
+        const int kl = (int)((n - 1) % kSeg);
+        if (kl % kSub != kSub - 1 && sub == 0) {
+            const AdjColumn col = adj_column(A, nrays, A.full_col ? (n - 1) / kSeg : 0);
+            col.mask[(long long)(kl / kSub) * nrays + pix] = mbits;
+        }
+        // the bound terms are maxima over the ray's samples: reduce over the group
+        for (int o = 1; o < GT; o <<= 1) {
+            max_am = fmaxf(max_am, __shfl_xor(max_am, o, 64));
+            max_q = fmaxf(max_q, __shfl_xor(max_q, o, 64));
+            max_qr = fmaxf(max_qr, __shfl_xor(max_qr, o, 64));
+        }
+        bound = fmaxf(max_am, fmaf(gamma, max_q, max_qr));
+        if (!isfinite(bound)) bound = __uint_as_float(0x7fc00000u);
+    }
+    if (inside && sub == 0) {
+        A.nsamp[pix] = (int)n;
+        A.ray_bound[pix] = bound;
+        if (a.image) reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
+    }
+    unsigned long long ns = sub == 0 ? (unsigned long long)n : 0ull;
+    for (int o = 32; o > 0; o >>= 1) ns += __shfl_xor(ns, o, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = ns;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int q = 0; q < kRThreads / 64; ++q) t += red[q];
+        if (t) {
+            atomicAdd(&A.hdr[0], t);
+            if (a.counter) atomicAdd(a.counter, t);
+        }
+    }
+}
+
 // The largest contribution bound of the call: each ray's bound per unit of
 // max |dL/dimage| times its pixel's max |dL/dimage| (NaN when that pixel's
 // gradient is not finite on a ray that has samples), into header word 1.
@@ -1554,6 +1729,28 @@ hipError_t noalloc_scratch(int device, void** ptr)
 
 hipError_t launch_adjoint_forward(const AdjointLaunch& A, hipStream_t s)
 {
+    if (A.group == 2 || A.group == 4) {
+        const int G = A.group;
+        const int TW = G == 2 ? 16 : 8, TH = 8;  // the kernel's pixel tile
+        const long long nblocks =
+            (long long)((A.r.view.width + TW - 1) / TW) * ((A.r.view.height + TH - 1) / TH);
+        if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+        dim3 grid((unsigned)nblocks), block(kRThreads);
+        return with_field_type(A.r.field_type, [&](auto tag) {
+            using FT = typename decltype(tag)::type;
+            if (A.r.lc == 4) {
+                if (G == 2)
+                    hipLaunchKernelGGL((k_adjoint_forward_rg<FT, 4, 2>), grid, block, 0, s, A);
+                else
+                    hipLaunchKernelGGL((k_adjoint_forward_rg<FT, 4, 4>), grid, block, 0, s, A);
+            } else {
+                if (G == 2)
+                    hipLaunchKernelGGL((k_adjoint_forward_rg<FT, 1, 2>), grid, block, 0, s, A);
+                else
+                    hipLaunchKernelGGL((k_adjoint_forward_rg<FT, 1, 4>), grid, block, 0, s, A);
+            }
+        });
+    }
     dim3 block(kBX, kBY);
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
     return with_field_type(A.r.field_type, [&](auto tag) {
