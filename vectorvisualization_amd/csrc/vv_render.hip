@@ -1136,6 +1136,212 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
+// Phase 2 with GT lanes per ray (2 or 4) on the full layout, with
+// k_adjoint_forward_rg's pixel tiles.  Within a part the lanes take
+// consecutive samples from the end (lane g: sample top - g) and evaluate
+// them at once; the group then runs the gradient recurrence of
+// k_render_adjoint in its sample order (every lane of the group alike, on
+// the samples' premultiplied colour and clamp masks), each lane keeping the
+// state at its own sample, and each lane splats its own sample.  Integer
+// sums do not depend on the order of the splats, so dTF and dsigma equal
+// k_render_adjoint's bits.
+template <typename FT, int LC, int GT>
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_adjoint_rg(AdjointLaunch A)
+{
+    static_assert(GT == 2 || GT == 4, "quad groups only");
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ long long dtf[1024];
+    __shared__ long long win[kWin];
+    __shared__ int box[kNSub][6];
+    __shared__ int seg_max;
+    int kexp;
+    if (!adj_scale_exp(A.hdr, kexp)) return;  // zero or non-finite bound: nothing to add
+    const RenderLaunch& a = A.r;
+    const int tid = threadIdx.x;
+    if (tid == 0) seg_max = 0;
+    for (int q = tid; q < 1024; q += kRThreads) dtf[q] = 0;
+    load_tables(a.tf, tf, op);
+
+    const int sub = tid & (GT - 1);
+    const int gl0 = (tid & 63) & ~(GT - 1);
+    const int W = a.view.width, H = a.view.height;
+    constexpr int rw = 64 / GT;
+    constexpr int WW = GT == 2 ? 8 : 4, WH = rw / WW;
+    constexpr int TW = 2 * WW, TH = 2 * WH;
+    const int tiles_x = (W + TW - 1) / TW;
+    const int r_in = tid / GT;
+    const int w = r_in / rw, lr = r_in & (rw - 1);
+    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
+    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    const long long nrays = (long long)W * H;
+    const long long pix = (long long)j * W + i;
+    const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
+
+    Ray r;
+    const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
+    const long long n = active ? A.nsamp[pix] : 0;
+    const int nseg = (int)((n + kSeg - 1) / kSeg);
+    if (nseg && sub == 0) atomicMax(&seg_max, nseg);
+    __syncthreads();
+
+    float G[4] = {0.f, 0.f, 0.f, 0.f};
+    if (active) {
+        const float4 gi = reinterpret_cast<const float4*>(A.dimage)[pix];
+        G[0] = gi.x;
+        G[1] = gi.y;
+        G[2] = gi.z;
+        G[3] = gi.w;
+    }
+    const int smax = seg_max;
+    for (int c = smax - 1; c >= 0; --c) {
+        const bool mine = active && c < nseg;
+        const AdjColumn col = adj_column(A, nrays, c);
+        const long long k0 = (long long)c * kSeg;
+        const long long k1 = min(n, k0 + kSeg);
+        if (tid < kNSub) {
+            box[tid][0] = box[tid][1] = box[tid][2] = 0x7fffffff;
+            box[tid][3] = box[tid][4] = box[tid][5] = -1;
+        }
+        __syncthreads();
+        // the voxel box of each part's sigma splats (lanes take parts in turn)
+        if (mine && A.dsigma_fx) {
+            for (int m = sub; m < kNSub; m += GT) {
+                const long long ka = k0 + (long long)m * kSub;
+                const long long kb = min(k1, ka + kSub);
+                if (ka >= kb) continue;
+                const float4 ps = col.pos[(long long)m * nrays + pix];
+                float pe[3] = {ps.x, ps.y, ps.z};
+                for (long long t = ka + 1; t < kb; ++t)
+                    for (int q = 0; q < 3; ++q) pe[q] = fmaf(r.dir[q], a.step, pe[q]);
+                const float pf[3] = {ps.x, ps.y, ps.z};
+                for (int q = 0; q < 3; ++q) {
+                    float f;
+                    const int c0 = tc_clamp(pf[q], n3[q], f) - 1;
+                    const int c1 = tc_clamp(pe[q], n3[q], f) - 1;
+                    atomicMin(&box[m][q], max(min(c0, c1), 0));
+                    atomicMax(&box[m][3 + q], min(max(c0, c1) + 1, n3[q] - 1));
+                }
+            }
+        }
+        __syncthreads();
+        for (int m = kNSub - 1; m >= 0; --m) {
+            const int bx0 = box[m][0], by0 = box[m][1], bz0 = box[m][2];
+            const int wx = box[m][3] - bx0 + 1, wy = box[m][4] - by0 + 1, wz = box[m][5] - bz0 + 1;
+            const long long wvol = (box[m][3] >= 0) ? (long long)wx * wy * wz : 0;
+            const bool use_win = A.use_window && A.dsigma_fx && wvol > 0 && wvol <= kWin;
+            if (use_win)
+                for (int q = tid; q < (int)wvol; q += kRThreads) win[q] = 0;
+            __syncthreads();
+            const long long ka = k0 + (long long)m * kSub;
+            const long long kb = min(k1, ka + kSub);
+            if (mine && ka < kb) {
+                const float4 ps = col.pos[(long long)m * nrays + pix];
+                const unsigned long long mbits = col.mask[(long long)m * nrays + pix];
+                for (long long top = kb - 1; top >= ka; top -= GT) {
+                    const long long k = top - sub;
+                    const bool valid = k >= ka;
+                    int mask = 0;
+                    float dka = 0.0f;
+                    float pk[3] = {ps.x, ps.y, ps.z};
+                    float src[4] = {0.f, 0.f, 0.f, 0.f};
+                    Sample s;
+                    if (valid) {
+                        mask = (int)((mbits >> (4 * (int)(k - ka))) & 15ull);
+                        dka = col.da[(k - k0) * nrays + pix];
+                        // the sample position: the forward fma chain from the part's start
+                        for (long long t = ka; t < k; ++t)
+                            for (int q = 0; q < 3; ++q) pk[q] = fmaf(r.dir[q], a.step, pk[q]);
+                        eval_sample<FT, LC>(a, tf, op, r, pk, s);
+                        const float al = s.a;
+                        src[0] = s.rgb[0] * al;
+                        src[1] = s.rgb[1] * al;
+                        src[2] = s.rgb[2] * al;
+                        src[3] = al;
+                    }
+                    // the recurrence over the round's samples in k_render_adjoint's order
+                    float sv[GT][4], mv[GT][4];
+                    group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
+                    {
+                        const float mk[4] = {__int_as_float(mask), 0.f, 0.f, 0.f};
+                        group_gather<GT>(mk, mv, gl0, std::make_integer_sequence<int, GT>{});
+                    }
+                    float Gk[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int g = 0; g < GT; ++g) {
+                        if (top - g < ka) break;
+                        const int mg = __float_as_int(mv[g][0]);
+                        for (int q = 0; q < 4; ++q)
+                            if (mg & (1 << q)) G[q] = 0.0f;
+                        if (g == sub)
+                            for (int q = 0; q < 4; ++q) Gk[q] = G[q];
+                        const float gs = G[0] * sv[g][0] + G[1] * sv[g][1] + G[2] * sv[g][2] + G[3] * sv[g][3];
+                        G[3] = G[3] - gs;
+                    }
+                    if (valid) {
+                        const float t = 1.0f - dka;
+                        const float al = s.a;
+                        float dsrc[4];
+                        for (int q = 0; q < 4; ++q) dsrc[q] = t * Gk[q];
+                        float da = dsrc[3];
+                        float drgb[3];
+                        for (int q = 0; q < 3; ++q) {
+                            drgb[q] = dsrc[q] * al;
+                            da = da + dsrc[q] * s.rgb[q];
+                        }
+                        const float draw = s.aclamp ? 0.0f : da * dalpha_draw(a, s.araw);
+                        for (int q = 0; q < 3; ++q) splat_tf(dtf, q, s.tfu, drgb[q] * s.m[q], kexp);
+                        splat_tf(dtf, 3, s.tfu, draw * s.ao * s.sg, kexp);
+                        if (A.dsigma_fx) {
+                            const float dsg = draw * s.ao * s.tfa;
+                            float fx, fy, fz;
+                            const int px = tc_clamp(pk[0], n3[0], fx) - 1;
+                            const int py = tc_clamp(pk[1], n3[1], fy) - 1;
+                            const int pz = tc_clamp(pk[2], n3[2], fz) - 1;
+                            const int xs[2] = {max(px, 0), min(px + 1, n3[0] - 1)};
+                            const int ys[2] = {max(py, 0), min(py + 1, n3[1] - 1)};
+                            const int zs[2] = {max(pz, 0), min(pz + 1, n3[2] - 1)};
+                            const float wxs[2] = {1.0f - fx, fx}, wys[2] = {1.0f - fy, fy}, wzs[2] = {1.0f - fz, fz};
+                            for (int cz = 0; cz < 2; ++cz)
+                                for (int cy = 0; cy < 2; ++cy)
+                                    for (int cx = 0; cx < 2; ++cx) {
+                                        const long long v = to_fx(wxs[cx] * wys[cy] * wzs[cz] * dsg, kexp);
+                                        if (!v) continue;
+                                        if (use_win) {
+                                            const int o = ((zs[cz] - bz0) * wy + (ys[cy] - by0)) * wx + (xs[cx] - bx0);
+                                            atomicAdd((unsigned long long*)&win[o], (unsigned long long)v);
+                                        } else {
+                                            const long long o = ((long long)zs[cz] * n3[1] + ys[cy]) * n3[0] + xs[cx];
+                                            atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+                                        }
+                                    }
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+            // flush the window: one global integer atomic per touched voxel
+            if (use_win) {
+                for (int q = tid; q < (int)wvol; q += kRThreads) {
+                    const long long v = win[q];
+                    if (v) {
+                        const int ox = q % wx, oy = (q / wx) % wy, oz = q / (wx * wy);
+                        const long long o = ((long long)(bz0 + oz) * n3[1] + (by0 + oy)) * n3[0] + (bx0 + ox);
+                        atomicAdd((unsigned long long*)&A.dsigma_fx[o], (unsigned long long)v);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (int q = tid; q < 1024; q += kRThreads) {
+        const long long v = dtf[q];
+        if (v) atomicAdd((unsigned long long*)&A.dtf_fx[q], (unsigned long long)v);
+    }
+}
+
 // fixed point back to float: one rounding (int64 -> float), then the exact
 // power-of-two scale
 __global__ void k_fixed_to_float(const unsigned long long* hdr, const long long* src, float* dst, long long n)
@@ -1766,6 +1972,28 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
 {
     const long long nrays = (long long)A.r.view.width * A.r.view.height;
     hipLaunchKernelGGL(k_adjoint_bound, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, A);
+    if (A.full_col && (A.group == 2 || A.group == 4)) {
+        const int G = A.group;
+        const int TW = G == 2 ? 16 : 8, TH = 8;  // the kernel's pixel tile
+        const long long nblocks =
+            (long long)((A.r.view.width + TW - 1) / TW) * ((A.r.view.height + TH - 1) / TH);
+        if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
+        dim3 grid((unsigned)nblocks), block(kRThreads);
+        return with_field_type(A.r.field_type, [&](auto tag) {
+            using FT = typename decltype(tag)::type;
+            if (A.r.lc == 4) {
+                if (G == 2)
+                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 4, 2>), grid, block, 0, s, A);
+                else
+                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 4, 4>), grid, block, 0, s, A);
+            } else {
+                if (G == 2)
+                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 1, 2>), grid, block, 0, s, A);
+                else
+                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 1, 4>), grid, block, 0, s, A);
+            }
+        });
+    }
     dim3 block(kBX, kBY);
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
     return with_field_type(A.r.field_type, [&](auto tag) {
