@@ -50,8 +50,8 @@ def parse():
     ap.add_argument("--pmc-probe-fused", action="store_true",
                     help="only 3 fused on-the-fly LIC ray casts of the image (rocprof counter runs)")
     ap.add_argument("--pmc-probe-frame", action="store_true",
-                    help="only 3 frames (automatic lanes per ray), 3 frames with one lane per pixel and 3 "
-                         "TF/opacity adjoints of the image (rocprof counter runs)")
+                    help="only 3 frames and 3 TF/opacity adjoints of the image with the automatic lanes per "
+                         "ray, then the same with one lane per pixel (rocprof counter runs)")
     return ap.parse_args()
 
 
@@ -222,14 +222,14 @@ def main():
         dimg = torch.randn(W * W * 4, dtype=torch.float32, device=dev) * 1e-3
         dtf = torch.empty(1024, dtype=torch.float32, device=dev)
         dsig = torch.empty(n ** 3, dtype=torch.float32, device=dev)
-        for _ in range(3):
-            ctx.render(cam, out=image)
-        ctx.set_tuning("render.volume_group", 1)  # the one-lane-per-pixel caster for comparison
-        for _ in range(3):
-            ctx.render(cam, out=image)
+        # the automatic lanes per ray, then one lane per pixel for comparison
+        for G in (0, 1):
+            ctx.set_tuning("render.volume_group", G)
+            for _ in range(3):
+                ctx.render(cam, out=image)
+            for _ in range(3):
+                ctx.render_adjoint(cam, dimg, dtf, dsig)
         ctx.set_tuning("render.volume_group", 0)
-        for _ in range(3):
-            ctx.render_adjoint(cam, dimg, dtf, dsig)
         ctx.synchronize()
         ctx.close()
         print("[bench] frame pmc probe done", file=sys.stderr, flush=True)
