@@ -379,6 +379,50 @@ __device__ __forceinline__ void group_gather(const float src[4], float (*sv)[4],
     }
 }
 
+// Pixel tiles of the G-lane ray casters (k_render_rg and the adjoint's
+// grouped walks): a block's 256/G rays form a TW x TH tile of 2 x 2 wave
+// tiles, each wave's 64/G rays a WW x WH tile (WW = WH or 2 WH); blocks take
+// the tiles of the image row by row.
+struct RgTile {
+    int WW, WH, TW, TH;
+};
+
+__host__ __device__ constexpr RgTile rg_tile(int G)
+{
+    int lw = 0;  // log2 of the rays per wave
+    while ((2 << lw) <= 64 / G) ++lw;
+    const int ww = 1 << ((lw + 1) >> 1);
+    return RgTile{ww, (64 / G) / ww, 2 * ww, 2 * ((64 / G) / ww)};
+}
+
+constexpr bool rg_tiles_fit()
+{
+    for (int G = 1; G <= 64; G *= 2) {
+        const RgTile t = rg_tile(G);
+        if (t.WW * t.WH * G != 64 || t.TW * t.TH * G != kRThreads || t.WW < t.WH || t.WW > 2 * t.WH) return false;
+    }
+    return true;
+}
+static_assert(rg_tiles_fit(), "a wave tile holds 64/G rays and a block tile 256/G");
+
+// blocks of a width x height image
+inline long long rg_blocks(int G, int width, int height)
+{
+    const RgTile t = rg_tile(G);
+    return (long long)((width + t.TW - 1) / t.TW) * ((height + t.TH - 1) / t.TH);
+}
+
+// pixel (i, j) of thread tid of block b
+__device__ __forceinline__ void rg_pixel(int G, int width, int b, int tid, int& i, int& j)
+{
+    const RgTile t = rg_tile(G);
+    const int tiles_x = (width + t.TW - 1) / t.TW;
+    const int rw = 64 / G, r_in = tid / G;
+    const int w = r_in / rw, lr = r_in % rw;
+    i = (b % tiles_x) * t.TW + (w & 1) * t.WW + lr % t.WW;
+    j = (b / tiles_x) * t.TH + (w >> 1) * t.WH + lr / t.WW;
+}
+
 // The LIC-volume ray caster with G lanes per ray (G a power of two, 2..64),
 // for views whose longest rays would otherwise set the frame time: in each
 // round lane g of a ray's group evaluates sample base + g, stepping from the
@@ -406,19 +450,8 @@ __global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_r
     const int sub = tid & (G - 1);
     const int gl0 = (tid & 63) & ~(G - 1);  // first lane of the group in the wave
     const int W = a.view.width, H = a.view.height;
-    const int nr = kRThreads / G;
-    const int lg = 31 - __builtin_clz(nr);
-    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
-    const int tiles_x = (W + TW - 1) / TW;
-    // each wave's 64/G rays form a square-ish WW x WH tile, the block's four
-    // waves a 2 x 2 arrangement of them (TW = 2 WW, TH = 2 WH)
-    const int r_in = tid / G;
-    const int rw = 64 / G;
-    const int lw = 31 - __builtin_clz(rw);
-    const int WW = 1 << ((lw + 1) >> 1), WH = rw / WW;
-    const int w = r_in / rw, lr = r_in & (rw - 1);
-    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
-    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    int i, j;
+    rg_pixel(G, W, (int)blockIdx.x, tid, i, j);
     const bool has_pix = i < W && j < H;
     float dest[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned long long ns = 0;
@@ -768,16 +801,8 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward_rg(AdjointLaunch 
     const int sub = tid & (GT - 1);
     const int gl0 = (tid & 63) & ~(GT - 1);
     const int W = a.view.width, H = a.view.height;
-    // k_render_rg's pixel tiles: each wave's 64/GT rays a square-ish WW x WH
-    // tile, the block's four waves 2 x 2 of them (TW = 2 WW, TH = 2 WH)
-    constexpr int rw = 64 / GT;
-    constexpr int WW = GT == 2 ? 8 : 4, WH = rw / WW;
-    constexpr int TW = 2 * WW, TH = 2 * WH;
-    const int tiles_x = (W + TW - 1) / TW;
-    const int r_in = tid / GT;
-    const int w = r_in / rw, lr = r_in & (rw - 1);
-    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
-    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    int i, j;
+    rg_pixel(GT, W, (int)blockIdx.x, tid, i, j);
     const long long nrays = (long long)W * H;
     const long long pix = (long long)j * W + i;
     float4* ck = reinterpret_cast<float4*>(A.ckpt);  // compact: [c][2][ray] checkpoint (pos, D)
@@ -1167,14 +1192,8 @@ k_render_adjoint_rg(AdjointLaunch A)
     const int sub = tid & (GT - 1);
     const int gl0 = (tid & 63) & ~(GT - 1);
     const int W = a.view.width, H = a.view.height;
-    constexpr int rw = 64 / GT;
-    constexpr int WW = GT == 2 ? 8 : 4, WH = rw / WW;
-    constexpr int TW = 2 * WW, TH = 2 * WH;
-    const int tiles_x = (W + TW - 1) / TW;
-    const int r_in = tid / GT;
-    const int w = r_in / rw, lr = r_in & (rw - 1);
-    const int i = (int)(blockIdx.x % tiles_x) * TW + (w & 1) * WW + (lr & (WW - 1));
-    const int j = (int)(blockIdx.x / tiles_x) * TH + (w >> 1) * WH + lr / WW;
+    int i, j;
+    rg_pixel(GT, W, (int)blockIdx.x, tid, i, j);
     const long long nrays = (long long)W * H;
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
@@ -1760,11 +1779,7 @@ hipError_t launch_render(const RenderLaunch& R, hipStream_t s)
 hipError_t launch_render_rg(const RenderLaunch& R, int G, hipStream_t s)
 {
     if (G < 2 || G > 64 || (G & (G - 1))) return hipErrorInvalidValue;
-    const int nr = kRThreads / G;
-    int lg = 0;
-    while ((1 << (lg + 1)) <= nr) ++lg;
-    const int TW = 1 << ((lg + 1) >> 1), TH = nr / TW;
-    const long long nblocks = (long long)((R.view.width + TW - 1) / TW) * ((R.view.height + TH - 1) / TH);
+    const long long nblocks = rg_blocks(G, R.view.width, R.view.height);
     if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
     dim3 grid((unsigned)nblocks), block(kRThreads);
 #define VV_RG_LAUNCH(LC)                                                                    \
@@ -1937,9 +1952,7 @@ hipError_t launch_adjoint_forward(const AdjointLaunch& A, hipStream_t s)
 {
     if (A.group == 2 || A.group == 4) {
         const int G = A.group;
-        const int TW = G == 2 ? 16 : 8, TH = 8;  // the kernel's pixel tile
-        const long long nblocks =
-            (long long)((A.r.view.width + TW - 1) / TW) * ((A.r.view.height + TH - 1) / TH);
+        const long long nblocks = rg_blocks(G, A.r.view.width, A.r.view.height);
         if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
         dim3 grid((unsigned)nblocks), block(kRThreads);
         return with_field_type(A.r.field_type, [&](auto tag) {
@@ -1974,9 +1987,7 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
     hipLaunchKernelGGL(k_adjoint_bound, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, A);
     if (A.full_col && (A.group == 2 || A.group == 4)) {
         const int G = A.group;
-        const int TW = G == 2 ? 16 : 8, TH = 8;  // the kernel's pixel tile
-        const long long nblocks =
-            (long long)((A.r.view.width + TW - 1) / TW) * ((A.r.view.height + TH - 1) / TH);
+        const long long nblocks = rg_blocks(G, A.r.view.width, A.r.view.height);
         if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
         dim3 grid((unsigned)nblocks), block(kRThreads);
         return with_field_type(A.r.field_type, [&](auto tag) {
