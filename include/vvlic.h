@@ -366,10 +366,14 @@ vv_status vv_batch_add_view(vv_batch* b, int timestep, const vv_camera* cam, flo
 vv_status vv_batch_set_field(vv_batch* b, int timestep, const float* v0, const float* v1,
                              float lerp_t, int on_device);
 vv_status vv_batch_info(const vv_batch* b, int* ntimesteps, int* nviews);
-/* times LIC tile shapes x brick (super-tile) edges per timestep size and the
- * render tile on the first view; picks the fastest (cached per size) */
+/* times LIC tile shapes x brick (super-tile) edges per timestep size, and
+ * the ray caster on the first view (one lane per pixel over five pixel
+ * tiles, or 2 or 4 lanes per ray); picks the fastest (LIC choices cached per
+ * size).  Before it runs, a batch renders with 4 lanes per ray. */
 vv_status vv_batch_autotune(vv_batch* b);
-/* tile x,y,z, brick tiles x,y,z of a timestep, then render tile x,y */
+/* tile x,y,z, brick tiles x,y,z of a timestep, then the ray caster's pixel
+ * tile x,y: 16 x 8 with 2 lanes per ray, 8 x 8 with 4, any other shape with
+ * one lane per pixel */
 vv_status vv_batch_get_tuning(const vv_batch* b, int timestep, int out8[8]);
 vv_status vv_batch_run(vv_batch* b);
 /* per view: milliseconds from the start of the last run to the end of that

@@ -80,7 +80,8 @@ struct vv_batch {
     // descriptors as last uploaded to jobs / first / rviews / vfirst
     std::vector<unsigned char> up_jobs, up_views;
     std::vector<int> up_first, up_vfirst;
-    int render_block[2] = {16, 16};
+    int render_block[2] = {16, 16};  // pixel tile with one lane per pixel
+    int render_group = 4;            // lanes per ray of the ray caster: 1, 2 or 4
     bool ran = false;
 };
 
@@ -477,8 +478,9 @@ vv_status vv_batch_run(vv_batch* b)
                 if (s) return s;
                 b->host_views[k].counter = samples + k;
                 b->slot[v] = (int)k;
-                const int tiles =
-                    ((b->views[v].cam.width + bx - 1) / bx) * ((b->views[v].cam.height + by - 1) / by);
+                const int vw = b->views[v].cam.width, vh = b->views[v].cam.height;
+                const int tiles = b->render_group > 1 ? (int)rg_blocks(b->render_group, vw, vh)
+                                                      : ((vw + bx - 1) / bx) * ((vh + by - 1) / by);
                 b->host_vfirst.push_back(b->host_vfirst.back() + tiles);
                 ++k;
             }
@@ -488,7 +490,7 @@ vv_status vv_batch_run(vv_batch* b)
         b->ngroups = (int)nsteps;
         int cus = 0;
         HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        const int render_grid = cus * render_batch_blocks_per_cu(ftype, 1, bx * by);
+        const int render_grid = cus * render_batch_blocks_per_cu(ftype, 1, bx * by, b->render_group);
         // Descriptors of this run (the graph's kernels read them from device
         // memory).  They are uploaded only when they differ from the last
         // upload, after the previous run has finished reading them; the
@@ -519,7 +521,7 @@ vv_status vv_batch_run(vv_batch* b)
             b->up_vfirst = b->host_vfirst;
             b->up_views.assign(views_bytes, views_bytes + views_size);
         }
-        std::vector<long long> key = {(long long)nsteps, (long long)nv, render_grid, bx, by, ftype, same,
+        std::vector<long long> key = {(long long)nsteps, (long long)nv, render_grid, bx, by, b->render_group, ftype, same,
                                       (long long)(intptr_t)b->jobs.p, (long long)(intptr_t)b->first.p,
                                       (long long)(intptr_t)b->rviews.p, (long long)(intptr_t)b->vfirst.p,
                                       (long long)(intptr_t)b->ctr.p};
@@ -576,6 +578,7 @@ vv_status vv_batch_run(vv_batch* b)
                 RB.begun = begun + b->gview[g];
                 RB.block[0] = bx;
                 RB.block[1] = by;
+                RB.group = b->render_group;
                 RB.grid = render_grid;
                 RB.field_type = ftype;
                 RB.lc = 1;
@@ -743,11 +746,12 @@ vv_status vv_batch_autotune(vv_batch* b)
             std::lock_guard<std::mutex> lk(g_tune_mu);
             g_lic_tune[key] = pick;
         }
-        // render tile, on the first view
+        // the ray caster, on the first view: one lane per pixel over five pixel
+        // tiles, or 2 or 4 lanes per ray
         if (!b->views.empty()) {
-            static const int blocks[][2] = {{16, 16}, {32, 8}, {8, 8}, {64, 4}, {16, 4}, {32, 4}};
+            static const int blocks[][2] = {{16, 16}, {32, 8}, {64, 4}, {16, 4}, {32, 4}};
             float best = 1e30f;
-            int pick[2] = {16, 16};
+            int pick[3] = {16, 16, 4};
             for (const auto& bl : blocks) {
                 b->render_block[0] = bl[0];
                 b->render_block[1] = bl[1];
@@ -761,10 +765,24 @@ vv_status vv_batch_autotune(vv_batch* b)
                     best = ms;
                     pick[0] = bl[0];
                     pick[1] = bl[1];
+                    pick[2] = 1;
+                }
+            }
+            for (int G : {2, 4}) {
+                RenderLaunch R;
+                s = build_view_launch(b, b->views[0], R);
+                if (s) return s;
+                hipError_t e = hipSuccess;
+                const float ms = time_ms(c->stream, [&] { return launch_render_rg(R, G, c->stream); }, 3, &e);
+                if (e) return fail(c, VV_EDEVICE, "autotune render: %s", hipGetErrorString(e));
+                if (ms < best) {
+                    best = ms;
+                    pick[2] = G;
                 }
             }
             b->render_block[0] = pick[0];
             b->render_block[1] = pick[1];
+            b->render_group = pick[2];
         }
         release_graph(b);
         return VV_OK;
@@ -780,8 +798,9 @@ vv_status vv_batch_get_tuning(const vv_batch* b, int timestep, int out8[8])
             out8[a] = st->tile[a];
             out8[3 + a] = st->super[a];
         }
-        out8[6] = b->render_block[0];
-        out8[7] = b->render_block[1];
+        const RgTile t = rg_tile(b->render_group);
+        out8[6] = b->render_group > 1 ? t.TW : b->render_block[0];
+        out8[7] = b->render_group > 1 ? t.TH : b->render_block[1];
         return VV_OK;
     });
 }

@@ -106,6 +106,28 @@ struct RenderLaunch {
     int block[2];                 // pixel tile (threads), 0: 16x16
 };
 
+// Pixel tiles of the G-lane ray casters (G a power of two, 1..64): a block's
+// 256/G rays form a TW x TH tile of 2 x 2 wave tiles, each wave's 64/G rays
+// a WW x WH tile (WW = WH or 2 WH).
+struct RgTile {
+    int WW, WH, TW, TH;
+};
+
+__host__ __device__ constexpr RgTile rg_tile(int G)
+{
+    int lw = 0;  // log2 of the rays per wave
+    while ((2 << lw) <= 64 / G) ++lw;
+    const int ww = 1 << ((lw + 1) >> 1);
+    return RgTile{ww, (64 / G) / ww, 2 * ww, 2 * ((64 / G) / ww)};
+}
+
+// tiles (blocks) of a width x height image
+inline long long rg_blocks(int G, int width, int height)
+{
+    const RgTile t = rg_tile(G);
+    return (long long)((width + t.TW - 1) / t.TW) * ((height + t.TH - 1) / t.TH);
+}
+
 struct RenderBatchLaunch {
     const RenderLaunch* views;    // device array, one per view
     const int* first;             // device: first work item of each view
@@ -113,7 +135,8 @@ struct RenderBatchLaunch {
     unsigned* counter;            // device work counter, zero before launch
     unsigned long long* done;     // device: completion stamp per view
     unsigned long long* begun;    // device: start stamp per view (all ones before launch)
-    int block[2];
+    int block[2];                 // pixel tile with one lane per pixel (group 1)
+    int group;                    // lanes per ray: 1, or 2 / 4 over rg_tile(group) tiles of 256 threads
     int grid;                     // persistent blocks
     int field_type, lc;           // shared by all views
 };
@@ -237,7 +260,7 @@ hipError_t launch_stamp(unsigned long long* t0, hipStream_t s);
 size_t lic_args_size();
 hipError_t lic_args_fill(const LicLaunch& L, void* host_args, int* ntiles);
 hipError_t launch_lic_batch(const LicBatchLaunch& B, hipStream_t s);
-int render_batch_blocks_per_cu(int field_type, int lc, int threads);
+int render_batch_blocks_per_cu(int field_type, int lc, int threads, int group);
 // view-aligned slicing (ViewSlicing): planes v.(p - center) = d_i with
 // d_i = -D/2 + (i + 1/2) D/n, i = 0..n-1, front to back
 struct SliceSetup {

@@ -379,22 +379,8 @@ __device__ __forceinline__ void group_gather(const float src[4], float (*sv)[4],
     }
 }
 
-// Pixel tiles of the G-lane ray casters (k_render_rg and the adjoint's
-// grouped walks): a block's 256/G rays form a TW x TH tile of 2 x 2 wave
-// tiles, each wave's 64/G rays a WW x WH tile (WW = WH or 2 WH); blocks take
-// the tiles of the image row by row.
-struct RgTile {
-    int WW, WH, TW, TH;
-};
-
-__host__ __device__ constexpr RgTile rg_tile(int G)
-{
-    int lw = 0;  // log2 of the rays per wave
-    while ((2 << lw) <= 64 / G) ++lw;
-    const int ww = 1 << ((lw + 1) >> 1);
-    return RgTile{ww, (64 / G) / ww, 2 * ww, 2 * ((64 / G) / ww)};
-}
-
+// Pixel tiles of the G-lane ray casters (rg_tile): blocks take the tiles of
+// the image row by row.
 constexpr bool rg_tiles_fit()
 {
     for (int G = 1; G <= 64; G *= 2) {
@@ -404,13 +390,6 @@ constexpr bool rg_tiles_fit()
     return true;
 }
 static_assert(rg_tiles_fit(), "a wave tile holds 64/G rays and a block tile 256/G");
-
-// blocks of a width x height image
-inline long long rg_blocks(int G, int width, int height)
-{
-    const RgTile t = rg_tile(G);
-    return (long long)((width + t.TW - 1) / t.TW) * ((height + t.TH - 1) / t.TH);
-}
 
 // pixel (i, j) of thread tid of block b
 __device__ __forceinline__ void rg_pixel(int G, int width, int b, int tid, int& i, int& j)
@@ -438,20 +417,17 @@ __device__ __forceinline__ void rg_pixel(int G, int width, int b, int tid, int& 
 // through shuffles issued back to back and one ballot before the ordered
 // compositing, instead of a shuffle round trip per sample (GT = 0: G at run
 // time, for 16 lanes and more).
+// One ray of a G-lane group, the group of thread tid (tid's lane in its
+// wave fixes the group): the rounds, the ordered compositing and the image
+// store of k_render_rg.  Returns the ray's sample count in the group's first
+// lane and 0 in the others.  Every lane of the group must call it together.
 template <typename FT, int LC, int GT>
-__global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_rt)
+__device__ __forceinline__ unsigned long long render_ray_rg(const RenderLaunch& a, const float* tf, const float* op,
+                                                            int G, int tid, int i, int j)
 {
-    const int G = GT > 0 ? GT : G_rt;
-    __shared__ float tf[1024];
-    __shared__ float op[256];
-    __shared__ unsigned long long red[kRThreads / 64];
-    load_tables(a.tf, tf, op);
-    const int tid = threadIdx.x;
     const int sub = tid & (G - 1);
     const int gl0 = (tid & 63) & ~(G - 1);  // first lane of the group in the wave
     const int W = a.view.width, H = a.view.height;
-    int i, j;
-    rg_pixel(G, W, (int)blockIdx.x, tid, i, j);
     const bool has_pix = i < W && j < H;
     float dest[4] = {0.f, 0.f, 0.f, 0.f};
     unsigned long long ns = 0;
@@ -534,7 +510,22 @@ __global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_r
         const long long pix = (long long)j * W + i;
         reinterpret_cast<float4*>(a.image)[pix] = make_float4(dest[0], dest[1], dest[2], dest[3]);
     }
-    if (a.counter) add_samples(a.counter, sub == 0 ? ns : 0ull, red);
+    return sub == 0 ? ns : 0ull;
+}
+
+template <typename FT, int LC, int GT>
+__global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_rt)
+{
+    const int G = GT > 0 ? GT : G_rt;
+    __shared__ float tf[1024];
+    __shared__ float op[256];
+    __shared__ unsigned long long red[kRThreads / 64];
+    load_tables(a.tf, tf, op);
+    const int tid = threadIdx.x;
+    int i, j;
+    rg_pixel(G, a.view.width, (int)blockIdx.x, tid, i, j);
+    const unsigned long long ns = render_ray_rg<FT, LC, GT>(a, tf, op, G, tid, i, j);
+    if (a.counter) add_samples(a.counter, ns, red);
 }
 
 // Persistent ray caster over the pixel tiles of many views.  Blocks take
@@ -542,7 +533,9 @@ __global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_r
 // different amounts); the start of each view's first tile and the end of its
 // last tile are recorded as the smallest and largest wall-clock stamps its
 // tiles write.
-template <typename FT, int LC>
+// GT > 0: GT lanes per ray (2 or 4) in 256-thread blocks over rg_tile(GT)
+// pixel tiles; GT = 0: one lane per pixel over blockDim pixel tiles.
+template <typename FT, int LC, int GT>
 __global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* views, const int* first,
                                                            int nviews, int total, unsigned* counter,
                                                            unsigned long long* done, unsigned long long* begun)
@@ -568,11 +561,17 @@ __global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* 
             cur = v;
         }
         const int t = item - first[v];
-        const int tx = (a.view.width + blockDim.x - 1) / blockDim.x;
-        const int i = (t % tx) * blockDim.x + threadIdx.x;
-        const int j = (t / tx) * blockDim.y + threadIdx.y;
         unsigned long long ns = 0;
-        if (i < a.view.width && j < a.view.height) ns = render_pixel<FT, LC>(a, tf, op, i, j);
+        if constexpr (GT > 0) {
+            int i, j;
+            rg_pixel(GT, a.view.width, t, tid, i, j);
+            ns = render_ray_rg<FT, LC, GT>(a, tf, op, GT, tid, i, j);
+        } else {
+            const int tx = (a.view.width + blockDim.x - 1) / blockDim.x;
+            const int i = (t % tx) * blockDim.x + threadIdx.x;
+            const int j = (t / tx) * blockDim.y + threadIdx.y;
+            if (i < a.view.width && j < a.view.height) ns = render_pixel<FT, LC>(a, tf, op, i, j);
+        }
         if (a.counter) add_samples(a.counter, ns, red);
         __syncthreads();
         if (tid == 0) atomicMax(&done[v], (unsigned long long)wall_clock64());
@@ -2019,18 +2018,26 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
 hipError_t launch_render_batch(const RenderBatchLaunch& B, hipStream_t s)
 {
     const int bx = B.block[0], by = B.block[1];
-    if (bx * by > kRThreads || (bx * by) % 64 || B.nviews < 1) return hipErrorInvalidValue;
-    dim3 block(bx, by);
+    if (B.nviews < 1 || (B.group != 1 && B.group != 2 && B.group != 4)) return hipErrorInvalidValue;
+    if (B.group == 1 && (bx * by > kRThreads || (bx * by) % 64)) return hipErrorInvalidValue;
+    dim3 block = B.group > 1 ? dim3(kRThreads) : dim3(bx, by);
     dim3 grid((unsigned)B.grid);
+#define VV_RB_LAUNCH(LC, GT)                                                                               \
+    hipLaunchKernelGGL((k_render_batch<FT, LC, GT>), grid, block, 0, s, B.views, B.first, B.nviews, B.total, \
+                       B.counter, B.done, B.begun)
     return with_field_type(B.field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
-        if (B.lc == 4)
-            hipLaunchKernelGGL((k_render_batch<FT, 4>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
-                               B.counter, B.done, B.begun);
-        else
-            hipLaunchKernelGGL((k_render_batch<FT, 1>), grid, block, 0, s, B.views, B.first, B.nviews, B.total,
-                               B.counter, B.done, B.begun);
+        if (B.lc == 4) {
+            if (B.group == 4) VV_RB_LAUNCH(4, 4);
+            else if (B.group == 2) VV_RB_LAUNCH(4, 2);
+            else VV_RB_LAUNCH(4, 0);
+        } else {
+            if (B.group == 4) VV_RB_LAUNCH(1, 4);
+            else if (B.group == 2) VV_RB_LAUNCH(1, 2);
+            else VV_RB_LAUNCH(1, 0);
+        }
     });
+#undef VV_RB_LAUNCH
 }
 
 hipError_t launch_stamp(unsigned long long* t0, hipStream_t s)
@@ -2051,14 +2058,19 @@ hipError_t launch_fixed_to_float(const unsigned long long* hdr, const long long*
 
 namespace vv {
 
-int render_batch_blocks_per_cu(int field_type, int lc, int threads)
+int render_batch_blocks_per_cu(int field_type, int lc, int threads, int group)
 {
     int n = 0;
     hipError_t e = hipSuccess;
     const hipError_t d = with_field_type(field_type, [&](auto tag) {
         using FT = typename decltype(tag)::type;
-        e = lc == 4 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<FT, 4>, threads, 0)
-                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_render_batch<FT, 1>, threads, 0);
+        auto occ = [&](auto kernel, int nt) { e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, nt, 0); };
+        if (group == 4)
+            lc == 4 ? occ(k_render_batch<FT, 4, 4>, kRThreads) : occ(k_render_batch<FT, 1, 4>, kRThreads);
+        else if (group == 2)
+            lc == 4 ? occ(k_render_batch<FT, 4, 2>, kRThreads) : occ(k_render_batch<FT, 1, 2>, kRThreads);
+        else
+            lc == 4 ? occ(k_render_batch<FT, 4, 0>, threads) : occ(k_render_batch<FT, 1, 0>, threads);
     });
     return (d == hipSuccess && e == hipSuccess && n > 0) ? n : 1;
 }
