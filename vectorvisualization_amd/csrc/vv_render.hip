@@ -412,15 +412,17 @@ __device__ __forceinline__ void rg_pixel(int G, int width, int b, int tid, int& 
 // tests only its own position: compositing stops at the first lane whose
 // sample is not valid, and that lane's test is the one render_pixel makes
 // at the same step.  Images and sample counts equal k_render's for every G.
-// A block's 256/G rays form a TW x TH pixel tile (TW = TH or 2 TH).  GT > 0
-// fixes G at compile time: the group's G samples then reach every lane
-// through shuffles issued back to back and one ballot before the ordered
-// compositing, instead of a shuffle round trip per sample (GT = 0: G at run
-// time, for 16 lanes and more).
-// One ray of a G-lane group, the group of thread tid (tid's lane in its
-// wave fixes the group): the rounds, the ordered compositing and the image
-// store of k_render_rg.  Returns the ray's sample count in the group's first
-// lane and 0 in the others.  Every lane of the group must call it together.
+// A block's 256/G rays form an rg_tile(G) pixel tile.  GT > 0 fixes G at
+// compile time: the group's G samples then reach every lane through
+// shuffles issued back to back (DPP quad permutations for 2 and 4 lanes)
+// and one ballot before the ordered compositing, instead of a shuffle round
+// trip per sample (GT = 0: G at run time, for 16 lanes and more).  The batch
+// ray caster runs the same routine per ray.
+//
+// render_ray_rg: one ray of a G-lane group, the group of thread tid (tid's
+// lane in its wave fixes the group): the rounds, the ordered compositing and
+// the image store.  Returns the ray's sample count in the group's first lane
+// and 0 in the others.  Every lane of the group must call it together.
 template <typename FT, int LC, int GT>
 __device__ __forceinline__ unsigned long long render_ray_rg(const RenderLaunch& a, const float* tf, const float* op,
                                                             int G, int tid, int i, int j)
