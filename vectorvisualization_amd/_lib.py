@@ -833,6 +833,9 @@ class Batch:
         self._ctx._chk(LIB.vv_batch_autotune(self._h))
 
     def tuning(self, timestep: int):
+        """LIC tile x,y,z and brick tiles x,y,z of a timestep, then the ray
+        caster's pixel tile x,y (16 x 8: 2 lanes per ray, 8 x 8: 4 lanes per
+        ray, any other shape: one lane per pixel)."""
         out = (C.c_int * 8)()
         self._ctx._chk(LIB.vv_batch_get_tuning(self._h, int(timestep), out))
         return list(out)
