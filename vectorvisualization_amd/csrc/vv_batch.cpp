@@ -9,10 +9,10 @@
 //   k_lic_batch     every LIC tile of every timestep
 //   k_render_batch  every pixel tile of every view, dynamically scheduled
 //
-// The LIC volumes run one after the other on the context's stream; each
-// timestep's views render on a second stream that the capture forks to
-// after that timestep's LIC volume, so a step's views render beside the next
-// step's LIC volume, and the capture joins it back at the end.
+// Everything runs on the context's stream: each timestep's LIC volume, then
+// that timestep's views.  (Rendering a step's views on a second stream beside
+// the next step's LIC volume paid while the ray caster ran one lane per
+// pixel; with lanes per ray the two kernels only slow each other down.)
 //
 // The reference animates an unsteady field by re-encoding, re-uploading and
 // recomputing the LIC volume frame by frame (VV/3DLIC.cpp:129-172,
@@ -71,12 +71,6 @@ struct vv_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<long long> graph_key;
-    // the render branch of the graph: each time step's views render on a
-    // second stream (joined into the capture by events) while the next time
-    // step's LIC volume is computed
-    hipStream_t side = nullptr;
-    std::vector<hipEvent_t> fork;
-    hipEvent_t join = nullptr;
     // descriptors as last uploaded to jobs / first / rviews / vfirst
     std::vector<unsigned char> up_jobs, up_views;
     std::vector<int> up_first, up_vfirst;
@@ -235,9 +229,6 @@ vv_status vv_batch_destroy(vv_batch* b)
         (void)hipSetDevice(b->ctx->device);
         (void)hipStreamSynchronize(b->ctx->stream);
         release_graph(b);
-        for (hipEvent_t ev : b->fork) (void)hipEventDestroy(ev);
-        if (b->join) (void)hipEventDestroy(b->join);
-        if (b->side) (void)hipStreamDestroy(b->side);
         for (BatchStep* st : b->steps) {
             st->field.release();
             st->noise.release();
@@ -544,15 +535,7 @@ vv_status vv_batch_run(vv_batch* b)
         }
         if (!b->exec || key != b->graph_key) {
             release_graph(b);
-            if (!b->side) HIPCHK(c, hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking));
-            if (!b->join) HIPCHK(c, hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
-            while (b->fork.size() < nsteps) {
-                hipEvent_t ev = nullptr;
-                HIPCHK(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-                b->fork.push_back(ev);
-            }
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            bool side_used = false;
             hipError_t e = launch_stamp(t0, c->stream);
             if (!e) e = launch_fill_u32(ctr, 0u, (long long)(head + 4 * nv), c->stream);
             if (!e) e = launch_fill_u32((uint32_t*)begun, 0xffffffffu, (long long)(2 * nv), c->stream);
@@ -582,16 +565,8 @@ vv_status vv_batch_run(vv_batch* b)
                 RB.grid = render_grid;
                 RB.field_type = ftype;
                 RB.lc = 1;
-                // views of this step after its LIC volume, beside the next step's LIC
-                e = hipEventRecord(b->fork[g], c->stream);
-                if (!e) e = hipStreamWaitEvent(b->side, b->fork[g], 0);
-                if (!e) e = launch_render_batch(RB, b->side);
-                side_used = side_used || !e;
-            }
-            if (side_used) {
-                const hipError_t ej = hipEventRecord(b->join, b->side);
-                const hipError_t ew = ej ? ej : hipStreamWaitEvent(c->stream, b->join, 0);
-                if (!e) e = ew;
+                // the views of this step, after its LIC volume
+                e = launch_render_batch(RB, c->stream);
             }
             hipGraph_t gr = nullptr;
             hipError_t e2 = hipStreamEndCapture(c->stream, &gr);
