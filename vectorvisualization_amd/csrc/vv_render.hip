@@ -536,9 +536,12 @@ __global__ void __launch_bounds__(kRThreads) k_render_rg(RenderLaunch a, int G_r
 // last tile are recorded as the smallest and largest wall-clock stamps its
 // tiles write.
 // GT > 0: GT lanes per ray (2 or 4) in 256-thread blocks over rg_tile(GT)
-// pixel tiles; GT = 0: one lane per pixel over blockDim pixel tiles.
+// pixel tiles; GT = 0: one lane per pixel over blockDim pixel tiles.  Four
+// waves per SIMD (at most 128 VGPRs): the scalar-LIC grouped kernels spill 2-4
+// registers and still run the batch faster than at 143 VGPRs and 3 waves.
 template <typename FT, int LC, int GT>
-__global__ void __launch_bounds__(kRThreads) k_render_batch(const RenderLaunch* views, const int* first,
+__global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_render_batch(const RenderLaunch* views, const int* first,
                                                            int nviews, int total, unsigned* counter,
                                                            unsigned long long* done, unsigned long long* begun)
 {
