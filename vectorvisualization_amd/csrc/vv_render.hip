@@ -849,6 +849,7 @@ __global__ void __launch_bounds__(kRThreads) k_adjoint_forward_rg(AdjointLaunch 
         float sv[GT][4], rs[GT];
         group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
         {
+            // rsum of every sample of the round (only channel 0 is used)
             float one[4] = {rsum, 0.f, 0.f, 0.f}, got[GT][4];
             group_gather<GT>(one, got, gl0, std::make_integer_sequence<int, GT>{});
             for (int g = 0; g < GT; ++g) rs[g] = got[g][0];
@@ -1287,6 +1288,8 @@ k_render_adjoint_rg(AdjointLaunch A)
                     float sv[GT][4], mv[GT][4];
                     group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
                     {
+                        // the 4-bit clamp masks travel as raw bits through the
+                        // float broadcast (a register move, no arithmetic)
                         const float mk[4] = {__int_as_float(mask), 0.f, 0.f, 0.f};
                         group_gather<GT>(mk, mv, gl0, std::make_integer_sequence<int, GT>{});
                     }
