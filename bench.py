@@ -353,22 +353,23 @@ def main():
                 vcam.quat[k] = q[k]
             _, vns = ctx.render(vcam, out=image, count_samples=True)
 
-            def rec_tf():
+            def rec_adj(ds):
                 ctx.render_record(vcam, out=image)
-                ctx.render_adjoint(vcam, dimg, dtf, None)
+                ctx.render_adjoint(vcam, dimg, dtf, ds)
             row = dict(quat=[round(x, 4) for x in q], ray_samples=int(vns))
             for tag, G in (("", 0), ("one_lane_per_pixel_", 1)):
                 ctx.set_tuning("render.volume_group", G)
                 row[f"{tag}ms"] = timed(lambda: ctx.render(vcam, out=image))
                 row[f"{tag}record_ms"] = timed(lambda: ctx.render_record(vcam, out=image))
-                row[f"{tag}record_tf_adjoint_ms"] = timed(rec_tf)
+                row[f"{tag}record_tf_adjoint_ms"] = timed(lambda: rec_adj(None))
+                row[f"{tag}record_adjoint_ms"] = timed(lambda: rec_adj(dsig))
             ctx.set_tuning("render.volume_group", 0)
             views.append(row)
         out["frame_views"] = {
             "what": f"LIC-volume ray cast at {W}^2 (FP32 field): the default view, then three seeded "
-                    "rotations; frame, recorded frame (vv_render_record) and recorded frame + TF adjoint "
-                    "in ms, with the automatic lanes per ray and with one lane per pixel (the adjoint's "
-                    "backward walk keeps one lane per ray either way)",
+                    "rotations; frame, recorded frame (vv_render_record), recorded frame + TF adjoint and "
+                    "recorded frame + TF and opacity-scale adjoint in ms, with the automatic lanes per ray "
+                    "and with one lane per pixel",
             "views": views,
         }
         out["pipeline"] = {
