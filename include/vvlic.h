@@ -503,9 +503,8 @@ vv_status vv_anim_destroy(vv_anim* a);
  * "render.volume_group" (lanes per ray of vv_render, a power of two up to
  * 64; 1 = one lane per pixel; 0 = 4 for views of 2^18 rays or more, about
  * 2^20 lanes in total below; the forward pass of vv_render_record and of the
- * adjoints, and the adjoints' backward walk when their segment columns fit
- * the full scratch layout, take the same number up to 4; gradients do not
- * depend on it), "render.block_x", "render.block_y" (pixel
+ * adjoints and the adjoints' backward walk take the same number up to 4;
+ * gradients do not depend on it), "render.block_x", "render.block_y" (pixel
  * tile of vv_render with one lane per pixel, 64 to 256 threads in
  * multiples of 64), "render.ray_group" (lanes per ray of vv_render_fused /
  * vv_render_slices, a power of two up to 64; 0 = enough lanes for about

@@ -1166,8 +1166,10 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
     }
 }
 
-// Phase 2 with GT lanes per ray (2 or 4) on the full layout, with
-// k_adjoint_forward_rg's pixel tiles.  Within a part the lanes take
+// Phase 2 with GT lanes per ray (2 or 4), with k_adjoint_forward_rg's pixel
+// tiles; COMPACT (the compact layout): the group re-runs each earlier segment
+// from its checkpoint first (a variant of its own, so the full layout's
+// kernel keeps its registers).  Within a part the lanes take
 // consecutive samples from the end (lane g: sample top - g) and evaluate
 // them at once; the group then runs the gradient recurrence of
 // k_render_adjoint in its sample order (every lane of the group alike, on
@@ -1175,7 +1177,7 @@ __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(
 // state at its own sample, and each lane splats its own sample.  Integer
 // sums do not depend on the order of the splats, so dTF and dsigma equal
 // k_render_adjoint's bits.
-template <typename FT, int LC, int GT>
+template <typename FT, int LC, int GT, bool COMPACT>
 __global__ void __launch_bounds__(kRThreads) __attribute__((amdgpu_waves_per_eu(4)))
 k_render_adjoint_rg(AdjointLaunch A)
 {
@@ -1202,6 +1204,7 @@ k_render_adjoint_rg(AdjointLaunch A)
     const long long nrays = (long long)W * H;
     const long long pix = (long long)j * W + i;
     const int* n3 = a.lic.n;  // the opacity scale lives on the LIC grid
+    const float4* ck = reinterpret_cast<const float4*>(A.ckpt);  // compact: [c][2][ray] checkpoint (pos, D)
 
     Ray r;
     const bool active = i < W && j < H && ray_setup(a.view, i, j, r);
@@ -1227,6 +1230,66 @@ k_render_adjoint_rg(AdjointLaunch A)
         if (tid < kNSub) {
             box[tid][0] = box[tid][1] = box[tid][2] = 0x7fffffff;
             box[tid][3] = box[tid][4] = box[tid][5] = -1;
+        }
+        // compact layout: re-run the segment from its checkpoint into the
+        // column (the forward pass left each ray's last segment there), the
+        // group's lanes taking consecutive samples as in k_adjoint_forward_rg;
+        // slot kk is written by lane kk mod GT, the lane that wrote it in
+        // the forward pass, and read after the block barrier below
+        if (COMPACT && mine && c < nseg - 1) {
+            const float4 cp = ck[(2 * c) * nrays + pix];
+            const float4 cd = ck[(2 * c + 1) * nrays + pix];
+            float base[3] = {cp.x, cp.y, cp.z};
+            float d[4] = {cd.x, cd.y, cd.z, cd.w};
+            unsigned long long mbits = 0;
+            for (long long bk = k0; bk < k1; bk += GT) {
+                const long long k = bk + sub;
+                const bool ok = k < k1;
+                float pos[3] = {base[0], base[1], base[2]};
+                for (int t = 0; t < sub; ++t)
+                    for (int q = 0; q < 3; ++q) pos[q] = fmaf(r.dir[q], a.step, pos[q]);
+                float src[4] = {0.f, 0.f, 0.f, 0.f};
+                if (ok) {
+                    Sample s;
+                    eval_sample<FT, LC>(a, tf, op, r, pos, s);
+                    src[0] = s.rgb[0] * s.a;
+                    src[1] = s.rgb[1] * s.a;
+                    src[2] = s.rgb[2] * s.a;
+                    src[3] = s.a;
+                }
+                float sv[GT][4];
+                group_gather<GT>(src, sv, gl0, std::make_integer_sequence<int, GT>{});
+                float mine_da = 0.0f;
+#pragma unroll
+                for (int g = 0; g < GT; ++g) {
+                    if (bk + g >= k1) break;
+                    if (g == sub) mine_da = d[3];
+                    const float t = 1.0f - d[3];
+                    int mask = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        const float u = fmaf(t, sv[g][q], d[q]);
+                        if (u < 0.0f || u > 1.0f) mask |= 1 << q;
+                        d[q] = clamp01(u);
+                    }
+                    mbits |= (unsigned long long)mask << (4 * (int)((bk + g - k0) % kSub));
+                }
+                if (ok) {
+                    const int kk = (int)(k - k0);
+                    if (kk % kSub == 0)
+                        col.pos[(long long)(kk / kSub) * nrays + pix] = make_float4(pos[0], pos[1], pos[2], 0.f);
+                    col.da[(long long)kk * nrays + pix] = mine_da;
+                }
+                // a part's masks once its last sample (or the segment's) is in
+                const long long last = min(bk + GT, k1) - 1;
+                const int kl = (int)(last - k0);
+                if (kl % kSub == kSub - 1 || last == k1 - 1) {
+                    if (sub == 0) col.mask[(long long)(kl / kSub) * nrays + pix] = mbits;
+                    mbits = 0;
+                }
+                float nxt[3];
+                for (int q = 0; q < 3; ++q) nxt[q] = fmaf(r.dir[q], a.step, pos[q]);
+                for (int q = 0; q < 3; ++q) base[q] = group_bcast<GT, GT - 1>(nxt[q]);
+            }
         }
         __syncthreads();
         // the voxel box of each part's sigma splats (lanes take parts in turn)
@@ -1992,25 +2055,33 @@ hipError_t launch_render_adjoint(const AdjointLaunch& A, hipStream_t s)
 {
     const long long nrays = (long long)A.r.view.width * A.r.view.height;
     hipLaunchKernelGGL(k_adjoint_bound, dim3((unsigned)((nrays + 255) / 256)), dim3(256), 0, s, A);
-    if (A.full_col && (A.group == 2 || A.group == 4)) {
+    if (A.group == 2 || A.group == 4) {
         const int G = A.group;
         const long long nblocks = rg_blocks(G, A.r.view.width, A.r.view.height);
         if (nblocks > 0x7fffffffLL) return hipErrorInvalidValue;
         dim3 grid((unsigned)nblocks), block(kRThreads);
+#define VV_ADJ_RG(LC, GT)                                                                        \
+    do {                                                                                         \
+        if (A.full_col)                                                                          \
+            hipLaunchKernelGGL((k_render_adjoint_rg<FT, LC, GT, false>), grid, block, 0, s, A); \
+        else                                                                                     \
+            hipLaunchKernelGGL((k_render_adjoint_rg<FT, LC, GT, true>), grid, block, 0, s, A);  \
+    } while (0)
         return with_field_type(A.r.field_type, [&](auto tag) {
             using FT = typename decltype(tag)::type;
             if (A.r.lc == 4) {
                 if (G == 2)
-                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 4, 2>), grid, block, 0, s, A);
+                    VV_ADJ_RG(4, 2);
                 else
-                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 4, 4>), grid, block, 0, s, A);
+                    VV_ADJ_RG(4, 4);
             } else {
                 if (G == 2)
-                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 1, 2>), grid, block, 0, s, A);
+                    VV_ADJ_RG(1, 2);
                 else
-                    hipLaunchKernelGGL((k_render_adjoint_rg<FT, 1, 4>), grid, block, 0, s, A);
+                    VV_ADJ_RG(1, 4);
             }
         });
+#undef VV_ADJ_RG
     }
     dim3 block(kBX, kBY);
     dim3 grid((A.r.view.width + kBX - 1) / kBX, (A.r.view.height + kBY - 1) / kBY);
