@@ -436,11 +436,15 @@ __device__ __forceinline__ float3 field_from_axes(const CachedArgs& c, FieldCach
         } else {
             const unsigned base = __umul24((unsigned)((int)mz + 1), c.fsz) + __umul24((unsigned)((int)my + 1), c.fsy) +
                                   (unsigned)((int)mx + 1);
-            const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+            // one byte offset for the four rows; each row's start is a
+            // block-uniform pointer (scalar arithmetic)
+            const unsigned boff = base << FieldLoad<FT>::kShift;
+            const size_t off[4] = {0u, (size_t)c.fsy << FieldLoad<FT>::kShift, (size_t)c.fsz << FieldLoad<FT>::kShift,
+                                   (size_t)(c.fsz + c.fsy) << FieldLoad<FT>::kShift};
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 RowLoad<FT, NCH> row;
-                row.load(c.fr, (base + off[k]) << FieldLoad<FT>::kShift);
+                row.load(c.fr + off[k], boff);
                 row.store(fc, k);
             }
         }
@@ -470,15 +474,17 @@ __device__ __forceinline__ float3 field_and_noise(const LicArgs& a, const Cached
     if (fmiss) {
         const unsigned fb = __umul24(clamp_idx(az.fl, c.fnm1[2]), c.fsz) +
                             __umul24(clamp_idx(ay.fl, c.fnm1[1]), c.fsy) + clamp_idx(ax.fl, c.fnm1[0]);
-        const unsigned off[4] = {0u, c.fsy, c.fsz, c.fsz + c.fsy};
+        const unsigned boff = fb << FieldLoad<FT>::kShift;
+        const size_t off[4] = {0u, (size_t)c.fsy << FieldLoad<FT>::kShift, (size_t)c.fsz << FieldLoad<FT>::kShift,
+                               (size_t)(c.fsz + c.fsy) << FieldLoad<FT>::kShift};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if constexpr (kHalf) {
-                hrows[k].load(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift);
+                hrows[k].load(c.fr + off[k], boff);
             } else {
                 // float caches hold the raw x1 texel in dx until both fills are in flight
                 float t0[3], t1[3];
-                FieldLoad<FT>::template pair<3>(c.fr, (fb + off[k]) << FieldLoad<FT>::kShift, t0, t1);
+                FieldLoad<FT>::template pair<3>(c.fr + off[k], boff, t0, t1);
 #pragma unroll
                 for (int ch = 0; ch < 3; ++ch) {
                     fc.v0[ch][k] = t0[ch];
