@@ -3,7 +3,8 @@ per ray (tuning key render.volume_group): every lanes-per-ray setting must
 give the one-lane kernels' images, sample counts and gradients bit for bit,
 whatever ends the rays (leaving the box, dest.a > 0.95, the sample limit),
 for every field storage, RGBA LIC volumes with gradient shading, an opacity
-scale, panned views and both adjoint scratch layouts."""
+scale, panned views, views smaller than one pixel tile and both adjoint
+scratch layouts."""
 import numpy as np
 import pytest
 import torch
@@ -197,3 +198,17 @@ def test_adjoint_forward_groups_identical_compact_layout(ctx):
     assert np.count_nonzero(outs[1][0]) > 0 and np.count_nonzero(outs[1][1]) > 0
     np.testing.assert_array_equal(outs[4][0], outs[1][0])
     np.testing.assert_array_equal(outs[4][1], outs[1][1])
+
+
+def test_volume_groups_tiny_and_thin_views(ctx):
+    # views smaller than one pixel tile: most lanes of a block have no pixel
+    c = lic_setup(ctx, n=12, kind="rankine", L=4, noise_kind=vv.NOISE_UNIFORM, gradient_scale=3.0)
+    p = c["params"]
+    p.tf_index = vv.TF_INDEX_MAG
+    p.step_size_vol = 1.0 / 64.0
+    ctx.set_lic_params(p)
+    ctx.compute_lic()
+    ctx.set_tf(*vv.fields.tf_magnitude(0.1, 0.6))
+    for w, h in ((1, 1), (1, 7), (3, 1), (65, 3)):
+        imgs = _all_groups(ctx, _camera(ctx, w, h, quat=(0.0, 0.0, 0.0, 1.0)))
+        _assert_identical(imgs, f"{w}x{h}")
