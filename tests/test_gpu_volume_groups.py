@@ -212,3 +212,32 @@ def test_volume_groups_tiny_and_thin_views(ctx):
     for w, h in ((1, 1), (1, 7), (3, 1), (65, 3)):
         imgs = _all_groups(ctx, _camera(ctx, w, h, quat=(0.0, 0.0, 0.0, 1.0)))
         _assert_identical(imgs, f"{w}x{h}")
+
+
+def test_adjoint_groups_tiny_and_thin_views(ctx):
+    # the adjoint's passes with 1, 2 and 4 lanes per ray on views smaller
+    # than one pixel tile: image, samples and both gradients equal
+    nv = _adjoint_scene(ctx)
+    rgba, op = vv.fields.tf_random(3)
+    for w, h in ((1, 1), (3, 1), (65, 3)):
+        cam = _camera(ctx, w, h, quat=(0.0, 0.0, 0.0, 1.0))
+        dimg = torch.randn(w * h * 4, dtype=torch.float32, device="cuda",
+                           generator=torch.Generator(device="cuda").manual_seed(w * 7 + h))
+        outs = {}
+        for G in (1, 2, 4):
+            ctx.set_tuning("render.volume_group", G)
+            ctx.set_tf(rgba, op)
+            img, ns = ctx.render_record(cam, count_samples=True)
+            dtf = torch.empty(1024, dtype=torch.float32, device="cuda")
+            dsg = torch.empty(nv, dtype=torch.float32, device="cuda")
+            ctx.render_adjoint(cam, dimg, dtf, dsg)
+            ctx.synchronize()
+            outs[G] = (img, ns, dtf.cpu().numpy(), dsg.cpu().numpy())
+        ctx.set_tuning("render.volume_group", 0)
+        ref = outs[1]
+        assert ref[1] > 0 and np.count_nonzero(ref[2]) > 0, (w, h)
+        for G, got in outs.items():
+            np.testing.assert_array_equal(got[0], ref[0], err_msg=f"{w}x{h} G={G} image")
+            assert got[1] == ref[1], (w, h, G)
+            np.testing.assert_array_equal(got[2], ref[2], err_msg=f"{w}x{h} G={G} dtf")
+            np.testing.assert_array_equal(got[3], ref[3], err_msg=f"{w}x{h} G={G} dsigma")
