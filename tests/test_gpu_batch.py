@@ -31,6 +31,10 @@ VIEWS = [  # timestep, width, height, yaw, pitch
     (2, 80, 80, -1.1, 0.4),
     (0, 33, 57, 2.5, -0.3),
     (2, 128, 96, 3.0, 0.1),
+    # views smaller than one pixel tile of the batch caster
+    (1, 1, 1, 0.3, 0.1),
+    (2, 3, 1, -0.4, 0.0),
+    (0, 5, 2, 1.2, 0.2),
 ]
 
 
