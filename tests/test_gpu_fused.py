@@ -266,3 +266,29 @@ def test_slices_blend_line_illumination(ctx):
     img, ref, d, nsamp, total = _blend_case(ctx, c, 36, 30, illum=vv.ILLUM_MALLO, luts=orc.illum_luts())
     assert total > 0
     assert d.max() <= 1 and np.count_nonzero(d) <= 0.001 * d.size
+
+
+@pytest.mark.parametrize("slices", [False, True])
+def test_ray_groups_identical_for_tiny_views(ctx, slices):
+    # views smaller than one pixel tile of the ray-group kernels: most lanes
+    # of a block have no pixel
+    c = lic_setup(ctx, n=12, kind="rankine", L=4, noise_kind=vv.NOISE_UNIFORM, gradient_scale=3.0)
+    p = c["params"]
+    p.tf_index = vv.TF_INDEX_MAG
+    p.step_size_vol = 1.0 / 64.0
+    ctx.set_lic_params(p)
+    cen = ctx.field_geometry()[2]
+    ctx.set_tf(*vv.fields.tf_magnitude(0.1, 0.6))
+    render = ctx.render_slices if slices else ctx.render_fused
+    for w, h in ((1, 1), (1, 7), (3, 1), (65, 3)):
+        cam = vv.default_camera(w, h, cen)
+        imgs = {}
+        for G in (1, 2, 8, 64):
+            ctx.set_tuning("render.ray_group", G)
+            imgs[G] = render(cam, count_samples=True)
+        ctx.set_tuning("render.ray_group", 0)
+        ref, nref = imgs[1]
+        assert nref > 0, (w, h)
+        for G, (img, n) in imgs.items():
+            np.testing.assert_array_equal(img, ref, err_msg=f"{w}x{h} G={G}")
+            assert n == nref, (w, h, G, n, nref)
